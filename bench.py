@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 tensor-quadrature Coulomb3D apply on one MI355X.
+
+Workload (the reference's config 1 shape, which fits one GPU): density = sum
+of 4 normalised Gaussians (exponents U[0.5,5] bohr^-2, centres U[-2,2]^3) on a
+97^3 cube over [-8,8]^3 built from 7-point LIP cells (16 cells, h = 1/6);
+40-point t-quadrature (20 Gauss-Legendre points on [0,2] + 20 logarithmic on
+[2,500]); coda on.  One step = one apply of the operator to the resident cube
+(Operator3D_transform_cube, src/bubbles/function3d.F90:4230-4349) with input
+and output already in HBM.
+
+Prints one JSON line: throughput in Gpts/s (output points per second), the
+roofline entry of the dominant kernel (algorithmic FP64 FLOPs per launch over
+its event-timed average duration, against the 78.6 TF FP64 MFMA peak), the
+error against the CPU oracle and the analytic potential, and the CPU baseline
+(the oracle's single-threaded FP64 apply on this host).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Coulomb/Helmholtz apply Gpts/s & %MFMA peak; SCF time-to-1e-6 Ha vs plain FP64"
+FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-reps", type=int, default=2)
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the apply would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+
+    n_cell, nlip, T = 16, 7, 40
+    ax = O.Axis.ncell_fixed(-8.0, 8.0, nlip, n_cell)
+    N = ax.n
+    rng = np.random.default_rng(0xDA6E0001)
+    alpha = rng.uniform(0.5, 5.0, 4)
+    centers = rng.uniform(-2.0, 2.0, (4, 3))
+    q = np.ones(4)
+    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
+
+    # the library's own context stream carries the kernels and the timing events
+    ctx = dage_amd.Context(0)
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, nlip)
+    quad = dage_amd.GaussQuad(nlin=20, nlog=20, tlin=2.0, tlog=500.0)
+    op = dage_amd.Coulomb3D(ctx, grid, grid, quad)
+
+    d_in = dage_amd.DeviceCube(ctx, rho.shape, rho)
+    d_out = dage_amd.DeviceCube(ctx, rho.shape)
+
+    for _ in range(args.warmup):
+        op.transform_cuda_cube(d_in, d_out)
+    ctx.synchronize()
+
+    ctx.profile(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.time_begin()
+    for _ in range(args.steps):
+        op.transform_cuda_cube(d_in, d_out)
+    ms_total = ctx.time_end()
+    wall = time.perf_counter() - t0
+    ms_step = ms_total / args.steps
+
+    kernels = {}
+    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel"):
+        tot, n = ctx.profile_query(name)
+        kernels[name] = (tot / max(n, 1), n // args.steps)
+    ctx.profile(False)
+
+    V = d_out.download()
+    pts = float(N) ** 3
+    gpts = pts / (ms_step * 1e-3) / 1e9
+    flop_mode = 2.0 * T * float(N) ** 4  # each mode product, per apply
+    flops_apply = 3.0 * flop_mode
+
+    # dominant kernel: the GEMM pass with the largest time per apply
+    per_apply = {k: v[0] * v[1] for k, v in kernels.items() if k.startswith("gemm")}
+    dom = max(per_apply, key=per_apply.get)
+    dom_ms, dom_launches = kernels[dom]
+    dom_flops = flop_mode / max(dom_launches, 1)
+    achieved_tf = dom_flops / (dom_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "kernel": dom,
+        "achieved": round(achieved_tf, 3),
+        "peak": FP64_MFMA_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tf / FP64_MFMA_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(dom_ms, 4),
+        "algorithmic_flops_per_launch": dom_flops,
+    }
+
+    # parity against the oracle (same operator, CPU) and the analytic potential
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, quad.tp)
+    w = O.coulomb_weights(quad.tw)
+    cpu_times = []
+    reps = 1 if args.no_cpu_baseline else max(1, args.cpu_reps)
+    ref = None
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        ref = O.transform_cube(fx, fy, fz, w, O.coda(500.0), rho)
+        cpu_times.append(time.perf_counter() - c0)
+    err_oracle = float(np.max(np.abs(V - ref)) / np.max(np.abs(ref)))
+    Va = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
+    err_analytic = float(np.max(np.abs(V - Va)) / np.max(np.abs(Va)))
+    cpu_s = min(cpu_times)
+    cpu_baseline = None
+    if not args.no_cpu_baseline:
+        cpu_baseline = {
+            "value": round(pts / cpu_s / 1e9, 6),
+            "unit": "Gpts/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"full 97^3 T=40 apply with the oracle's C restatement of "
+                      f"Operator3D_transform_cube (1 thread, best of {reps}: {cpu_s:.2f} s)",
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": round(gpts, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "config 1: plain Coulomb3D apply, FP64, 97^3 cube over [-8,8]^3, "
+                        "nlip 7, T=40 (20 lin [0,2] + 20 log [2,500]), 4 Gaussians, coda on",
+            "n": N, "T": T, "nlip": nlip, "precision": "fp64",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+        "apply_tflops": round(flops_apply / (ms_step * 1e-3) / 1e12, 3),
+        "apply_mfma_frac": round(flops_apply / (ms_step * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4),
+        "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kernels.items()},
+        "max_rel_err_vs_oracle": err_oracle,
+        "max_rel_err_vs_analytic": err_analytic,
+        "wall_s_timed": round(wall, 4),
+    }
+    print(json.dumps(line), flush=True)
+    for obj in (d_in, d_out, op, grid, ctx):
+        obj.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,14 @@
+"""dage_amd -- MI355X (gfx950) implementation of DAGE's operator-application
+hot path: the tensor-quadrature Coulomb3D/Helmholtz3D apply on the cube.
+
+The product is the C-ABI library dage_amd/libdage_ops.so (include/dage_ops.h);
+this package is its host-side mirror of the reference's Operator3D API.
+"""
+from ._lib import DAGE_OP_NO_CODA, DAGE_PREC_FP64, DageError, build, load
+from .operators import (Context, Coulomb3D, DeviceCube, GaussQuad, Grid3D,
+                        Helmholtz3D, Operator3D)
+
+__all__ = [
+    "Context", "Coulomb3D", "DeviceCube", "GaussQuad", "Grid3D", "Helmholtz3D", "Operator3D",
+    "DageError", "DAGE_OP_NO_CODA", "DAGE_PREC_FP64", "build", "load",
+]

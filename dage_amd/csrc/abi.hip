@@ -1,0 +1,591 @@
+// C-ABI entry points (include/dage_ops.h).  Argument checking, handle
+// lifetime, host<->device staging and profiling live here; the numerics are in
+// host_math.cpp, matrices.hip, apply_f64.hip and reduce.hip.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "dage_internal.hpp"
+
+namespace dage {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+const char *last_error() { return g_err; }
+
+namespace {
+std::mutex g_ref_mutex;
+}
+
+void ctx_retain(dage_ctx ctx) {
+    std::lock_guard<std::mutex> lock(g_ref_mutex);
+    ++ctx->refs;
+}
+
+void ctx_release(dage_ctx ctx) {
+    bool last;
+    {
+        std::lock_guard<std::mutex> lock(g_ref_mutex);
+        last = (--ctx->refs == 0);
+    }
+    if (!last) return;
+    (void)hipSetDevice(ctx->device);
+    // an external stream set by the caller may already be gone; only the
+    // library's own stream is known to be alive here
+    (void)hipStreamSynchronize(ctx->own_stream);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    if (ctx->timer_start) (void)hipEventDestroy(ctx->timer_start);
+    if (ctx->timer_stop) (void)hipEventDestroy(ctx->timer_stop);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+void prof_begin(dage_ctx ctx, const char *name, hipEvent_t *start) {
+    (void)name;
+    *start = nullptr;
+    if (!ctx->profiling) return;
+    if (hipEventCreate(start) != hipSuccess) {
+        *start = nullptr;
+        return;
+    }
+    (void)hipEventRecord(*start, ctx->stream);
+}
+
+void prof_end(dage_ctx ctx, const char *name, hipEvent_t start) {
+    if (!ctx->profiling || !start) return;
+    hipEvent_t stop;
+    if (hipEventCreate(&stop) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return;
+    }
+    (void)hipEventRecord(stop, ctx->stream);
+    ctx->pending.push_back({start, stop, name});
+}
+
+int prof_resolve(dage_ctx ctx) {
+    if (ctx->pending.empty()) return DAGE_OK;
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.start, p.stop);
+        auto &e = ctx->profile[p.name];
+        e.total_ms += ms;
+        e.launches += 1;
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    ctx->pending.clear();
+    return DAGE_OK;
+}
+
+}  // namespace dage
+
+using namespace dage;
+
+namespace {
+
+int use_device(dage_ctx ctx) {
+    DAGE_REQUIRE(ctx != nullptr, DAGE_ERR_ARG, "null context handle");
+    DAGE_HIP_CHECK(hipSetDevice(ctx->device));
+    return DAGE_OK;
+}
+
+void free_operator_buffers(dage_operator op) {
+    double **bufs[] = {&op->d_fx, &op->d_fy, &op->d_fz, &op->d_gz,
+                       &op->d_w,  &op->d_v,  &op->d_in, &op->d_out};
+    for (double **b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    op->ws_bytes = 0;
+}
+
+int alloc_dev(double **p, size_t elems) {
+    DAGE_HIP_CHECK(hipMalloc((void **)p, sizeof(double) * (elems ? elems : 1)));
+    return DAGE_OK;
+}
+
+bool same_axis(const Axis &a, const Axis &b) {
+    return a.ncell == b.ncell && a.nlip == b.nlip && a.grid_type == b.grid_type &&
+           a.points == b.points;
+}
+
+int upload_weights(dage_operator op) {
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_w, op->w.data(), sizeof(double) * op->ntp,
+                                  hipMemcpyHostToDevice, op->ctx->stream));
+    // op->w is a member vector that outlives the copy; later host writes to
+    // it go through set_weights, which synchronises first.
+    return DAGE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dage_abi_version(void) { return DAGE_ABI_VERSION; }
+
+const char *dage_last_error(void) { return last_error(); }
+
+// ------------------------------------------------------------------ context
+int dage_ctx_create(int device, dage_ctx *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    DAGE_HIP_CHECK(hipGetDeviceCount(&count));
+    DAGE_REQUIRE(device >= 0 && device < count, DAGE_ERR_ARG,
+                 "device %d out of range (%d visible)", device, count);
+    DAGE_HIP_CHECK(hipSetDevice(device));
+    auto *ctx = new (std::nothrow) dage_ctx_s();
+    DAGE_REQUIRE(ctx != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        set_error("hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
+        return DAGE_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return DAGE_OK;
+}
+
+int dage_ctx_destroy(dage_ctx ctx) {
+    if (!ctx) return DAGE_OK;
+    ctx_release(ctx);
+    return DAGE_OK;
+}
+
+int dage_ctx_set_stream(dage_ctx ctx, void *stream) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_TRY(prof_resolve(ctx));
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return DAGE_OK;
+}
+
+int dage_ctx_get_stream(dage_ctx ctx, void **stream) {
+    DAGE_REQUIRE(ctx && stream, DAGE_ERR_ARG, "null argument");
+    *stream = ctx->stream;
+    return DAGE_OK;
+}
+
+int dage_ctx_synchronize(dage_ctx ctx) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return DAGE_OK;
+}
+
+int dage_ctx_profile(dage_ctx ctx, int enable) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_TRY(prof_resolve(ctx));
+    ctx->profile.clear();
+    ctx->profiling = enable != 0;
+    return DAGE_OK;
+}
+
+int dage_ctx_profile_query(dage_ctx ctx, const char *name, double *total_ms, int *launches) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(name && total_ms && launches, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(prof_resolve(ctx));
+    auto it = ctx->profile.find(name);
+    *total_ms = it == ctx->profile.end() ? 0.0 : it->second.total_ms;
+    *launches = it == ctx->profile.end() ? 0 : it->second.launches;
+    return DAGE_OK;
+}
+
+int dage_ctx_time_begin(dage_ctx ctx) {
+    DAGE_TRY(use_device(ctx));
+    if (!ctx->timer_start) DAGE_HIP_CHECK(hipEventCreate(&ctx->timer_start));
+    if (!ctx->timer_stop) DAGE_HIP_CHECK(hipEventCreate(&ctx->timer_stop));
+    DAGE_HIP_CHECK(hipEventRecord(ctx->timer_start, ctx->stream));
+    return DAGE_OK;
+}
+
+int dage_ctx_time_end(dage_ctx ctx, double *ms) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(ms != nullptr, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(ctx->timer_start && ctx->timer_stop, DAGE_ERR_STATE,
+                 "dage_ctx_time_end without dage_ctx_time_begin");
+    DAGE_HIP_CHECK(hipEventRecord(ctx->timer_stop, ctx->stream));
+    DAGE_HIP_CHECK(hipEventSynchronize(ctx->timer_stop));
+    float f = 0.f;
+    DAGE_HIP_CHECK(hipEventElapsedTime(&f, ctx->timer_start, ctx->timer_stop));
+    *ms = f;
+    return DAGE_OK;
+}
+
+// -------------------------------------------------------------- device cube
+int dage_cube_create(dage_ctx ctx, const int shape[3], dage_cube *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(shape && shape[0] > 0 && shape[1] > 0 && shape[2] > 0, DAGE_ERR_ARG,
+                 "cube shape must be positive");
+    auto *c = new (std::nothrow) dage_cube_s();
+    DAGE_REQUIRE(c != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    c->ctx = ctx;
+    for (int a = 0; a < 3; ++a) c->shape[a] = shape[a];
+    hipError_t e = hipMalloc((void **)&c->d, sizeof(double) * c->count());
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipMalloc of a %d x %d x %d cube failed: %s", shape[0], shape[1], shape[2],
+                  hipGetErrorString(e));
+        return DAGE_ERR_ALLOC;
+    }
+    ctx_retain(ctx);
+    *out = c;
+    return DAGE_OK;
+}
+
+int dage_cube_destroy(dage_cube c) {
+    if (!c) return DAGE_OK;
+    dage_ctx ctx = c->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(c->d);
+    delete c;
+    ctx_release(ctx);
+    return DAGE_OK;
+}
+
+int dage_cube_upload(dage_cube c, const double *host) {
+    DAGE_REQUIRE(c && host, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(c->ctx));
+    DAGE_HIP_CHECK(hipMemcpyAsync(c->d, host, sizeof(double) * c->count(),
+                                  hipMemcpyHostToDevice, c->ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+    return DAGE_OK;
+}
+
+int dage_cube_download(dage_cube c, double *host) {
+    DAGE_REQUIRE(c && host, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(c->ctx));
+    DAGE_HIP_CHECK(hipMemcpyAsync(host, c->d, sizeof(double) * c->count(),
+                                  hipMemcpyDeviceToHost, c->ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+    return DAGE_OK;
+}
+
+int dage_cube_set_to_zero(dage_cube c) {
+    DAGE_REQUIRE(c != nullptr, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(c->ctx));
+    DAGE_TRY(fill_zero(c->ctx, c->d, c->count()));
+    return DAGE_OK;
+}
+
+int dage_cube_device_pointer(dage_cube c, double **dev) {
+    DAGE_REQUIRE(c && dev, DAGE_ERR_ARG, "null argument");
+    *dev = c->d;
+    return DAGE_OK;
+}
+
+// --------------------------------------------------------------- quadrature
+int dage_gauss_quad(int nlin, int nlog, double tstart, double tlin, double tlog, double *tp,
+                    double *tw) {
+    DAGE_REQUIRE(nlin >= 1 && nlog >= 0 && tp && tw, DAGE_ERR_ARG, "invalid quadrature sizes");
+    DAGE_REQUIRE(tlin > tstart && (nlog == 0 || (tlog > tlin && tlin > 0.0)), DAGE_ERR_ARG,
+                 "quadrature needs tstart < tlin < tlog");
+    gauss_quad(nlin, nlog, tstart, tlin, tlog, tp, tw);
+    return DAGE_OK;
+}
+
+// --------------------------------------------------------------------- grid
+int dage_grid3d_create(dage_ctx ctx, const double qmin[3], const int ncell[3], int nlip,
+                       const double *sx, const double *sy, const double *sz, int grid_type,
+                       dage_grid3d *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(qmin && ncell && sx && sy && sz, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(nlip >= 2 && nlip <= 7, DAGE_ERR_UNSUPPORTED,
+                 "nlip %d outside 2..7 (the reference's closed forms stop at expo_6)", nlip);
+    DAGE_REQUIRE(grid_type == 1 || (grid_type == 2 && (nlip == 6 || nlip == 7)),
+                 DAGE_ERR_UNSUPPORTED, "grid_type %d with nlip %d is not supported", grid_type,
+                 nlip);
+    const double *steps[3] = {sx, sy, sz};
+    for (int a = 0; a < 3; ++a) {
+        DAGE_REQUIRE(ncell[a] >= 1, DAGE_ERR_ARG, "ncell[%d] = %d", a, ncell[a]);
+        for (int c = 0; c < ncell[a]; ++c)
+            DAGE_REQUIRE(steps[a][c] > 0.0 && std::isfinite(steps[a][c]), DAGE_ERR_ARG,
+                         "step %d of axis %d is not positive", c, a);
+    }
+    auto *g = new (std::nothrow) dage_grid3d_s();
+    DAGE_REQUIRE(g != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    g->ctx = ctx;
+    ctx_retain(ctx);
+    for (int a = 0; a < 3; ++a) {
+        if (make_axis(qmin[a], ncell[a], nlip, steps[a], grid_type, &g->axis[a])) {
+            ctx_release(ctx);
+            delete g;
+            set_error("invalid grid axis %d", a);
+            return DAGE_ERR_ARG;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        const Axis &ax = g->axis[a];
+        double base[kMaxNlip];
+        lip_integrals(nlip, grid_type, base);
+        std::vector<double> w(ax.n(), 0.0);
+        for (int c = 0; c < ax.ncell; ++c)
+            for (int j = 0; j < nlip; ++j) w[c * (nlip - 1) + j] += base[j] * ax.steps[c];
+        if (hipMalloc((void **)&g->d_weights[a], sizeof(double) * ax.n()) != hipSuccess ||
+            hipMemcpy(g->d_weights[a], w.data(), sizeof(double) * ax.n(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            for (auto &p : g->d_weights)
+                if (p) (void)hipFree(p);
+            ctx_release(ctx);
+            delete g;
+            set_error("device allocation for grid weights failed");
+            return DAGE_ERR_ALLOC;
+        }
+    }
+    *out = g;
+    return DAGE_OK;
+}
+
+int dage_grid3d_create_equidistant(dage_ctx ctx, const double ranges[6], int nlip,
+                                   double stepmax, int grid_type, dage_grid3d *out) {
+    DAGE_REQUIRE(ranges && stepmax > 0.0 && nlip >= 2, DAGE_ERR_ARG, "invalid arguments");
+    double qmin[3];
+    int ncell[3];
+    std::vector<double> steps[3];
+    for (int a = 0; a < 3; ++a) {
+        const double r0 = ranges[2 * a], r1 = ranges[2 * a + 1];
+        DAGE_REQUIRE(r1 > r0, DAGE_ERR_ARG, "empty range on axis %d", a);
+        ncell[a] = (int)std::ceil((r1 - r0) / (stepmax * (nlip - 1)));
+        const double s = (r1 - r0) / (ncell[a] * (nlip - 1));
+        steps[a].assign(ncell[a], s);
+        qmin[a] = r0;
+    }
+    return dage_grid3d_create(ctx, qmin, ncell, nlip, steps[0].data(), steps[1].data(),
+                              steps[2].data(), grid_type, out);
+}
+
+int dage_grid3d_destroy(dage_grid3d g) {
+    if (!g) return DAGE_OK;
+    dage_ctx ctx = g->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : g->d_weights)
+        if (p) (void)hipFree(p);
+    delete g;
+    ctx_release(ctx);
+    return DAGE_OK;
+}
+
+int dage_grid3d_shape(dage_grid3d g, int shape[3]) {
+    DAGE_REQUIRE(g && shape, DAGE_ERR_ARG, "null argument");
+    for (int a = 0; a < 3; ++a) shape[a] = g->shape(a);
+    return DAGE_OK;
+}
+
+int dage_grid3d_coord(dage_grid3d g, int axis, double *points) {
+    DAGE_REQUIRE(g && points && axis >= 0 && axis < 3, DAGE_ERR_ARG, "invalid argument");
+    std::memcpy(points, g->axis[axis].points.data(), sizeof(double) * g->shape(axis));
+    return DAGE_OK;
+}
+
+int dage_grid3d_weights(dage_grid3d g, int axis, double *weights) {
+    DAGE_REQUIRE(g && weights && axis >= 0 && axis < 3, DAGE_ERR_ARG, "invalid argument");
+    DAGE_TRY(use_device(g->ctx));
+    DAGE_HIP_CHECK(hipMemcpy(weights, g->d_weights[axis], sizeof(double) * g->shape(axis),
+                             hipMemcpyDeviceToHost));
+    return DAGE_OK;
+}
+
+// ----------------------------------------------------------------- operator
+int dage_coulomb3d_create(dage_ctx ctx, dage_grid3d gin, dage_grid3d gout, const double *tp,
+                          const double *tw, int ntp, double tlog, int flags,
+                          dage_operator *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(gin && gout && tp && tw && ntp >= 1, DAGE_ERR_ARG, "invalid arguments");
+    DAGE_REQUIRE(gin->ctx == ctx && gout->ctx == ctx, DAGE_ERR_ARG,
+                 "grids belong to another context");
+    DAGE_REQUIRE(tlog > 0.0, DAGE_ERR_ARG, "tlog must be positive");
+    for (int a = 0; a < 3; ++a)
+        DAGE_REQUIRE(gin->axis[a].nlip == gout->axis[a].nlip &&
+                         gin->axis[a].grid_type == gout->axis[a].grid_type,
+                     DAGE_ERR_ARG, "input and output grids must share nlip and grid_type");
+    for (int i = 0; i < ntp; ++i)
+        DAGE_REQUIRE(std::isfinite(tp[i]) && tp[i] >= 0.0 && std::isfinite(tw[i]), DAGE_ERR_ARG,
+                     "t-point %d is not a finite non-negative value", i);
+    auto *op = new (std::nothrow) dage_operator_s();
+    DAGE_REQUIRE(op != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    op->ctx = ctx;
+    op->gin = gin;
+    op->gout = gout;
+    op->ntp = ntp;
+    op->tp.assign(tp, tp + ntp);
+    op->tw.assign(tw, tw + ntp);
+    op->w.resize(ntp);
+    const double c = two_over_sqrtpi();
+    for (int i = 0; i < ntp; ++i) op->w[i] = c * tw[i];
+    op->coda = pi() / (tlog * tlog);
+    op->flags = flags;
+    bool same = true;
+    for (int a = 0; a < 3; ++a) {
+        op->din[a] = gin->shape(a);
+        op->dout[a] = gout->shape(a);
+        same = same && same_axis(gin->axis[a], gout->axis[a]);
+    }
+    if (!same && !(flags & DAGE_OP_NO_CODA)) {
+        delete op;
+        set_error("the coda term needs identical input and output grids (cube_project onto "
+                  "another grid is not implemented); pass DAGE_OP_NO_CODA");
+        return DAGE_ERR_UNSUPPORTED;
+    }
+    const size_t nfx = (size_t)op->dout[0] * op->din[0] * ntp;
+    const size_t nfy = (size_t)op->din[1] * op->dout[1] * ntp;
+    const size_t nfz = (size_t)op->din[2] * op->dout[2] * ntp;
+    int rc = alloc_dev(&op->d_fx, nfx);
+    if (rc == DAGE_OK) rc = alloc_dev(&op->d_fy, nfy);
+    if (rc == DAGE_OK) rc = alloc_dev(&op->d_fz, nfz);
+    if (rc == DAGE_OK) rc = alloc_dev(&op->d_gz, nfz);
+    if (rc == DAGE_OK) rc = alloc_dev(&op->d_w, ntp);
+    if (rc == DAGE_OK) rc = upload_weights(op);
+    if (rc == DAGE_OK) rc = generate_matrices(op);
+    if (rc == DAGE_OK) rc = relayout_fz(op);
+    if (rc == DAGE_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            set_error("operator setup failed: %s", hipGetErrorString(e));
+            rc = DAGE_ERR_HIP;
+        }
+    }
+    if (rc != DAGE_OK) {
+        free_operator_buffers(op);
+        delete op;
+        return rc;
+    }
+    // the grids are only read while the matrices are generated
+    op->gin = op->gout = nullptr;
+    ctx_retain(ctx);
+    *out = op;
+    return DAGE_OK;
+}
+
+int dage_operator_set_weights(dage_operator op, const double *w) {
+    DAGE_REQUIRE(op && w, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(op->ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(op->ctx->stream));
+    op->w.assign(w, w + op->ntp);
+    return upload_weights(op);
+}
+
+int dage_operator_get_weights(dage_operator op, double *w) {
+    DAGE_REQUIRE(op && w, DAGE_ERR_ARG, "null argument");
+    std::memcpy(w, op->w.data(), sizeof(double) * op->ntp);
+    return DAGE_OK;
+}
+
+int dage_operator_set_energy(dage_operator op, double energy) {
+    DAGE_REQUIRE(op != nullptr, DAGE_ERR_ARG, "null operator");
+    DAGE_REQUIRE(std::isfinite(energy), DAGE_ERR_ARG, "energy is not finite");
+    DAGE_TRY(use_device(op->ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(op->ctx->stream));
+    const double c = two_over_sqrtpi();
+    for (int i = 0; i < op->ntp; ++i) {
+        double wi = op->tw[i];
+        if (energy < -0.005) wi = wi * std::exp(2.0 * energy / (4.0 * (op->tp[i] * op->tp[i])));
+        op->w[i] = c * wi;
+    }
+    return upload_weights(op);
+}
+
+int dage_operator_set_matrices(dage_operator op, const double *fx, const double *fy,
+                               const double *fz) {
+    DAGE_REQUIRE(op && fx && fy && fz, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(op->ctx));
+    const size_t T = op->ntp;
+    hipStream_t s = op->ctx->stream;
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fx, fx, 8 * (size_t)op->dout[0] * op->din[0] * T,
+                                  hipMemcpyHostToDevice, s));
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fy, fy, 8 * (size_t)op->din[1] * op->dout[1] * T,
+                                  hipMemcpyHostToDevice, s));
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fz, fz, 8 * (size_t)op->din[2] * op->dout[2] * T,
+                                  hipMemcpyHostToDevice, s));
+    DAGE_TRY(relayout_fz(op));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    return DAGE_OK;
+}
+
+int dage_operator_get_matrices(dage_operator op, double *fx, double *fy, double *fz) {
+    DAGE_REQUIRE(op && fx && fy && fz, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(op->ctx));
+    const size_t T = op->ntp;
+    hipStream_t s = op->ctx->stream;
+    DAGE_HIP_CHECK(hipMemcpyAsync(fx, op->d_fx, 8 * (size_t)op->dout[0] * op->din[0] * T,
+                                  hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipMemcpyAsync(fy, op->d_fy, 8 * (size_t)op->din[1] * op->dout[1] * T,
+                                  hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipMemcpyAsync(fz, op->d_fz, 8 * (size_t)op->din[2] * op->dout[2] * T,
+                                  hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    return DAGE_OK;
+}
+
+int dage_operator_apply_device(dage_operator op, const double *in, double *out, int precision) {
+    DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(precision == DAGE_PREC_FP64, DAGE_ERR_UNSUPPORTED,
+                 "precision mode %d is not implemented", precision);
+    DAGE_REQUIRE((const void *)in != (const void *)out, DAGE_ERR_ARG,
+                 "in-place apply is not supported");
+    DAGE_TRY(use_device(op->ctx));
+    return apply_f64(op, in, out);
+}
+
+int dage_operator_apply(dage_operator op, const double *in, double *out, int precision) {
+    DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(op->ctx));
+    const size_t nin = (size_t)op->din[0] * op->din[1] * op->din[2];
+    const size_t nout = (size_t)op->dout[0] * op->dout[1] * op->dout[2];
+    if (!op->d_in) DAGE_TRY(alloc_dev(&op->d_in, nin));
+    if (!op->d_out) DAGE_TRY(alloc_dev(&op->d_out, nout));
+    hipStream_t s = op->ctx->stream;
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_in, in, 8 * nin, hipMemcpyHostToDevice, s));
+    DAGE_TRY(dage_operator_apply_device(op, op->d_in, op->d_out, precision));
+    DAGE_HIP_CHECK(hipMemcpyAsync(out, op->d_out, 8 * nout, hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    return DAGE_OK;
+}
+
+int dage_operator_workspace_bytes(dage_operator op, size_t *bytes) {
+    DAGE_REQUIRE(op && bytes, DAGE_ERR_ARG, "null argument");
+    return apply_f64_workspace(op, bytes);
+}
+
+int dage_operator_destroy(dage_operator op) {
+    if (!op) return DAGE_OK;
+    dage_ctx ctx = op->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_operator_buffers(op);
+    delete op;
+    ctx_release(ctx);
+    return DAGE_OK;
+}
+
+// ---------------------------------------------------------------- cube sums
+int dage_cube_integrate_device(dage_grid3d g, const double *f, const double *h, double *result) {
+    DAGE_REQUIRE(g && f && result, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(g->ctx));
+    return cube_integrate(g, f, h, result);
+}
+
+}  // extern "C"

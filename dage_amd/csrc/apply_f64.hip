@@ -1,0 +1,392 @@
+// FP64 tensor-quadrature apply on gfx950 matrix cores.
+//
+// out(xo,yo,zo) = sum_t w_t sum_{xi,yi,zi} Fx_t(xo,xi) Fy_t(yi,yo) Fz_t(zi,zo)
+//                 in(xi,yi,zi)  + coda*in
+// (Operator3D_transform_cube, src/bubbles/function3d.F90:4230-4349, and the
+// cuBLAS path it replaces, function3d.F90:4354-4480 / cuda_blas.cu:141-330).
+//
+// Three passes of one tiled FP64 MFMA GEMM (v_mfma_f64_16x16x4f64, LDS-staged
+// operand panels, register-prefetched next panel):
+//   1. x mode, batched over t:  V_t(xo,yi,zi) = sum_xi Fx_t(xo,xi) in(xi,yi,zi)
+//   2. y mode, batched over t:  U((xo,yo),(zi,t)) = w_t sum_yi V_t(xo,yi,zi) Fy_t(yi,yo)
+//   3. z mode with the t-sum folded into K = (zi,t):
+//        out((xo,yo),zo) = sum_(zi,t) U((xo,yo),(zi,t)) Gz((zi,t),zo) + coda*in
+//      split over K for occupancy, then a deterministic reduction adds the
+//      splits and the coda term.
+// t is processed in chunks so that V and U stay within a workspace budget.
+#include "dage_internal.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace dage {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;  // 4 wavefronts of 64
+constexpr int kBK = 16;        // K panel depth staged in LDS
+
+struct Gemm {
+    const double *A;
+    const double *B;
+    double *C;
+    int M, N, K;
+    int M0;                   // m = m_lo + M0*m_hi (two-level row index)
+    long sAm0, sAm1, sAk;     // A(m,k)
+    long sBk, sBn;            // B(k,n)
+    long sCm0, sCm1, sCn;     // C(m,n)
+    long bA, bB, bC;          // strides per blockIdx.z
+    const double *alpha;      // alpha[blockIdx.z] (null: 1)
+    int Ksplit;               // >0: blockIdx.z selects a K range of this size
+    int accumulate;           // C += result instead of C = result
+};
+
+// C tile of (64*MW) x (16*NB) per block; wavefront w owns rows
+// [w*16*MW, (w+1)*16*MW) and all 16*NB columns.  MFMA f64 16x16x4 operand
+// layout on gfx950 (measured with a one-hot probe): lane l holds A(l%16, l/16)
+// and B(l/16, l%16); result register r of lane l is D(4*r + l/16, l%16).
+template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG>
+__global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
+    constexpr int BM = 64 * MW;
+    constexpr int BN = 16 * NB;
+    constexpr int A_PER = BM * kBK / kThreads;  // elements per thread
+    constexpr int B_PER = BN * kBK / kThreads;
+    constexpr int LDA = BM + 1;  // +1 double breaks power-of-two strides
+    constexpr int LDB = BN + 1;
+    __shared__ double As[kBK * LDA];
+    __shared__ double Bs[kBK * LDB];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int m0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
+
+    const double *A = g.A;
+    const double *B = g.B;
+    double *C = g.C;
+    double alpha = 1.0;
+    int kbeg = 0, kend = g.K;
+    if (g.Ksplit > 0) {
+        kbeg = blockIdx.z * g.Ksplit;
+        kend = min(g.K, kbeg + g.Ksplit);
+        A += (long)kbeg * g.sAk;
+        B += (long)kbeg * g.sBk;
+        C += (long)blockIdx.z * g.bC;
+    } else {
+        A += (long)blockIdx.z * g.bA;
+        B += (long)blockIdx.z * g.bB;
+        C += (long)blockIdx.z * g.bC;
+        if (g.alpha) alpha = g.alpha[blockIdx.z];
+    }
+    const int K = kend - kbeg;
+
+    // Per-slot global offsets (fixed across K panels).
+    long a_off[A_PER];
+    int a_k[A_PER];
+    int a_sm[A_PER];  // LDS index of the element
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+        const int idx = tid + r * kThreads;
+        int m, k;
+        if (A_MCONTIG) { m = idx % BM; k = idx / BM; }
+        else           { k = idx % kBK; m = idx / kBK; }
+        const int gm = m0 + m;
+        a_k[r] = (gm < g.M) ? k : -1 - k;  // negative: row out of range
+        const int mlo = gm % g.M0, mhi = gm / g.M0;
+        a_off[r] = (long)mlo * g.sAm0 + (long)mhi * g.sAm1 + (long)k * g.sAk;
+        a_sm[r] = k * LDA + m;
+    }
+    long b_off[B_PER];
+    int b_k[B_PER];
+    int b_sm[B_PER];
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+        const int idx = tid + r * kThreads;
+        int n, k;
+        if (B_KCONTIG) { k = idx % kBK; n = idx / kBK; }
+        else           { n = idx % BN; k = idx / BN; }
+        const int gn = n0 + n;
+        b_k[r] = (gn < g.N) ? k : -1 - k;
+        b_off[r] = (long)k * g.sBk + (long)gn * g.sBn;
+        b_sm[r] = k * LDB + n;
+    }
+
+    double ra[A_PER], rb[B_PER];
+    auto load_panel = [&](int kp) {
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) {
+            const int k = a_k[r];
+            ra[r] = (k >= 0 && kp + k < K) ? A[a_off[r] + (long)kp * g.sAk] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) {
+            const int k = b_k[r];
+            rb[r] = (k >= 0 && kp + k < K) ? B[b_off[r] + (long)kp * g.sBk] : 0.0;
+        }
+    };
+    auto store_panel = [&]() {
+#pragma unroll
+        for (int r = 0; r < A_PER; ++r) As[a_sm[r]] = ra[r];
+#pragma unroll
+        for (int r = 0; r < B_PER; ++r) Bs[b_sm[r]] = rb[r];
+    };
+
+    d4 acc[MW][NB];
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+    const int wrow = wave * 16 * MW;
+
+    load_panel(0);
+    for (int kp = 0; kp < K; kp += kBK) {
+        store_panel();
+        __syncthreads();
+        if (kp + kBK < K) load_panel(kp + kBK);
+        const int ksteps = min(kBK, K - kp + 3) / 4;  // skip all-padding steps
+        for (int ks = 0; ks < ksteps; ++ks) {
+            const int kk = ks * 4 + lk;
+            double a[MW], b[NB];
+#pragma unroll
+            for (int i = 0; i < MW; ++i) a[i] = As[kk * LDA + wrow + i * 16 + lrow];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) b[j] = Bs[kk * LDB + j * 16 + lrow];
+#pragma unroll
+            for (int i = 0; i < MW; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // Epilogue: register r of lane l is D(4*r + l/16, l%16) of each 16x16 block.
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + wrow + i * 16 + 4 * r + lk;
+            if (gm >= g.M) continue;
+            const long crow = (long)(gm % g.M0) * g.sCm0 + (long)(gm / g.M0) * g.sCm1;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int gn = n0 + j * 16 + lrow;
+                if (gn >= g.N) continue;
+                double *dst = C + crow + (long)gn * g.sCn;
+                const double v = alpha * acc[i][j][r];
+                *dst = g.accumulate ? *dst + v : v;
+            }
+        }
+    }
+}
+
+// out[i] = sum_s part[s*len + i] + coda*in[i]  (in may be null)
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const double *part, int splits,
+                                                            long len, const double *in,
+                                                            double coda, double *out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len;
+         i += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < splits; ++k) s += part[(long)k * len + i];
+        if (in) s += coda * in[i];
+        out[i] = s;
+    }
+}
+
+// Fz (nzi, nzo, T) -> Gz((zi + nzi*t), zo)
+__global__ void __launch_bounds__(256) relayout_fz_kernel(const double *fz, double *gz, int nzi,
+                                                          int nzo, int T) {
+    const long total = (long)nzi * nzo * T;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int zi = (int)(e % nzi);
+        const long r = e / nzi;
+        const int zo = (int)(r % nzo);
+        const int t = (int)(r / nzo);
+        gz[(long)zi + (long)nzi * t + (long)nzi * T * zo] = fz[e];
+    }
+}
+
+// Pick the column-block count NB (<= 8) that wastes the least MFMA width on
+// an output dimension of n points.
+int pick_nb(int n) {
+    int best = 8;
+    double best_cost = 1e30;
+    for (int nb = 8; nb >= 1; --nb) {
+        const int bn = 16 * nb;
+        const int tiles = (n + bn - 1) / bn;
+        const double cost = (double)tiles * bn / n + 0.02 * tiles;  // waste + per-tile cost
+        if (cost < best_cost - 1e-12) { best_cost = cost; best = nb; }
+    }
+    return best;
+}
+
+template <int MW, int NB, bool AM, bool BK>
+void launch_t(const Gemm &g, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((gemm_f64_kernel<MW, NB, AM, BK>), grid, dim3(kThreads), 0, s, g);
+}
+
+template <bool AM, bool BK>
+int launch_nb(const Gemm &g, int nb, int zdim, hipStream_t s) {
+    constexpr int MW = 2;
+    const int BM = 64 * MW;
+    const dim3 grid((g.M + BM - 1) / BM, (g.N + 16 * nb - 1) / (16 * nb), zdim);
+    switch (nb) {
+    case 1: launch_t<MW, 1, AM, BK>(g, grid, s); break;
+    case 2: launch_t<MW, 2, AM, BK>(g, grid, s); break;
+    case 3: launch_t<MW, 3, AM, BK>(g, grid, s); break;
+    case 4: launch_t<MW, 4, AM, BK>(g, grid, s); break;
+    case 5: launch_t<MW, 5, AM, BK>(g, grid, s); break;
+    case 6: launch_t<MW, 6, AM, BK>(g, grid, s); break;
+    case 7: launch_t<MW, 7, AM, BK>(g, grid, s); break;
+    default: launch_t<MW, 8, AM, BK>(g, grid, s); break;
+    }
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, bool a_mcontig, bool b_kcontig,
+                int zdim) {
+    const int nb = pick_nb(g.N);
+    hipEvent_t ev;
+    prof_begin(ctx, name, &ev);
+    int rc;
+    if (a_mcontig && b_kcontig) rc = launch_nb<true, true>(g, nb, zdim, ctx->stream);
+    else if (a_mcontig) rc = launch_nb<true, false>(g, nb, zdim, ctx->stream);
+    else if (b_kcontig) rc = launch_nb<false, true>(g, nb, zdim, ctx->stream);
+    else rc = launch_nb<false, false>(g, nb, zdim, ctx->stream);
+    prof_end(ctx, name, ev);
+    return rc;
+}
+
+// Workspace layout: V | U | P (split-K partials)
+struct Plan {
+    int tchunk;
+    int splits;
+    size_t v_elems, u_elems, p_elems;
+};
+
+Plan make_plan(dage_operator op) {
+    const long nyi = op->din[1], nzi = op->din[2];
+    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
+    size_t budget = 4ull << 30;  // bytes for V + U
+    if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
+    const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
+    Plan p;
+    p.tchunk = (int)std::max<size_t>(1, std::min<size_t>(op->ntp, budget / per_t));
+    const long M3 = nxo * nyo;
+    const int mtiles = (int)((M3 + 127) / 128);
+    const int nb = pick_nb((int)nzo);
+    const int ntiles = (int)((nzo + 16 * nb - 1) / (16 * nb));
+    const long K3 = nzi * p.tchunk;
+    int splits = std::max(1, (1024 + mtiles * ntiles - 1) / (mtiles * ntiles));
+    splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 64));
+    p.splits = splits;
+    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk;
+    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk;
+    p.p_elems = (size_t)M3 * nzo * splits;
+    return p;
+}
+
+}  // namespace
+
+int apply_f64_workspace(dage_operator op, size_t *bytes) {
+    const Plan p = make_plan(op);
+    *bytes = 8 * (p.v_elems + p.u_elems + p.p_elems);
+    return DAGE_OK;
+}
+
+int relayout_fz(dage_operator op) {
+    const int nzi = op->din[2], nzo = op->dout[2], T = op->ntp;
+    const long total = (long)nzi * nzo * T;
+    const int grid = (int)std::min<long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(relayout_fz_kernel, dim3(grid), dim3(256), 0, op->ctx->stream, op->d_fz,
+                       op->d_gz, nzi, nzo, T);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int apply_f64(dage_operator op, const double *in, double *out) {
+    const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
+    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
+    const int T = op->ntp;
+    const Plan plan = make_plan(op);
+    size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
+    if (need > op->ws_bytes) {
+        if (op->d_v) DAGE_HIP_CHECK(hipFree(op->d_v));
+        op->d_v = nullptr;
+        op->ws_bytes = 0;
+        DAGE_HIP_CHECK(hipMalloc((void **)&op->d_v, need));
+        op->ws_bytes = need;
+    }
+    double *V = op->d_v;
+    double *U = V + plan.v_elems;
+    double *P = U + plan.u_elems;
+    hipStream_t s = op->ctx->stream;
+    const long M3 = nxo * nyo;
+
+    for (int t0 = 0; t0 < T; t0 += plan.tchunk) {
+        const int tc = std::min(plan.tchunk, T - t0);
+        // pass 1: V_t^T((yi,zi), xo) = in^T((yi,zi), xi) . Fx_t^T(xi, xo)
+        Gemm g1{};
+        g1.A = in;
+        g1.B = op->d_fx + (long)t0 * nxo * nxi;
+        g1.C = V;
+        g1.M = (int)(nyi * nzi); g1.N = (int)nxo; g1.K = (int)nxi;
+        g1.M0 = g1.M; g1.sAm0 = nxi; g1.sAm1 = 0; g1.sAk = 1;
+        g1.sBk = nxo; g1.sBn = 1;
+        g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1;
+        g1.bA = 0; g1.bB = nxo * nxi; g1.bC = nxo * nyi * nzi;
+        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_xmode", g1, false, false, tc));
+
+        // pass 2: U((xo,yo),(zi,t)) = w_t V_t((xo,zi), yi) . Fy_t(yi, yo)
+        Gemm g2{};
+        g2.A = V;
+        g2.B = op->d_fy + (long)t0 * nyi * nyo;
+        g2.C = U;
+        g2.M = (int)(nxo * nzi); g2.N = (int)nyo; g2.K = (int)nyi;
+        g2.M0 = (int)nxo; g2.sAm0 = 1; g2.sAm1 = nxo * nyi; g2.sAk = nxo;
+        g2.sBk = 1; g2.sBn = nyi;
+        g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo;
+        g2.bA = nxo * nyi * nzi; g2.bB = nyi * nyo; g2.bC = nxo * nyo * nzi;
+        g2.alpha = op->d_w + t0;
+        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_ymode", g2, true, true, tc));
+
+        // pass 3: P_s((xo,yo), zo) (+)= U((xo,yo), k) . Gz(k, zo), k in split s
+        const int K3 = (int)(nzi * tc);
+        const int ks = (K3 + plan.splits - 1) / plan.splits;
+        const int splits = (K3 + ks - 1) / ks;
+        Gemm g3{};
+        g3.A = U;
+        g3.B = op->d_gz + (long)nzi * t0;
+        g3.C = P;
+        g3.M = (int)M3; g3.N = (int)nzo; g3.K = K3;
+        g3.M0 = g3.M; g3.sAm0 = 1; g3.sAm1 = 0; g3.sAk = M3;
+        g3.sBk = 1; g3.sBn = (long)nzi * T;
+        g3.sCm0 = 1; g3.sCm1 = 0; g3.sCn = M3;
+        g3.bC = M3 * nzo;
+        g3.Ksplit = ks;
+        g3.accumulate = (t0 > 0);
+        if (t0 == 0 && splits < plan.splits)
+            DAGE_TRY(fill_zero(op->ctx, P + (size_t)splits * M3 * nzo,
+                               (size_t)(plan.splits - splits) * M3 * nzo));
+        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_zmode", g3, true, true, splits));
+    }
+    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
+    const long len = M3 * nzo;
+    const int grid = (int)std::min<long>((len + 255) / 256, 65536);
+    hipEvent_t ev;
+    prof_begin(op->ctx, "splitk_reduce_kernel", &ev);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid), dim3(256), 0, s, P, plan.splits, len,
+                       coda ? in : nullptr, op->coda, out);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(op->ctx, "splitk_reduce_kernel", ev);
+    return DAGE_OK;
+}
+
+}  // namespace dage

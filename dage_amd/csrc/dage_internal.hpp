@@ -1,0 +1,155 @@
+// Internal declarations of the gfx950 operator-application library.
+// Not part of the C-ABI: the public surface is include/dage_ops.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dage_ops.h"
+
+namespace dage {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char *fmt, ...);
+
+#define DAGE_HIP_CHECK(expr)                                                  \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::dage::set_error("%s failed: %s (%s:%d)", #expr,                 \
+                              hipGetErrorString(e_), __FILE__, __LINE__);     \
+            return DAGE_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+#define DAGE_REQUIRE(cond, code, ...)                                         \
+    do {                                                                      \
+        if (!(cond)) {                                                        \
+            ::dage::set_error(__VA_ARGS__);                                   \
+            return (code);                                                    \
+        }                                                                     \
+    } while (0)
+
+#define DAGE_TRY(expr)                                                        \
+    do {                                                                      \
+        int rc_ = (expr);                                                     \
+        if (rc_ != DAGE_OK) return rc_;                                       \
+    } while (0)
+
+// ------------------------------------------------------------ host helpers
+// Restatements used by the library itself (quadrature, LIP basis, grids).
+// Their reference routines are cited where they are defined (host_math.cpp).
+void gauleg(int ngp, double *x, double *w);
+void gauss_quad(int nlin, int nlog, double tstart, double tlin, double tlog,
+                double *tp, double *tw);
+int gauss_lobatto_grid(int n, double left, double right, double *tp);
+void lip_first_last(int nlip, double *first, double *last);
+int lip_coeffs(int nlip, int grid_type, double *c);  // c[j + nlip*k]
+int lip_integrals(int nlip, int grid_type, double *v);
+double two_over_sqrtpi();
+double pi();
+
+constexpr int kMaxNlip = 8;
+
+struct Axis {
+    double qmin = 0.0;
+    int ncell = 0;
+    int nlip = 0;
+    int grid_type = 1;
+    std::vector<double> steps;   // ncell
+    std::vector<double> points;  // ncell*(nlip-1)+1
+    std::vector<double> celld;   // ncell
+    int n() const { return ncell * (nlip - 1) + 1; }
+};
+int make_axis(double qmin, int ncell, int nlip, const double *steps,
+              int grid_type, Axis *ax);
+
+// --------------------------------------------------------------- profiling
+struct ProfileEntry {
+    double total_ms = 0.0;
+    int launches = 0;
+};
+
+}  // namespace dage
+
+struct dage_ctx_s {
+    // Grids and operators hold a reference; the context is freed when the
+    // last holder (including the creator's dage_ctx_destroy) lets go.
+    int refs = 1;
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    // pending (start, stop, name) event triples, resolved on query
+    struct Pending {
+        hipEvent_t start, stop;
+        std::string name;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, dage::ProfileEntry> profile;
+    hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+};
+
+struct dage_cube_s {
+    dage_ctx ctx = nullptr;
+    int shape[3] = {0, 0, 0};
+    double *d = nullptr;
+    size_t count() const { return (size_t)shape[0] * shape[1] * shape[2]; }
+};
+
+struct dage_grid3d_s {
+    dage_ctx ctx = nullptr;
+    dage::Axis axis[3];
+    double *d_weights[3] = {nullptr, nullptr, nullptr};  // LIP weights (device)
+    int shape(int a) const { return axis[a].n(); }
+};
+
+struct dage_operator_s {
+    dage_ctx ctx = nullptr;
+    dage_grid3d gin = nullptr, gout = nullptr;  // set during creation only
+    int ntp = 0;
+    std::vector<double> tp, tw, w;  // host copies
+    double coda = 0.0;
+    int flags = 0;
+    int din[3] = {0, 0, 0}, dout[3] = {0, 0, 0};
+    double *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;  // reference layouts
+    double *d_gz = nullptr;  // Fz relaid as ((zi + nzi*t), zo)
+    double *d_w = nullptr;   // weights (device)
+    double *d_v = nullptr;   // x-mode intermediate
+    double *d_u = nullptr;   // y-mode intermediate, ((xo,yo), (zi,t))
+    double *d_in = nullptr, *d_out = nullptr;  // staging for host applies
+    size_t ws_bytes = 0;
+};
+
+namespace dage {
+
+// Context reference counting (grids and operators keep their context alive).
+void ctx_retain(dage_ctx ctx);
+void ctx_release(dage_ctx ctx);
+
+// Event-bracketed launch helpers (record only when profiling is on).
+void prof_begin(dage_ctx ctx, const char *name, hipEvent_t *start);
+void prof_end(dage_ctx ctx, const char *name, hipEvent_t start);
+int prof_resolve(dage_ctx ctx);
+
+// ----------------------------------------------------------------- kernels
+// Operator matrix generation (matrices.hip).
+int generate_matrices(dage_operator op);
+// FP64 apply (apply_f64.hip).
+int apply_f64(dage_operator op, const double *in_dev, double *out_dev);
+int apply_f64_workspace(dage_operator op, size_t *bytes);
+// Cube integral (reduce.hip).
+int cube_integrate(dage_grid3d g, const double *f, const double *h,
+                   double *result);
+// Relayout Fz (nzi, nzo, T) -> Gz ((zi + nzi*t), zo).
+int relayout_fz(dage_operator op);
+// Zero n doubles with a fill kernel on the context stream (graph-safe, unlike
+// a memset node).
+int fill_zero(dage_ctx ctx, double *p, size_t n);
+
+}  // namespace dage

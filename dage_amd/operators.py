@@ -1,0 +1,337 @@
+"""Host-side mirror of DAGE's Operator3D API over the gfx950 C-ABI.
+
+Names and argument meaning follow the reference:
+  GaussQuad(nlin, nlog, tlin, tlog, tstart)     src/bubbles/gauss.F90:81-122
+  Grid3D.step / Grid3D.equidistant              src/bubbles/grid.F90:652-751
+  Coulomb3D(gridin, gridout, gauss)             src/bubbles/coulomb.F90:109-195
+  Operator3D.transform_cube / .apply            src/bubbles/function3d.F90:4230-4349
+  Operator3D.transform_cuda_cube                src/bubbles/function3d.F90:4354-4480
+  Operator3D.set_transformation_weights         src/bubbles/function3d.F90:320-336
+  Helmholtz3D.set_energy                        src/bubbles/helmholtz3d.F90:411-444
+
+Cubes are numpy arrays in Fortran order (x fastest) for the host path and
+DeviceCube objects (the library's CudaCube, same layout) for the device path.
+Errors raise DageError with the library's message, in place of the
+reference's print-and-exit.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DAGE_OP_NO_CODA, DAGE_PREC_FP64, call
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One HIP device and one stream (replaces the reference's StreamContainer)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        call("dage_ctx_create", device, ctypes.byref(h))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr: int | None):
+        call("dage_ctx_set_stream", self.handle, ctypes.c_void_p(stream_ptr or 0))
+
+    @property
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        call("dage_ctx_get_stream", self.handle, ctypes.byref(s))
+        return s.value or 0
+
+    def synchronize(self):
+        call("dage_ctx_synchronize", self.handle)
+
+    def profile(self, enable: bool):
+        call("dage_ctx_profile", self.handle, 1 if enable else 0)
+
+    def profile_query(self, name: str):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        call("dage_ctx_profile_query", self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def time_begin(self):
+        call("dage_ctx_time_begin", self.handle)
+
+    def time_end(self) -> float:
+        ms = ctypes.c_double()
+        call("dage_ctx_time_end", self.handle, ctypes.byref(ms))
+        return ms.value
+
+
+class DeviceCube:
+    """Device-resident cube (CudaCube, src/bubbles/cube.cu:1182-1269)."""
+
+    def __init__(self, ctx: Context, shape, host=None):
+        self.shape = tuple(int(s) for s in shape)
+        sh = (ctypes.c_int * 3)(*self.shape)
+        h = ctypes.c_void_p()
+        call("dage_cube_create", ctx.handle, sh, ctypes.byref(h))
+        self.handle = h
+        self.ctx = ctx
+        if host is not None:
+            self.upload(host)
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_cube_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ptr(self) -> int:
+        p = ctypes.c_void_p()
+        call("dage_cube_device_pointer", self.handle, ctypes.byref(p))
+        return p.value
+
+    def upload(self, host):
+        a = np.asfortranarray(host, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"host cube shape {a.shape} != {self.shape}")
+        call("dage_cube_upload", self.handle, a.ctypes.data_as(_dp))
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.shape, order="F")
+        call("dage_cube_download", self.handle, out.ctypes.data_as(_dp))
+        return out
+
+    def set_to_zero(self):
+        call("dage_cube_set_to_zero", self.handle)
+
+
+class GaussQuad:
+    """t-quadrature: nlin Gauss-Legendre points on [tstart, tlin] plus nlog on a
+    logarithmic map of [tlin, tlog].  Defaults as the reference (10, 8, 2, 1000)."""
+
+    def __init__(self, nlin=10, nlog=8, tlin=2.0, tlog=1.0e3, tstart=0.0):
+        n = nlin + nlog
+        self.tp = np.zeros(n)
+        self.tw = np.zeros(n)
+        self.tlog = float(tlog)
+        call("dage_gauss_quad", nlin, nlog, float(tstart), float(tlin), float(tlog), _ptr(self.tp),
+             _ptr(self.tw))
+
+    def get_tpoints(self):
+        return self.tp.copy()
+
+    def get_weights(self):
+        return self.tw.copy()
+
+    def get_npoints(self):
+        return len(self.tp)
+
+    def get_tlog(self):
+        return self.tlog
+
+
+class Grid3D:
+    """Cell grid of LIP cells (nlip points per cell, shared end points)."""
+
+    def __init__(self, ctx: Context, qmin, steps, nlip: int, grid_type: int = 1):
+        steps = [_f64(s) for s in steps]
+        qmin = _f64(qmin)
+        ncell = (ctypes.c_int * 3)(*[len(s) for s in steps])
+        h = ctypes.c_void_p()
+        call("dage_grid3d_create", ctx.handle, _ptr(qmin), ncell, nlip, _ptr(steps[0]),
+             _ptr(steps[1]), _ptr(steps[2]), grid_type, ctypes.byref(h))
+        self.handle = h
+        self.ctx = ctx
+        self.nlip = nlip
+        self.grid_type = grid_type
+        self._steps = steps
+
+    @classmethod
+    def equidistant(cls, ctx, ranges, nlip=7, stepmax=0.2, grid_type=1):
+        """Grid3D_init_equidistant: ranges = [(xmin,xmax),(ymin,ymax),(zmin,zmax)]."""
+        steps, qmin = [], []
+        for r0, r1 in ranges:
+            nc = int(np.ceil((r1 - r0) / (stepmax * (nlip - 1))))
+            steps.append(np.full(nc, (r1 - r0) / (nc * (nlip - 1))))
+            qmin.append(r0)
+        return cls(ctx, qmin, steps, nlip, grid_type)
+
+    @classmethod
+    def ncell(cls, ctx, ranges, ncell, nlip=7, grid_type=1):
+        """Grid3D_init_equidistant_ncell (grid.F90:753-808)."""
+        steps = [np.full(n, (r1 - r0) / (n * (nlip - 1))) for (r0, r1), n in zip(ranges, ncell)]
+        return cls(ctx, [r[0] for r in ranges], steps, nlip, grid_type)
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_grid3d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_shape(self):
+        s = (ctypes.c_int * 3)()
+        call("dage_grid3d_shape", self.handle, s)
+        return tuple(s)
+
+    def get_coord(self, axis: int) -> np.ndarray:
+        out = np.zeros(self.get_shape()[axis])
+        call("dage_grid3d_coord", self.handle, axis, _ptr(out))
+        return out
+
+    def get_ints(self, axis: int) -> np.ndarray:
+        out = np.zeros(self.get_shape()[axis])
+        call("dage_grid3d_weights", self.handle, axis, _ptr(out))
+        return out
+
+    def integrate(self, f: DeviceCube, g: DeviceCube | None = None) -> float:
+        """sum w_x w_y w_z f [g] over the cube (FP64 on the GPU)."""
+        shape = self.get_shape()
+        for c in (f, g):
+            if c is not None and c.shape != shape:
+                raise ValueError(f"cube shape {c.shape} != grid shape {shape}")
+        r = ctypes.c_double()
+        call("dage_cube_integrate_device", self.handle, ctypes.c_void_p(f.ptr),
+             ctypes.c_void_p(g.ptr if g is not None else 0), ctypes.byref(r))
+        return r.value
+
+
+class Operator3D:
+    """Tensor-quadrature operator: sum_t w_t Fx_t (x) Fy_t (x) Fz_t  [+ coda]."""
+
+    def __init__(self, handle, ctx, gridin, gridout, ntp):
+        self.handle = handle
+        self.ctx = ctx
+        self.gridin = gridin
+        self.gridout = gridout
+        self.ntp = ntp
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_operator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_dims(self):
+        return self.gridin.get_shape(), self.gridout.get_shape()
+
+    def set_transformation_weights(self, w):
+        w = _f64(w)
+        if len(w) != self.ntp:
+            raise ValueError(f"expected {self.ntp} weights, got {len(w)}")
+        call("dage_operator_set_weights", self.handle, _ptr(w))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.ntp)
+        call("dage_operator_get_weights", self.handle, _ptr(w))
+        return w
+
+    def get_matrices(self):
+        (nxi, nyi, nzi), (nxo, nyo, nzo) = self.get_dims()
+        T = self.ntp
+        fx = np.zeros((nxo, nxi, T), order="F")
+        fy = np.zeros((nyi, nyo, T), order="F")
+        fz = np.zeros((nzi, nzo, T), order="F")
+        call("dage_operator_get_matrices", self.handle, fx.ctypes.data_as(_dp),
+             fy.ctypes.data_as(_dp), fz.ctypes.data_as(_dp))
+        return fx, fy, fz
+
+    def set_matrices(self, fx, fy, fz):
+        (nxi, nyi, nzi), (nxo, nyo, nzo) = self.get_dims()
+        T = self.ntp
+        arrs = []
+        for a, shape in ((fx, (nxo, nxi, T)), (fy, (nyi, nyo, T)), (fz, (nzi, nzo, T))):
+            a = np.asfortranarray(a, dtype=np.float64)
+            if a.shape != shape:
+                raise ValueError(f"matrix shape {a.shape} != {shape}")
+            arrs.append(a)
+        call("dage_operator_set_matrices", self.handle, *[a.ctypes.data_as(_dp) for a in arrs])
+
+    def transform_cube(self, cubein, precision: int = DAGE_PREC_FP64) -> np.ndarray:
+        """Host cube in, host cube out (uploads, applies, downloads)."""
+        nin, nout = self.get_dims()
+        cubein = np.asfortranarray(cubein, dtype=np.float64)
+        if cubein.shape != tuple(nin):
+            raise ValueError(f"cube shape {cubein.shape} != grid shape {nin}")
+        out = np.zeros(nout, order="F")
+        call("dage_operator_apply", self.handle, cubein.ctypes.data_as(_dp),
+             out.ctypes.data_as(_dp), precision)
+        return out
+
+    apply = transform_cube
+
+    def transform_cuda_cube(self, cubein: DeviceCube, cubeout: DeviceCube,
+                            precision: int = DAGE_PREC_FP64):
+        """Device cubes in and out; asynchronous on the context stream."""
+        nin, nout = self.get_dims()
+        if cubein.shape != tuple(nin) or cubeout.shape != tuple(nout):
+            raise ValueError(f"device cubes {cubein.shape}->{cubeout.shape} do not match "
+                             f"the operator {tuple(nin)}->{tuple(nout)}")
+        call("dage_operator_apply_device", self.handle, ctypes.c_void_p(cubein.ptr),
+             ctypes.c_void_p(cubeout.ptr), precision)
+
+    def workspace_bytes(self) -> int:
+        n = ctypes.c_size_t()
+        call("dage_operator_workspace_bytes", self.handle, ctypes.byref(n))
+        return n.value
+
+
+class Coulomb3D(Operator3D):
+    """Coulomb operator 1/r (Coulomb3D_init_explicit_sub, coulomb.F90:109-176)."""
+
+    def __init__(self, ctx: Context, gridin: Grid3D, gridout: Grid3D | None = None,
+                 gauss: GaussQuad | None = None, coda: bool = True):
+        gridout = gridout or gridin
+        gauss = gauss or GaussQuad()
+        h = ctypes.c_void_p()
+        flags = 0 if coda else DAGE_OP_NO_CODA
+        call("dage_coulomb3d_create", ctx.handle, gridin.handle, gridout.handle, _ptr(gauss.tp),
+             _ptr(gauss.tw), len(gauss.tp), gauss.tlog, flags, ctypes.byref(h))
+        super().__init__(h, ctx, gridin, gridout, len(gauss.tp))
+        self.gaussian_quadrature = gauss
+
+
+class Helmholtz3D(Coulomb3D):
+    """Helmholtz operator exp(-k r)/r with k^2 = -2E (helmholtz3d.F90:367-481):
+    the Coulomb operator with re-weighted t-quadrature."""
+
+    def __init__(self, ctx, gridin, gridout=None, gauss=None, energy: float = 0.0, coda=True):
+        super().__init__(ctx, gridin, gridout, gauss, coda)
+        self.energy = 0.0
+        self.set_energy(energy)
+
+    def set_energy(self, energy: float):
+        call("dage_operator_set_energy", self.handle, float(energy))
+        self.energy = float(energy)

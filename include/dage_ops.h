@@ -1,0 +1,186 @@
+/*
+ * dage_ops.h -- C-ABI of the MI355X (gfx950) operator-application library.
+ *
+ * Drop-in boundary for DAGE's Operator3D GPU path.  Each entry point names
+ * the reference routine or bind(C) interface it replaces (paths relative to
+ * the reference repository root).  A Fortran ISO_C_BINDING module that
+ * mirrors the reference's Operator3D API on top of these calls is in
+ * dage_amd/fortran/dage_ops_mod.F90.
+ *
+ * Conventions
+ *  - Every function returns an int status: DAGE_OK (0) or a DAGE_ERR_* code.
+ *    dage_last_error() returns the message of the last failure on the
+ *    calling host thread.  (The reference prints and calls exit(-1),
+ *    src/bubbles/streamcontainer.h:67-95.)
+ *  - Cubes are Fortran column-major real(8) arrays, x fastest:
+ *    index = x + nx*(y + ny*z).  Indices are 0-based in C.
+ *  - Host pointers are owned by the caller; device memory allocated by the
+ *    library is owned by the library.  *_device entry points take device
+ *    pointers (hipMalloc'd, caller-owned) and enqueue on the context stream
+ *    without synchronising.
+ *  - A context is one HIP device plus one stream.  Handles may be used from
+ *    several host threads when each thread uses its own context (the
+ *    reference calls the CUDA path from 3 OpenMP threads,
+ *    src/gbfmm/gbfmm_coulomb3d.F90:767-785).
+ */
+#ifndef DAGE_OPS_H
+#define DAGE_OPS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DAGE_ABI_VERSION 1
+
+enum {
+    DAGE_OK = 0,
+    DAGE_ERR_ARG = 1,         /* invalid argument / shape */
+    DAGE_ERR_HIP = 2,         /* HIP runtime error */
+    DAGE_ERR_ALLOC = 3,       /* device or host allocation failed */
+    DAGE_ERR_STATE = 4,       /* handle in the wrong state */
+    DAGE_ERR_UNSUPPORTED = 5  /* feature outside the implemented scope */
+};
+
+/* Precision modes of the apply. */
+enum {
+    DAGE_PREC_FP64 = 0        /* FP64 MFMA throughout */
+};
+
+/* Operator creation flags. */
+enum {
+    DAGE_OP_NO_CODA = 1       /* omit coda*rho, as the reference CUDA path
+                                 does (src/bubbles/function3d.F90:4473-4479) */
+};
+
+typedef struct dage_ctx_s *dage_ctx;
+typedef struct dage_grid3d_s *dage_grid3d;
+typedef struct dage_operator_s *dage_operator;
+typedef struct dage_cube_s *dage_cube;
+
+/* ----------------------------------------------------------------- library */
+int dage_abi_version(void);
+const char *dage_last_error(void);
+
+/* ----------------------------------------------------------------- context
+ * Replaces streamcontainer_init(StreamContainer**, int)
+ * (src/bubbles/streamcontainer.cpp:377) for one device. */
+int dage_ctx_create(int device, dage_ctx *ctx);
+int dage_ctx_destroy(dage_ctx ctx);
+/* Use a caller-provided hipStream_t (NULL restores the context's own). */
+int dage_ctx_set_stream(dage_ctx ctx, void *hip_stream);
+int dage_ctx_get_stream(dage_ctx ctx, void **hip_stream);
+/* Replaces cuda_sync_all_ (src/bubbles/cublas_iface.cpp) for this stream. */
+int dage_ctx_synchronize(dage_ctx ctx);
+/* Per-kernel timing with HIP events on the context stream (the reference's
+ * bigben/NVTX ranges, src/bubbles/chrono.F90:85-200,
+ * src/bubbles/cuda_profiling.cu:58-90).  enable != 0 starts recording and
+ * clears the counters. */
+int dage_ctx_profile(dage_ctx ctx, int enable);
+/* Total milliseconds and launch count of kernel `name` since profiling was
+ * enabled (synchronises the stream). */
+int dage_ctx_profile_query(dage_ctx ctx, const char *name, double *total_ms,
+                           int *launches);
+
+/* Event timer on the context stream: begin records a start event; end
+ * records a stop event, synchronises and returns the elapsed milliseconds. */
+int dage_ctx_time_begin(dage_ctx ctx);
+int dage_ctx_time_end(dage_ctx ctx, double *ms);
+
+/* -------------------------------------------------------------- device cube
+ * Device-resident cube, x fastest, replacing the reference's CudaCube
+ * (src/bubbles/cube.cu:1182-1269; Fortran side src/bubbles/cuda.F90:70-200):
+ * cudacube_init_cuda, cudacube_upload_cuda, cudacube_download_cuda,
+ * cudacube_set_to_zero_cuda, cudacube_destroy_cuda.  The memory is one
+ * contiguous allocation (no pitch, no multi-device split). */
+int dage_cube_create(dage_ctx ctx, const int shape[3], dage_cube *cube);
+int dage_cube_destroy(dage_cube cube);
+/* Synchronises the context stream before returning (the host buffer may be
+ * reused immediately). */
+int dage_cube_upload(dage_cube cube, const double *host);
+/* Synchronises the context stream before returning. */
+int dage_cube_download(dage_cube cube, double *host);
+int dage_cube_set_to_zero(dage_cube cube);
+/* Device address of the cube, for the *_device entry points. */
+int dage_cube_device_pointer(dage_cube cube, double **dev);
+
+/* -------------------------------------------------------------- quadrature
+ * GaussQuadInit (src/bubbles/gauss.F90:81-122, gauleg_int :307-329,
+ * gauleg :337-384).  tp, tw receive nlin+nlog values. */
+int dage_gauss_quad(int nlin, int nlog, double tstart, double tlin,
+                    double tlog, double *tp, double *tw);
+
+/* -------------------------------------------------------------------- grid
+ * Grid3D_init_step (src/bubbles/grid.F90:652-702) with Grid1D_init_step
+ * (:446-537).  steps_x/y/z hold ncell[axis] cell steps.  grid_type 1 is
+ * equidistant cells, 2 Gauss-Lobatto cells (nlip 6 or 7).  Replaces
+ * grid3d_init_cuda / grid1d_init_cuda (src/bubbles/grid.cu:498-521). */
+int dage_grid3d_create(dage_ctx ctx, const double qmin[3], const int ncell[3],
+                       int nlip, const double *steps_x, const double *steps_y,
+                       const double *steps_z, int grid_type,
+                       dage_grid3d *grid);
+/* Grid3D_init_equidistant (src/bubbles/grid.F90:704-751); ranges is the
+ * Fortran (2,3) array {xmin,xmax,ymin,ymax,zmin,zmax}. */
+int dage_grid3d_create_equidistant(dage_ctx ctx, const double ranges[6],
+                                   int nlip, double stepmax, int grid_type,
+                                   dage_grid3d *grid);
+int dage_grid3d_destroy(dage_grid3d grid);
+int dage_grid3d_shape(dage_grid3d grid, int shape[3]);
+/* Grid1D_get_coord (grid.F90) of one axis (0,1,2) into host memory. */
+int dage_grid3d_coord(dage_grid3d grid, int axis, double *points);
+/* Grid1D_get_ints (grid.F90:1398-1424): LIP integration weights. */
+int dage_grid3d_weights(dage_grid3d grid, int axis, double *weights);
+
+/* ---------------------------------------------------------------- operator
+ * Coulomb3D_init_explicit_sub (src/bubbles/coulomb.F90:109-176): w =
+ * (2/sqrt(pi))*tw, coda = pi/tlog^2, and the 1D operator matrices of
+ * make_Coulomb3D_matrices (src/bubbles/exponentials.F90:602-639), generated
+ * on the GPU in FP64 from the reference's erf closed forms; then
+ * Operator3D_cuda_init (src/bubbles/function3d.F90:4172-4209).
+ * gridin/gridout must share nlip and grid_type. */
+int dage_coulomb3d_create(dage_ctx ctx, dage_grid3d gridin,
+                          dage_grid3d gridout, const double *tp,
+                          const double *tw, int ntp, double tlog, int flags,
+                          dage_operator *op);
+/* Operator3D_set_transformation_weights (function3d.F90:320-336). */
+int dage_operator_set_weights(dage_operator op, const double *w);
+int dage_operator_get_weights(dage_operator op, double *w);
+/* Helmholtz3D_set_energy (src/bubbles/helmholtz3d.F90:411-444): w_t =
+ * (2/sqrt(pi)) tw_t exp(2E/(4 t^2)) for E < -0.005, else Coulomb weights. */
+int dage_operator_set_energy(dage_operator op, double energy);
+/* Upload externally built matrices, as Operator3D_cuda_init uploads
+ * f(X_), f(Y_), f(Z_) (function3d.F90:4200-4206).  fx: (nxo,nxi,T), fy:
+ * (nyi,nyo,T), fz: (nzi,nzo,T), host memory. */
+int dage_operator_set_matrices(dage_operator op, const double *fx,
+                               const double *fy, const double *fz);
+/* Download the matrices (same layouts) to host memory. */
+int dage_operator_get_matrices(dage_operator op, double *fx, double *fy,
+                               double *fz);
+/* Operator3D_transform_cube (function3d.F90:4230-4349): cubeout =
+ * sum_t w_t (Fx_t x Fy_t x Fz_t) cubein + coda*cubein.  Host buffers: the
+ * cube is uploaded, transformed and downloaded, then the stream is
+ * synchronised. */
+int dage_operator_apply(dage_operator op, const double *cubein,
+                        double *cubeout, int precision);
+/* Operator3D_transform_cuda_cube (function3d.F90:4354-4480) on device
+ * buffers; asynchronous on the context stream. */
+int dage_operator_apply_device(dage_operator op, const double *cubein_dev,
+                               double *cubeout_dev, int precision);
+/* Bytes of device workspace the operator holds for its intermediates. */
+int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
+int dage_operator_destroy(dage_operator op);
+
+/* ---------------------------------------------------------------- cube sums
+ * Integrator3D reduce3d (src/bubbles/integrator.cu:320-373) /
+ * SerialInfo_integrate_cube (src/bubbles/parallel_info.F90:512-530), and the
+ * cube part of Function3D_dot_product (function3d.F90:1497-1530):
+ * result = sum_ijk wx_i wy_j wz_k f_ijk g_ijk (g may be NULL).  Device
+ * buffers; synchronises and writes *result on the host. */
+int dage_cube_integrate_device(dage_grid3d grid, const double *f_dev,
+                               const double *g_dev, double *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DAGE_OPS_H */

@@ -1,0 +1,272 @@
+"""CPU FP64 oracle for DAGE's operator-application hot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker: tests/, the smoke
+entry point and the CPU-baseline leg of bench.py load it to compute the
+expected results.  The product (dage_amd) never imports it.
+
+It wraps oracle/liboracle.so (a C restatement built from dage_oracle.c, see
+that file for the reference routine behind each function) with numpy.
+Arrays follow the reference's Fortran layout: column-major, x fastest.
+
+Parity status of each function (what pins it) is listed in the module
+docstrings of tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    d, i = ctypes.c_double, ctypes.c_int
+    sig = {
+        "dage_oracle_pi": (d, []),
+        "dage_oracle_gauleg": (None, [i, _dp, _dp]),
+        "dage_oracle_gauss_quad": (None, [i, i, d, d, d, _dp, _dp]),
+        "dage_oracle_gauss_lobatto_grid": (i, [i, d, d, _dp]),
+        "dage_oracle_lip_first_last": (None, [i, i, _dp, _dp]),
+        "dage_oracle_lip_coeffs": (i, [i, i, _dp]),
+        "dage_oracle_lip_integrals": (i, [i, i, _dp]),
+        "dage_oracle_grid1d": (i, [d, i, i, _dp, i, _dp, _dp]),
+        "dage_oracle_equidistant_cells": (None, [d, d, i, d, _ip, _dp]),
+        "dage_oracle_grid1d_ints": (i, [i, i, _dp, i, _dp]),
+        "dage_oracle_integrals": (None, [d, d, d, d, i, _dp]),
+        "dage_oracle_expo": (d, [i, d, d, d, d]),
+        "dage_oracle_gauss_expo": (d, [i, d, d, d, d]),
+        "dage_oracle_exponentials": (i, [d, i, _dp, i, i, _dp, i, _dp, i, i, _dp]),
+        "dage_oracle_transform_cube": (i, [_ip, _ip, i, _dp, _dp, _dp, _dp, d, _dp, _dp, i, i, i]),
+        "dage_oracle_helmholtz_weights": (None, [_dp, _dp, i, d, _dp]),
+        "dage_oracle_gauss_density": (None, [i, _dp, _dp, _dp, _dp, i, _dp, i, _dp, i, _dp]),
+        "dage_oracle_gauss_potential": (None, [i, _dp, _dp, _dp, _dp, i, _dp, i, _dp, i, _dp]),
+        "dage_oracle_cube_integral": (d, [_dp, i, _dp, i, _dp, i, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ quadrature
+def gauleg(n: int):
+    x, w = np.zeros(n), np.zeros(n)
+    _load().dage_oracle_gauleg(n, _p(x), _p(w))
+    return x, w
+
+
+def gauss_quad(nlin=10, nlog=8, tstart=0.0, tlin=2.0, tlog=1.0e3):
+    """GaussQuadInit defaults: nlin 10, nlog 8, tstart 0, tlin 2, tlog 1000."""
+    n = nlin + nlog
+    tp, tw = np.zeros(n), np.zeros(n)
+    _load().dage_oracle_gauss_quad(nlin, nlog, tstart, tlin, tlog, _p(tp), _p(tw))
+    return tp, tw
+
+
+def gauss_lobatto_grid(n: int, left: float, right: float):
+    tp = np.zeros(n)
+    if _load().dage_oracle_gauss_lobatto_grid(n, left, right, _p(tp)):
+        raise ValueError("gauss_lobatto_grid supports n=6,7 only")
+    return tp
+
+
+# ------------------------------------------------------------------ LIP / grid
+def lip_first_last(nlip: int, grid_type: int = 1):
+    f, l = ctypes.c_double(), ctypes.c_double()
+    _load().dage_oracle_lip_first_last(nlip, grid_type, ctypes.byref(f), ctypes.byref(l))
+    return f.value, l.value
+
+
+def lip_coeffs(nlip: int, grid_type: int = 1) -> np.ndarray:
+    c = np.zeros(nlip * nlip)
+    if _load().dage_oracle_lip_coeffs(nlip, grid_type, _p(c)):
+        raise ValueError("unsupported LIP basis")
+    return c.reshape((nlip, nlip), order="F")
+
+
+def lip_integrals(nlip: int, grid_type: int = 1) -> np.ndarray:
+    v = np.zeros(nlip)
+    if _load().dage_oracle_lip_integrals(nlip, grid_type, _p(v)):
+        raise ValueError("unsupported LIP basis")
+    return v
+
+
+@dataclass
+class Axis:
+    """One Grid1D: qmin, per-cell steps, nlip, grid_type (grid.F90:446-537)."""
+
+    qmin: float
+    steps: np.ndarray
+    nlip: int
+    grid_type: int = 1
+    points: np.ndarray = field(init=False)
+    celld: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        self.steps = _f64(self.steps)
+        ncell = len(self.steps)
+        self.points = np.zeros(ncell * (self.nlip - 1) + 1)
+        self.celld = np.zeros(ncell)
+        if _load().dage_oracle_grid1d(self.qmin, ncell, self.nlip, _p(self.steps),
+                                      self.grid_type, _p(self.points), _p(self.celld)):
+            raise ValueError("unsupported grid")
+
+    @property
+    def ncell(self) -> int:
+        return len(self.steps)
+
+    @property
+    def n(self) -> int:
+        return len(self.points)
+
+    @classmethod
+    def equidistant(cls, r0, r1, nlip, stepmax, grid_type=1):
+        nc, st = ctypes.c_int(), ctypes.c_double()
+        _load().dage_oracle_equidistant_cells(r0, r1, nlip, stepmax, ctypes.byref(nc), ctypes.byref(st))
+        return cls(r0, np.full(nc.value, st.value), nlip, grid_type)
+
+    @classmethod
+    def ncell_fixed(cls, r0, r1, nlip, ncell, grid_type=1):
+        """Grid3D_init_equidistant_ncell per axis (grid.F90:753-808)."""
+        return cls(r0, np.full(ncell, (r1 - r0) / (ncell * (nlip - 1))), nlip, grid_type)
+
+    def ints(self) -> np.ndarray:
+        v = np.zeros(self.n)
+        _load().dage_oracle_grid1d_ints(self.ncell, self.nlip, _p(self.steps), self.grid_type, _p(v))
+        return v
+
+
+# ---------------------------------------------------------- matrix generation
+def integrals(a, b, t, p, n) -> np.ndarray:
+    v = np.zeros(n)
+    _load().dage_oracle_integrals(a, b, t, p, n, _p(v))
+    return v
+
+
+def expo(n, a, b, t, p) -> float:
+    return _load().dage_oracle_expo(n, a, b, t, p)
+
+
+def gauss_expo(n, a, b, t, p) -> float:
+    return _load().dage_oracle_gauss_expo(n, a, b, t, p)
+
+
+def exponentials(ax_in: Axis, ax_out: Axis, tp, transpose: bool) -> np.ndarray:
+    tp = _f64(tp)
+    nin, nout, T = ax_in.n, ax_out.n, len(tp)
+    out = np.zeros(nin * nout * T)
+    rc = _load().dage_oracle_exponentials(ax_in.qmin, ax_in.ncell, _p(ax_in.steps), ax_in.nlip,
+                                          ax_in.grid_type, _p(ax_out.points), nout, _p(tp), T,
+                                          1 if transpose else 0, _p(out))
+    if rc:
+        raise RuntimeError("exponentials failed")
+    shape = (nin, nout, T) if transpose else (nout, nin, T)
+    return out.reshape(shape, order="F")
+
+
+def coulomb3d_matrices(axes_in, axes_out, tp):
+    """make_Coulomb3D_matrices (exponentials.F90:602-639): Fx (out,in,T),
+    Fy and Fz stored transposed (in,out,T)."""
+    return (exponentials(axes_in[0], axes_out[0], tp, False),
+            exponentials(axes_in[1], axes_out[1], tp, True),
+            exponentials(axes_in[2], axes_out[2], tp, True))
+
+
+# ----------------------------------------------------------------- the apply
+def transform_cube(fx, fy, fz, w, coda, cube_in, t0=0, t1=None, add_coda=True):
+    fx, fy, fz = (np.asfortranarray(a, dtype=np.float64) for a in (fx, fy, fz))
+    cube_in = np.asfortranarray(cube_in, dtype=np.float64)
+    w = _f64(w)
+    T = fx.shape[2]
+    di = (ctypes.c_int * 3)(*cube_in.shape)
+    do = (ctypes.c_int * 3)(fx.shape[0], fy.shape[1], fz.shape[1])
+    out = np.zeros((do[0], do[1], do[2]), order="F")
+    rc = _load().dage_oracle_transform_cube(
+        di, do, T, fx.ctypes.data_as(_dp), fy.ctypes.data_as(_dp), fz.ctypes.data_as(_dp), _p(w),
+        float(coda), cube_in.ctypes.data_as(_dp), out.ctypes.data_as(_dp), int(t0),
+        int(T if t1 is None else t1), 1 if add_coda else 0)
+    if rc:
+        raise RuntimeError(f"transform_cube failed ({rc})")
+    return out
+
+
+def coulomb_weights(tw):
+    """Coulomb3D w = (2/sqrt(pi)) tw (coulomb.F90:150)."""
+    tw = _f64(tw)
+    w = np.zeros_like(tw)
+    _load().dage_oracle_helmholtz_weights(_p(tw), _p(tw), len(tw), 0.0, _p(w))
+    return w
+
+
+def helmholtz_weights(tp, tw, energy):
+    tp, tw = _f64(tp), _f64(tw)
+    w = np.zeros_like(tw)
+    _load().dage_oracle_helmholtz_weights(_p(tp), _p(tw), len(tw), float(energy), _p(w))
+    return w
+
+
+def coda(tlog):
+    """Coulomb3D coda = PI/tlog**2 (coulomb.F90:151)."""
+    return _load().dage_oracle_pi() / (tlog * tlog)
+
+
+# ------------------------------------------------------------- test densities
+def gauss_density(q, alpha, centers, x, y, z):
+    q, alpha = _f64(q), _f64(alpha)
+    c = _f64(np.asarray(centers).reshape(-1, 3))
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    cube = np.zeros((len(x), len(y), len(z)), order="F")
+    _load().dage_oracle_gauss_density(len(q), _p(q), _p(alpha), _p(c), _p(x), len(x), _p(y), len(y),
+                                      _p(z), len(z), cube.ctypes.data_as(_dp))
+    return cube
+
+
+def gauss_potential(q, alpha, centers, x, y, z):
+    q, alpha = _f64(q), _f64(alpha)
+    c = _f64(np.asarray(centers).reshape(-1, 3))
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    cube = np.zeros((len(x), len(y), len(z)), order="F")
+    _load().dage_oracle_gauss_potential(len(q), _p(q), _p(alpha), _p(c), _p(x), len(x), _p(y), len(y),
+                                        _p(z), len(z), cube.ctypes.data_as(_dp))
+    return cube
+
+
+def cube_integral(wx, wy, wz, f, g=None) -> float:
+    wx, wy, wz = _f64(wx), _f64(wy), _f64(wz)
+    f = np.asfortranarray(f, dtype=np.float64)
+    gp = None
+    if g is not None:
+        g = np.asfortranarray(g, dtype=np.float64)
+        gp = g.ctypes.data_as(_dp)
+    return _load().dage_oracle_cube_integral(_p(wx), len(wx), _p(wy), len(wy), _p(wz), len(wz),
+                                             f.ctypes.data_as(_dp), gp)

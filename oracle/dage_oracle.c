@@ -1,0 +1,696 @@
+/*
+ * dage_oracle.c -- CPU FP64 restatement of DAGE's tensor-quadrature
+ * Coulomb3D/Helmholtz3D operator (the hot path's oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP library; the
+ * product never links or calls it.  See dage_oracle.h for the reference
+ * routine each function restates.  The formulas are kept term for term in the
+ * reference's order (including its cancellation-prone closed forms) because
+ * the GPU library must reproduce the reference's numbers, not better ones.
+ *
+ * Built with -ffp-contract=off so that every product and sum rounds as
+ * written.
+ */
+#include "dage_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* globals.F90:54-61: PI=dacos(-1), SQRTPI=dsqrt(PI), TWOOVERSQRTPI=2/SQRTPI */
+static double o_pi(void) { return acos(-1.0); }
+static double o_sqrtpi(void) { return sqrt(o_pi()); }
+
+double dage_oracle_pi(void) { return o_pi(); }
+
+/* ---------------------------------------------------------------- quadrature
+ * gauss.F90:337-384.  Roots of P_n by Newton from the cos() guess; the weight
+ * uses the derivative pp evaluated before the last Newton update, exactly as
+ * the reference loop leaves it. */
+void dage_oracle_gauleg(int ngp, double *xabsc, double *weig) {
+    const double EPS = 3.0e-15;
+    const double pi = o_pi();
+    int m = (ngp + 1) / 2;
+    for (int i = 1; i <= m; ++i) {
+        double z = cos(pi * (i - 0.25) / (ngp + 0.5));
+        double z1, pp, p1, p2, p3;
+        for (;;) {
+            p1 = 1.0;
+            p2 = 0.0;
+            for (int j = 1; j <= ngp; ++j) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = ngp * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+            if (!(fabs(z - z1) > EPS)) break;
+        }
+        xabsc[i - 1] = -z;
+        xabsc[ngp - i] = z;
+        weig[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp);
+        weig[ngp - i] = weig[i - 1];
+    }
+}
+
+/* gauss.F90:307-329 (gauleg_int), linear or logarithmic map of [-1,1]. */
+static void gauleg_int(int n, double *tp, double *tw, double a, double b,
+                       int log_p) {
+    dage_oracle_gauleg(n, tp, tw);
+    for (int i = 0; i < n; ++i) {
+        if (log_p) {
+            tp[i] = a * pow(b / a, 0.5 * (tp[i] + 1.0));
+            tw[i] = 0.5 * log(b / a) * tw[i] * tp[i];
+        } else {
+            tp[i] = 0.5 * ((b + a) + (b - a) * tp[i]);
+            tw[i] = 0.5 * (b - a) * tw[i];
+        }
+    }
+}
+
+/* gauss.F90:81-122 */
+void dage_oracle_gauss_quad(int nlin, int nlog, double tstart, double tlin,
+                            double tlog, double *tp, double *tw) {
+    gauleg_int(nlin, tp, tw, tstart, tlin, 0);
+    gauleg_int(nlog, tp + nlin, tw + nlin, tlin, tlog, 1);
+}
+
+/* gauss.F90:409-446 */
+int dage_oracle_gauss_lobatto_grid(int n, double left, double right,
+                                   double *tp) {
+    if (n == 6) {
+        tp[0] = -1.0;
+        tp[1] = -sqrt((7.0 + 2 * sqrt(7.0)) / 21.0);
+        tp[2] = -sqrt((7.0 - 2 * sqrt(7.0)) / 21.0);
+        tp[3] = sqrt((7.0 - 2 * sqrt(7.0)) / 21.0);
+        tp[4] = sqrt((7.0 + 2 * sqrt(7.0)) / 21.0);
+        tp[5] = 1.0;
+    } else if (n == 7) {
+        tp[0] = -1.0;
+        tp[1] = -sqrt((15.0 + 2 * sqrt(15.0)) / 33.0);
+        tp[2] = -sqrt((15.0 - 2 * sqrt(15.0)) / 33.0);
+        tp[3] = 0.0;
+        tp[4] = sqrt((15.0 - 2 * sqrt(15.0)) / 33.0);
+        tp[5] = sqrt((15.0 + 2 * sqrt(15.0)) / 33.0);
+        tp[6] = 1.0;
+    } else {
+        return -1;
+    }
+    for (int i = 0; i < n; ++i)
+        tp[i] = 0.5 * ((right + left) + (right - left) * tp[i]);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ LIP basis
+ * lip.F90:118-142: first = -(nlip-1)/2, last = nlip/2 in integer arithmetic
+ * for the equidistant (1) and Gauss-Lobatto (2) cells. */
+void dage_oracle_lip_first_last(int nlip, int grid_type, double *first,
+                                double *last) {
+    (void)grid_type;
+    *first = (double)(-((nlip - 1) / 2));
+    *last = (double)(nlip / 2);
+}
+
+/* lip.F90:285-344: product form of the Lagrange polynomials, built up one
+ * node at a time in decreasing-power coefficient order. */
+int dage_oracle_lip_coeffs(int nlip, int grid_type, double *c) {
+    int n = nlip;
+    double nodes[16];
+    if (n < 2 || n > 15) return -1;
+    if (grid_type == 1) {
+        for (int i = 1; i <= n; ++i) nodes[i - 1] = (double)(i - (n + 1) / 2);
+    } else if (grid_type == 2) {
+        double first, last;
+        dage_oracle_lip_first_last(n, grid_type, &first, &last);
+        if (dage_oracle_gauss_lobatto_grid(n, first, last, nodes)) return -1;
+    } else {
+        return -1;
+    }
+    memset(c, 0, sizeof(double) * n * n);
+    for (int j = 0; j < n; ++j) c[j + n * (n - 1)] = 1.0;
+    for (int i = 1; i <= n; ++i) {
+        double x_i = nodes[i - 1];
+        for (int j = 1; j <= n; ++j) {
+            if (i == j) continue;
+            double dx = (grid_type == 1) ? (double)(j - i)
+                                         : nodes[j - 1] - nodes[i - 1];
+            double *row = c + (j - 1);
+            /* p(j,1:n-1) = (p(j,2:n) - p(j,1:n-1)*x_i)/dx, old values on the
+             * right; ascending k reads p(k+1) before it is overwritten. */
+            for (int k = 0; k < n - 1; ++k)
+                row[n * k] = (row[n * (k + 1)] - row[n * k] * x_i) / dx;
+            row[n * (n - 1)] = -row[n * (n - 1)] * x_i / dx;
+        }
+    }
+    return 0;
+}
+
+/* lip.F90:367-415: integral coefficients c(:,k)/(n-k+1), then Horner at
+ * last minus Horner at first. */
+int dage_oracle_lip_integrals(int nlip, int grid_type, double *integrals) {
+    int n = nlip;
+    double c[256], ic[256];
+    double first, last;
+    if (dage_oracle_lip_coeffs(n, grid_type, c)) return -1;
+    dage_oracle_lip_first_last(n, grid_type, &first, &last);
+    for (int k = 0; k < n; ++k)          /* 0-based k <-> Fortran column k+1 */
+        for (int j = 0; j < n; ++j) ic[j + n * k] = c[j + n * k] / (n - k);
+    for (int j = 0; j < n; ++j) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int k = 0; k < n; ++k) t1 = (t1 + ic[j + n * k]) * first;
+        for (int k = 0; k < n; ++k) t2 = (t2 + ic[j + n * k]) * last;
+        integrals[j] = t2 - t1;
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- grid
+ * grid.F90:446-537: points accumulate from the start of each cell. */
+int dage_oracle_grid1d(double qmin, int ncell, int nlip, const double *steps,
+                       int grid_type, double *points, double *celld) {
+    double templ[16];
+    if (nlip < 2 || nlip > 15) return -1;
+    if (grid_type == 1) {
+        for (int k = 1; k <= nlip - 1; ++k) templ[k] = (double)k;
+    } else if (grid_type == 2) {
+        double gl[16];
+        if (dage_oracle_gauss_lobatto_grid(nlip, 0.0, (double)(nlip - 1), gl))
+            return -1;
+        for (int k = 1; k <= nlip - 1; ++k) templ[k] = gl[k];
+    } else {
+        return -1;
+    }
+    int ip = 0;
+    points[0] = qmin;
+    for (int ic = 0; ic < ncell; ++ic) {
+        celld[ic] = points[ip];
+        for (int k = 1; k <= nlip - 1; ++k) {
+            ++ip;
+            points[ip] = celld[ic] + templ[k] * steps[ic];
+        }
+    }
+    return 0;
+}
+
+/* grid.F90:539-567 */
+void dage_oracle_equidistant_cells(double r0, double r1, int nlip,
+                                   double stepmax, int *ncell, double *step) {
+    int nc = (int)ceil((r1 - r0) / (stepmax * (nlip - 1)));
+    *ncell = nc;
+    *step = (r1 - r0) / (nc * (nlip - 1));
+}
+
+/* grid.F90:1398-1424 */
+int dage_oracle_grid1d_ints(int ncell, int nlip, const double *steps,
+                            int grid_type, double *ints) {
+    double base[16];
+    if (dage_oracle_lip_integrals(nlip, grid_type, base)) return -1;
+    int n = ncell * (nlip - 1) + 1;
+    for (int i = 0; i < n; ++i) ints[i] = 0.0;
+    for (int ic = 0; ic < ncell; ++ic)
+        for (int j = 0; j < nlip; ++j)
+            ints[ic * (nlip - 1) + j] += base[j] * steps[ic];
+    return 0;
+}
+
+/* ------------------------------------------------------- exponential integrals
+ * exponentials.F90:695-1297.  Closed forms of int_a^b x^n e^{-t^2(x-p)^2} dx,
+ * restated term by term.  Ea/Eb stand for dexp(-t**2*(a-p)**2) and
+ * dexp(-t**2*(b-p)**2); ErA/ErB for derf(a*t-t*p) and derf(b*t-t*p). */
+static double series4(double q) {
+    return q + q * q / 2.0 + q * q * q / 6.0 + q * q * q * q / 24.0;
+}
+
+static double expo_0(double a, double b, double t, double p) {
+    const double thr = 1.0e-6;
+    if (t == 0.0) return b - a;
+    if (t < thr) return b - a;
+    return o_sqrtpi() * (erf(b * t - t * p) - erf(a * t - t * p)) / t / 2.0;
+}
+
+static double expo_1(double a, double b, double t, double p) {
+    const double thr = 1.0e-6;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (b * b - a * a) / 2.0;
+    double v1 = p * SP * erf(t * (b - p));
+    double v2;
+    if (t > 1.0e-3) {
+        v2 = (exp(-(t * t) * ((a - p) * (a - p))) -
+              exp(-(t * t) * ((b - p) * (b - p)))) / t;
+    } else {
+        double q = -(t * t) * ((a - p) * (a - p));
+        double r = -(t * t) * ((b - p) * (b - p));
+        v2 = (q + q * q / 2.0 + q * q * q / 6.0 + q * q * q * q / 24.0 - r -
+              r * r / 2.0 - r * r * r / 6.0 - r * r * r * r / 24.0) / t;
+    }
+    double v3 = -(p * SP * erf(a * t - t * p));
+    return (v1 + v2 + v3) / t / 2.0;
+}
+
+static double expo_2(double a, double b, double t, double p) {
+    const double thr = 0.5e-4;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (b * b * b - a * a * a) / 3.0;
+    double v1 = p * p * SP * erf(b * t - t * p) / 2.0;
+    double v2, v3;
+    if (t > 1.0e-3) {
+        double Ea = exp(-(t * t) * ((a - p) * (a - p)));
+        double Eb = exp(-(t * t) * ((b - p) * (b - p)));
+        v2 = (Ea * a - Eb * b);
+        v3 = (Ea - Eb) * p / t / 2.0;
+    } else {
+        double q = -(t * t) * ((a - p) * (a - p));
+        double r = -(t * t) * ((b - p) * (b - p));
+        v2 = series4(q) * a - series4(r) * b;
+        v2 = (v2 + a - b);
+        v3 = series4(q) - series4(r);
+        v3 = v3 * p / t / 2.0;
+    }
+    double v5 = SP * (erf(t * (b - p)) - erf(t * (a - p))) / t / 2.0;
+    v2 = (v2 + v5) / t / 2.0;
+    v5 = 0.0;
+    double v4 = -(p * p * SP * erf(a * t - t * p) / 2.0);
+    return (v1 + v2 + v3 + v4 + v5) / t;
+}
+
+static double expo_3(double a, double b, double t, double p) {
+    const double thr = 1.0e-4;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (b * b * b * b - a * a * a * a) / 4.0;
+    double v1, v2, v3, v5;
+    if (t > 1.0e-3) {
+        double Ea = exp(-(t * t) * ((a - p) * (a - p)));
+        double Eb = exp(-(t * t) * ((b - p) * (b - p)));
+        v1 = 2.0 * (Ea * (a * a) - Eb * (b * b)) / (t * t);
+        v2 = 2.0 * p * (a * Ea - b * Eb) * t;
+        v3 = 2.0 * (Ea * (p * p) - Eb * (p * p)) / (t * t);
+        v5 = 2.0 * (Ea - Eb) / t;
+    } else {
+        double q = -(t * t) * ((a - p) * (a - p));
+        double r = -(t * t) * ((b - p) * (b - p));
+        v1 = (1.0 + series4(q)) * a * a - (1.0 + series4(r)) * b * b;
+        v1 = 2.0 * v1 / t / t;
+        v2 = series4(q) * a - series4(r) * b;
+        v2 = v2 + a - b;
+        v2 = 2.0 * v2 * p * t;
+        v3 = series4(q) - series4(r);
+        v3 = 2.0 * v3 * p * p / t / t;
+        v5 = series4(q) - series4(r);
+        v5 = 2.0 * v5 / t;
+    }
+    double dE = erf(b * t - t * p) - erf(a * t - t * p);
+    double v4 = 2.0 * (p * p * p) * SP * dE / t;
+    double v6 = 3.0 * p * SP * dE;
+    v6 = (v6 + v2 + v5) / t / t / t;
+    v2 = 0.0;
+    v5 = 0.0;
+    return 0.25 * (v1 + v2 + v3 + v4 + v5 + v6);
+}
+
+static double expo_4(double a, double b, double t, double p) {
+    const double thr = 1.0e-3;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (pow(b, 5) - pow(a, 5)) / 5.0;
+    double v1, v2, v3, v4, v7, v8;
+    double p2 = p * p, p3 = p * p * p;
+    if (t > 1.0e-3) {
+        double Ea = exp(-(t * t) * ((a - p) * (a - p)));
+        double Eb = exp(-(t * t) * ((b - p) * (b - p)));
+        v1 = 4.0 * (Ea * (a * a * a) - Eb * (b * b * b));
+        v2 = 4.0 * (Ea * p * (a * a) - Eb * p * (b * b));
+        v3 = 4.0 * (Ea * p2 * a - Eb * p2 * b);
+        v4 = 4.0 * (Ea * p3 - Eb * p3);
+        v7 = 10.0 * p * (Ea - Eb) / t / t;
+        v8 = 6.0 * (Ea * a - Eb * b);
+    } else {
+        double q = -(t * t) * ((a - p) * (a - p));
+        double r = -(t * t) * ((b - p) * (b - p));
+        double Sq = 1.0 + series4(q), Sr = 1.0 + series4(r);
+        v1 = 4.0 * (Sq * (a * a * a) - Sr * (b * b * b));
+        v2 = 4.0 * (Sq * p * a * a - Sr * p * b * b);
+        v3 = 4.0 * (Sq * p * p * a - Sr * p * p * b);
+        v4 = 4.0 * (series4(q) - series4(r)) * p3;
+        v7 = 10.0 * (series4(q) - series4(r)) * p / t / t;
+        v8 = 6.0 * (Sq * a - Sr * b);
+    }
+    double dE = erf(a * t - t * p) - erf(b * t - t * p);
+    double v5 = -(4.0 * t * (p2 * p2) * SP * dE);
+    double v6 = -(12.0 / t * p2 * SP * dE);
+    double v9 = -(3.0 * SP * dE / t);
+    v2 = v2 / (t * t);
+    v5 = (v4 + v5) / (t * t);
+    v7 = (v3 + v6 + v7) / (t * t);
+    v8 = ((v8 + v9) / t / t + v1) / (t * t);
+    v1 = v3 = v4 = v6 = v9 = 0.0;
+    return 0.125 * (v1 + v2 + v3 + v4 + v5 + v6 + v7 + v8 + v9);
+}
+
+static double expo_5(double a, double b, double t, double p) {
+    const double thr = 2.5e-3;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (pow(b, 6) - pow(a, 6)) / 6.0;
+    double v1, v2, v3, v4, v5, v8, v9, v11, v12;
+    double p2 = p * p, p3 = p * p * p, p4 = p2 * p2, p5 = p4 * p;
+    double a2 = a * a, a3 = a * a * a, a4 = a2 * a2;
+    double b2 = b * b, b3 = b * b * b, b4 = b2 * b2;
+    double t2 = t * t, t4 = t2 * t2;
+    if (t > 1.0e-2) {
+        double Ea = exp(-(t * t) * ((a - p) * (a - p)));
+        double Eb = exp(-(t * t) * ((b - p) * (b - p)));
+        v1 = 4.0 * (Ea * a4 - Eb * b4);
+        v2 = 4.0 * (Ea * p * a3 - Eb * p * b3);
+        v3 = 4.0 * (Ea * p2 * a2 - Eb * p2 * b2);
+        v4 = 4.0 * (Ea * p3 * a - Eb * p3 * b);
+        v5 = 4.0 * (Ea * p4 - Eb * p4);
+        v8 = 18.0 * (Ea - Eb) * p2 / t2;
+        v9 = 14.0 * (Ea * p * a - Eb * p * b) / t2;
+        v11 = 8.0 * (Ea * a2 - Eb * b2) / t2;
+        v12 = 8.0 * (Ea - Eb) / t4;
+    } else {
+        double q = -(t * t) * ((a - p) * (a - p));
+        double r = -(t * t) * ((b - p) * (b - p));
+        double Sq = 1.0 + series4(q), Sr = 1.0 + series4(r);
+        v1 = 4.0 * (Sq * a4 - Sr * b4);
+        v2 = 4.0 * (Sq * p * a3 - Sr * p * b3);
+        v3 = 4.0 * (Sq * p2 * a2 - Sr * p2 * b2);
+        v4 = 4.0 * (Sq * p3 * a - Sr * p3 * b);
+        v5 = 4.0 * ((series4(q)) * p4 - (series4(r)) * p4);
+        v8 = 18.0 * (((series4(q)) - (series4(r))) * p2 / t2);
+        v9 = 14.0 * ((Sq * a * p - Sr * b * p) / t2);
+        v11 = 8.0 * ((Sq * a * a - Sr * b * b) / t2);
+        v12 = 8.0 * (((series4(q)) - (series4(r))) / t4);
+    }
+    double dE = erf(a * t - t * p) - erf(b * t - t * p);
+    double v6 = -(4.0 * t * p5 * SP * dE);
+    double v7 = -(20.0 / t * p3 * SP * dE);
+    double v10 = -(15.0 * p * SP * dE / (t2 * t));
+    double value = v1 + v2 + v3 + v4 + v5 + v6;
+    value = value + v7 + v8 + v9 + v10 + v11 + v12;
+    value = value / t2;
+    return 0.125 * value;
+}
+
+static double expo_6(double a, double b, double t, double p) {
+    const double thr = 5.0e-3;
+    const double SP = o_sqrtpi();
+    if (t == 0.0 || t < thr) return (pow(b, 7) - pow(a, 7)) / 7.0;
+    double p2 = p * p, p3 = p * p * p, p4 = p2 * p2, p5 = p4 * p,
+           p6 = p3 * p3;
+    double a2 = a * a, a3 = a * a * a, a4 = a2 * a2, a5 = a4 * a;
+    double b2 = b * b, b3 = b * b * b, b4 = b2 * b2, b5 = b4 * b;
+    double Ea = exp(-(t * t) * ((a - p) * (a - p)));
+    double Eb = exp(-(t * t) * ((b - p) * (b - p)));
+    double dE = erf(b * t - t * p) - erf(a * t - t * p);
+    double v1 = 30.0 * (Ea * a - Eb * b);
+    double v2 = 15.0 * SP * dE;
+    v1 = (v1 + v2 / t) / t / t / t;
+    v2 = 0.0;
+    double v3 = 90.0 * p2 * SP * dE / t / t;
+    double v4 = 56.0 * (Ea - Eb) * p3 / t;
+    double v5 = 66.0 * (Ea - Eb) * p / t / t / t;
+    double v6 = 8.0 * (Ea - Eb) * p5 * t;
+    double v7 = 20.0 * (Ea * a3 - Eb * b3) / t;
+    double v8 = 36.0 * (Ea * a2 - Eb * b2) * p / t;
+    double v9 = 48.0 * (Ea * a - Eb * b) * p2 / t;
+    double v10 = 8.0 * p6 * SP * dE * (t * t);
+    double v11 = 8.0 * (Ea * a4 - Eb * b4) * p * t;
+    double v12 = 8.0 * (Ea * a3 - Eb * b3) * p2 * t;
+    double v13 = 8.0 * (Ea * a5 - Eb * b5) * t;
+    double v14 = 8.0 * (Ea * a - Eb * b) * p4 * t;
+    double v15 = 8.0 * (Ea * a2 - Eb * b2) * p3 * t;
+    double v16 = 60.0 * p4 * SP * dE;
+    double value = v1 + v2 + v3 + v4 + v5 + v6 + v7 + v8;
+    value = value + v9 + v10 + v11 + v12 + v13 + v14 + v15 + v16;
+    value = value / (t * t * t);
+    return 0.0625 * value;
+}
+
+double dage_oracle_expo(int n, double a, double b, double t, double p) {
+    switch (n) {
+    case 0: return expo_0(a, b, t, p);
+    case 1: return expo_1(a, b, t, p);
+    case 2: return expo_2(a, b, t, p);
+    case 3: return expo_3(a, b, t, p);
+    case 4: return expo_4(a, b, t, p);
+    case 5: return expo_5(a, b, t, p);
+    case 6: return expo_6(a, b, t, p);
+    default: return 1.23456789e10; /* expo_7, exponentials.F90:1299-1318 */
+    }
+}
+
+/* exponentials.F90:2287-2429: 4 equal sub-intervals x 12-point
+ * Gauss-Legendre with the reference's 15-digit node/weight literals. */
+double dage_oracle_gauss_expo(int n, double a, double b, double t, double p) {
+    static const double wk[12] = {
+        0.047175336386512, 0.106939325995318, 0.160078328543346,
+        0.203167426723066, 0.233492536538355, 0.249147045813403,
+        0.249147045813403, 0.233492536538355, 0.203167426723066,
+        0.160078328543346, 0.106939325995318, 0.047175336386512};
+    static const double tk[12] = {
+        -0.981560634246719, -0.904117256370475, -0.769902674194305,
+        -0.587317954286617, -0.367831498998180, -0.125233408511469,
+        0.125233408511469,  0.367831498998180,  0.587317954286617,
+        0.769902674194305,  0.904117256370475,  0.981560634246719};
+    if (t == 0.0) {
+        if (n == 0) return b - a;
+        return (pow(b, n + 1) - pow(a, n + 1)) / (n + 1);
+    }
+    const int interval = 4;
+    double limit[5];
+    for (int i = 1; i <= interval + 1; ++i)
+        limit[i - 1] = ((interval + 1 - i) * a + (i - 1) * b) / interval;
+    double summa = 0.0;
+    for (int i = 1; i <= interval; ++i) {
+        double a0 = limit[i - 1], a1 = limit[i];
+        double pr = 1.0 - a1 * 2.0 / (a1 - a0);
+        double qr = 2.0 / (a1 - a0);
+        for (int j = 0; j < 12; ++j) {
+            double tp = (tk[j] - pr) / qr;
+            double tw = wk[j] / qr;
+            double e = exp(-(t * t) * (tp - p) * (tp - p));
+            if (n > 0) {
+                double tpn = 1.0;
+                for (int k = 0; k < n; ++k) tpn *= tp;
+                summa = summa + tpn * e * tw;
+            } else {
+                summa = summa + e * tw;
+            }
+        }
+    }
+    return summa;
+}
+
+/* exponentials.F90:647-666 */
+void dage_oracle_integrals(double a, double b, double t, double p, int n,
+                           double *values) {
+    for (int i = 0; i < n; ++i)
+        values[n - 1 - i] = (t < 0.1) ? dage_oracle_gauss_expo(i, a, b, t, p)
+                                      : dage_oracle_expo(i, a, b, t, p);
+}
+
+/* exponentials.F90:178-346 for an empty PBC string: one image (idisp = 0),
+ * pst0 = x2cell(x2, icell) (grid.F90:1768-1776), LIP combination of the
+ * monomial integrals times the cell scale, accumulated cell by cell. */
+int dage_oracle_exponentials(double qmin_in, int ncell_in,
+                             const double *steps_in, int nlip, int grid_type,
+                             const double *points_out, int nout,
+                             const double *tp, int ntp, int transpose,
+                             double *out) {
+    int nin = ncell_in * (nlip - 1) + 1;
+    double *pin = (double *)malloc(sizeof(double) * nin);
+    double *celld = (double *)malloc(sizeof(double) * ncell_in);
+    double lipc[256], vals[16], temp[16];
+    double first, last;
+    if (!pin || !celld) { free(pin); free(celld); return -1; }
+    if (dage_oracle_grid1d(qmin_in, ncell_in, nlip, steps_in, grid_type, pin,
+                           celld) ||
+        dage_oracle_lip_coeffs(nlip, grid_type, lipc)) {
+        free(pin); free(celld);
+        return -1;
+    }
+    dage_oracle_lip_first_last(nlip, grid_type, &first, &last);
+    memset(out, 0, sizeof(double) * (size_t)nin * nout * ntp);
+    for (int it = 0; it < ntp; ++it) {
+        double t = tp[it];
+        for (int ic = 0; ic < ncell_in; ++ic) {
+            double h = steps_in[ic];
+            double shft = t * h;
+            int fg = ic * (nlip - 1);
+            for (int ix = 0; ix < nout; ++ix) {
+                double pst0 = (points_out[ix] - celld[ic]) / h + first;
+                dage_oracle_integrals(first, last, shft, pst0, nlip, vals);
+                for (int j = 0; j < nlip; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < nlip; ++k) s += lipc[j + nlip * k] * vals[k];
+                    temp[j] = s * h;
+                }
+                for (int j = 0; j < nlip; ++j) {
+                    size_t idx = transpose
+                        ? (size_t)(fg + j) + (size_t)nin * (ix + (size_t)nout * it)
+                        : (size_t)ix + (size_t)nout * ((fg + j) + (size_t)nin * it);
+                    out[idx] += temp[j];
+                }
+            }
+        }
+    }
+    free(pin);
+    free(celld);
+    return 0;
+}
+
+/* -------------------------------------------------------------- tensor apply
+ * function3d.F90:4301-4314: for each t, per input z-slice
+ *   tmp1 = Fx_t . in(:,:,iz);  tmp_u(:,:,iz) = tmp1 . Fy_t
+ * then per output y-row  out(:,iy,:) += w_t * tmp_u(:,iy,:) . Fz_t,
+ * followed by out += coda*in (:4327-4332). */
+int dage_oracle_transform_cube(const int di[3], const int dout[3], int ntp,
+                               const double *fx, const double *fy,
+                               const double *fz, const double *w, double coda,
+                               const double *in, double *out, int t0, int t1,
+                               int add_coda) {
+    const size_t nxi = di[0], nyi = di[1], nzi = di[2];
+    const size_t nxo = dout[0], nyo = dout[1], nzo = dout[2];
+    double *tmp1 = (double *)malloc(sizeof(double) * nxo * nyi);
+    double *tmpu = (double *)malloc(sizeof(double) * nxo * nyo * nzi);
+    double *m = (double *)malloc(sizeof(double) * nxo * nzo);
+    if (!tmp1 || !tmpu || !m) { free(tmp1); free(tmpu); free(m); return -1; }
+    memset(out, 0, sizeof(double) * nxo * nyo * nzo);
+    for (int ip = t0; ip < t1 && ip < ntp; ++ip) {
+        const double *Fx = fx + nxo * nxi * ip;
+        const double *Fy = fy + nyi * nyo * ip;
+        const double *Fz = fz + nzi * nzo * ip;
+        for (size_t iz = 0; iz < nzi; ++iz) {
+            const double *slice = in + nxi * nyi * iz;
+            memset(tmp1, 0, sizeof(double) * nxo * nyi);
+            for (size_t yi = 0; yi < nyi; ++yi)
+                for (size_t xi = 0; xi < nxi; ++xi) {
+                    double s = slice[xi + nxi * yi];
+                    const double *col = Fx + nxo * xi;
+                    double *dst = tmp1 + nxo * yi;
+                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += col[xo] * s;
+                }
+            double *tu = tmpu + nxo * nyo * iz;
+            memset(tu, 0, sizeof(double) * nxo * nyo);
+            for (size_t yo = 0; yo < nyo; ++yo)
+                for (size_t yi = 0; yi < nyi; ++yi) {
+                    double f = Fy[yi + nyi * yo];
+                    const double *src = tmp1 + nxo * yi;
+                    double *dst = tu + nxo * yo;
+                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
+                }
+        }
+        for (size_t iy = 0; iy < nyo; ++iy) {
+            memset(m, 0, sizeof(double) * nxo * nzo);
+            for (size_t zo = 0; zo < nzo; ++zo)
+                for (size_t zi = 0; zi < nzi; ++zi) {
+                    double f = Fz[zi + nzi * zo];
+                    const double *src = tmpu + nxo * (iy + nyo * zi);
+                    double *dst = m + nxo * zo;
+                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
+                }
+            for (size_t zo = 0; zo < nzo; ++zo) {
+                double *o = out + nxo * (iy + nyo * zo);
+                const double *src = m + nxo * zo;
+                for (size_t xo = 0; xo < nxo; ++xo) o[xo] = o[xo] + w[ip] * src[xo];
+            }
+        }
+    }
+    if (add_coda && coda != 0.0) {
+        if (nxi != nxo || nyi != nyo || nzi != nzo) {
+            free(tmp1); free(tmpu); free(m);
+            return -2; /* cube_project onto a different grid: not restated */
+        }
+        size_t n = nxo * nyo * nzo;
+        for (size_t i = 0; i < n; ++i) out[i] = out[i] + coda * in[i];
+    }
+    free(tmp1);
+    free(tmpu);
+    free(m);
+    return 0;
+}
+
+/* helmholtz3d.F90:411-444, coulomb.F90:150 */
+void dage_oracle_helmholtz_weights(const double *tp, const double *tw, int ntp,
+                                   double energy, double *w) {
+    const double two_over_sqrtpi = 2.0 / o_sqrtpi();
+    for (int i = 0; i < ntp; ++i) {
+        double wi = tw[i];
+        if (energy < -0.005) wi = wi * exp(2.0 * energy / (4.0 * (tp[i] * tp[i])));
+        w[i] = two_over_sqrtpi * wi;
+    }
+}
+
+/* gauss_generator.F90:291-318 */
+void dage_oracle_gauss_density(int ngau, const double *q, const double *alpha,
+                               const double *c, const double *x, int nx,
+                               const double *y, int ny, const double *z,
+                               int nz, double *cube) {
+    const double pi = o_pi();
+    double *ex = (double *)malloc(sizeof(double) * (nx + ny + nz));
+    double *ey = ex + nx, *ez = ey + ny;
+    memset(cube, 0, sizeof(double) * (size_t)nx * ny * nz);
+    for (int p = 0; p < ngau; ++p) {
+        double norm = pow(alpha[p] / pi, 1.5);
+        for (int i = 0; i < nx; ++i) ex[i] = exp(-alpha[p] * ((x[i] - c[3 * p]) * (x[i] - c[3 * p])));
+        for (int j = 0; j < ny; ++j) ey[j] = exp(-alpha[p] * ((y[j] - c[3 * p + 1]) * (y[j] - c[3 * p + 1])));
+        for (int k = 0; k < nz; ++k) ez[k] = exp(-alpha[p] * ((z[k] - c[3 * p + 2]) * (z[k] - c[3 * p + 2])));
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    size_t id = i + (size_t)nx * (j + (size_t)ny * k);
+                    cube[id] = cube[id] + q[p] * norm * ex[i] * ey[j] * ez[k];
+                }
+    }
+    free(ex);
+}
+
+/* gauss_generator.F90:384-429 (gausspot: erf(sqrt(a) r)/r with a series
+ * near r = 0 and 1/r far away). */
+static double gausspot(double alpha, double r) {
+    double sa = sqrt(alpha);
+    double temp = r * sa;
+    if (temp < 0.01)
+        return 2.0 / o_sqrtpi() * sa *
+               (1.0 - 0.33333333333333333 * (temp * temp) + 0.1 * (temp * temp * temp * temp));
+    if (temp < 10.0) return erf(r * sa) / r;
+    return 1.0 / r;
+}
+
+void dage_oracle_gauss_potential(int ngau, const double *q,
+                                 const double *alpha, const double *c,
+                                 const double *x, int nx, const double *y,
+                                 int ny, const double *z, int nz,
+                                 double *cube) {
+    memset(cube, 0, sizeof(double) * (size_t)nx * ny * nz);
+    for (int p = 0; p < ngau; ++p)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double dx = c[3 * p] - x[i], dy = c[3 * p + 1] - y[j],
+                           dz = c[3 * p + 2] - z[k];
+                    double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    size_t id = i + (size_t)nx * (j + (size_t)ny * k);
+                    cube[id] = cube[id] + q[p] * gausspot(alpha[p], r);
+                }
+}
+
+double dage_oracle_cube_integral(const double *wx, int nx, const double *wy,
+                                 int ny, const double *wz, int nz,
+                                 const double *f, const double *g) {
+    double total = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        double sk = 0.0;
+        for (int j = 0; j < ny; ++j) {
+            double sj = 0.0;
+            for (int i = 0; i < nx; ++i) {
+                size_t id = i + (size_t)nx * (j + (size_t)ny * k);
+                double v = g ? f[id] * g[id] : f[id];
+                sj += wx[i] * v;
+            }
+            sk += wy[j] * sj;
+        }
+        total += wz[k] * sk;
+    }
+    return total;
+}

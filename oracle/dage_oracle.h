@@ -1,0 +1,126 @@
+/*
+ * dage_oracle.h -- CPU FP64 restatement of DAGE's operator-application hot
+ * path (tensor-quadrature Coulomb3D/Helmholtz3D apply).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * library in dage_amd/ is compared against.  Only tests/, the smoke entry
+ * point and the CPU-baseline leg of bench.py may load it; the product path
+ * never calls it and has no CPU fallback.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the reference repository root).  Arrays are Fortran column-major, 0-based
+ * in C.
+ */
+#ifndef DAGE_ORACLE_H
+#define DAGE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants as src/bubbles/globals.F90:54-61 defines them. */
+double dage_oracle_pi(void);
+
+/* gauleg: src/bubbles/gauss.F90:337-384 (Newton on P_n, EPS 3e-15). */
+void dage_oracle_gauleg(int ngp, double *xabsc, double *weig);
+
+/* GaussQuadInit + gauleg_int: src/bubbles/gauss.F90:81-122, 307-329.
+ * tp, tw have nlin+nlog entries. */
+void dage_oracle_gauss_quad(int nlin, int nlog, double tstart, double tlin,
+                            double tlog, double *tp, double *tw);
+
+/* gauss_lobatto_grid: src/bubbles/gauss.F90:409-446 (n = 6, 7 only).
+ * Returns 0 on success, -1 for an unsupported n. */
+int dage_oracle_gauss_lobatto_grid(int n, double left, double right,
+                                   double *tp);
+
+/* LIPBasisInit first/last: src/bubbles/lip.F90:118-142. */
+void dage_oracle_lip_first_last(int nlip, int grid_type, double *first,
+                                double *last);
+
+/* LIPBasis_coeffs(0): src/bubbles/lip.F90:285-344.  coeffs is nlip x nlip
+ * column-major; coeffs[j + nlip*k] is the coefficient of x^(nlip-1-k) of
+ * basis polynomial j.  Returns -1 for an unsupported grid type. */
+int dage_oracle_lip_coeffs(int nlip, int grid_type, double *coeffs);
+
+/* LIPBasis_integrals: src/bubbles/lip.F90:367-415. */
+int dage_oracle_lip_integrals(int nlip, int grid_type, double *integrals);
+
+/* Grid1D_init_step: src/bubbles/grid.F90:446-537.  points has
+ * ncell*(nlip-1)+1 entries, celld has ncell entries. */
+int dage_oracle_grid1d(double qmin, int ncell, int nlip, const double *steps,
+                       int grid_type, double *points, double *celld);
+
+/* Grid1D_init_equidistant cell count and step: src/bubbles/grid.F90:539-567
+ * (also Grid3D_init_equidistant :704-751). */
+void dage_oracle_equidistant_cells(double r0, double r1, int nlip,
+                                   double stepmax, int *ncell, double *step);
+
+/* Grid1D_get_ints: src/bubbles/grid.F90:1398-1424 (whole axis). */
+int dage_oracle_grid1d_ints(int ncell, int nlip, const double *steps,
+                            int grid_type, double *ints);
+
+/* integrals(a,b,t,p,n): src/bubbles/exponentials.F90:647-666.
+ * values[n-1-i] = int_a^b x^i exp(-t^2 (x-p)^2) dx. */
+void dage_oracle_integrals(double a, double b, double t, double p, int n,
+                           double *values);
+
+/* expo_0 .. expo_6 closed forms: src/bubbles/exponentials.F90:695-1297. */
+double dage_oracle_expo(int n, double a, double b, double t, double p);
+
+/* gauss_expo_n: src/bubbles/exponentials.F90:2287-2429. */
+double dage_oracle_gauss_expo(int n, double a, double b, double t, double p);
+
+/* exponentials(): src/bubbles/exponentials.F90:178-346 for one axis, no
+ * periodic images.  Input axis: (qmin_in, ncell_in, steps_in); output axis
+ * points (points_out, nout).  When transpose == 0 the result is
+ * (nout, nin, T) column-major (the X matrix); otherwise (nin, nout, T) (the
+ * Y and Z matrices, exponentials.F90:629-634). */
+int dage_oracle_exponentials(double qmin_in, int ncell_in,
+                             const double *steps_in, int nlip, int grid_type,
+                             const double *points_out, int nout,
+                             const double *tp, int ntp, int transpose,
+                             double *out);
+
+/* Operator3D_transform_cube: src/bubbles/function3d.F90:4230-4349, CPU
+ * branch (loop :4301-4314), then out += coda*in (:4327-4332) when coda != 0
+ * (same input/output grid, where cube_project is the identity copy,
+ * :4143-4146).
+ * fx: (nxo,nxi,T), fy: (nyi,nyo,T), fz: (nzi,nzo,T); in: (nxi,nyi,nzi);
+ * out: (nxo,nyo,nzo), overwritten.  Only t-points [t0, t1) are summed
+ * (t0=0, t1=T for the full operator); the coda is added only when
+ * add_coda != 0. */
+int dage_oracle_transform_cube(const int dims_in[3], const int dims_out[3],
+                               int ntp, const double *fx, const double *fy,
+                               const double *fz, const double *w, double coda,
+                               const double *in, double *out, int t0, int t1,
+                               int add_coda);
+
+/* Helmholtz3D_set_energy weights: src/bubbles/helmholtz3d.F90:411-444 and
+ * Coulomb3D w = (2/sqrt(pi))*tw, coulomb.F90:150. */
+void dage_oracle_helmholtz_weights(const double *tp, const double *tw, int ntp,
+                                   double energy, double *w);
+
+/* GaussGenerator_gen_cubef: src/bubbles/gauss_generator.F90:291-318. */
+void dage_oracle_gauss_density(int ngau, const double *q, const double *alpha,
+                               const double *centers, const double *x, int nx,
+                               const double *y, int ny, const double *z,
+                               int nz, double *cube);
+
+/* GaussPotGenerator_gen_cubef: src/bubbles/gauss_generator.F90:384-429. */
+void dage_oracle_gauss_potential(int ngau, const double *q,
+                                 const double *alpha, const double *centers,
+                                 const double *x, int nx, const double *y,
+                                 int ny, const double *z, int nz,
+                                 double *cube);
+
+/* Cube integral sum_{ijk} wx_i wy_j wz_k f_ijk (reduce3d / integrate_cube,
+ * src/bubbles/integrator.cu:320-373, parallel_info.F90 integrate_cube). */
+double dage_oracle_cube_integral(const double *wx, int nx, const double *wy,
+                                 int ny, const double *wz, int nz,
+                                 const double *f, const double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
