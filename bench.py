@@ -37,7 +37,33 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=2)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="time a plain device copy of a 257^3 cube instead (the known-size "
+                        "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
+
+
+def copy_calibration(args):
+    import numpy as np
+
+    import dage_amd
+    n = 257
+    ctx = dage_amd.Context(0)
+    a = dage_amd.DeviceCube(ctx, (n, n, n), np.ones((n, n, n), order="F"))
+    b = dage_amd.DeviceCube(ctx, (n, n, n))
+    for _ in range(args.warmup):
+        b.copy_from(a)
+    ctx.time_begin()
+    for _ in range(args.steps):
+        b.copy_from(a)
+    ms = ctx.time_end() / args.steps
+    nbytes = 8 * n ** 3
+    print(json.dumps({"mode": "copy_calibration", "kernel": "cube_copy_kernel",
+                      "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "ms_per_launch": round(ms, 4),
+                      "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    for obj in (a, b, ctx):
+        obj.close()
 
 
 def main():
@@ -46,6 +72,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md for how the apply would shard).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args)
+        return
 
     import numpy as np
 

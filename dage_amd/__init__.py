@@ -5,10 +5,10 @@ The product is the C-ABI library dage_amd/libdage_ops.so (include/dage_ops.h);
 this package is its host-side mirror of the reference's Operator3D API.
 """
 from ._lib import DAGE_OP_NO_CODA, DAGE_PREC_FP64, DageError, build, load
-from .operators import (Context, Coulomb3D, DeviceCube, GaussQuad, Grid3D,
-                        Helmholtz3D, Operator3D)
+from .operators import (Bubbles, Context, Coulomb3D, DeviceCube, GaussQuad, Grid3D,
+                        Helmholtz3D, Operator3D, radial_grid_steps)
 
 __all__ = [
-    "Context", "Coulomb3D", "DeviceCube", "GaussQuad", "Grid3D", "Helmholtz3D", "Operator3D",
-    "DageError", "DAGE_OP_NO_CODA", "DAGE_PREC_FP64", "build", "load",
+    "Bubbles", "Context", "Coulomb3D", "DeviceCube", "GaussQuad", "Grid3D", "Helmholtz3D", "Operator3D",
+    "radial_grid_steps", "DageError", "DAGE_OP_NO_CODA", "DAGE_PREC_FP64", "build", "load",
 ]

@@ -49,6 +49,7 @@ _SIGNATURES = {
     "dage_cube_download": [_c_p, _dp],
     "dage_cube_set_to_zero": [_c_p],
     "dage_cube_device_pointer": [_c_p, ctypes.POINTER(_c_p)],
+    "dage_cube_copy": [_c_p, _c_p],
     "dage_gauss_quad": [_int, _int, _dbl, _dbl, _dbl, _dp, _dp],
     "dage_grid3d_create": [_c_p, _dp, _ip, _int, _dp, _dp, _dp, _int, ctypes.POINTER(_c_p)],
     "dage_grid3d_create_equidistant": [_c_p, _dp, _int, _dbl, _int, ctypes.POINTER(_c_p)],
@@ -67,6 +68,11 @@ _SIGNATURES = {
     "dage_operator_workspace_bytes": [_c_p, ctypes.POINTER(ctypes.c_size_t)],
     "dage_operator_destroy": [_c_p],
     "dage_cube_integrate_device": [_c_p, _c_p, _c_p, _dp],
+    "dage_bubbles_create": [_c_p, _int, _int, _int, _int, ctypes.POINTER(_c_p)],
+    "dage_bubbles_set": [_c_p, _int, _dp, _dbl, _int, _dp, _dp],
+    "dage_bubbles_inject_device": [_c_p, _c_p, _int, _dbl, _c_p],
+    "dage_bubbles_destroy": [_c_p],
+    "dage_radial_grid_steps": [_dbl, _int, _int, _dbl, _dp],
 }
 _RESTYPES = {"dage_last_error": ctypes.c_char_p}
 

@@ -287,6 +287,17 @@ int dage_cube_set_to_zero(dage_cube c) {
     return DAGE_OK;
 }
 
+int dage_cube_copy(dage_cube dst, dage_cube src) {
+    DAGE_REQUIRE(dst && src, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(dst->ctx == src->ctx, DAGE_ERR_ARG, "cubes belong to different contexts");
+    DAGE_REQUIRE(dst->shape[0] == src->shape[0] && dst->shape[1] == src->shape[1] &&
+                     dst->shape[2] == src->shape[2],
+                 DAGE_ERR_ARG, "cube shapes differ");
+    DAGE_TRY(use_device(dst->ctx));
+    if (dst == src) return DAGE_OK;
+    return copy_device(dst->ctx, dst->d, src->d, dst->count());
+}
+
 int dage_cube_device_pointer(dage_cube c, double **dev) {
     DAGE_REQUIRE(c && dev, DAGE_ERR_ARG, "null argument");
     *dev = c->d;
@@ -344,8 +355,13 @@ int dage_grid3d_create(dage_ctx ctx, const double qmin[3], const int ncell[3], i
             for (int j = 0; j < nlip; ++j) w[c * (nlip - 1) + j] += base[j] * ax.steps[c];
         if (hipMalloc((void **)&g->d_weights[a], sizeof(double) * ax.n()) != hipSuccess ||
             hipMemcpy(g->d_weights[a], w.data(), sizeof(double) * ax.n(),
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc((void **)&g->d_points[a], sizeof(double) * ax.n()) != hipSuccess ||
+            hipMemcpy(g->d_points[a], ax.points.data(), sizeof(double) * ax.n(),
                       hipMemcpyHostToDevice) != hipSuccess) {
             for (auto &p : g->d_weights)
+                if (p) (void)hipFree(p);
+            for (auto &p : g->d_points)
                 if (p) (void)hipFree(p);
             ctx_release(ctx);
             delete g;
@@ -381,6 +397,8 @@ int dage_grid3d_destroy(dage_grid3d g) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &p : g->d_weights)
+        if (p) (void)hipFree(p);
+    for (auto &p : g->d_points)
         if (p) (void)hipFree(p);
     delete g;
     ctx_release(ctx);
@@ -577,6 +595,120 @@ int dage_operator_destroy(dage_operator op) {
     (void)hipStreamSynchronize(ctx->stream);
     free_operator_buffers(op);
     delete op;
+    ctx_release(ctx);
+    return DAGE_OK;
+}
+
+// ------------------------------------------------------------------ bubbles
+int dage_radial_grid_steps(double z, int ncell, int nlip, double r_max, double *steps) {
+    DAGE_REQUIRE(steps && z > 0.0 && ncell >= 1 && nlip >= 2 && r_max > 0.0, DAGE_ERR_ARG,
+                 "invalid radial grid parameters");
+    double x0 = 0.0;
+    const double cst = 8.0 * std::pow(z, -1.5);
+    const double dx = r_max / ncell;
+    for (int i = 1; i <= ncell; ++i) {
+        const double x1 = cst / ((r_max + cst) / (i * dx) - 1.0);
+        steps[i - 1] = (x1 - x0) / (nlip - 1);
+        x0 = x1;
+    }
+    return DAGE_OK;
+}
+
+int dage_bubbles_create(dage_ctx ctx, int nbub, int lmax, int k, int nlip, dage_bubbles *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(nbub >= 0, DAGE_ERR_ARG, "negative bubble count");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxBubbleLmax, DAGE_ERR_UNSUPPORTED,
+                 "bubbles lmax %d outside 0..%d", lmax, kMaxBubbleLmax);
+    DAGE_REQUIRE(nlip >= 2 && nlip <= 7, DAGE_ERR_UNSUPPORTED, "bubble nlip %d outside 2..7",
+                 nlip);
+    auto *b = new (std::nothrow) dage_bubbles_s();
+    DAGE_REQUIRE(b != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    b->ctx = ctx;
+    b->nbub = nbub;
+    b->lmax = lmax;
+    b->k = k;
+    b->nlip = nlip;
+    b->host.resize(nbub);
+    if (nbub > 0 &&
+        hipMalloc((void **)&b->d_desc, sizeof(BubbleDev) * nbub) != hipSuccess) {
+        delete b;
+        set_error("device allocation of bubble descriptors failed");
+        return DAGE_ERR_ALLOC;
+    }
+    ctx_retain(ctx);
+    *out = b;
+    return DAGE_OK;
+}
+
+int dage_bubbles_set(dage_bubbles b, int ibub, const double center[3], double charge, int ncell,
+                     const double *steps, const double *bf) {
+    DAGE_REQUIRE(b && center && steps && bf, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(ibub >= 0 && ibub < b->nbub, DAGE_ERR_ARG, "bubble index %d outside 0..%d",
+                 ibub, b->nbub - 1);
+    DAGE_REQUIRE(ncell >= 1, DAGE_ERR_ARG, "radial grid needs at least one cell");
+    DAGE_TRY(use_device(b->ctx));
+    Axis ax;
+    DAGE_REQUIRE(make_axis(0.0, ncell, b->nlip, steps, 1, &ax) == 0, DAGE_ERR_ARG,
+                 "invalid radial grid");
+    for (int c = 0; c < ncell; ++c)
+        DAGE_REQUIRE(steps[c] > 0.0 && std::isfinite(steps[c]), DAGE_ERR_ARG,
+                     "radial step %d is not positive", c);
+    const int npts = ax.n();
+    const int nlm = (b->lmax + 1) * (b->lmax + 1);
+    // r^k folded into the radial values on the grid (Bubbles_rpow,
+    // bubbles.F90:3363-3378: r(1)^k = 0 for k < 0)
+    std::vector<double> buf(2 * (size_t)ncell + (size_t)npts * nlm);
+    std::memcpy(buf.data(), ax.celld.data(), sizeof(double) * ncell);
+    std::memcpy(buf.data() + ncell, steps, sizeof(double) * ncell);
+    double *f = buf.data() + 2 * ncell;
+    for (int lm = 0; lm < nlm; ++lm)
+        for (int i = 0; i < npts; ++i) {
+            const double v = bf[i + (size_t)npts * lm];
+            double rp = 1.0;
+            if (b->k > 0) rp = std::pow(ax.points[i], b->k);
+            else if (b->k < 0) rp = (i == 0) ? 0.0 : std::pow(ax.points[i], b->k);
+            f[i + (size_t)npts * lm] = (b->k == 0) ? v : v * rp;
+        }
+    BubbleHost &h = b->host[ibub];
+    DAGE_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    if (h.d_buf) DAGE_HIP_CHECK(hipFree(h.d_buf));
+    h.d_buf = nullptr;
+    h.set = false;
+    DAGE_HIP_CHECK(hipMalloc((void **)&h.d_buf, sizeof(double) * buf.size()));
+    DAGE_HIP_CHECK(hipMemcpy(h.d_buf, buf.data(), sizeof(double) * buf.size(),
+                             hipMemcpyHostToDevice));
+    for (int a = 0; a < 3; ++a) h.center[a] = center[a];
+    h.charge = charge;
+    h.ncell = ncell;
+    h.qmax = ax.points[npts - 1];
+    h.set = true;
+    b->dirty = true;
+    return DAGE_OK;
+}
+
+int dage_bubbles_inject_device(dage_bubbles b, dage_grid3d grid, int lmin, double multiplier,
+                               double *cube) {
+    DAGE_REQUIRE(b && grid && cube, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(grid->ctx == b->ctx, DAGE_ERR_ARG, "grid belongs to another context");
+    DAGE_REQUIRE(lmin >= 0, DAGE_ERR_ARG, "negative lmin");
+    DAGE_TRY(use_device(b->ctx));
+    for (int i = 0; i < b->nbub; ++i)
+        DAGE_REQUIRE(b->host[i].set, DAGE_ERR_STATE, "bubble %d has not been set", i);
+    if (b->nbub == 0 || lmin > b->lmax) return DAGE_OK;
+    return bubbles_inject(b, grid, lmin, multiplier, cube);
+}
+
+int dage_bubbles_destroy(dage_bubbles b) {
+    if (!b) return DAGE_OK;
+    dage_ctx ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &h : b->host)
+        if (h.d_buf) (void)hipFree(h.d_buf);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    delete b;
     ctx_release(ctx);
     return DAGE_OK;
 }

@@ -46,16 +46,22 @@ struct Gemm {
 // [w*16*MW, (w+1)*16*MW) and all 16*NB columns.  MFMA f64 16x16x4 operand
 // layout on gfx950 (measured with a one-hot probe): lane l holds A(l%16, l/16)
 // and B(l/16, l%16); result register r of lane l is D(4*r + l/16, l%16).
-template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG>
+//
+// Operand panels of kBK k-values are double-buffered in LDS (one barrier per
+// panel); the next panel's global loads are issued before the current panel's
+// MFMAs, from clamped addresses so no load sits behind a branch.  Values
+// outside the matrix are loaded as zeros, so padded k-steps add nothing.
+// PASS only distinguishes the three apply passes in profiles.
+template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG, int PASS>
 __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     constexpr int BM = 64 * MW;
     constexpr int BN = 16 * NB;
     constexpr int A_PER = BM * kBK / kThreads;  // elements per thread
-    constexpr int B_PER = BN * kBK / kThreads;
+    constexpr int B_PER = NB;                   // BN * kBK / kThreads
     constexpr int LDA = BM + 1;  // +1 double breaks power-of-two strides
     constexpr int LDB = BN + 1;
-    __shared__ double As[kBK * LDA];
-    __shared__ double Bs[kBK * LDB];
+    __shared__ double As[2][kBK * LDA];
+    __shared__ double Bs[2][kBK * LDB];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -82,55 +88,59 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     }
     const int K = kend - kbeg;
 
-    // Per-slot global offsets (fixed across K panels).
-    long a_off[A_PER];
-    int a_k[A_PER];
-    int a_sm[A_PER];  // LDS index of the element
-#pragma unroll
-    for (int r = 0; r < A_PER; ++r) {
-        const int idx = tid + r * kThreads;
-        int m, k;
-        if (A_MCONTIG) { m = idx % BM; k = idx / BM; }
-        else           { k = idx % kBK; m = idx / kBK; }
-        const int gm = m0 + m;
-        a_k[r] = (gm < g.M) ? k : -1 - k;  // negative: row out of range
-        const int mlo = gm % g.M0, mhi = gm / g.M0;
-        a_off[r] = (long)mlo * g.sAm0 + (long)mhi * g.sAm1 + (long)k * g.sAk;
-        a_sm[r] = k * LDA + m;
+    // Loader geometry: every thread loads A_PER elements of the A panel and NB
+    // of the B panel at fixed (m, k) / (k, n) steps from one base position, so
+    // a slot's address is base + slot * compile-time multiple of a stride.
+    //   A m-contiguous: m = tid % BM, k = tid / BM + slot * (256 / BM)
+    //   A k-contiguous: k = tid % 16, m = tid / 16 + slot * 16 (one-level M)
+    //   B k-contiguous: k = tid % 16, n = tid / 16 + slot * 16
+    //   B n-contiguous: n = tid % 16 + slot * 16, k = tid / 16
+    constexpr int A_DK = A_MCONTIG ? kThreads / BM : 0;
+    constexpr int A_DM = A_MCONTIG ? 0 : kThreads / kBK;
+    const int a_m = A_MCONTIG ? tid % BM : tid / kBK;
+    const int a_k = A_MCONTIG ? tid / BM : tid % kBK;
+    long a_base;
+    bool a_row_ok;  // m-contiguous: the single row of this thread is inside M
+    {
+        const int gm = m0 + a_m;
+        a_row_ok = gm < g.M;
+        const int mlo = a_row_ok ? gm % g.M0 : 0, mhi = a_row_ok ? gm / g.M0 : 0;
+        a_base = (long)mlo * g.sAm0 + (long)mhi * g.sAm1 + (long)a_k * g.sAk;
     }
-    long b_off[B_PER];
-    int b_k[B_PER];
-    int b_sm[B_PER];
-#pragma unroll
-    for (int r = 0; r < B_PER; ++r) {
-        const int idx = tid + r * kThreads;
-        int n, k;
-        if (B_KCONTIG) { k = idx % kBK; n = idx / kBK; }
-        else           { n = idx % BN; k = idx / BN; }
-        const int gn = n0 + n;
-        b_k[r] = (gn < g.N) ? k : -1 - k;
-        b_off[r] = (long)k * g.sBk + (long)gn * g.sBn;
-        b_sm[r] = k * LDB + n;
-    }
+    const long a_step = A_MCONTIG ? (long)A_DK * g.sAk : (long)A_DM * g.sAm0;
+    const int a_sm0 = a_k * LDA + a_m;
+    constexpr int a_sstep = A_MCONTIG ? A_DK * LDA : A_DM;
+
+    const int bk = B_KCONTIG ? tid % kBK : tid / 16;
+    const int bn = B_KCONTIG ? tid / kBK : tid % 16;
+    const long b_base = (long)bk * g.sBk + (long)(n0 + bn) * g.sBn;
+    const long b_step = 16 * g.sBn;
+    const int b_sm0 = bk * LDB + bn;
 
     double ra[A_PER], rb[B_PER];
     auto load_panel = [&](int kp) {
+        const int lim = K - kp;  // valid k within this panel: [0, lim)
+        const long koff = (long)kp * g.sAk;
 #pragma unroll
         for (int r = 0; r < A_PER; ++r) {
-            const int k = a_k[r];
-            ra[r] = (k >= 0 && kp + k < K) ? A[a_off[r] + (long)kp * g.sAk] : 0.0;
+            const bool ok = A_MCONTIG ? (a_row_ok && a_k + r * A_DK < lim)
+                                      : (a_k < lim && m0 + a_m + r * A_DM < g.M);
+            const double v = A[ok ? a_base + r * a_step + koff : 0];
+            ra[r] = ok ? v : 0.0;
         }
+        const long kboff = (long)kp * g.sBk;
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
-            const int k = b_k[r];
-            rb[r] = (k >= 0 && kp + k < K) ? B[b_off[r] + (long)kp * g.sBk] : 0.0;
+            const bool ok = bk < lim && n0 + bn + 16 * r < g.N;
+            const double v = B[ok ? b_base + r * b_step + kboff : 0];
+            rb[r] = ok ? v : 0.0;
         }
     };
-    auto store_panel = [&]() {
+    auto store_panel = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < A_PER; ++r) As[a_sm[r]] = ra[r];
+        for (int r = 0; r < A_PER; ++r) As[buf][a_sm0 + r * a_sstep] = ra[r];
 #pragma unroll
-        for (int r = 0; r < B_PER; ++r) Bs[b_sm[r]] = rb[r];
+        for (int r = 0; r < B_PER; ++r) Bs[buf][b_sm0 + 16 * r] = rb[r];
     };
 
     d4 acc[MW][NB];
@@ -143,29 +153,42 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const int lk = lane >> 4;
     const int wrow = wave * 16 * MW;
 
-    load_panel(0);
-    for (int kp = 0; kp < K; kp += kBK) {
-        store_panel();
+    auto kstep = [&](const double *as, const double *bs, int ks) {
+        const int kk = ks * 4 + lk;
+        double a[MW], b[NB];
+#pragma unroll
+        for (int i = 0; i < MW; ++i) a[i] = as[kk * LDA + wrow + i * 16 + lrow];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) b[j] = bs[kk * LDB + j * 16 + lrow];
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+
+    const int npanels = (K + kBK - 1) / kBK;
+    if (npanels > 0) {
+        load_panel(0);
+        store_panel(0);
         __syncthreads();
-        if (kp + kBK < K) load_panel(kp + kBK);
-        const int ksteps = min(kBK, K - kp + 3) / 4;  // skip all-padding steps
-        for (int ks = 0; ks < ksteps; ++ks) {
-            const int kk = ks * 4 + lk;
-            double a[MW], b[NB];
+    }
+    for (int p = 0; p < npanels; ++p) {
+        const int buf = p & 1;
+        const bool more = p + 1 < npanels;
+        if (more) load_panel((p + 1) * kBK);
+        const double *as = As[buf];
+        const double *bs = Bs[buf];
+        // uniform bound: the last panel skips its all-padding k-steps
+        const int ksteps = more ? kBK / 4 : (K - p * kBK + 3) / 4;
 #pragma unroll
-            for (int i = 0; i < MW; ++i) a[i] = As[kk * LDA + wrow + i * 16 + lrow];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) b[j] = Bs[kk * LDB + j * 16 + lrow];
-#pragma unroll
-            for (int i = 0; i < MW; ++i)
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+        for (int ks = 0; ks < kBK / 4; ++ks)
+            if (ks < ksteps) kstep(as, bs, ks);
+        if (more) store_panel(buf ^ 1);
         __syncthreads();
     }
 
-    // Epilogue: register r of lane l is D(4*r + l/16, l%16) of each 16x16 block.
+    // Epilogue.  Register r of lane l holds D(4*r + l/16, l%16).
 #pragma unroll
     for (int i = 0; i < MW; ++i) {
 #pragma unroll
@@ -226,40 +249,43 @@ int pick_nb(int n) {
     return best;
 }
 
-template <int MW, int NB, bool AM, bool BK>
+constexpr int kMW = 1;  // 64-row tiles: 2-3 waves per SIMD at NB = 7
+
+template <int NB, bool AM, bool BK, int PASS>
 void launch_t(const Gemm &g, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((gemm_f64_kernel<MW, NB, AM, BK>), grid, dim3(kThreads), 0, s, g);
+    hipLaunchKernelGGL((gemm_f64_kernel<kMW, NB, AM, BK, PASS>), grid, dim3(kThreads), 0, s, g);
 }
 
-template <bool AM, bool BK>
+template <bool AM, bool BK, int PASS>
 int launch_nb(const Gemm &g, int nb, int zdim, hipStream_t s) {
-    constexpr int MW = 2;
-    const int BM = 64 * MW;
+    const int BM = 64 * kMW;
     const dim3 grid((g.M + BM - 1) / BM, (g.N + 16 * nb - 1) / (16 * nb), zdim);
     switch (nb) {
-    case 1: launch_t<MW, 1, AM, BK>(g, grid, s); break;
-    case 2: launch_t<MW, 2, AM, BK>(g, grid, s); break;
-    case 3: launch_t<MW, 3, AM, BK>(g, grid, s); break;
-    case 4: launch_t<MW, 4, AM, BK>(g, grid, s); break;
-    case 5: launch_t<MW, 5, AM, BK>(g, grid, s); break;
-    case 6: launch_t<MW, 6, AM, BK>(g, grid, s); break;
-    case 7: launch_t<MW, 7, AM, BK>(g, grid, s); break;
-    default: launch_t<MW, 8, AM, BK>(g, grid, s); break;
+    case 1: launch_t<1, AM, BK, PASS>(g, grid, s); break;
+    case 2: launch_t<2, AM, BK, PASS>(g, grid, s); break;
+    case 3: launch_t<3, AM, BK, PASS>(g, grid, s); break;
+    case 4: launch_t<4, AM, BK, PASS>(g, grid, s); break;
+    case 5: launch_t<5, AM, BK, PASS>(g, grid, s); break;
+    case 6: launch_t<6, AM, BK, PASS>(g, grid, s); break;
+    case 7: launch_t<7, AM, BK, PASS>(g, grid, s); break;
+    default: launch_t<8, AM, BK, PASS>(g, grid, s); break;
     }
     DAGE_HIP_CHECK(hipGetLastError());
     return DAGE_OK;
 }
 
-int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, bool a_mcontig, bool b_kcontig,
-                int zdim) {
+// The three passes have fixed operand layouts: x mode reads k-contiguous A and
+// n-contiguous B; y and z modes read m-contiguous A and k-contiguous B.
+template <int PASS>
+int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim) {
+    // the k-contiguous A loader steps rows linearly: one-level row index only
+    DAGE_REQUIRE(PASS != 0 || g.M0 >= g.M, DAGE_ERR_STATE, "x-mode GEMM needs a one-level M");
     const int nb = pick_nb(g.N);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
     int rc;
-    if (a_mcontig && b_kcontig) rc = launch_nb<true, true>(g, nb, zdim, ctx->stream);
-    else if (a_mcontig) rc = launch_nb<true, false>(g, nb, zdim, ctx->stream);
-    else if (b_kcontig) rc = launch_nb<false, true>(g, nb, zdim, ctx->stream);
-    else rc = launch_nb<false, false>(g, nb, zdim, ctx->stream);
+    if constexpr (PASS == 0) rc = launch_nb<false, false, PASS>(g, nb, zdim, ctx->stream);
+    else rc = launch_nb<true, true, PASS>(g, nb, zdim, ctx->stream);
     prof_end(ctx, name, ev);
     return rc;
 }
@@ -280,7 +306,7 @@ Plan make_plan(dage_operator op) {
     Plan p;
     p.tchunk = (int)std::max<size_t>(1, std::min<size_t>(op->ntp, budget / per_t));
     const long M3 = nxo * nyo;
-    const int mtiles = (int)((M3 + 127) / 128);
+    const int mtiles = (int)((M3 + 64 * kMW - 1) / (64 * kMW));
     const int nb = pick_nb((int)nzo);
     const int ntiles = (int)((nzo + 16 * nb - 1) / (16 * nb));
     const long K3 = nzi * p.tchunk;
@@ -342,7 +368,7 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         g1.sBk = nxo; g1.sBn = 1;
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1;
         g1.bA = 0; g1.bB = nxo * nxi; g1.bC = nxo * nyi * nzi;
-        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_xmode", g1, false, false, tc));
+        DAGE_TRY(launch_gemm<0>(op->ctx, "gemm_f64_xmode", g1, tc));
 
         // pass 2: U((xo,yo),(zi,t)) = w_t V_t((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
@@ -355,7 +381,7 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo;
         g2.bA = nxo * nyi * nzi; g2.bB = nyi * nyo; g2.bC = nxo * nyo * nzi;
         g2.alpha = op->d_w + t0;
-        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_ymode", g2, true, true, tc));
+        DAGE_TRY(launch_gemm<1>(op->ctx, "gemm_f64_ymode", g2, tc));
 
         // pass 3: P_s((xo,yo), zo) (+)= U((xo,yo), k) . Gz(k, zo), k in split s
         const int K3 = (int)(nzi * tc);
@@ -375,7 +401,7 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         if (t0 == 0 && splits < plan.splits)
             DAGE_TRY(fill_zero(op->ctx, P + (size_t)splits * M3 * nzo,
                                (size_t)(plan.splits - splits) * M3 * nzo));
-        DAGE_TRY(launch_gemm(op->ctx, "gemm_f64_zmode", g3, true, true, splits));
+        DAGE_TRY(launch_gemm<2>(op->ctx, "gemm_f64_zmode", g3, splits));
     }
     const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
     const long len = M3 * nzo;

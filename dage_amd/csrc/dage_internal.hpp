@@ -106,7 +106,39 @@ struct dage_grid3d_s {
     dage_ctx ctx = nullptr;
     dage::Axis axis[3];
     double *d_weights[3] = {nullptr, nullptr, nullptr};  // LIP weights (device)
+    double *d_points[3] = {nullptr, nullptr, nullptr};   // grid points (device)
     int shape(int a) const { return axis[a].n(); }
+};
+
+namespace dage {
+constexpr int kMaxBubbleLmax = 6;
+
+// Device view of one bubble (read by the inject kernel).
+struct BubbleDev {
+    double cx, cy, cz;
+    double qmax;           // last radial grid point
+    const double *celld;   // ncell cell starts
+    const double *steps;   // ncell cell steps
+    const double *f;       // npts x (lmax+1)^2, r^k folded in, point fastest
+    int ncell, npts;
+};
+
+struct BubbleHost {
+    double center[3] = {0.0, 0.0, 0.0};
+    double charge = 0.0;
+    int ncell = 0;
+    double qmax = 0.0;
+    double *d_buf = nullptr;  // celld | steps | f
+    bool set = false;
+};
+}  // namespace dage
+
+struct dage_bubbles_s {
+    dage_ctx ctx = nullptr;
+    int nbub = 0, lmax = 0, k = 0, nlip = 7;
+    std::vector<dage::BubbleHost> host;
+    dage::BubbleDev *d_desc = nullptr;
+    bool dirty = true;
 };
 
 struct dage_operator_s {
@@ -148,8 +180,14 @@ int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);
 // Relayout Fz (nzi, nzo, T) -> Gz ((zi + nzi*t), zo).
 int relayout_fz(dage_operator op);
+// Bubbles (inject.hip, abi.hip).
+int bubbles_sync_device(dage_bubbles bub);
+int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multiplier,
+                   double *cube);
 // Zero n doubles with a fill kernel on the context stream (graph-safe, unlike
 // a memset node).
 int fill_zero(dage_ctx ctx, double *p, size_t n);
+// dst[i] = src[i] with a streaming copy kernel.
+int copy_device(dage_ctx ctx, double *dst, const double *src, size_t n);
 
 }  // namespace dage

@@ -1,0 +1,194 @@
+// Bubbles inject: cube(x,y,z) += multiplier * sum_bubbles sum_{l>=lmin,m}
+//                 interp(r^k f_lm)(r) * S_lm(x-c)/r^l
+// (Bubbles_eval_3dgrid, src/bubbles/bubbles.F90:2148-2211, and the CUDA
+// Bubbles::inject it dispatches to, src/bubbles/bubbles_cuda.cu:3090-3340).
+//
+// One fused launch for all bubbles and all (l,m): one thread per cube point,
+// x fastest across the 64 lanes of a wavefront so the read-modify-write of the
+// cube is coalesced.  Per bubble the thread finds the radial cell by the
+// reference's bisection (grid.F90:1212-1246), evaluates the nlip cell
+// polynomials once by Horner (lip.F90:551-566), and reuses them for every
+// (l,m); the Racah solid harmonics come from the recursion of
+// harmonics.F90:586-659, fully unrolled for the compile-time lmax.
+#include <cmath>
+#include <cstring>
+
+#include "dage_internal.hpp"
+
+namespace dage {
+namespace {
+
+struct InjectArgs {
+    const double *gx, *gy, *gz;  // cube grid points
+    int nx, ny, nz;
+    const BubbleDev *bubs;
+    int nbub;
+    int lmin;
+    int nlip;
+    double first;
+    double multiplier;
+    double *cube;
+    double lip[kMaxNlip * kMaxNlip];  // lip[j + nlip*k], decreasing powers
+};
+
+template <int LMAX>
+__device__ inline void solid_harmonics(double x, double y, double z, double *S) {
+    S[0] = 1.0;
+    if constexpr (LMAX >= 1) {
+        S[1] = y;
+        S[2] = z;
+        S[3] = x;
+    }
+    const double r2 = x * x + y * y + z * z;
+#pragma unroll
+    for (int l = 2; l <= LMAX; ++l) {
+        const int c = l * l + l, p = (l - 1) * (l - 1) + (l - 1), q = (l - 2) * (l - 2) + (l - 2);
+        const double f = sqrt((2 * l - 1.0) / (2.0 * l));
+        S[c - l] = f * (y * S[p + (l - 1)] + x * S[p - (l - 1)]);
+#pragma unroll
+        for (int m = -(l - 1); m <= l - 1; ++m) {
+            double v = (2 * l - 1.0) / sqrt(1.0 * (l + m) * (l - m)) * (z * S[p + m]);
+            if (l - (m < 0 ? -m : m) > 1)
+                v = v - sqrt((l + m - 1.0) * (l - m - 1.0) / ((l + m) * (l - m))) * (r2 * S[q + m]);
+            S[c + m] = v;
+        }
+        S[c + l] = f * (x * S[p + (l - 1)] - y * S[p - (l - 1)]);
+    }
+}
+
+// Grid1D_get_icell: 1-based cell of r, 0 outside [0, qmax].
+__device__ inline int radial_icell(const double *celld, int ncell, double qmax, double r) {
+    if (!(r >= 0.0 && r <= qmax)) return 0;  // also rejects NaN
+    if (celld[ncell - 1] <= r) return ncell;
+    int icell = ncell / 2, maxi = ncell - 1, mini = 1;
+    for (int it = 0; it < 64; ++it) {  // bisection over <= 2^63 cells
+        if (icell < 1 || icell > ncell - 1) return 0;
+        if (r >= celld[icell]) {
+            if (r < celld[icell + 1]) return icell + 1;
+            mini = icell + 2;
+            icell = (mini + maxi) / 2;
+        } else {
+            if (r >= celld[icell - 1]) return icell;
+            maxi = icell - 1;
+            icell = (mini + maxi) / 2;
+        }
+    }
+    return 0;
+}
+
+template <int LMAX>
+__global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= A.nx || iy >= A.ny) return;
+    const double px = A.gx[ix], py = A.gy[iy], pz = A.gz[iz];
+    constexpr double kEps = 2.220446049250313e-16;  // epsilon(0.d0)
+    double total = 0.0;
+    for (int b = 0; b < A.nbub; ++b) {
+        const BubbleDev B = A.bubs[b];
+        const double dx = px - B.cx, dy = py - B.cy, dz = pz - B.cz;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const int ic = radial_icell(B.celld, B.ncell, B.qmax, r);
+        if (ic == 0) continue;
+        const double crd = (r - B.celld[ic - 1]) / B.steps[ic - 1] + A.first;
+        double polys[kMaxNlip];
+#pragma unroll
+        for (int j = 0; j < kMaxNlip; ++j) {
+            if (j >= A.nlip) break;
+            double v = A.lip[j];
+            for (int q = 1; q < A.nlip; ++q) v = v * crd + A.lip[j + A.nlip * q];
+            polys[j] = v;
+        }
+        double S[(LMAX + 1) * (LMAX + 1)];
+        solid_harmonics<LMAX>(dx, dy, dz, S);
+        const double *f = B.f + (long)(ic - 1) * (A.nlip - 1);
+        double res = 0.0;
+        double rl = 1.0;
+#pragma unroll
+        for (int l = 0; l <= LMAX; ++l) {
+            if (l >= A.lmin) {
+                const double den = (r > kEps) ? rl : 1.0;
+#pragma unroll
+                for (int m = -l; m <= l; ++m) {
+                    const int lm = l * l + l + m;
+                    const double *fl = f + (long)lm * B.npts;
+                    double fv = 0.0;
+                    for (int j = 0; j < A.nlip; ++j) fv += fl[j] * polys[j];
+                    res += fv * (S[lm] / den);
+                }
+            }
+            rl *= r;
+        }
+        total += res;
+    }
+    const long id = ix + (long)A.nx * (iy + (long)A.ny * iz);
+    A.cube[id] = A.cube[id] + A.multiplier * total;
+}
+
+}  // namespace
+
+int bubbles_sync_device(dage_bubbles bub) {
+    if (!bub->dirty) return DAGE_OK;
+    std::vector<BubbleDev> desc(bub->nbub);
+    for (int b = 0; b < bub->nbub; ++b) {
+        const BubbleHost &h = bub->host[b];
+        BubbleDev &d = desc[b];
+        d.cx = h.center[0];
+        d.cy = h.center[1];
+        d.cz = h.center[2];
+        d.qmax = h.qmax;
+        d.ncell = h.ncell;
+        d.npts = h.ncell * (bub->nlip - 1) + 1;
+        d.celld = h.d_buf;
+        d.steps = h.d_buf + h.ncell;
+        d.f = h.d_buf + 2 * h.ncell;
+    }
+    DAGE_HIP_CHECK(hipMemcpy(bub->d_desc, desc.data(), sizeof(BubbleDev) * bub->nbub,
+                             hipMemcpyHostToDevice));
+    bub->dirty = false;
+    return DAGE_OK;
+}
+
+int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multiplier,
+                   double *cube) {
+    dage_ctx ctx = bub->ctx;
+    DAGE_TRY(bubbles_sync_device(bub));
+    InjectArgs A{};
+    A.gx = grid->d_points[0];
+    A.gy = grid->d_points[1];
+    A.gz = grid->d_points[2];
+    A.nx = grid->shape(0);
+    A.ny = grid->shape(1);
+    A.nz = grid->shape(2);
+    A.bubs = bub->d_desc;
+    A.nbub = bub->nbub;
+    A.lmin = lmin;
+    A.nlip = bub->nlip;
+    double last;
+    lip_first_last(bub->nlip, &A.first, &last);
+    DAGE_REQUIRE(lip_coeffs(bub->nlip, 1, A.lip) == 0, DAGE_ERR_ARG, "unsupported bubble nlip");
+    A.multiplier = multiplier;
+    A.cube = cube;
+    const dim3 block(64, 4, 1);
+    const dim3 g((A.nx + 63) / 64, (A.ny + 3) / 4, A.nz);
+    hipEvent_t ev;
+    prof_begin(ctx, "bubbles_inject_kernel", &ev);
+    switch (bub->lmax) {
+    case 0: hipLaunchKernelGGL(bubbles_inject_kernel<0>, g, block, 0, ctx->stream, A); break;
+    case 1: hipLaunchKernelGGL(bubbles_inject_kernel<1>, g, block, 0, ctx->stream, A); break;
+    case 2: hipLaunchKernelGGL(bubbles_inject_kernel<2>, g, block, 0, ctx->stream, A); break;
+    case 3: hipLaunchKernelGGL(bubbles_inject_kernel<3>, g, block, 0, ctx->stream, A); break;
+    case 4: hipLaunchKernelGGL(bubbles_inject_kernel<4>, g, block, 0, ctx->stream, A); break;
+    case 5: hipLaunchKernelGGL(bubbles_inject_kernel<5>, g, block, 0, ctx->stream, A); break;
+    case 6: hipLaunchKernelGGL(bubbles_inject_kernel<6>, g, block, 0, ctx->stream, A); break;
+    default:
+        set_error("bubbles lmax %d outside 0..%d", bub->lmax, kMaxBubbleLmax);
+        return DAGE_ERR_UNSUPPORTED;
+    }
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "bubbles_inject_kernel", ev);
+    return DAGE_OK;
+}
+
+}  // namespace dage

@@ -51,7 +51,35 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(double *p, size_t n) {
         p[i] = 0.0;
 }
 
+__global__ void __launch_bounds__(256) cube_copy_kernel(double2 *dst, const double2 *src,
+                                                        size_t n2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void cube_copy_tail_kernel(double *dst, const double *src, size_t i) { dst[i] = src[i]; }
+
 }  // namespace
+
+int copy_device(dage_ctx ctx, double *dst, const double *src, size_t n) {
+    if (n == 0) return DAGE_OK;
+    const size_t n2 = n / 2;
+    hipEvent_t ev;
+    prof_begin(ctx, "cube_copy_kernel", &ev);
+    if (n2) {
+        const size_t blocks = (n2 + 255) / 256;
+        const int grid = (int)(blocks < 8192 ? blocks : 8192);
+        hipLaunchKernelGGL(cube_copy_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                           (double2 *)dst, (const double2 *)src, n2);
+    }
+    if (n & 1)
+        hipLaunchKernelGGL(cube_copy_tail_kernel, dim3(1), dim3(1), 0, ctx->stream, dst, src,
+                           n - 1);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "cube_copy_kernel", ev);
+    return DAGE_OK;
+}
 
 int fill_zero(dage_ctx ctx, double *p, size_t n) {
     if (n == 0) return DAGE_OK;

@@ -127,6 +127,9 @@ class DeviceCube:
     def set_to_zero(self):
         call("dage_cube_set_to_zero", self.handle)
 
+    def copy_from(self, src: "DeviceCube"):
+        call("dage_cube_copy", self.handle, src.handle)
+
 
 class GaussQuad:
     """t-quadrature: nlin Gauss-Legendre points on [tstart, tlin] plus nlog on a
@@ -335,3 +338,51 @@ class Helmholtz3D(Coulomb3D):
     def set_energy(self, energy: float):
         call("dage_operator_set_energy", self.handle, float(energy))
         self.energy = float(energy)
+
+
+def radial_grid_steps(z: float, ncell: int, nlip: int = 7, r_max: float = 20.0) -> np.ndarray:
+    """Grid1D_init_radial cell steps (grid.F90:569-603) for ncell cells."""
+    st = np.zeros(ncell)
+    call("dage_radial_grid_steps", float(z), ncell, nlip, float(r_max), _ptr(st))
+    return st
+
+
+class Bubbles:
+    """Atom-centred radial functions x real solid harmonics (bubbles.F90:90-230).
+
+    bf for each bubble is (npts, (lmax+1)**2) in Fortran order with lm index
+    l*l+l+m; the radial part used is r**k * f_lm(r)."""
+
+    def __init__(self, ctx: Context, nbub: int, lmax: int, k: int = 0, nlip: int = 7):
+        h = ctypes.c_void_p()
+        call("dage_bubbles_create", ctx.handle, nbub, lmax, k, nlip, ctypes.byref(h))
+        self.handle = h
+        self.ctx = ctx
+        self.nbub, self.lmax, self.k, self.nlip = nbub, lmax, k, nlip
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_bubbles_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bubble(self, ibub: int, center, charge: float, steps, bf):
+        steps = _f64(steps)
+        npts = len(steps) * (self.nlip - 1) + 1
+        bf = np.asfortranarray(bf, dtype=np.float64)
+        if bf.shape != (npts, (self.lmax + 1) ** 2):
+            raise ValueError(f"bf shape {bf.shape} != {(npts, (self.lmax + 1) ** 2)}")
+        call("dage_bubbles_set", self.handle, ibub, _ptr(_f64(center)), float(charge), len(steps),
+             _ptr(steps), bf.ctypes.data_as(_dp))
+
+    def inject(self, grid: Grid3D, cube: DeviceCube, lmin: int = 0, multiplier: float = 1.0):
+        """cube += multiplier * bubbles (l >= lmin) evaluated on grid."""
+        if cube.shape != grid.get_shape():
+            raise ValueError(f"cube shape {cube.shape} != grid shape {grid.get_shape()}")
+        call("dage_bubbles_inject_device", self.handle, grid.handle, lmin, float(multiplier),
+             ctypes.c_void_p(cube.ptr))

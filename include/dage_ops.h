@@ -58,6 +58,7 @@ typedef struct dage_ctx_s *dage_ctx;
 typedef struct dage_grid3d_s *dage_grid3d;
 typedef struct dage_operator_s *dage_operator;
 typedef struct dage_cube_s *dage_cube;
+typedef struct dage_bubbles_s *dage_bubbles;
 
 /* ----------------------------------------------------------------- library */
 int dage_abi_version(void);
@@ -104,6 +105,9 @@ int dage_cube_download(dage_cube cube, double *host);
 int dage_cube_set_to_zero(dage_cube cube);
 /* Device address of the cube, for the *_device entry points. */
 int dage_cube_device_pointer(dage_cube cube, double **dev);
+/* dst = src on the device (same shape), with a streaming copy kernel
+ * ("cube_copy_kernel"); asynchronous on the context stream. */
+int dage_cube_copy(dage_cube dst, dage_cube src);
 
 /* -------------------------------------------------------------- quadrature
  * GaussQuadInit (src/bubbles/gauss.F90:81-122, gauleg_int :307-329,
@@ -170,6 +174,33 @@ int dage_operator_apply_device(dage_operator op, const double *cubein_dev,
 /* Bytes of device workspace the operator holds for its intermediates. */
 int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
 int dage_operator_destroy(dage_operator op);
+
+/* ----------------------------------------------------------------- bubbles
+ * Atom-centred radial functions times real solid harmonics
+ * (src/bubbles/bubbles.F90:90-230).  Replaces bubbles_init_cuda,
+ * bubble_init_cuda, bubble_upload_all_cuda and bubbles_inject_cuda
+ * (src/bubbles/bubbles_cuda.cu:3715-3776).  lmax and k are shared by all
+ * bubbles (r^k f_lm(r) is the radial part); radial grids start at r = 0 with
+ * equidistant-cell LIP basis of nlip points. */
+int dage_bubbles_create(dage_ctx ctx, int nbub, int lmax, int k, int nlip,
+                        dage_bubbles *bubbles);
+/* Bubble ibub (0-based): centre, nuclear charge (informational), radial grid
+ * (ncell cells with the given steps) and radial functions bf, an
+ * npts x (lmax+1)^2 column-major array (npts = ncell*(nlip-1)+1, lm index
+ * l*l+l+m), as Fortran holds bf(ibub)%p. */
+int dage_bubbles_set(dage_bubbles bubbles, int ibub, const double center[3],
+                     double charge, int ncell, const double *steps,
+                     const double *bf);
+/* Bubbles_eval_3dgrid / bubbles_inject_cuda: cube_dev += multiplier *
+ * sum_b sum_{l>=lmin} interp(r^k f_lm)(r) S_lm(r-c_b)/r^l on the points of
+ * grid (asynchronous on the context stream). */
+int dage_bubbles_inject_device(dage_bubbles bubbles, dage_grid3d grid, int lmin,
+                               double multiplier, double *cube_dev);
+int dage_bubbles_destroy(dage_bubbles bubbles);
+/* Grid1D_init_radial (src/bubbles/grid.F90:569-603) cell steps for ncell
+ * cells: x_i = c/((r_max+c)/(i*dx)-1), c = 8 z^-1.5, dx = r_max/ncell. */
+int dage_radial_grid_steps(double z, int ncell, int nlip, double r_max,
+                           double *steps);
 
 /* ---------------------------------------------------------------- cube sums
  * Integrator3D reduce3d (src/bubbles/integrator.cu:320-373) /

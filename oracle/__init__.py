@@ -61,6 +61,11 @@ def _load():
         "dage_oracle_gauss_density": (None, [i, _dp, _dp, _dp, _dp, i, _dp, i, _dp, i, _dp]),
         "dage_oracle_gauss_potential": (None, [i, _dp, _dp, _dp, _dp, i, _dp, i, _dp, i, _dp]),
         "dage_oracle_cube_integral": (d, [_dp, i, _dp, i, _dp, i, _dp, _dp]),
+        "dage_oracle_radial_steps": (None, [d, i, i, d, _dp]),
+        "dage_oracle_get_icell": (i, [_dp, i, d, d, d]),
+        "dage_oracle_solid_harmonics": (None, [i, d, d, d, _dp]),
+        "dage_oracle_bubbles_eval_grid": (i, [i, _dp, i, i, i, _ip, i, ctypes.POINTER(_dp),
+                                              ctypes.POINTER(_dp), _dp, i, _dp, i, _dp, i, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -270,3 +275,58 @@ def cube_integral(wx, wy, wz, f, g=None) -> float:
         gp = g.ctypes.data_as(_dp)
     return _load().dage_oracle_cube_integral(_p(wx), len(wx), _p(wy), len(wy), _p(wz), len(wz),
                                              f.ctypes.data_as(_dp), gp)
+
+
+# -------------------------------------------------------------------- bubbles
+def radial_steps(z: float, ncell: int, nlip: int = 7, r_max: float = 20.0) -> np.ndarray:
+    """Grid1D_init_radial cell steps for a given cell count (grid.F90:569-603)."""
+    st = np.zeros(ncell)
+    _load().dage_oracle_radial_steps(float(z), ncell, nlip, float(r_max), _p(st))
+    return st
+
+
+def solid_harmonics(lmax: int, x: float, y: float, z: float) -> np.ndarray:
+    """Racah real regular solid harmonics, index l*l + l + m."""
+    S = np.zeros((lmax + 1) ** 2)
+    _load().dage_oracle_solid_harmonics(lmax, x, y, z, _p(S))
+    return S
+
+
+def get_icell(ax: "Axis", x: float) -> int:
+    return _load().dage_oracle_get_icell(_p(ax.celld), ax.ncell, ax.points[0], ax.points[-1], x)
+
+
+@dataclass
+class BubbleSet:
+    """Bubbles: centres (nbub,3), per-bubble radial steps (from r=0) and radial
+    functions bf (npts, (lmax+1)^2) in Fortran order; global lmax and k."""
+
+    centers: np.ndarray
+    steps: list
+    bf: list
+    lmax: int
+    k: int = 0
+    nlip: int = 7
+
+    def axis(self, b: int) -> Axis:
+        return Axis(0.0, self.steps[b], self.nlip)
+
+
+def bubbles_eval_grid(bub: BubbleSet, x, y, z, lmin: int = 0, cube=None):
+    """Bubbles_eval_3dgrid CPU path: returns cube + sum of bubbles on the grid."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    nb = len(bub.steps)
+    centers = _f64(np.asarray(bub.centers).reshape(nb, 3))
+    steps = [_f64(s) for s in bub.steps]
+    bfs = [np.asfortranarray(b, dtype=np.float64) for b in bub.bf]
+    ncell = (ctypes.c_int * nb)(*[len(s) for s in steps])
+    sp = (_dp * nb)(*[_p(s) for s in steps])
+    bp = (_dp * nb)(*[b.ctypes.data_as(_dp) for b in bfs])
+    out = np.zeros((len(x), len(y), len(z)), order="F") if cube is None else \
+        np.array(cube, dtype=np.float64, order="F", copy=True)
+    rc = _load().dage_oracle_bubbles_eval_grid(nb, _p(centers), bub.lmax, bub.k, lmin, ncell,
+                                               bub.nlip, sp, bp, _p(x), len(x), _p(y), len(y),
+                                               _p(z), len(z), out.ctypes.data_as(_dp))
+    if rc:
+        raise RuntimeError("bubbles_eval_grid failed")
+    return out

@@ -694,3 +694,130 @@ double dage_oracle_cube_integral(const double *wx, int nx, const double *wy,
     }
     return total;
 }
+
+/* ------------------------------------------------------------------- bubbles
+ * grid.F90:569-603 (Grid1D_init_radial) */
+void dage_oracle_radial_steps(double z, int ncell, int nlip, double r_max,
+                              double *steps) {
+    double x0 = 0.0;
+    const double cst = 8.0 * pow(z, -1.5);
+    const double dx = r_max / ncell;
+    for (int i = 1; i <= ncell; ++i) {
+        double x1 = cst / ((r_max + cst) / (i * dx) - 1.0);
+        steps[i - 1] = (x1 - x0) / (nlip - 1);
+        x0 = x1;
+    }
+}
+
+/* grid.F90:1212-1246 */
+int dage_oracle_get_icell(const double *celld, int ncell, double qmin,
+                          double qmax, double x) {
+    if (x > qmax || x < qmin) return 0;
+    if (celld[ncell - 1] <= x) return ncell;
+    int icell = ncell / 2, maxi = ncell - 1, mini = 1;
+    for (;;) {
+        if (x >= celld[icell]) {          /* celld(icell+1), 1-based */
+            if (x < celld[icell + 1]) return icell + 1;
+            mini = icell + 2;
+            icell = (mini + maxi) / 2;
+        } else {
+            if (x >= celld[icell - 1]) return icell;
+            maxi = icell - 1;
+            icell = (mini + maxi) / 2;
+        }
+    }
+}
+
+/* harmonics.F90:586-659: S_00 = 1, S_1 = (y, z, x), then for l >= 2
+ *   S_l,-l = sqrt((2l-1)/2l) (y S_l-1,l-1 + x S_l-1,-(l-1))
+ *   S_l,m  = (2l-1)/sqrt((l+m)(l-m)) z S_l-1,m
+ *            - sqrt((l+m-1)(l-m-1)/((l+m)(l-m))) r^2 S_l-2,m   (|m| < l-1)
+ *   S_l,l  = sqrt((2l-1)/2l) (x S_l-1,l-1 - y S_l-1,-(l-1))
+ * evaluated on values (the reference builds the same polynomials). */
+void dage_oracle_solid_harmonics(int lmax, double x, double y, double z,
+                                 double *S) {
+    S[0] = 1.0;
+    if (lmax < 1) return;
+    S[1] = y;
+    S[2] = z;
+    S[3] = x;
+    const double r2 = x * x + y * y + z * z;
+    for (int l = 2; l <= lmax; ++l) {
+        const int c = l * l + l, p = (l - 1) * (l - 1) + (l - 1), q = (l - 2) * (l - 2) + (l - 2);
+        const double f = sqrt((2 * l - 1.0) / (2.0 * l));
+        S[c - l] = f * (y * S[p + (l - 1)] + x * S[p - (l - 1)]);
+        for (int m = -(l - 1); m <= l - 1; ++m) {
+            double v = (2 * l - 1.0) / sqrt(1.0 * (l + m) * (l - m)) * (z * S[p + m]);
+            if (l - abs(m) > 1)
+                v = v - sqrt((l + m - 1.0) * (l - m - 1.0) / ((l + m) * (l - m))) * (r2 * S[q + m]);
+            S[c + m] = v;
+        }
+        S[c + l] = f * (x * S[p + (l - 1)] - y * S[p - (l - 1)]);
+    }
+}
+
+int dage_oracle_bubbles_eval_grid(int nbub, const double *centers, int lmax,
+                                  int k, int lmin, const int *ncell, int nlip,
+                                  const double *const *steps,
+                                  const double *const *bf, const double *x,
+                                  int nx, const double *y, int ny,
+                                  const double *z, int nz, double *cube) {
+    const int nlm = (lmax + 1) * (lmax + 1);
+    double lipc[256], first, last, S[256], polys[16];
+    if (lmax > 14 || dage_oracle_lip_coeffs(nlip, 1, lipc)) return -1;
+    dage_oracle_lip_first_last(nlip, 1, &first, &last);
+    const double eps = 2.220446049250313e-16; /* epsilon(0.d0), harmonics.F90:1345 */
+    for (int b = 0; b < nbub; ++b) {
+        const int nc = ncell[b], npts = nc * (nlip - 1) + 1;
+        double *pts = (double *)malloc(sizeof(double) * npts);
+        double *celld = (double *)malloc(sizeof(double) * nc);
+        double *fk = (double *)malloc(sizeof(double) * (size_t)npts * nlm);
+        if (!pts || !celld || !fk) { free(pts); free(celld); free(fk); return -1; }
+        dage_oracle_grid1d(0.0, nc, nlip, steps[b], 1, pts, celld);
+        /* bubble_values = bf * rpow (bubbles.F90:2118-2136, rpow :3363-3378) */
+        for (int lm = 0; lm < nlm; ++lm)
+            for (int i = 0; i < npts; ++i) {
+                double rp = 1.0;
+                if (k > 0) rp = pow(pts[i], k);
+                else if (k < 0) rp = (i == 0) ? 0.0 : pow(pts[i], k);
+                fk[i + (size_t)npts * lm] = (k == 0) ? bf[b][i + (size_t)npts * lm]
+                                                     : bf[b][i + (size_t)npts * lm] * rp;
+            }
+        const double qmax = pts[npts - 1];
+        const double cx = centers[3 * b], cy = centers[3 * b + 1], cz = centers[3 * b + 2];
+        for (int kk = 0; kk < nz; ++kk)
+            for (int jj = 0; jj < ny; ++jj)
+                for (int ii = 0; ii < nx; ++ii) {
+                    const double dx = x[ii] - cx, dy = y[jj] - cy, dz = z[kk] - cz;
+                    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    const int ic = dage_oracle_get_icell(celld, nc, 0.0, qmax, r);
+                    double res = 0.0;
+                    if (ic > 0) {
+                        const double crd = (r - celld[ic - 1]) / steps[b][ic - 1] + first;
+                        for (int j = 0; j < nlip; ++j) {  /* eval_polys, lip.F90:551-566 */
+                            double v = lipc[j];
+                            for (int q = 1; q < nlip; ++q) v = v * crd + lipc[j + nlip * q];
+                            polys[j] = v;
+                        }
+                        dage_oracle_solid_harmonics(lmax, dx, dy, dz, S);
+                        const int start = (ic - 1) * (nlip - 1);
+                        for (int l = lmin; l <= lmax; ++l) {
+                            const double rl = (r > eps) ? pow(r, l) : 1.0;
+                            for (int m = -l; m <= l; ++m) {
+                                const int lm = l * l + l + m;
+                                double fv = 0.0;
+                                for (int j = 0; j < nlip; ++j)
+                                    fv += fk[start + j + (size_t)npts * lm] * polys[j];
+                                res += fv * (S[lm] / rl);
+                            }
+                        }
+                    }
+                    const size_t id = ii + (size_t)nx * (jj + (size_t)ny * kk);
+                    cube[id] = cube[id] + res;
+                }
+        free(pts);
+        free(celld);
+        free(fk);
+    }
+    return 0;
+}

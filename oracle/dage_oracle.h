@@ -120,6 +120,36 @@ double dage_oracle_cube_integral(const double *wx, int nx, const double *wy,
                                  int ny, const double *wz, int nz,
                                  const double *f, const double *g);
 
+/* Grid1D_init_radial cell steps (src/bubbles/grid.F90:569-603) for a given
+ * cell count: x_i = c/((r_max+c)/(i dx) - 1), c = 8 z^-1.5, dx = r_max/ncell,
+ * step_i = (x_i - x_{i-1})/(nlip-1).  (The reference takes ncell =
+ * int(n0 z^0.25).) */
+void dage_oracle_radial_steps(double z, int ncell, int nlip, double r_max,
+                              double *steps);
+
+/* Grid1D_get_icell (src/bubbles/grid.F90:1212-1246): 1-based cell index of
+ * x by bisection over the cell starts, 0 outside [qmin, qmax]. */
+int dage_oracle_get_icell(const double *celld, int ncell, double qmin,
+                          double qmax, double x);
+
+/* Racah-normalised real regular solid harmonics S_lm(x,y,z), l = 0..lmax,
+ * S[l*l + l + m] (src/bubbles/harmonics.F90:586-659 recursion). */
+void dage_oracle_solid_harmonics(int lmax, double x, double y, double z,
+                                 double *S);
+
+/* Bubbles_eval_3dgrid CPU path (src/bubbles/bubbles.F90:2148-2211 ->
+ * Bubbles_eval_all :2017-2073 -> Bubbles_simple_eval_pick :2075-2146):
+ * cube[i,j,k] += sum_b sum_{l>=lmin} interp(r^k f_lm)(r) * S_lm/r^l
+ * with SimpleInterpolator1D_eval (src/bubbles/evaluators.F90:508-550).
+ * Bubble b has radial grid (ncell[b] cells of nlip points from r = 0, steps
+ * steps[b]) and radial functions bf[b] (npts x (lmax+1)^2, column-major). */
+int dage_oracle_bubbles_eval_grid(int nbub, const double *centers, int lmax,
+                                  int k, int lmin, const int *ncell, int nlip,
+                                  const double *const *steps,
+                                  const double *const *bf, const double *x,
+                                  int nx, const double *y, int ny,
+                                  const double *z, int nz, double *cube);
+
 #ifdef __cplusplus
 }
 #endif

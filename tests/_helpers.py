@@ -54,3 +54,24 @@ def random_gaussians(rng, n, alpha_range=(0.5, 5.0), box=2.0, logalpha=False):
 def rel_maxdiff(a, b) -> float:
     """max |a-b| / max |b|: the potential error measure used throughout."""
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def slater_bubbles(rng, natom, lmax=4, zset=tuple(range(1, 9)), box=4.0, ncell=33, nlip=7,
+                   r_max=20.0, k=0):
+    """Config-2 style bubbles: Z uniform in zset, centres U[-box,box]^3, radial
+    grid of ncell cells to r_max, f_lm(r) = c_lm r^l exp(-Z r), c_lm ~ U[-1,1]."""
+    Z = rng.choice(np.asarray(zset, float), natom)
+    centers = rng.uniform(-box, box, (natom, 3))
+    steps, bfs = [], []
+    for a in range(natom):
+        st = O.radial_steps(Z[a], ncell, nlip, r_max)
+        r = O.Axis(0.0, st, nlip).points
+        c = rng.uniform(-1.0, 1.0, (lmax + 1) ** 2)
+        bf = np.zeros((len(r), (lmax + 1) ** 2), order="F")
+        for l in range(lmax + 1):
+            for m in range(-l, l + 1):
+                lm = l * l + l + m
+                bf[:, lm] = c[lm] * r ** l * np.exp(-Z[a] * r)
+        steps.append(st)
+        bfs.append(bf)
+    return O.BubbleSet(centers, steps, bfs, lmax, k, nlip), Z
