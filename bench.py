@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Coulomb/Helmholtz apply Gpts/s & %MFMA peak; SCF time-to-1e-6 Ha vs plain FP64"
 FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# FP64 flops per (point, bubble) of the inject at lmax 4, nlip 7: 49 Horner,
+# 25 x 7 radial dot products, ~75 for the solid-harmonic recursion, 25 scalings
+FLOP_PER_POINT_BUBBLE = 2 * 49 + 2 * 25 * 7 + 75 + 2 * 25
 
 
 def parse():
@@ -37,10 +40,24 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=2)
+    p.add_argument("--no-inject", dest="inject", action="store_false",
+                   help="skip the bubbles-inject measurement")
     p.add_argument("--copy-calibration", action="store_true",
                    help="time a plain device copy of a 257^3 cube instead (the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/hbm_traffic.json: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+    factors from the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def copy_calibration(args):
@@ -138,7 +155,7 @@ def main():
         "peak": FP64_MFMA_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / FP64_MFMA_PEAK_TFLOPS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom),
         "avg_launch_ms": round(dom_ms, 4),
         "algorithmic_flops_per_launch": dom_flops,
     }
@@ -195,9 +212,76 @@ def main():
         "max_rel_err_vs_analytic": err_analytic,
         "wall_s_timed": round(wall, 4),
     }
-    print(json.dumps(line), flush=True)
-    for obj in (d_in, d_out, op, grid, ctx):
+    for obj in (d_in, d_out, op, grid):
         obj.close()
+    if args.inject:
+        line["inject"] = bench_inject(ctx, args)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_inject(ctx, args):
+    """Bubbles inject at the config-2 shape: 241^3 cube over [-12,12]^3,
+    4 atoms (Z in 1..8, centres in [-4,4]^3), 199-point radial grids to 20
+    bohr, lmax 4, f_lm = c_lm r^l exp(-Z r).  One fused launch per step."""
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    rng = np.random.default_rng(0xDA6E0002)
+    ax = O.Axis.ncell_fixed(-12.0, 12.0, 7, 40)
+    n = ax.n
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    lmax, natom = 4, 4
+    bub = dage_amd.Bubbles(ctx, natom, lmax, 0, 7)
+    Z = rng.integers(1, 9, natom).astype(float)
+    centers = rng.uniform(-4.0, 4.0, (natom, 3))
+    host = []
+    for a in range(natom):
+        st = dage_amd.radial_grid_steps(Z[a], 33, 7, 20.0)
+        r = O.Axis(0.0, st, 7).points
+        c = rng.uniform(-1.0, 1.0, (lmax + 1) ** 2)
+        bf = np.zeros((len(r), (lmax + 1) ** 2), order="F")
+        for l in range(lmax + 1):
+            for m in range(-l, l + 1):
+                bf[:, l * l + l + m] = c[l * l + l + m] * r ** l * np.exp(-Z[a] * r)
+        bub.set_bubble(a, centers[a], Z[a], st, bf)
+        host.append((st, bf))
+    cube = dage_amd.DeviceCube(ctx, (n, n, n))
+    cube.set_to_zero()
+    for _ in range(max(1, args.warmup)):
+        bub.inject(grid, cube)
+    steps = max(3, args.steps // 2)
+    ctx.profile(True)
+    ctx.time_begin()
+    for _ in range(steps):
+        bub.inject(grid, cube)
+    ms = ctx.time_end() / steps
+    kms, kn = ctx.profile_query("bubbles_inject_kernel")
+    ctx.profile(False)
+    kernel_ms = kms / max(kn, 1)
+    # parity on a slab of the cube: z-planes 0..7 through the oracle
+    cube.set_to_zero()
+    bub.inject(grid, cube)
+    got = cube.download()
+    bset = O.BubbleSet(centers, [h[0] for h in host], [h[1] for h in host], lmax, 0, 7)
+    ref = O.bubbles_eval_grid(bset, ax.points, ax.points, ax.points[:8])
+    err = float(np.max(np.abs(got[:, :, :8] - ref)) / np.max(np.abs(ref)))
+    nbytes = 16.0 * n ** 3  # cube read-modify-write per launch
+    for obj in (cube, bub, grid):
+        obj.close()
+    return {
+        "workload": "config 2 bubbles: 241^3 cube over [-12,12]^3, 4 atoms, lmax 4, "
+                    "199-point radial grids, fused all-atom inject",
+        "ms_per_launch": round(kernel_ms, 4),
+        "Gpts_s": round(n ** 3 / (ms * 1e-3) / 1e9, 3),
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / (kernel_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(nbytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("bubbles_inject_kernel")},
+        "fp64_valu_tflops": round(FLOP_PER_POINT_BUBBLE * natom * n ** 3 / (kernel_ms * 1e-3) / 1e12, 2),
+        "max_rel_err_vs_oracle_slab": err,
+    }
 
 
 if __name__ == "__main__":

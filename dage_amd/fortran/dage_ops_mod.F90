@@ -1,0 +1,267 @@
+!> ISO_C_BINDING interface to libdage_ops.so (include/dage_ops.h) and a thin
+!! Operator3D-style wrapper, so Fortran callers can swap the reference's
+!! CUDA path (src/bubbles/cuda.F90, function3d.F90 Operator3D_cuda_init /
+!! transform_cuda_cube) for the gfx950 library.
+!!
+!! Handles are opaque type(c_ptr).  Every C function returns an integer
+!! status; the wrappers below stop with the library's message on failure,
+!! which is how the reference reports errors (streamcontainer.h:67-73).
+!! Cubes are ordinary real(8) arrays cube(nx,ny,nz) in Fortran order.
+module dage_ops_m
+    use, intrinsic :: iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: DAGE_OK = 0
+    integer(c_int), parameter, public :: DAGE_PREC_FP64 = 0
+    integer(c_int), parameter, public :: DAGE_OP_NO_CODA = 1
+
+    public :: dage_gauss_quad, dage_ctx_create, dage_ctx_destroy, dage_ctx_synchronize
+    public :: dage_grid3d_create, dage_grid3d_destroy, dage_grid3d_shape
+    public :: dage_grid3d_coord, dage_grid3d_weights
+    public :: dage_coulomb3d_create, dage_operator_set_weights, dage_operator_set_energy
+    public :: dage_operator_apply, dage_operator_destroy, dage_last_error_f
+    public :: dage_bubbles_create, dage_bubbles_set, dage_bubbles_destroy
+    public :: DageOperator3D, dage_check
+
+    interface
+        integer(c_int) function dage_gauss_quad(nlin, nlog, tstart, tlin, tlog, tp, tw) &
+                bind(C, name="dage_gauss_quad")
+            import :: c_int, c_double
+            integer(c_int), value :: nlin, nlog
+            real(c_double), value :: tstart, tlin, tlog
+            real(c_double)        :: tp(*), tw(*)
+        end function
+
+        integer(c_int) function dage_ctx_create(device, ctx) bind(C, name="dage_ctx_create")
+            import :: c_int, c_ptr
+            integer(c_int), value :: device
+            type(c_ptr)           :: ctx
+        end function
+
+        integer(c_int) function dage_ctx_destroy(ctx) bind(C, name="dage_ctx_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function dage_ctx_synchronize(ctx) bind(C, name="dage_ctx_synchronize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function dage_grid3d_create(ctx, qmin, ncell, nlip, sx, sy, sz, &
+                                                   grid_type, grid) bind(C, name="dage_grid3d_create")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: ctx
+            real(c_double)        :: qmin(3)
+            integer(c_int)        :: ncell(3)
+            integer(c_int), value :: nlip, grid_type
+            real(c_double)        :: sx(*), sy(*), sz(*)
+            type(c_ptr)           :: grid
+        end function
+
+        integer(c_int) function dage_grid3d_destroy(grid) bind(C, name="dage_grid3d_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grid
+        end function
+
+        integer(c_int) function dage_grid3d_coord(grid, axis, points) bind(C, name="dage_grid3d_coord")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: grid
+            integer(c_int), value :: axis
+            real(c_double)        :: points(*)
+        end function
+
+        integer(c_int) function dage_grid3d_weights(grid, axis, weights) &
+                bind(C, name="dage_grid3d_weights")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: grid
+            integer(c_int), value :: axis
+            real(c_double)        :: weights(*)
+        end function
+
+        integer(c_int) function dage_grid3d_shape(grid, shape) bind(C, name="dage_grid3d_shape")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grid
+            integer(c_int)     :: shape(3)
+        end function
+
+        integer(c_int) function dage_coulomb3d_create(ctx, gridin, gridout, tp, tw, ntp, tlog, &
+                                                      flags, op) bind(C, name="dage_coulomb3d_create")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: ctx, gridin, gridout
+            real(c_double)        :: tp(*), tw(*)
+            integer(c_int), value :: ntp, flags
+            real(c_double), value :: tlog
+            type(c_ptr)           :: op
+        end function
+
+        integer(c_int) function dage_operator_set_weights(op, w) &
+                bind(C, name="dage_operator_set_weights")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: op
+            real(c_double)     :: w(*)
+        end function
+
+        integer(c_int) function dage_operator_set_energy(op, energy) &
+                bind(C, name="dage_operator_set_energy")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: op
+            real(c_double), value :: energy
+        end function
+
+        integer(c_int) function dage_operator_apply(op, cubein, cubeout, precision) &
+                bind(C, name="dage_operator_apply")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: op
+            real(c_double)        :: cubein(*), cubeout(*)
+            integer(c_int), value :: precision
+        end function
+
+        integer(c_int) function dage_operator_destroy(op) bind(C, name="dage_operator_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: op
+        end function
+
+        integer(c_int) function dage_bubbles_create(ctx, nbub, lmax, k, nlip, bubbles) &
+                bind(C, name="dage_bubbles_create")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: ctx
+            integer(c_int), value :: nbub, lmax, k, nlip
+            type(c_ptr)           :: bubbles
+        end function
+
+        integer(c_int) function dage_bubbles_set(bubbles, ibub, center, charge, ncell, steps, bf) &
+                bind(C, name="dage_bubbles_set")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: bubbles
+            integer(c_int), value :: ibub, ncell
+            real(c_double)        :: center(3)
+            real(c_double), value :: charge
+            real(c_double)        :: steps(*), bf(*)
+        end function
+
+        integer(c_int) function dage_bubbles_destroy(bubbles) bind(C, name="dage_bubbles_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: bubbles
+        end function
+
+        type(c_ptr) function dage_last_error() bind(C, name="dage_last_error")
+            import :: c_ptr
+        end function
+    end interface
+
+    !> Operator3D-shaped handle: gridin -> gridout with a t-quadrature
+    !! (coulomb.F90:109-176; function3d.F90:295-336).
+    type :: DageOperator3D
+        type(c_ptr) :: ctx = c_null_ptr
+        type(c_ptr) :: gridin = c_null_ptr
+        type(c_ptr) :: gridout = c_null_ptr
+        type(c_ptr) :: op = c_null_ptr
+        integer     :: dims_in(3) = 0, dims_out(3) = 0
+    contains
+        procedure :: transform_cube => DageOperator3D_transform_cube
+        procedure :: set_energy     => DageOperator3D_set_energy
+        procedure :: destroy        => DageOperator3D_destroy
+    end type
+
+    interface DageOperator3D
+        module procedure :: DageOperator3D_init_coulomb
+    end interface
+
+contains
+
+    !> Message of the last failed call on this thread.
+    function dage_last_error_f() result(msg)
+        character(len=:), allocatable :: msg
+        character(kind=c_char), pointer :: chars(:)
+        integer :: n
+        type(c_ptr) :: p
+        p = dage_last_error()
+        call c_f_pointer(p, chars, [1024])
+        n = 0
+        do while (n < 1024)
+            if (chars(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: msg)
+        msg = transfer(chars(1:n), msg)
+    end function
+
+    subroutine dage_check(status, what)
+        integer(c_int), intent(in)   :: status
+        character(len=*), intent(in) :: what
+        if (status /= DAGE_OK) then
+            write(*, '(a, a, a, i0, a, a)') "dage: ", what, " failed (", status, "): ", &
+                dage_last_error_f()
+            stop 1
+        end if
+    end subroutine
+
+    !> Coulomb3D on an equidistant-cell grid: qmin, ncell, one step per axis.
+    function DageOperator3D_init_coulomb(device, qmin, ncell, step, nlip, nlin, nlog, tlin, &
+                                         tlog, coda) result(new)
+        integer, intent(in)      :: device, ncell(3), nlip, nlin, nlog
+        real(8), intent(in)      :: qmin(3), step(3), tlin, tlog
+        logical, intent(in)      :: coda
+        type(DageOperator3D)     :: new
+        real(c_double), allocatable :: sx(:), sy(:), sz(:), tp(:), tw(:)
+        integer(c_int) :: shp(3), ncl(3), flags
+        real(c_double) :: qm(3)
+
+        call dage_check(dage_ctx_create(int(device, c_int), new%ctx), "dage_ctx_create")
+        allocate(sx(ncell(1)), sy(ncell(2)), sz(ncell(3)))
+        sx = step(1)
+        sy = step(2)
+        sz = step(3)
+        qm = qmin
+        ncl = ncell
+        call dage_check(dage_grid3d_create(new%ctx, qm, ncl, int(nlip, c_int), sx, sy, sz, &
+                                           1_c_int, new%gridin), "dage_grid3d_create")
+        new%gridout = new%gridin
+        call dage_check(dage_grid3d_shape(new%gridin, shp), "dage_grid3d_shape")
+        new%dims_in = shp
+        new%dims_out = shp
+        allocate(tp(nlin + nlog), tw(nlin + nlog))
+        call dage_check(dage_gauss_quad(int(nlin, c_int), int(nlog, c_int), 0.0_c_double, &
+                                        real(tlin, c_double), real(tlog, c_double), tp, tw), &
+                        "dage_gauss_quad")
+        flags = 0
+        if (.not. coda) flags = DAGE_OP_NO_CODA
+        call dage_check(dage_coulomb3d_create(new%ctx, new%gridin, new%gridout, tp, tw, &
+                                              int(nlin + nlog, c_int), real(tlog, c_double), &
+                                              flags, new%op), "dage_coulomb3d_create")
+    end function
+
+    !> Operator3D_transform_cube (function3d.F90:4230-4349) on the GPU.
+    function DageOperator3D_transform_cube(self, cubein) result(cubeout)
+        class(DageOperator3D), intent(in) :: self
+        real(8), intent(in)               :: cubein(:,:,:)
+        real(8), allocatable, target      :: cubeout(:,:,:)
+        real(c_double), allocatable       :: tmp(:,:,:)
+        allocate(cubeout(self%dims_out(1), self%dims_out(2), self%dims_out(3)))
+        tmp = cubein
+        call dage_check(dage_operator_apply(self%op, tmp, cubeout, DAGE_PREC_FP64), &
+                        "dage_operator_apply")
+    end function
+
+    !> Helmholtz3D_set_energy (helmholtz3d.F90:411-444).
+    subroutine DageOperator3D_set_energy(self, energy)
+        class(DageOperator3D), intent(inout) :: self
+        real(8), intent(in)                  :: energy
+        call dage_check(dage_operator_set_energy(self%op, real(energy, c_double)), &
+                        "dage_operator_set_energy")
+    end subroutine
+
+    subroutine DageOperator3D_destroy(self)
+        class(DageOperator3D), intent(inout) :: self
+        integer(c_int) :: rc
+        rc = dage_operator_destroy(self%op)
+        rc = dage_grid3d_destroy(self%gridin)
+        rc = dage_ctx_destroy(self%ctx)
+        self%op = c_null_ptr
+        self%gridin = c_null_ptr
+        self%gridout = c_null_ptr
+        self%ctx = c_null_ptr
+    end subroutine
+end module
