@@ -130,8 +130,21 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = ms_total / args.steps
 
+    # mixed precision (split-BF16, 6 products) on the same operator and cube
+    d_mix = dage_amd.DeviceCube(ctx, rho.shape)
+    mixed = {}
+    for mode, label in ((dage_amd.DAGE_PREC_BF16X6, "bf16x6"), (dage_amd.DAGE_PREC_BF16X3, "bf16x3")):
+        for _ in range(args.warmup):
+            op.transform_cuda_cube(d_in, d_mix, precision=mode)
+        ctx.time_begin()
+        for _ in range(args.steps):
+            op.transform_cuda_cube(d_in, d_mix, precision=mode)
+        mms = ctx.time_end() / args.steps
+        mixed[label] = (mms, d_mix.download())
+
     kernels = {}
-    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel"):
+    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+                 "gemm_mixed_xmode", "gemm_mixed_ymode", "gemm_mixed_zmode"):
         tot, n = ctx.profile_query(name)
         kernels[name] = (tot / max(n, 1), n // args.steps)
     ctx.profile(False)
@@ -142,8 +155,8 @@ def main():
     flop_mode = 2.0 * T * float(N) ** 4  # each mode product, per apply
     flops_apply = 3.0 * flop_mode
 
-    # dominant kernel: the GEMM pass with the largest time per apply
-    per_apply = {k: v[0] * v[1] for k, v in kernels.items() if k.startswith("gemm")}
+    # dominant kernel: the FP64 GEMM pass with the largest time per apply
+    per_apply = {k: v[0] * v[1] for k, v in kernels.items() if k.startswith("gemm_f64")}
     dom = max(per_apply, key=per_apply.get)
     dom_ms, dom_launches = kernels[dom]
     dom_flops = flop_mode / max(dom_launches, 1)
@@ -207,12 +220,18 @@ def main():
         "cpu_baseline": cpu_baseline,
         "apply_tflops": round(flops_apply / (ms_step * 1e-3) / 1e12, 3),
         "apply_mfma_frac": round(flops_apply / (ms_step * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4),
-        "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kernels.items()},
+        "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kernels.items() if v[1] > 0},
+        "mixed": {label: {"ms_per_apply": round(mms, 4),
+                          "Gpts_s": round(pts / (mms * 1e-3) / 1e9, 4),
+                          "speedup_vs_fp64": round(ms_step / mms, 3),
+                          "dense_fp64_equiv_tflops": round(flops_apply / (mms * 1e-3) / 1e12, 2),
+                          "max_rel_err_vs_oracle": float(np.max(np.abs(vm - ref)) / np.max(np.abs(ref)))}
+                  for label, (mms, vm) in mixed.items()},
         "max_rel_err_vs_oracle": err_oracle,
         "max_rel_err_vs_analytic": err_analytic,
         "wall_s_timed": round(wall, 4),
     }
-    for obj in (d_in, d_out, op, grid):
+    for obj in (d_in, d_out, d_mix, op, grid):
         obj.close()
     if args.inject:
         line["inject"] = bench_inject(ctx, args)

@@ -4,11 +4,12 @@ hot path: the tensor-quadrature Coulomb3D/Helmholtz3D apply on the cube.
 The product is the C-ABI library dage_amd/libdage_ops.so (include/dage_ops.h);
 this package is its host-side mirror of the reference's Operator3D API.
 """
-from ._lib import DAGE_OP_NO_CODA, DAGE_PREC_FP64, DageError, build, load
+from ._lib import (DAGE_OP_NO_CODA, DAGE_PREC_BF16X3, DAGE_PREC_BF16X6, DAGE_PREC_FP64,
+                   DageError, build, load)
 from .operators import (Bubbles, Context, Coulomb3D, DeviceCube, GaussQuad, Grid3D,
                         Helmholtz3D, Operator3D, radial_grid_steps)
 
 __all__ = [
     "Bubbles", "Context", "Coulomb3D", "DeviceCube", "GaussQuad", "Grid3D", "Helmholtz3D", "Operator3D",
-    "radial_grid_steps", "DageError", "DAGE_OP_NO_CODA", "DAGE_PREC_FP64", "build", "load",
+    "radial_grid_steps", "DageError", "DAGE_OP_NO_CODA", "DAGE_PREC_FP64", "DAGE_PREC_BF16X6", "DAGE_PREC_BF16X3", "build", "load",
 ]

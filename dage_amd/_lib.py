@@ -22,6 +22,8 @@ DAGE_ERR_STATE = 4
 DAGE_ERR_UNSUPPORTED = 5
 
 DAGE_PREC_FP64 = 0
+DAGE_PREC_BF16X6 = 1
+DAGE_PREC_BF16X3 = 2
 DAGE_OP_NO_CODA = 1
 
 _c_p = ctypes.c_void_p

@@ -112,6 +112,9 @@ void free_operator_buffers(dage_operator op) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
+    if (op->d_planes) (void)hipFree(op->d_planes);
+    op->d_planes = nullptr;
+    op->planes_valid = false;
     op->ws_bytes = 0;
 }
 
@@ -539,6 +542,7 @@ int dage_operator_set_matrices(dage_operator op, const double *fx, const double 
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fz, fz, 8 * (size_t)op->din[2] * op->dout[2] * T,
                                   hipMemcpyHostToDevice, s));
     DAGE_TRY(relayout_fz(op));
+    op->planes_valid = false;
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
     return DAGE_OK;
 }
@@ -560,11 +564,14 @@ int dage_operator_get_matrices(dage_operator op, double *fx, double *fy, double 
 
 int dage_operator_apply_device(dage_operator op, const double *in, double *out, int precision) {
     DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
-    DAGE_REQUIRE(precision == DAGE_PREC_FP64, DAGE_ERR_UNSUPPORTED,
-                 "precision mode %d is not implemented", precision);
+    DAGE_REQUIRE(precision == DAGE_PREC_FP64 || precision == DAGE_PREC_BF16X6 ||
+                     precision == DAGE_PREC_BF16X3,
+                 DAGE_ERR_UNSUPPORTED, "precision mode %d is not implemented", precision);
     DAGE_REQUIRE((const void *)in != (const void *)out, DAGE_ERR_ARG,
                  "in-place apply is not supported");
     DAGE_TRY(use_device(op->ctx));
+    if (precision == DAGE_PREC_BF16X6) return apply_mixed(op, in, out, 6);
+    if (precision == DAGE_PREC_BF16X3) return apply_mixed(op, in, out, 3);
     return apply_f64(op, in, out);
 }
 

@@ -141,6 +141,15 @@ struct dage_bubbles_s {
     bool dirty = true;
 };
 
+// Layout of the split-BF16 matrix planes (apply_mixed.hip): per mode, column
+// count padded to the block width and K padded to a multiple of 32.
+struct MixedLayout {
+    int nbx = 0, nby = 0, nbz = 0;
+    int npx = 0, npy = 0, npz = 0;
+    int kpx = 0, kpy = 0, kpz = 0;
+    size_t off_x = 0, off_y = 0, off_z = 0;  // element offsets into the plane buffer
+};
+
 struct dage_operator_s {
     dage_ctx ctx = nullptr;
     dage_grid3d gin = nullptr, gout = nullptr;  // set during creation only
@@ -156,6 +165,9 @@ struct dage_operator_s {
     double *d_u = nullptr;   // y-mode intermediate, ((xo,yo), (zi,t))
     double *d_in = nullptr, *d_out = nullptr;  // staging for host applies
     size_t ws_bytes = 0;
+    void *d_planes = nullptr;   // split-BF16 matrix planes (built on first mixed apply)
+    bool planes_valid = false;  // cleared when the matrices change
+    MixedLayout mixed;
 };
 
 namespace dage {
@@ -175,6 +187,8 @@ int generate_matrices(dage_operator op);
 // FP64 apply (apply_f64.hip).
 int apply_f64(dage_operator op, const double *in_dev, double *out_dev);
 int apply_f64_workspace(dage_operator op, size_t *bytes);
+// Mixed-precision apply (apply_mixed.hip): npair = 6 (bf16x6) or 3 (bf16x3).
+int apply_mixed(dage_operator op, const double *in_dev, double *out_dev, int npair);
 // Cube integral (reduce.hip).
 int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);

@@ -45,7 +45,12 @@ enum {
 
 /* Precision modes of the apply. */
 enum {
-    DAGE_PREC_FP64 = 0        /* FP64 MFMA throughout */
+    DAGE_PREC_FP64 = 0,       /* FP64 MFMA throughout */
+    DAGE_PREC_BF16X6 = 1,     /* split-BF16, 3 terms per operand, 6 MFMA
+                                 products, FP32 sums of 32 products promoted
+                                 to FP64; FP32 intermediates (~5e-8 of
+                                 max|V| at config 1) */
+    DAGE_PREC_BF16X3 = 2      /* split-BF16, 2 terms, 3 products (~2e-6) */
 };
 
 /* Operator creation flags. */

@@ -821,3 +821,184 @@ int dage_oracle_bubbles_eval_grid(int nbub, const double *centers, int lmax,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------- FMM
+ * multipole.F90:33-560: the interaction (M2L) and translation (M2M/L2L)
+ * matrices for Racah-normalised real solid harmonics.  Vectors are indexed
+ * lm = l*l + l + m (0-based); N = (lmax+1)^2. */
+static double o_fact(int n) { /* product([(1.d0*i, i=1,n)]), 1 for n <= 0 */
+    double p = 1.0;
+    for (int i = 1; i <= n; ++i) p *= (double)i;
+    return p;
+}
+static int o_sgn(int n) { return (n % 2 == 0) ? 1 : -1; } /* (-1)**n */
+static int o_delta(int i, int j) { return i == j; }
+static double o_cc(int l, int m) { /* multipole.F90:512-526 */
+    if (m == 0) return 1.0 / o_fact(l);
+    if (m > 0) return o_sgn(m) / sqrt(2.0 * o_fact(l - m) * o_fact(l + m));
+    return o_sgn(m) * o_cc(l, -m);
+}
+static double o_cs(int l, int m) { /* multipole.F90:528-542 */
+    if (m == 0) return 0.0;
+    if (m > 0) return o_sgn(m) / sqrt(2.0 * o_fact(l - m) * o_fact(l + m));
+    return -o_sgn(m) * o_cs(l, -m);
+}
+static double o_cst(int l, int m) { return o_fact(l - m) * o_fact(l + m) * o_cs(l, m); }
+static double o_cct(int l, int m) { return o_fact(l - m) * o_fact(l + m) * o_cc(l, m); }
+static double o_d(int l, int m) { /* multipole.F90:564-573 */
+    return sqrt(o_fact(l + m) * o_fact(l - m) * (2.0 - o_delta(m, 0)));
+}
+static void o_lm(int idx, int *l, int *m) { /* 0-based lm -> (l, m) */
+    int L = 0;
+    while ((L + 1) * (L + 1) <= idx) ++L;
+    *l = L;
+    *m = idx - L * L - L;
+}
+
+/* InteractionMatrix_compute_coeffs + eval_pick (multipole.F90:104-183,
+ * 268-312): T (N x N, column-major) with L_target = T(dR) q_source, where
+ * dR = source centre - target centre. */
+int dage_oracle_fmm_interaction(int lmax, const double dR[3], double *T) {
+    const int N = (lmax + 1) * (lmax + 1);
+    const int L2 = 2 * lmax;
+    double *I = (double *)malloc(sizeof(double) * (L2 + 1) * (L2 + 1));
+    if (!I) return -1;
+    dage_oracle_solid_harmonics(L2, dR[0], dR[1], dR[2], I);
+    const double r = sqrt(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    for (int j = 0; j <= L2; ++j)
+        for (int k = -j; k <= j; ++k) I[j * j + j + k] /= pow(r, 2 * j + 1);
+    const double ij[3] = {1.0, 0.5, 0.25};
+    for (int s = 0; s < N; ++s) {
+        int j, k;
+        o_lm(s, &j, &k);
+        for (int i = s; i < N; ++i) {
+            int l, m;
+            o_lm(i, &l, &m);
+            const double c = 2.0 * o_sgn(j) * ij[o_delta(m, 0) + o_delta(k, 0)];
+            int M1, M2;
+            double c1, c2;
+            const int LL = l + j;
+            if (m >= 0 && k >= 0) {
+                M1 = abs(m + k); c1 = o_cct(LL, m + k);
+                M2 = abs(m - k); c2 = o_sgn(k) * o_cct(LL, m - k);
+            } else if (m >= 0 && k < 0) {
+                M1 = -abs(m + k); c1 = o_cst(LL, m + k);
+                M2 = -abs(m - k); c2 = -o_sgn(k) * o_cst(LL, m - k);
+                c1 = -c1 * o_sgn(k); c2 = -c2 * o_sgn(k);
+            } else if (m < 0 && k >= 0) {
+                M1 = -abs(m + k); c1 = o_cst(LL, m + k);
+                M2 = -abs(m - k); c2 = o_sgn(k) * o_cst(LL, m - k);
+                c1 = -c1 * o_sgn(m); c2 = -c2 * o_sgn(m);
+            } else {
+                M1 = abs(m + k); c1 = -o_cct(LL, m + k);
+                M2 = abs(m - k); c2 = o_sgn(k) * o_cct(LL, m - k);
+                c1 = c1 * o_sgn(m + k); c2 = c2 * o_sgn(m + k);
+            }
+            c1 *= c; c2 *= c;
+            c1 = o_sgn(m) * c1 / o_d(l, m); c2 = o_sgn(m) * c2 / o_d(l, m);
+            c1 = o_sgn(k) * c1 / o_d(j, k); c2 = o_sgn(k) * c2 / o_d(j, k);
+            const double w = c1 * I[LL * LL + LL + M1] + c2 * I[LL * LL + LL + M2];
+            T[i + (size_t)N * s] = w;
+            T[s + (size_t)N * i] = o_sgn(l - j) * w;
+        }
+    }
+    free(I);
+    return 0;
+}
+
+/* TranslationMatrix_compute_coeffs + eval (multipole.F90:349-470): packed
+ * unit-lower-triangular W (N(N+1)/2, column-major packed) for dR = from - to. */
+int dage_oracle_fmm_translation(int lmax, const double dR[3], double *W) {
+    const int N = (lmax + 1) * (lmax + 1);
+    double *S = (double *)malloc(sizeof(double) * N);
+    if (!S) return -1;
+    dage_oracle_solid_harmonics(lmax, dR[0], dR[1], dR[2], S);
+    const double ij[2] = {1.0, 0.5};
+    for (int s = 0; s < N; ++s) {
+        int j, k;
+        o_lm(s, &j, &k);
+        for (int i = s; i < N; ++i) {
+            int l, m;
+            o_lm(i, &l, &m);
+            const int r = i + (2 * N - s - 1) * s / 2;
+            const int d = l - j;
+            int ok1 = 0, ok2 = 0, M1 = 0, M2 = 0;
+            double c1 = 0.0, c2 = 0.0;
+            if (m >= 0 && k >= 0) {
+                if (abs(m - k) <= d) { ok1 = 1; M1 = abs(m - k); c1 = o_cc(d, m - k) * ij[o_delta(0, k)]; }
+                if (abs(m + k) <= d) { ok2 = 1; M2 = abs(m + k); c2 = o_sgn(k) * o_cc(d, m + k) * ij[o_delta(0, k)]; }
+            } else if (m >= 0 && k < 0) {
+                if (abs(m - k) <= d) { ok1 = 1; M1 = -abs(m - k); c1 = o_cs(d, m - k) * o_sgn(k); }
+                if (abs(m + k) <= d) { ok2 = 1; M2 = -abs(m + k); c2 = -o_cs(d, m + k); }
+            } else if (m < 0 && k >= 0) {
+                if (abs(m - k) <= d) { ok1 = 1; M1 = -abs(m - k); c1 = -o_cs(d, m - k) * o_sgn(m) * ij[o_delta(0, k)]; }
+                if (abs(m + k) <= d) { ok2 = 1; M2 = -abs(m + k); c2 = -o_cs(d, m + k) * ij[o_delta(0, k)] * o_sgn(m + k); }
+            } else {
+                if (abs(m - k) <= d) { ok1 = 1; M1 = abs(m - k); c1 = o_cc(d, m - k) * o_sgn(m - k); }
+                if (abs(m + k) <= d) { ok2 = 1; M2 = abs(m + k); c2 = -o_cc(d, m + k) * o_sgn(m); }
+            }
+            const double sc = o_d(l, m) * o_sgn(m - k) / o_d(j, k);
+            c1 *= sc;
+            c2 *= sc;
+            double w = 0.0;
+            if (ok1) w = c1 * S[d * d + d + M1];
+            if (ok2) w = w + c2 * S[d * d + d + M2];
+            W[r] = w;
+        }
+    }
+    free(S);
+    return 0;
+}
+
+/* dtpmv('L', 'N'|'T', 'U', N, W, q, 1) as TranslationMatrix_apply calls it
+ * (multipole.F90:475-492): q := W q or W^T q, unit diagonal. */
+void dage_oracle_fmm_translate(int lmax, const double *W, int transpose, double *q) {
+    const int N = (lmax + 1) * (lmax + 1);
+    double *out = (double *)malloc(sizeof(double) * N);
+    if (!out) return;
+    for (int i = 0; i < N; ++i) out[i] = q[i];
+    for (int s = 0; s < N; ++s)
+        for (int i = s + 1; i < N; ++i) {
+            const double w = W[i + (2 * N - s - 1) * s / 2];
+            if (transpose) out[s] += w * q[i];
+            else out[i] += w * q[s];
+        }
+    for (int i = 0; i < N; ++i) q[i] = out[i];
+    free(out);
+}
+
+/* Box multipoles as the GPU evaluator forms them (integrator.cu:1122-1196):
+ * q_lm = sum_p rho_p wx wy wz S_lm(r_p - centre) over the box sub-grid. */
+void dage_oracle_box_multipoles(int lmax, const double *rho, int ld_x, int ld_y,
+                                const double *x, const double *wx, int nx,
+                                const double *y, const double *wy, int ny,
+                                const double *z, const double *wz, int nz,
+                                const double centre[3], double *q) {
+    const int N = (lmax + 1) * (lmax + 1);
+    double S[(24 + 1) * (24 + 1)];
+    for (int i = 0; i < N; ++i) q[i] = 0.0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double v = rho[i + (size_t)ld_x * (j + (size_t)ld_y * k)] * wx[i] * wy[j] * wz[k];
+                dage_oracle_solid_harmonics(lmax, x[i] - centre[0], y[j] - centre[1], z[k] - centre[2], S);
+                for (int n = 0; n < N; ++n) q[n] += S[n] * v;
+            }
+}
+
+/* Local-expansion evaluation (gbfmm_coulomb3d.cu:81-280, .F90:1701-1745):
+ * out(p) += sum_lm L_lm S_lm(r_p - centre) over an nx*ny*nz block. */
+void dage_oracle_eval_local_expansion(int lmax, const double *L, const double *x, int nx,
+                                      const double *y, int ny, const double *z, int nz,
+                                      const double centre[3], double *out, int ld_x, int ld_y) {
+    const int N = (lmax + 1) * (lmax + 1);
+    double S[(24 + 1) * (24 + 1)];
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                dage_oracle_solid_harmonics(lmax, x[i] - centre[0], y[j] - centre[1], z[k] - centre[2], S);
+                double v = 0.0;
+                for (int n = 0; n < N; ++n) v += S[n] * L[n];
+                out[i + (size_t)ld_x * (j + (size_t)ld_y * k)] += v;
+            }
+}

@@ -150,6 +150,26 @@ int dage_oracle_bubbles_eval_grid(int nbub, const double *centers, int lmax,
                                   int nx, const double *y, int ny,
                                   const double *z, int nz, double *cube);
 
+/* ------------------------------------------------------------------- FMM */
+/* InteractionMatrix (multipole.F90:104-183, 268-312): T (N x N, column-major,
+ * N = (lmax+1)^2), L_target += T(dR) q_source with dR = source - target. */
+int dage_oracle_fmm_interaction(int lmax, const double dR[3], double *T);
+/* TranslationMatrix (multipole.F90:349-470): packed unit-lower W(dR). */
+int dage_oracle_fmm_translation(int lmax, const double dR[3], double *W);
+/* TranslationMatrix_apply (multipole.F90:475-492): q := W q (transpose 0) or
+ * W^T q (transpose 1). */
+void dage_oracle_fmm_translate(int lmax, const double *W, int transpose, double *q);
+/* Box multipoles (integrator.cu:1122-1196), lmax <= 24. */
+void dage_oracle_box_multipoles(int lmax, const double *rho, int ld_x, int ld_y,
+                                const double *x, const double *wx, int nx,
+                                const double *y, const double *wy, int ny,
+                                const double *z, const double *wz, int nz,
+                                const double centre[3], double *q);
+/* Local-expansion evaluation (gbfmm_coulomb3d.cu:81-280). */
+void dage_oracle_eval_local_expansion(int lmax, const double *L, const double *x, int nx,
+                                      const double *y, int ny, const double *z, int nz,
+                                      const double centre[3], double *out, int ld_x, int ld_y);
+
 #ifdef __cplusplus
 }
 #endif
