@@ -75,6 +75,15 @@ _SIGNATURES = {
     "dage_bubbles_inject_device": [_c_p, _c_p, _int, _dbl, _c_p],
     "dage_bubbles_destroy": [_c_p],
     "dage_radial_grid_steps": [_dbl, _int, _int, _dbl, _dp],
+    "dage_gbfmm_create": [_c_p, _c_p, _int, _int, _dp, _dp, _int, ctypes.POINTER(_c_p)],
+    "dage_gbfmm_apply_device": [_c_p, _c_p, _c_p, _int],
+    "dage_gbfmm_apply": [_c_p, _dp, _dp, _int],
+    "dage_gbfmm_box_count": [_c_p, _ip],
+    "dage_gbfmm_get_multipoles": [_c_p, _dp],
+    "dage_gbfmm_get_local_expansions": [_c_p, _dp],
+    "dage_gbfmm_destroy": [_c_p],
+    "dage_fmm_interaction_matrix": [_int, _dp, _dp],
+    "dage_fmm_translation_matrix": [_int, _dp, _dp],
 }
 _RESTYPES = {"dage_last_error": ctypes.c_char_p}
 

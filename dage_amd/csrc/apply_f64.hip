@@ -297,14 +297,14 @@ struct Plan {
     size_t v_elems, u_elems, p_elems;
 };
 
-Plan make_plan(dage_operator op) {
-    const long nyi = op->din[1], nzi = op->din[2];
-    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
+Plan make_plan(const ApplyMats &m) {
+    const long nyi = m.din[1], nzi = m.din[2];
+    const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
     size_t budget = 4ull << 30;  // bytes for V + U
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
     const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
     Plan p;
-    p.tchunk = (int)std::max<size_t>(1, std::min<size_t>(op->ntp, budget / per_t));
+    p.tchunk = (int)std::max<size_t>(1, std::min<size_t>(m.ntp, budget / per_t));
     const long M3 = nxo * nyo;
     const int mtiles = (int)((M3 + 64 * kMW - 1) / (64 * kMW));
     const int nb = pick_nb((int)nzo);
@@ -319,13 +319,27 @@ Plan make_plan(dage_operator op) {
     return p;
 }
 
-}  // namespace
-
-int apply_f64_workspace(dage_operator op, size_t *bytes) {
-    const Plan p = make_plan(op);
-    *bytes = 8 * (p.v_elems + p.u_elems + p.p_elems);
-    return DAGE_OK;
+// out(x,y,z) = sum_s part[s*len + (x + nxo*(y + nyo*z))] over the written
+// extent of a strided output view.
+__global__ void __launch_bounds__(256) splitk_reduce_view_kernel(const double *part, int splits,
+                                                                 long len, int nxo, int nyo,
+                                                                 int nx, int ny, int nz,
+                                                                 double *out, long ldx, long ldy) {
+    const long total = (long)nx * ny * nz;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % nx);
+        const long r = e / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        const long src = x + (long)nxo * (y + (long)nyo * z);
+        double s = 0.0;
+        for (int k = 0; k < splits; ++k) s += part[(long)k * len + src];
+        out[x + ldx * (y + ldy * z)] = s;
+    }
 }
+
+}  // namespace
 
 int relayout_fz(dage_operator op) {
     const int nzi = op->din[2], nzo = op->dout[2], T = op->ntp;
@@ -337,23 +351,47 @@ int relayout_fz(dage_operator op) {
     return DAGE_OK;
 }
 
-int apply_f64(dage_operator op, const double *in, double *out) {
-    const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
-    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
-    const int T = op->ntp;
-    const Plan plan = make_plan(op);
-    size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
-    if (need > op->ws_bytes) {
-        if (op->d_v) DAGE_HIP_CHECK(hipFree(op->d_v));
-        op->d_v = nullptr;
-        op->ws_bytes = 0;
-        DAGE_HIP_CHECK(hipMalloc((void **)&op->d_v, need));
-        op->ws_bytes = need;
+ApplyMats operator_mats(dage_operator op) {
+    ApplyMats m;
+    m.fx = op->d_fx;
+    m.fy = op->d_fy;
+    m.gz = op->d_gz;
+    m.w = op->d_w;
+    for (int i = 0; i < 3; ++i) {
+        m.din[i] = op->din[i];
+        m.dout[i] = op->dout[i];
     }
-    double *V = op->d_v;
+    m.ntp = op->ntp;
+    return m;
+}
+
+int apply_f64_workspace(dage_operator op, size_t *bytes) {
+    const Plan p = make_plan(operator_mats(op));
+    *bytes = 8 * (p.v_elems + p.u_elems + p.p_elems);
+    return DAGE_OK;
+}
+
+int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const OutView &ov,
+                   double coda, Workspace *ws) {
+    const long nxi = m.din[0], nyi = m.din[1], nzi = m.din[2];
+    const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
+    const int T = m.ntp;
+    DAGE_REQUIRE(ov.n[0] <= nxo && ov.n[1] <= nyo && ov.n[2] <= nzo && ov.ldx >= ov.n[0] &&
+                     ov.ldy >= ov.n[1],
+                 DAGE_ERR_ARG, "output view exceeds the operator's output grid");
+    const Plan plan = make_plan(m);
+    const size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
+    if (need > ws->bytes) {
+        if (ws->p) DAGE_HIP_CHECK(hipFree(ws->p));
+        ws->p = nullptr;
+        ws->bytes = 0;
+        DAGE_HIP_CHECK(hipMalloc((void **)&ws->p, need));
+        ws->bytes = need;
+    }
+    double *V = ws->p;
     double *U = V + plan.v_elems;
     double *P = U + plan.u_elems;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = ctx->stream;
     const long M3 = nxo * nyo;
 
     for (int t0 = 0; t0 < T; t0 += plan.tchunk) {
@@ -361,27 +399,27 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         // pass 1: V_t^T((yi,zi), xo) = in^T((yi,zi), xi) . Fx_t^T(xi, xo)
         Gemm g1{};
         g1.A = in;
-        g1.B = op->d_fx + (long)t0 * nxo * nxi;
+        g1.B = m.fx + (long)t0 * nxo * nxi;
         g1.C = V;
         g1.M = (int)(nyi * nzi); g1.N = (int)nxo; g1.K = (int)nxi;
         g1.M0 = g1.M; g1.sAm0 = nxi; g1.sAm1 = 0; g1.sAk = 1;
         g1.sBk = nxo; g1.sBn = 1;
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1;
         g1.bA = 0; g1.bB = nxo * nxi; g1.bC = nxo * nyi * nzi;
-        DAGE_TRY(launch_gemm<0>(op->ctx, "gemm_f64_xmode", g1, tc));
+        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc));
 
         // pass 2: U((xo,yo),(zi,t)) = w_t V_t((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
         g2.A = V;
-        g2.B = op->d_fy + (long)t0 * nyi * nyo;
+        g2.B = m.fy + (long)t0 * nyi * nyo;
         g2.C = U;
         g2.M = (int)(nxo * nzi); g2.N = (int)nyo; g2.K = (int)nyi;
         g2.M0 = (int)nxo; g2.sAm0 = 1; g2.sAm1 = nxo * nyi; g2.sAk = nxo;
         g2.sBk = 1; g2.sBn = nyi;
         g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo;
         g2.bA = nxo * nyi * nzi; g2.bB = nyi * nyo; g2.bC = nxo * nyo * nzi;
-        g2.alpha = op->d_w + t0;
-        DAGE_TRY(launch_gemm<1>(op->ctx, "gemm_f64_ymode", g2, tc));
+        g2.alpha = m.w + t0;
+        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc));
 
         // pass 3: P_s((xo,yo), zo) (+)= U((xo,yo), k) . Gz(k, zo), k in split s
         const int K3 = (int)(nzi * tc);
@@ -389,7 +427,7 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         const int splits = (K3 + ks - 1) / ks;
         Gemm g3{};
         g3.A = U;
-        g3.B = op->d_gz + (long)nzi * t0;
+        g3.B = m.gz + (long)nzi * t0;
         g3.C = P;
         g3.M = (int)M3; g3.N = (int)nzo; g3.K = K3;
         g3.M0 = g3.M; g3.sAm0 = 1; g3.sAm1 = 0; g3.sAk = M3;
@@ -399,20 +437,48 @@ int apply_f64(dage_operator op, const double *in, double *out) {
         g3.Ksplit = ks;
         g3.accumulate = (t0 > 0);
         if (t0 == 0 && splits < plan.splits)
-            DAGE_TRY(fill_zero(op->ctx, P + (size_t)splits * M3 * nzo,
+            DAGE_TRY(fill_zero(ctx, P + (size_t)splits * M3 * nzo,
                                (size_t)(plan.splits - splits) * M3 * nzo));
-        DAGE_TRY(launch_gemm<2>(op->ctx, "gemm_f64_zmode", g3, splits));
+        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits));
     }
-    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
     const long len = M3 * nzo;
-    const int grid = (int)std::min<long>((len + 255) / 256, 65536);
+    const bool whole = ov.n[0] == nxo && ov.n[1] == nyo && ov.n[2] == nzo && ov.ldx == nxo &&
+                       ov.ldy == nyo;
     hipEvent_t ev;
-    prof_begin(op->ctx, "splitk_reduce_kernel", &ev);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid), dim3(256), 0, s, P, plan.splits, len,
-                       coda ? in : nullptr, op->coda, out);
-    DAGE_HIP_CHECK(hipGetLastError());
-    prof_end(op->ctx, "splitk_reduce_kernel", ev);
+    if (whole) {
+        const int grid = (int)std::min<long>((len + 255) / 256, 65536);
+        prof_begin(ctx, "splitk_reduce_kernel", &ev);
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid), dim3(256), 0, s, P, plan.splits,
+                           len, coda != 0.0 ? in : nullptr, coda, ov.p);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "splitk_reduce_kernel", ev);
+    } else {
+        DAGE_REQUIRE(coda == 0.0, DAGE_ERR_ARG, "the coda term needs a whole-cube output");
+        const long n = (long)ov.n[0] * ov.n[1] * ov.n[2];
+        const int grid = (int)std::min<long>((n + 255) / 256, 65536);
+        prof_begin(ctx, "splitk_reduce_view_kernel", &ev);
+        hipLaunchKernelGGL(splitk_reduce_view_kernel, dim3(grid), dim3(256), 0, s, P,
+                           plan.splits, len, (int)nxo, (int)nyo, ov.n[0], ov.n[1], ov.n[2], ov.p,
+                           ov.ldx, ov.ldy);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "splitk_reduce_view_kernel", ev);
+    }
     return DAGE_OK;
+}
+
+int apply_f64(dage_operator op, const double *in, double *out) {
+    const ApplyMats m = operator_mats(op);
+    OutView ov;
+    ov.p = out;
+    ov.ldx = op->dout[0];
+    ov.ldy = op->dout[1];
+    for (int i = 0; i < 3; ++i) ov.n[i] = op->dout[i];
+    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
+    Workspace ws{op->d_v, op->ws_bytes};
+    const int rc = apply_f64_view(op->ctx, m, in, ov, coda ? op->coda : 0.0, &ws);
+    op->d_v = ws.p;
+    op->ws_bytes = ws.bytes;
+    return rc;
 }
 
 }  // namespace dage

@@ -186,9 +186,48 @@ int prof_resolve(dage_ctx ctx);
 int generate_matrices(dage_operator op);
 // FP64 apply (apply_f64.hip).
 int apply_f64(dage_operator op, const double *in_dev, double *out_dev);
+// One operator's matrices in the library's device layouts (Fx (nxo,nxi,T),
+// Fy (nyi,nyo,T), Gz ((zi + nzi*t), zo)) and weights.
+struct ApplyMats {
+    const double *fx = nullptr, *fy = nullptr, *gz = nullptr, *w = nullptr;
+    int din[3] = {0, 0, 0}, dout[3] = {0, 0, 0};
+    int ntp = 0;
+};
+// Strided destination: element (x,y,z) at p[x + ldx*(y + ldy*z)] for
+// x < n[0], y < n[1], z < n[2] (n <= the operator's output shape).
+struct OutView {
+    double *p = nullptr;
+    long ldx = 0, ldy = 0;
+    int n[3] = {0, 0, 0};
+};
+// Device scratch owned by the caller, grown on demand.
+struct Workspace {
+    double *p = nullptr;
+    size_t bytes = 0;
+};
+ApplyMats operator_mats(dage_operator op);
+// FP64 apply of contiguous `in` (din) into an output view; coda != 0 adds
+// coda*in and needs the whole, contiguous output.
+int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const OutView &out,
+                   double coda, Workspace *ws);
 int apply_f64_workspace(dage_operator op, size_t *bytes);
 // Mixed-precision apply (apply_mixed.hip): npair = 6 (bf16x6) or 3 (bf16x3).
 int apply_mixed(dage_operator op, const double *in_dev, double *out_dev, int npair);
+// FMM matrices (fmm_host.cpp).  Sparse term lists per matrix entry (row +
+// N*col): interaction entries combine I_{L,n}(D), translation entries S_{L,n}(d).
+struct FmmTables {
+    int lmax = 0;
+    std::vector<int> t_start, t_idx;
+    std::vector<double> t_val;
+    std::vector<int> w_start, w_idx;
+    std::vector<double> w_val;
+};
+const FmmTables &fmm_tables(int lmax);
+// T (N x N, column-major): L_target += T q_source, D = source - target centre.
+void fmm_interaction(const FmmTables &tab, const double D[3], double *T);
+// W (N x N, column-major, unit lower triangular): q(new) = W(old - new) q(old).
+void fmm_translation(const FmmTables &tab, const double d[3], double *W);
+void solid_harmonics_host(int lmax, double x, double y, double z, double *S);
 // Cube integral (reduce.hip).
 int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);

@@ -1,0 +1,769 @@
+// Grid-based fast multipole method for the Coulomb potential of a cube
+// (GBFMMCoulomb3D, src/gbfmm/gbfmm_coulomb3d.F90:306-1330, its CUDA side
+// gbfmm_coulomb3d.cu and the box geometry of gbfmm_parallel_info.F90).
+//
+// The cube's cells are split into 2^maxlevel boxes per axis.  For every box:
+//   near field  exact Coulomb operator (no coda) from the box and its nearest
+//               neighbours to the box's points, i.e. the operator built on
+//               the near-field sub-grid -> box sub-grid (Coulomb3D_get_suboperator,
+//               coulomb.F90:517-635), evaluated with the FP64 MFMA apply;
+//   far field   box multipoles q_lm (lmax, Racah solid harmonics, box sub-grid
+//               weights), translated up the tree (M2M), converted to local
+//               expansions from each interaction list (M2L), translated down
+//               (L2L) and evaluated on the box's points (LE).
+// Every output point belongs to exactly one box: the last point of a box is
+// left to the next box unless it is the last point of the cube
+// (cut_end_borders, gbfmm_coulomb3d.F90:470-473).
+//
+// Boxes are numbered as in the reference: at level 1 index x + 2y + 4z of
+// the 2x2x2 split, and each box's 8 children are numbered consecutively
+// (parent*8 + octant, octant = ox + 2 oy + 4 oz).  Multipole and local-
+// expansion arrays hold level maxlevel first, then maxlevel-1, ..., level 2.
+//
+// Device layout: q and L are (N, nbox_total) FP64 with N = (lmax+1)^2; the
+// per-level M2L matrices are full N x N column-major blocks keyed by the
+// integer box offset of the source; M2M/L2L use 8 N x N blocks per level.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "dage_internal.hpp"
+
+struct dage_gbfmm_s {
+    dage_ctx ctx = nullptr;
+    int maxlevel = 0, start_level = 2, lmax = 0, N = 0;
+    int nb = 0;  // boxes per axis at maxlevel
+    int npt[3] = {0, 0, 0};
+    int cells_per_box[3] = {0, 0, 0};
+    int nlip = 0;
+    double qmin[3] = {0, 0, 0}, qmax[3] = {0, 0, 0};
+    std::vector<int> level_offset;  // by level
+    int nbox_total = 0;
+    // per axis and box coordinate: box and near-field point ranges
+    std::vector<int> box_lo[3], box_n[3], box_cut[3], nf_lo[3], nf_n[3];
+    std::vector<dage_grid3d> grids;   // near-field and box sub-grids
+    std::vector<dage_operator> ops;   // ops[b]: near field (b,b,b) -> box (b,b,b)
+    std::vector<int> boxvec;          // 3 per maxlevel box, by box index
+    double *d_nf = nullptr;           // gathered near-field sub-cube
+    dage::Workspace ws;
+    // far field
+    double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr;
+    int zsplit = 1;
+    int *d_boxvec = nullptr;
+    int *d_geo = nullptr;             // [axis][box_lo, box_n, box_cut][nb]
+    double *d_wbox = nullptr;         // [axis][nb][maxn] box sub-grid weights
+    int wbox_ld = 0;
+    double *d_points = nullptr;       // [axis][maxpt] global points
+    int points_ld = 0;
+    double *d_centers = nullptr;      // 3 per maxlevel box
+    double *d_coef = nullptr;         // a[lm], b[lm], f[l] of the solid-harmonic recursion
+    std::vector<double *> d_T;        // per level: M2L matrices
+    std::vector<int *> d_list;        // per level: CSR start (nbox+1), src, mat
+    std::vector<int> list_len;
+    std::vector<double *> d_W;        // per child level: 8 translation matrices
+};
+
+namespace dage {
+namespace {
+
+constexpr int kMaxFmmLmax = 15;  // N <= 256: one thread per lm row
+
+int box_index(int vx, int vy, int vz, int level) {
+    int idx = 0;
+    for (int l = level - 1; l >= 0; --l)
+        idx = idx * 8 + ((vx >> l) & 1) + 2 * ((vy >> l) & 1) + 4 * ((vz >> l) & 1);
+    return idx;
+}
+
+void box_vector(int idx, int level, int *v) {
+    v[0] = v[1] = v[2] = 0;
+    for (int l = 0; l < level; ++l) {
+        const int o = (idx >> (3 * l)) & 7;
+        v[0] |= (o & 1) << l;
+        v[1] |= ((o >> 1) & 1) << l;
+        v[2] |= ((o >> 2) & 1) << l;
+    }
+}
+
+// Racah solid harmonics S_lm(x,y,z), l <= lmax, produced column by column
+// (fixed m, increasing l) so that only two previous values are live; each
+// value is handed to f(lm, value).  Same recursion and association as the
+// full-array form (harmonics.F90:586-659): coef holds a[lm], b[lm], f[l].
+template <typename F>
+__device__ inline void solid_harmonics_by_column(int lmax, double x, double y, double z,
+                                                 const double *coef, F &&f) {
+    const int N = (lmax + 1) * (lmax + 1);
+    const double *ca = coef, *cb = coef + N, *cf = coef + 2 * N;
+    const double r2 = x * x + y * y + z * z;
+    double dp = 1.0, dm = 0.0;  // S_{mm,mm}, S_{mm,-mm}
+    for (int mm = 0; mm <= lmax; ++mm) {
+        if (mm == 1) {
+            dp = x;
+            dm = y;
+        } else if (mm > 1) {
+            const double np = cf[mm] * (x * dp - y * dm);
+            const double nm = cf[mm] * (y * dp + x * dm);
+            dp = np;
+            dm = nm;
+        }
+        for (int sg = 0; sg < (mm ? 2 : 1); ++sg) {
+            const int m = sg ? -mm : mm;
+            double p2 = 0.0, p1 = sg ? dm : dp;
+            f(mm * mm + mm + m, p1);
+            for (int l = mm + 1; l <= lmax; ++l) {
+                const int lm = l * l + l + m;
+                double v = ca[lm] * (z * p1);
+                if (l - mm > 1) v = v - cb[lm] * (r2 * p2);
+                f(lm, v);
+                p2 = p1;
+                p1 = v;
+            }
+        }
+    }
+}
+
+struct MultArgs {
+    const double *rho;
+    long ldx, ldy;
+    const int *boxvec, *geo;
+    int nb;
+    const double *wbox;
+    int wbox_ld;
+    const double *points;
+    int points_ld;
+    const double *centers;
+    const double *coef;
+    int lmax, N, zsplit, nbox;
+    double *part;  // [zsplit][nbox][N]
+};
+
+// q_lm(box) = sum over the box sub-grid of rho * wx wy wz * S_lm(r - centre)
+// (GBFMMCoulomb3DMultipoleEvaluator, integrator.cu:1122-1196).  Block
+// (box, z-slab); each wavefront sums 64 points per lm with a butterfly and
+// lane lm%64 keeps the running total of lm, so the accumulators take 4
+// registers; the 4 wavefronts and the z-slabs are added in a fixed order.
+__global__ void __launch_bounds__(256) box_multipoles_kernel(MultArgs A) {
+    __shared__ double coef[3 * 256 + 32];
+    __shared__ double red[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ncoef = 2 * A.N + A.lmax + 1;
+    for (int i = tid; i < ncoef; i += 256) coef[i] = A.coef[i];
+    __syncthreads();
+    const int box = blockIdx.x;
+    const int bx = A.boxvec[3 * box], by = A.boxvec[3 * box + 1], bz = A.boxvec[3 * box + 2];
+    const int nb = A.nb;
+    const int x0 = A.geo[0 * 3 * nb + bx], nx = A.geo[0 * 3 * nb + nb + bx];
+    const int y0 = A.geo[1 * 3 * nb + by], ny = A.geo[1 * 3 * nb + nb + by];
+    const int z0 = A.geo[2 * 3 * nb + bz], nz = A.geo[2 * 3 * nb + nb + bz];
+    const int zper = (nz + A.zsplit - 1) / A.zsplit;
+    const int zb = blockIdx.y * zper, ze = min(nz, zb + zper);
+    const double *wx = A.wbox + (long)(0 * nb + bx) * A.wbox_ld;
+    const double *wy = A.wbox + (long)(1 * nb + by) * A.wbox_ld;
+    const double *wz = A.wbox + (long)(2 * nb + bz) * A.wbox_ld;
+    const double *px = A.points, *py = A.points + A.points_ld, *pz = A.points + 2 * A.points_ld;
+    const double cx = A.centers[3 * box], cy = A.centers[3 * box + 1], cz = A.centers[3 * box + 2];
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    const long total = (long)nx * ny * max(0, ze - zb);
+    for (long base = 0; base < total; base += 256) {
+        const long e = base + tid;
+        double val = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+        if (e < total) {
+            const int i = (int)(e % nx);
+            const long r = e / nx;
+            const int j = (int)(r % ny);
+            const int k = zb + (int)(r / ny);
+            val = A.rho[(x0 + i) + A.ldx * ((y0 + j) + A.ldy * (long)(z0 + k))] * wx[i] * wy[j] *
+                  wz[k];
+            dx = px[x0 + i] - cx;
+            dy = py[y0 + j] - cy;
+            dz = pz[z0 + k] - cz;
+        }
+        solid_harmonics_by_column(A.lmax, dx, dy, dz, coef, [&](int lm, double s) {
+            double v = s * val;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == (lm & 63)) {
+                switch (lm >> 6) {
+                case 0: acc0 += v; break;
+                case 1: acc1 += v; break;
+                case 2: acc2 += v; break;
+                default: acc3 += v; break;
+                }
+            }
+        });
+    }
+    red[wave][lane] = acc0;
+    red[wave][64 + lane] = acc1;
+    red[wave][128 + lane] = acc2;
+    red[wave][192 + lane] = acc3;
+    __syncthreads();
+    if (tid < A.N) {
+        const double s = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        A.part[((long)blockIdx.y * A.nbox + box) * A.N + tid] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) sum_parts_kernel(const double *part, int zsplit, long len,
+                                                        double *q) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len;
+         i += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < zsplit; ++k) s += part[(long)k * len + i];
+        q[i] = s;
+    }
+}
+
+// q(parent) = sum_children W_octant q(child)  (translate_multipoles,
+// gbfmm_coulomb3d.F90:1502-1558); W unit lower triangular.
+__global__ void __launch_bounds__(256) m2m_kernel(const double *W, const double *qchild,
+                                                  double *qparent, int N) {
+    const int P = blockIdx.x, i = threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int o = 0; o < 8; ++o) {
+        const double *Wo = W + (long)o * N * N;
+        const double *qc = qchild + (long)(8 * P + o) * N;
+        for (int s = 0; s <= i; ++s) acc += Wo[i + (long)N * s] * qc[s];
+    }
+    qparent[(long)P * N + i] = acc;
+}
+
+// L(target) += sum_list T(offset) q(source)  (calculate_local_farfield_potential,
+// gbfmm_coulomb3d.F90:1461-1500).
+__global__ void __launch_bounds__(256) m2l_kernel(const double *T, const int *start,
+                                                  const int *src, const int *mat, const double *q,
+                                                  double *L, int N) {
+    const int t = blockIdx.x, i = threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int e = start[t]; e < start[t + 1]; ++e) {
+        const double *Te = T + (long)mat[e] * N * N;
+        const double *qs = q + (long)src[e] * N;
+        for (int s = 0; s < N; ++s) acc += Te[i + (long)N * s] * qs[s];
+    }
+    L[(long)t * N + i] += acc;
+}
+
+// L(child) += W_octant^T L(parent)  (translate_potential, gbfmm_coulomb3d.F90:1562-1622).
+__global__ void __launch_bounds__(256) l2l_kernel(const double *W, const double *Lparent,
+                                                  double *Lchild, int N) {
+    const int c = blockIdx.x, i = threadIdx.x;
+    if (i >= N) return;
+    const double *Wo = W + (long)(c & 7) * N * N;
+    const double *Lp = Lparent + (long)(c >> 3) * N;
+    double acc = 0.0;
+    for (int s = i; s < N; ++s) acc += Wo[s + (long)N * i] * Lp[s];
+    Lchild[(long)c * N + i] += acc;
+}
+
+struct LeArgs {
+    double *out;
+    long ldx, ldy;
+    const int *boxvec, *geo;
+    int nb;
+    const double *points;
+    int points_ld;
+    const double *centers;
+    const double *coef;
+    const double *L;  // maxlevel section
+    int lmax, N;
+};
+
+// out(p) += sum_lm L_lm(box) S_lm(p - centre) over the box's own points
+// (GBFMMCoulomb3D_evaluate_le_grid, gbfmm_coulomb3d.cu:233-280).  Block
+// (box, z-plane of the box).
+__global__ void __launch_bounds__(256) local_expansion_kernel(LeArgs A) {
+    __shared__ double coef[3 * 256 + 32];
+    __shared__ double Ls[256];
+    const int tid = threadIdx.x;
+    const int box = blockIdx.x;
+    const int nb = A.nb;
+    const int bx = A.boxvec[3 * box], by = A.boxvec[3 * box + 1], bz = A.boxvec[3 * box + 2];
+    const int nzc = A.geo[2 * 3 * nb + 2 * nb + bz];
+    const int kz = blockIdx.y;
+    if (kz >= nzc) return;  // uniform per block
+    const int ncoef = 2 * A.N + A.lmax + 1;
+    for (int i = tid; i < ncoef; i += 256) coef[i] = A.coef[i];
+    for (int i = tid; i < A.N; i += 256) Ls[i] = A.L[(long)box * A.N + i];
+    __syncthreads();
+    const int x0 = A.geo[0 * 3 * nb + bx], nxc = A.geo[0 * 3 * nb + 2 * nb + bx];
+    const int y0 = A.geo[1 * 3 * nb + by], nyc = A.geo[1 * 3 * nb + 2 * nb + by];
+    const int z = A.geo[2 * 3 * nb + bz] + kz;
+    const double cx = A.centers[3 * box], cy = A.centers[3 * box + 1], cz = A.centers[3 * box + 2];
+    const double dz = A.points[2 * A.points_ld + z] - cz;
+    for (int e = tid; e < nxc * nyc; e += 256) {
+        const int i = e % nxc, j = e / nxc;
+        const double dx = A.points[x0 + i] - cx;
+        const double dy = A.points[A.points_ld + y0 + j] - cy;
+        double res = 0.0;
+        solid_harmonics_by_column(A.lmax, dx, dy, dz, coef,
+                                  [&](int lm, double s) { res += s * Ls[lm]; });
+        double *dst = A.out + (x0 + i) + A.ldx * ((y0 + j) + A.ldy * (long)z);
+        *dst = *dst + res;
+    }
+}
+
+__global__ void __launch_bounds__(256) gather_kernel(const double *src, long ldx, long ldy,
+                                                     int x0, int y0, int z0, int nx, int ny,
+                                                     int nz, double *dst) {
+    const long total = (long)nx * ny * nz;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % nx);
+        const long r = e / nx;
+        const int j = (int)(r % ny);
+        const int k = (int)(r / ny);
+        dst[e] = src[(x0 + i) + ldx * ((y0 + j) + ldy * (long)(z0 + k))];
+    }
+}
+
+int upload(void **dst, const void *src, size_t bytes) {
+    DAGE_HIP_CHECK(hipMalloc(dst, bytes ? bytes : 8));
+    if (bytes) DAGE_HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return DAGE_OK;
+}
+
+void free_gbfmm(dage_gbfmm f) {
+    for (dage_operator op : f->ops)
+        if (op) dage_operator_destroy(op);
+    for (dage_grid3d g : f->grids)
+        if (g) dage_grid3d_destroy(g);
+    void *bufs[] = {f->d_nf, f->ws.p, f->d_q, f->d_L, f->d_part, f->d_boxvec, f->d_geo,
+                    f->d_wbox, f->d_points, f->d_centers, f->d_coef};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    for (double *p : f->d_T)
+        if (p) (void)hipFree(p);
+    for (int *p : f->d_list)
+        if (p) (void)hipFree(p);
+    for (double *p : f->d_W)
+        if (p) (void)hipFree(p);
+    if (f->ctx) ctx_release(f->ctx);
+    delete f;
+}
+
+int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *tw, int ntp) {
+    const int L = f->maxlevel, nb = f->nb, lmax = f->lmax, N = f->N;
+    // ---- geometry per axis (get_box_cell_index_limits, get_nearfield_cell_index_limits)
+    int maxn = 0;
+    for (int a = 0; a < 3; ++a) {
+        const Axis &ax = grid->axis[a];
+        const int s = ax.ncell / nb, np1 = ax.nlip - 1;
+        f->cells_per_box[a] = s;
+        f->npt[a] = ax.n();
+        f->qmin[a] = ax.points.front();
+        f->qmax[a] = ax.points.back();
+        for (int b = 0; b < nb; ++b) {
+            f->box_lo[a].push_back(b * s * np1);
+            f->box_n[a].push_back(s * np1 + 1);
+            f->box_cut[a].push_back(b < nb - 1 ? s * np1 : s * np1 + 1);
+            const int c0 = std::max(0, b - 1) * s, c1 = std::min(nb, b + 2) * s;
+            f->nf_lo[a].push_back(c0 * np1);
+            f->nf_n[a].push_back((c1 - c0) * np1 + 1);
+        }
+        maxn = std::max(maxn, s * np1 + 1);
+    }
+    // ---- near-field operators on sub-grids, one per box coordinate
+    size_t nf_max = 0;
+    for (int b = 0; b < nb; ++b) {
+        double q_nf[3], q_box[3];
+        int n_nf[3], n_box[3];
+        std::vector<double> st_nf[3], st_box[3];
+        for (int a = 0; a < 3; ++a) {
+            const Axis &ax = grid->axis[a];
+            const int s = f->cells_per_box[a];
+            const int c0 = std::max(0, b - 1) * s, c1 = std::min(nb, b + 2) * s;
+            q_nf[a] = ax.celld[c0];
+            n_nf[a] = c1 - c0;
+            st_nf[a].assign(ax.steps.begin() + c0, ax.steps.begin() + c1);
+            q_box[a] = ax.celld[b * s];
+            n_box[a] = s;
+            st_box[a].assign(ax.steps.begin() + b * s, ax.steps.begin() + (b + 1) * s);
+        }
+        dage_grid3d gnf = nullptr, gbox = nullptr;
+        DAGE_TRY(dage_grid3d_create(f->ctx, q_nf, n_nf, f->nlip, st_nf[0].data(), st_nf[1].data(),
+                                    st_nf[2].data(), grid->axis[0].grid_type, &gnf));
+        f->grids.push_back(gnf);
+        DAGE_TRY(dage_grid3d_create(f->ctx, q_box, n_box, f->nlip, st_box[0].data(),
+                                    st_box[1].data(), st_box[2].data(), grid->axis[0].grid_type,
+                                    &gbox));
+        f->grids.push_back(gbox);
+        dage_operator op = nullptr;
+        DAGE_TRY(dage_coulomb3d_create(f->ctx, gnf, gbox, tp, tw, ntp, 1.0, DAGE_OP_NO_CODA, &op));
+        f->ops.push_back(op);
+    }
+    for (int bx = 0; bx < nb; ++bx)
+        for (int by = 0; by < nb; ++by)
+            for (int bz = 0; bz < nb; ++bz)
+                nf_max = std::max(nf_max, (size_t)f->nf_n[0][bx] * f->nf_n[1][by] * f->nf_n[2][bz]);
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_nf, 8 * nf_max));
+
+    // ---- box numbering and centres (get_box_vector, get_box_center)
+    const int nbox_max = nb * nb * nb;
+    f->boxvec.resize(3 * nbox_max);
+    std::vector<double> centers(3 * nbox_max);
+    for (int idx = 0; idx < nbox_max; ++idx) {
+        int v[3];
+        box_vector(idx, L, v);
+        for (int a = 0; a < 3; ++a) {
+            f->boxvec[3 * idx + a] = v[a];
+            const double size = (f->qmax[a] - f->qmin[a]) / (double)nb;
+            centers[3 * idx + a] = f->qmin[a] + (v[a] + 0.5) * size;
+        }
+    }
+    f->level_offset.assign(L + 1, 0);
+    for (int l = L - 1; l >= f->start_level; --l) f->level_offset[l] = f->level_offset[l + 1] + (1 << (3 * (l + 1)));
+    f->nbox_total = 0;
+    for (int l = f->start_level; l <= L; ++l) f->nbox_total += 1 << (3 * l);
+
+    std::vector<int> geo(9 * nb);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < nb; ++b) {
+            geo[a * 3 * nb + b] = f->box_lo[a][b];
+            geo[a * 3 * nb + nb + b] = f->box_n[a][b];
+            geo[a * 3 * nb + 2 * nb + b] = f->box_cut[a][b];
+        }
+    // box sub-grid weights (Grid1D_get_ints of the box sub-axis)
+    f->wbox_ld = maxn;
+    std::vector<double> wbox((size_t)3 * nb * maxn, 0.0);
+    double base[kMaxNlip];
+    lip_integrals(f->nlip, grid->axis[0].grid_type, base);
+    for (int a = 0; a < 3; ++a) {
+        const Axis &ax = grid->axis[a];
+        const int s = f->cells_per_box[a], np1 = ax.nlip - 1;
+        for (int b = 0; b < nb; ++b) {
+            double *w = &wbox[(size_t)(a * nb + b) * maxn];
+            for (int c = 0; c < s; ++c)
+                for (int j = 0; j < ax.nlip; ++j) w[c * np1 + j] += base[j] * ax.steps[b * s + c];
+        }
+    }
+    f->points_ld = std::max(f->npt[0], std::max(f->npt[1], f->npt[2]));
+    std::vector<double> pts((size_t)3 * f->points_ld, 0.0);
+    for (int a = 0; a < 3; ++a)
+        std::copy(grid->axis[a].points.begin(), grid->axis[a].points.end(),
+                  pts.begin() + (size_t)a * f->points_ld);
+    // recursion coefficients a[lm], b[lm], f[l]
+    std::vector<double> coef(2 * N + lmax + 1, 0.0);
+    for (int l = 0; l <= lmax; ++l) {
+        coef[2 * N + l] = std::sqrt((2 * l - 1.0) / (2.0 * l));
+        for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            if (std::abs(m) < l) {
+                coef[lm] = (2 * l - 1.0) / std::sqrt(1.0 * (l + m) * (l - m));
+                if (l - std::abs(m) > 1)
+                    coef[N + lm] = std::sqrt((l + m - 1.0) * (l - m - 1.0) / ((l + m) * (l - m)));
+            }
+        }
+    }
+    DAGE_TRY(upload((void **)&f->d_boxvec, f->boxvec.data(), 4 * f->boxvec.size()));
+    DAGE_TRY(upload((void **)&f->d_geo, geo.data(), 4 * geo.size()));
+    DAGE_TRY(upload((void **)&f->d_wbox, wbox.data(), 8 * wbox.size()));
+    DAGE_TRY(upload((void **)&f->d_points, pts.data(), 8 * pts.size()));
+    DAGE_TRY(upload((void **)&f->d_centers, centers.data(), 8 * centers.size()));
+    DAGE_TRY(upload((void **)&f->d_coef, coef.data(), 8 * coef.size()));
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_q, 8 * (size_t)N * f->nbox_total));
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_L, 8 * (size_t)N * f->nbox_total));
+    f->zsplit = std::max(1, std::min(maxn, (2048 + nbox_max - 1) / nbox_max));
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_part, 8 * (size_t)N * nbox_max * f->zsplit));
+
+    // ---- far-field matrices (init_interaction_matrices, get_local_farfield_indices)
+    const FmmTables &tab = fmm_tables(lmax);
+    f->d_T.assign(L + 1, nullptr);
+    f->d_list.assign(L + 1, nullptr);
+    f->list_len.assign(L + 1, 0);
+    f->d_W.assign(L + 1, nullptr);
+    std::vector<double> M((size_t)N * N);
+    for (int l = f->start_level; l <= L; ++l) {
+        const int n = 1 << l, nbox = n * n * n;
+        double size[3];
+        for (int a = 0; a < 3; ++a) size[a] = (f->qmax[a] - f->qmin[a]) / (double)n;
+        std::map<int, int> mat_id;
+        std::vector<int> start(nbox + 1, 0), src, mat;
+        for (int t = 0; t < nbox; ++t) {
+            int v[3], p[3];
+            box_vector(t, l, v);
+            for (int a = 0; a < 3; ++a) p[a] = v[a] >> 1;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (!dx && !dy && !dz) continue;
+                        const int q[3] = {p[0] + dx, p[1] + dy, p[2] + dz};
+                        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= n / 2 || q[1] >= n / 2 ||
+                            q[2] >= n / 2)
+                            continue;
+                        for (int o = 0; o < 8; ++o) {
+                            const int c[3] = {2 * q[0] + (o & 1), 2 * q[1] + ((o >> 1) & 1),
+                                              2 * q[2] + ((o >> 2) & 1)};
+                            const int ox = c[0] - v[0], oy = c[1] - v[1], oz = c[2] - v[2];
+                            if (std::abs(ox) <= 1 && std::abs(oy) <= 1 && std::abs(oz) <= 1)
+                                continue;  // nearest neighbour: near field
+                            const int key = (ox + 8) + 17 * ((oy + 8) + 17 * (oz + 8));
+                            auto it = mat_id.find(key);
+                            const int id = (it == mat_id.end()) ? (int)mat_id.size() : it->second;
+                            if (it == mat_id.end()) mat_id[key] = id;
+                            src.push_back(f->level_offset[l] + box_index(c[0], c[1], c[2], l));
+                            mat.push_back(id);
+                        }
+                    }
+            start[t + 1] = (int)src.size();
+        }
+        std::vector<double> Tall((size_t)N * N * std::max<size_t>(1, mat_id.size()));
+        for (auto &kv : mat_id) {
+            const int key = kv.first;
+            const int ox = key % 17 - 8, oy = (key / 17) % 17 - 8, oz = key / 289 - 8;
+            const double D[3] = {ox * size[0], oy * size[1], oz * size[2]};
+            fmm_interaction(tab, D, &Tall[(size_t)kv.second * N * N]);
+        }
+        DAGE_TRY(upload((void **)&f->d_T[l], Tall.data(), 8 * Tall.size()));
+        std::vector<int> lst;
+        lst.insert(lst.end(), start.begin(), start.end());
+        lst.insert(lst.end(), src.begin(), src.end());
+        lst.insert(lst.end(), mat.begin(), mat.end());
+        f->list_len[l] = (int)src.size();
+        DAGE_TRY(upload((void **)&f->d_list[l], lst.data(), 4 * lst.size()));
+        if (l > f->start_level) {  // child level l: W(child - parent) per octant
+            std::vector<double> Wall((size_t)8 * N * N);
+            for (int o = 0; o < 8; ++o) {
+                const double d[3] = {((o & 1) - 0.5) * size[0], (((o >> 1) & 1) - 0.5) * size[1],
+                                     (((o >> 2) & 1) - 0.5) * size[2]};
+                fmm_translation(tab, d, &Wall[(size_t)o * N * N]);
+            }
+            DAGE_TRY(upload((void **)&f->d_W[l], Wall.data(), 8 * Wall.size()));
+        }
+    }
+    return DAGE_OK;
+}
+
+int gbfmm_apply_f64(dage_gbfmm f, const double *rho, double *pot) {
+    dage_ctx ctx = f->ctx;
+    hipStream_t s = ctx->stream;
+    const int nb = f->nb, N = f->N, L = f->maxlevel;
+    const long ldx = f->npt[0], ldy = f->npt[1];
+    hipEvent_t ev;
+    // near field: every box writes its own (cut) points
+    for (int box = 0; box < nb * nb * nb; ++box) {
+        const int bx = f->boxvec[3 * box], by = f->boxvec[3 * box + 1], bz = f->boxvec[3 * box + 2];
+        const int nx = f->nf_n[0][bx], ny = f->nf_n[1][by], nz = f->nf_n[2][bz];
+        const long tot = (long)nx * ny * nz;
+        prof_begin(ctx, "gather_kernel", &ev);
+        hipLaunchKernelGGL(gather_kernel, dim3((int)std::min<long>((tot + 255) / 256, 65536)),
+                           dim3(256), 0, s, rho, ldx, ldy, f->nf_lo[0][bx], f->nf_lo[1][by],
+                           f->nf_lo[2][bz], nx, ny, nz, f->d_nf);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "gather_kernel", ev);
+        ApplyMats m;
+        m.fx = f->ops[bx]->d_fx;
+        m.fy = f->ops[by]->d_fy;
+        m.gz = f->ops[bz]->d_gz;
+        m.w = f->ops[0]->d_w;
+        m.din[0] = nx; m.din[1] = ny; m.din[2] = nz;
+        m.dout[0] = f->box_n[0][bx]; m.dout[1] = f->box_n[1][by]; m.dout[2] = f->box_n[2][bz];
+        m.ntp = f->ops[0]->ntp;
+        OutView ov;
+        ov.p = pot + f->box_lo[0][bx] + ldx * (f->box_lo[1][by] + ldy * (long)f->box_lo[2][bz]);
+        ov.ldx = ldx;
+        ov.ldy = ldy;
+        ov.n[0] = f->box_cut[0][bx]; ov.n[1] = f->box_cut[1][by]; ov.n[2] = f->box_cut[2][bz];
+        DAGE_TRY(apply_f64_view(ctx, m, f->d_nf, ov, 0.0, &f->ws));
+    }
+    // multipoles of the finest boxes
+    const int nbox_max = nb * nb * nb;
+    DAGE_TRY(fill_zero(ctx, f->d_q, (size_t)N * f->nbox_total));
+    DAGE_TRY(fill_zero(ctx, f->d_L, (size_t)N * f->nbox_total));
+    MultArgs ma;
+    ma.rho = rho; ma.ldx = ldx; ma.ldy = ldy;
+    ma.boxvec = f->d_boxvec; ma.geo = f->d_geo; ma.nb = nb;
+    ma.wbox = f->d_wbox; ma.wbox_ld = f->wbox_ld;
+    ma.points = f->d_points; ma.points_ld = f->points_ld;
+    ma.centers = f->d_centers; ma.coef = f->d_coef;
+    ma.lmax = f->lmax; ma.N = N; ma.zsplit = f->zsplit; ma.nbox = nbox_max;
+    ma.part = f->d_part;
+    prof_begin(ctx, "box_multipoles_kernel", &ev);
+    hipLaunchKernelGGL(box_multipoles_kernel, dim3(nbox_max, f->zsplit), dim3(256), 0, s, ma);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "box_multipoles_kernel", ev);
+    const long qlen = (long)N * nbox_max;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((int)std::min<long>((qlen + 255) / 256, 65536)),
+                       dim3(256), 0, s, f->d_part, f->zsplit, qlen, f->d_q);
+    DAGE_HIP_CHECK(hipGetLastError());
+    // upward pass
+    for (int l = L - 1; l >= f->start_level; --l) {
+        prof_begin(ctx, "m2m_kernel", &ev);
+        hipLaunchKernelGGL(m2m_kernel, dim3(1 << (3 * l)), dim3(256), 0, s, f->d_W[l + 1],
+                           f->d_q + (size_t)N * f->level_offset[l + 1],
+                           f->d_q + (size_t)N * f->level_offset[l], N);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "m2m_kernel", ev);
+    }
+    // far field per level, translated down
+    for (int l = f->start_level; l <= L; ++l) {
+        const int nbox = 1 << (3 * l);
+        if (l > f->start_level) {
+            prof_begin(ctx, "l2l_kernel", &ev);
+            hipLaunchKernelGGL(l2l_kernel, dim3(nbox), dim3(256), 0, s, f->d_W[l],
+                               f->d_L + (size_t)N * f->level_offset[l - 1],
+                               f->d_L + (size_t)N * f->level_offset[l], N);
+            DAGE_HIP_CHECK(hipGetLastError());
+            prof_end(ctx, "l2l_kernel", ev);
+        }
+        const int *lst = f->d_list[l];
+        prof_begin(ctx, "m2l_kernel", &ev);
+        hipLaunchKernelGGL(m2l_kernel, dim3(nbox), dim3(256), 0, s, f->d_T[l], lst,
+                           lst + nbox + 1, lst + nbox + 1 + f->list_len[l], f->d_q,
+                           f->d_L + (size_t)N * f->level_offset[l], N);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "m2l_kernel", ev);
+    }
+    // local expansions on the finest boxes' points
+    LeArgs la;
+    la.out = pot; la.ldx = ldx; la.ldy = ldy;
+    la.boxvec = f->d_boxvec; la.geo = f->d_geo; la.nb = nb;
+    la.points = f->d_points; la.points_ld = f->points_ld;
+    la.centers = f->d_centers; la.coef = f->d_coef;
+    la.L = f->d_L;  // level maxlevel has offset 0
+    la.lmax = f->lmax; la.N = N;
+    int maxz = 0;
+    for (int b = 0; b < nb; ++b) maxz = std::max(maxz, f->box_cut[2][b]);
+    prof_begin(ctx, "local_expansion_kernel", &ev);
+    hipLaunchKernelGGL(local_expansion_kernel, dim3(nbox_max, maxz), dim3(256), 0, s, la);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "local_expansion_kernel", ev);
+    return DAGE_OK;
+}
+
+int use_device(dage_ctx ctx) {
+    DAGE_REQUIRE(ctx != nullptr, DAGE_ERR_ARG, "null context handle");
+    DAGE_HIP_CHECK(hipSetDevice(ctx->device));
+    return DAGE_OK;
+}
+
+}  // namespace
+}  // namespace dage
+
+using namespace dage;
+
+extern "C" {
+
+int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax, const double *tp,
+                      const double *tw, int ntp, dage_gbfmm *out) {
+    DAGE_REQUIRE(out != nullptr, DAGE_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(grid && tp && tw && ntp > 0, DAGE_ERR_ARG, "null or empty argument");
+    DAGE_REQUIRE(grid->ctx == ctx, DAGE_ERR_ARG, "grid belongs to another context");
+    DAGE_REQUIRE(maxlevel >= 2 && maxlevel <= 6, DAGE_ERR_UNSUPPORTED,
+                 "maxlevel %d outside 2..6 (boxes start at level 2)", maxlevel);
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
+                 lmax, kMaxFmmLmax);
+    const int nb = 1 << maxlevel;
+    for (int a = 0; a < 3; ++a)
+        DAGE_REQUIRE(grid->axis[a].ncell % nb == 0, DAGE_ERR_ARG,
+                     "axis %d: %d cells do not split into %d boxes", a, grid->axis[a].ncell, nb);
+    auto *f = new (std::nothrow) dage_gbfmm_s();
+    DAGE_REQUIRE(f != nullptr, DAGE_ERR_ALLOC, "out of host memory");
+    f->ctx = ctx;
+    ctx_retain(ctx);
+    f->maxlevel = maxlevel;
+    f->lmax = lmax;
+    f->N = (lmax + 1) * (lmax + 1);
+    f->nb = nb;
+    f->nlip = grid->axis[0].nlip;
+    const int rc = build_gbfmm(f, grid, tp, tw, ntp);
+    if (rc != DAGE_OK) {
+        free_gbfmm(f);
+        return rc;
+    }
+    *out = f;
+    return DAGE_OK;
+}
+
+int dage_gbfmm_apply_device(dage_gbfmm f, const double *rho, double *pot, int precision) {
+    DAGE_REQUIRE(f && rho && pot, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(precision == DAGE_PREC_FP64, DAGE_ERR_UNSUPPORTED,
+                 "GBFMM runs in FP64 (precision %d requested)", precision);
+    DAGE_TRY(use_device(f->ctx));
+    return gbfmm_apply_f64(f, rho, pot);
+}
+
+int dage_gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
+    DAGE_REQUIRE(f && rho && pot, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(f->ctx));
+    const size_t n = (size_t)f->npt[0] * f->npt[1] * f->npt[2];
+    double *d_in = nullptr, *d_out = nullptr;
+    DAGE_HIP_CHECK(hipMalloc((void **)&d_in, 8 * n));
+    if (hipMalloc((void **)&d_out, 8 * n) != hipSuccess) {
+        (void)hipFree(d_in);
+        set_error("device allocation of %zu bytes failed", 8 * n);
+        return DAGE_ERR_HIP;
+    }
+    int rc = DAGE_OK;
+    if (hipMemcpyAsync(d_in, rho, 8 * n, hipMemcpyHostToDevice, f->ctx->stream) != hipSuccess) {
+        set_error("upload failed");
+        rc = DAGE_ERR_HIP;
+    }
+    if (rc == DAGE_OK) rc = dage_gbfmm_apply_device(f, d_in, d_out, precision);
+    if (rc == DAGE_OK &&
+        hipMemcpyAsync(pot, d_out, 8 * n, hipMemcpyDeviceToHost, f->ctx->stream) != hipSuccess) {
+        set_error("download failed");
+        rc = DAGE_ERR_HIP;
+    }
+    if (hipStreamSynchronize(f->ctx->stream) != hipSuccess && rc == DAGE_OK) {
+        set_error("stream synchronisation failed");
+        rc = DAGE_ERR_HIP;
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+int dage_gbfmm_box_count(dage_gbfmm f, int *nbox_total) {
+    DAGE_REQUIRE(f && nbox_total, DAGE_ERR_ARG, "null argument");
+    *nbox_total = f->nbox_total;
+    return DAGE_OK;
+}
+
+int dage_gbfmm_get_multipoles(dage_gbfmm f, double *q) {
+    DAGE_REQUIRE(f && q, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(f->ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpy(q, f->d_q, 8 * (size_t)f->N * f->nbox_total, hipMemcpyDeviceToHost));
+    return DAGE_OK;
+}
+
+int dage_gbfmm_get_local_expansions(dage_gbfmm f, double *L) {
+    DAGE_REQUIRE(f && L, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_device(f->ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpy(L, f->d_L, 8 * (size_t)f->N * f->nbox_total, hipMemcpyDeviceToHost));
+    return DAGE_OK;
+}
+
+int dage_gbfmm_destroy(dage_gbfmm f) {
+    if (!f) return DAGE_OK;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    free_gbfmm(f);
+    return DAGE_OK;
+}
+
+int dage_fmm_interaction_matrix(int lmax, const double dR[3], double *T) {
+    DAGE_REQUIRE(dR && T, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
+                 lmax, kMaxFmmLmax);
+    DAGE_REQUIRE(dR[0] != 0.0 || dR[1] != 0.0 || dR[2] != 0.0, DAGE_ERR_ARG,
+                 "the interaction matrix is undefined at zero separation");
+    fmm_interaction(fmm_tables(lmax), dR, T);
+    return DAGE_OK;
+}
+
+int dage_fmm_translation_matrix(int lmax, const double dR[3], double *W) {
+    DAGE_REQUIRE(dR && W, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
+                 lmax, kMaxFmmLmax);
+    fmm_translation(fmm_tables(lmax), dR, W);
+    return DAGE_OK;
+}
+
+}  // extern "C"

@@ -8,6 +8,8 @@ Names and argument meaning follow the reference:
   Operator3D.transform_cuda_cube                src/bubbles/function3d.F90:4354-4480
   Operator3D.set_transformation_weights         src/bubbles/function3d.F90:320-336
   Helmholtz3D.set_energy                        src/bubbles/helmholtz3d.F90:411-444
+  GBFMMCoulomb3D(grid, maxlevel, lmax, gauss)   src/gbfmm/gbfmm_coulomb3d.F90:306-506
+  InteractionMatrix / TranslationMatrix         src/bubbles/multipole.F90:33-560
 
 Cubes are numpy arrays in Fortran order (x fastest) for the host path and
 DeviceCube objects (the library's CudaCube, same layout) for the device path.
@@ -386,3 +388,75 @@ class Bubbles:
             raise ValueError(f"cube shape {cube.shape} != grid shape {grid.get_shape()}")
         call("dage_bubbles_inject_device", self.handle, grid.handle, lmin, float(multiplier),
              ctypes.c_void_p(cube.ptr))
+
+
+class GBFMMCoulomb3D:
+    """Grid-based FMM Coulomb operator on one grid (gbfmm_coulomb3d.F90:306-1330).
+
+    2**maxlevel boxes per axis, multipoles up to lmax, near field with the
+    t-quadrature `gauss` (the reference's default GaussQuad(25, 25))."""
+
+    def __init__(self, ctx: Context, grid: Grid3D, maxlevel: int = 2, lmax: int = 12,
+                 gauss: "GaussQuad | None" = None):
+        gauss = gauss if gauss is not None else GaussQuad(25, 25)
+        h = ctypes.c_void_p()
+        call("dage_gbfmm_create", ctx.handle, grid.handle, maxlevel, lmax, _ptr(gauss.tp),
+             _ptr(gauss.tw), gauss.get_npoints(), ctypes.byref(h))
+        self.handle = h
+        self.ctx, self.grid = ctx, grid
+        self.maxlevel, self.lmax = maxlevel, lmax
+        n = ctypes.c_int()
+        call("dage_gbfmm_box_count", h, ctypes.byref(n))
+        self.nbox_total = n.value
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_gbfmm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, rho: np.ndarray) -> np.ndarray:
+        """Potential of the density cube rho (host arrays, Fortran order)."""
+        shape = self.grid.get_shape()
+        rho = np.asfortranarray(rho, dtype=np.float64)
+        if rho.shape != shape:
+            raise ValueError(f"density shape {rho.shape} != grid shape {shape}")
+        pot = np.zeros(shape, order="F")
+        call("dage_gbfmm_apply", self.handle, rho.ctypes.data_as(_dp), pot.ctypes.data_as(_dp),
+             DAGE_PREC_FP64)
+        return pot
+
+    def apply_device(self, rho: "DeviceCube", pot: "DeviceCube"):
+        call("dage_gbfmm_apply_device", self.handle, ctypes.c_void_p(rho.ptr),
+             ctypes.c_void_p(pot.ptr), DAGE_PREC_FP64)
+
+    def multipole_moments(self) -> np.ndarray:
+        q = np.zeros(((self.lmax + 1) ** 2, self.nbox_total), order="F")
+        call("dage_gbfmm_get_multipoles", self.handle, q.ctypes.data_as(_dp))
+        return q
+
+    def local_expansions(self) -> np.ndarray:
+        L = np.zeros(((self.lmax + 1) ** 2, self.nbox_total), order="F")
+        call("dage_gbfmm_get_local_expansions", self.handle, L.ctypes.data_as(_dp))
+        return L
+
+
+def interaction_matrix(lmax: int, dR) -> np.ndarray:
+    """InteractionMatrix%eval: L_target += T @ q_source, dR = source - target."""
+    N = (lmax + 1) ** 2
+    T = np.zeros((N, N), order="F")
+    call("dage_fmm_interaction_matrix", lmax, _ptr(_f64(dR)), T.ctypes.data_as(_dp))
+    return T
+
+
+def translation_matrix(lmax: int, dR) -> np.ndarray:
+    """TranslationMatrix%eval unpacked: q(new) = W @ q(old), dR = old - new."""
+    N = (lmax + 1) ** 2
+    W = np.zeros((N, N), order="F")
+    call("dage_fmm_translation_matrix", lmax, _ptr(_f64(dR)), W.ctypes.data_as(_dp))
+    return W

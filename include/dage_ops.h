@@ -64,6 +64,7 @@ typedef struct dage_grid3d_s *dage_grid3d;
 typedef struct dage_operator_s *dage_operator;
 typedef struct dage_cube_s *dage_cube;
 typedef struct dage_bubbles_s *dage_bubbles;
+typedef struct dage_gbfmm_s *dage_gbfmm;
 
 /* ----------------------------------------------------------------- library */
 int dage_abi_version(void);
@@ -215,6 +216,44 @@ int dage_radial_grid_steps(double z, int ncell, int nlip, double r_max,
  * buffers; synchronises and writes *result on the host. */
 int dage_cube_integrate_device(dage_grid3d grid, const double *f_dev,
                                const double *g_dev, double *result);
+
+/* -------------------------------------------------------------------- GBFMM
+ * Grid-based fast multipole Coulomb potential of a cube
+ * (src/gbfmm/gbfmm_coulomb3d.F90:306-1330; GPU side gbfmm_coulomb3d.cu:
+ * 550-676, whose gbfmmcoulomb3d_init_cuda / _calculate_multipole_moments_cuda
+ * / _evaluate_potential_le_cuda / _destroy_cuda these replace).
+ * dage_gbfmm_create (GBFMMCoulomb3D_init + init_nearfield +
+ * init_interaction_matrices): the grid's cells split into 2^maxlevel boxes
+ * per axis (every axis's cell count must divide), multipoles up to lmax,
+ * near-field operators built on the sub-grids with the t-quadrature
+ * (tp, tw, ntp) and Coulomb weights 2/sqrt(pi)*tw, no coda. */
+int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax,
+                      const double *tp, const double *tw, int ntp,
+                      dage_gbfmm *gbfmm);
+/* GBFMMCoulomb3D_calculate_potential, cube part (gbfmm_coulomb3d.F90:650-821):
+ * pot = near field + local expansions, on the grid's points (device buffers,
+ * asynchronous on the context stream).  precision: DAGE_PREC_FP64. */
+int dage_gbfmm_apply_device(dage_gbfmm gbfmm, const double *rho_dev,
+                            double *pot_dev, int precision);
+/* Host-buffer variant (uploads, applies, downloads, synchronises). */
+int dage_gbfmm_apply(dage_gbfmm gbfmm, const double *rho, double *pot,
+                     int precision);
+/* Number of boxes over levels 2..maxlevel (the columns of the arrays below). */
+int dage_gbfmm_box_count(dage_gbfmm gbfmm, int *nbox_total);
+/* After an apply: multipole moments and local expansions, (lmax+1)^2 x
+ * nbox_total column-major, boxes in the reference's order (level maxlevel
+ * first, children of a box consecutive), as multipole_moments(:, :) and
+ * farfield_potential(:, :) of gbfmm_coulomb3d.F90. */
+int dage_gbfmm_get_multipoles(dage_gbfmm gbfmm, double *q);
+int dage_gbfmm_get_local_expansions(dage_gbfmm gbfmm, double *L);
+int dage_gbfmm_destroy(dage_gbfmm gbfmm);
+/* InteractionMatrix%eval (src/bubbles/multipole.F90:268-312): T, (lmax+1)^2
+ * square column-major, with L_target += T q_source for dR = source centre -
+ * target centre.  Host computation. */
+int dage_fmm_interaction_matrix(int lmax, const double dR[3], double *T);
+/* TranslationMatrix%eval (multipole.F90:444-470) unpacked: W, (lmax+1)^2
+ * square column-major, unit lower triangular, q(new) = W(old - new) q(old). */
+int dage_fmm_translation_matrix(int lmax, const double dR[3], double *W);
 
 #ifdef __cplusplus
 }

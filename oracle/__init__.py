@@ -66,6 +66,13 @@ def _load():
         "dage_oracle_solid_harmonics": (None, [i, d, d, d, _dp]),
         "dage_oracle_bubbles_eval_grid": (i, [i, _dp, i, i, i, _ip, i, ctypes.POINTER(_dp),
                                               ctypes.POINTER(_dp), _dp, i, _dp, i, _dp, i, _dp]),
+        "dage_oracle_fmm_interaction": (i, [i, _dp, _dp]),
+        "dage_oracle_fmm_translation": (i, [i, _dp, _dp]),
+        "dage_oracle_fmm_translate": (None, [i, _dp, i, _dp]),
+        "dage_oracle_box_multipoles": (None, [i, _dp, i, i, _dp, _dp, i, _dp, _dp, i, _dp, _dp, i,
+                                              _dp, _dp]),
+        "dage_oracle_eval_local_expansion": (None, [i, _dp, _dp, i, _dp, i, _dp, i, _dp, _dp, i,
+                                                    i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -330,3 +337,182 @@ def bubbles_eval_grid(bub: BubbleSet, x, y, z, lmin: int = 0, cube=None):
     if rc:
         raise RuntimeError("bubbles_eval_grid failed")
     return out
+
+
+# ------------------------------------------------------------------------ FMM
+def fmm_interaction(lmax: int, dR) -> np.ndarray:
+    """InteractionMatrix%eval (multipole.F90:268-312): L_target += T q_source,
+    dR = source centre - target centre."""
+    N = (lmax + 1) ** 2
+    T = np.zeros(N * N)
+    if _load().dage_oracle_fmm_interaction(lmax, _p(_f64(dR)), _p(T)):
+        raise MemoryError
+    return T.reshape((N, N), order="F")
+
+
+def fmm_translation_packed(lmax: int, dR) -> np.ndarray:
+    """TranslationMatrix%eval (multipole.F90:444-470): packed unit-lower W."""
+    N = (lmax + 1) ** 2
+    W = np.zeros(N * (N + 1) // 2)
+    if _load().dage_oracle_fmm_translation(lmax, _p(_f64(dR)), _p(W)):
+        raise MemoryError
+    return W
+
+
+def fmm_translate(lmax: int, W: np.ndarray, q, transpose=False) -> np.ndarray:
+    """dtpmv('L', 'N'|'T', 'U') as TranslationMatrix_apply uses it."""
+    q = _f64(q).copy()
+    _load().dage_oracle_fmm_translate(lmax, _p(_f64(W)), 1 if transpose else 0, _p(q))
+    return q
+
+
+def fmm_translation_full(lmax: int, dR) -> np.ndarray:
+    """The packed W unpacked to a square matrix with its unit diagonal."""
+    N = (lmax + 1) ** 2
+    Wp = fmm_translation_packed(lmax, dR)
+    W = np.eye(N)
+    r = 0
+    for s in range(N):
+        for i in range(s, N):
+            if i > s:
+                W[i, s] = Wp[r]
+            r += 1
+    return W
+
+
+def sub_axis(ax: Axis, c0: int, c1: int) -> Axis:
+    """Grid1D_get_subgrid (grid.F90:1474-1506) for cells [c0, c1) (0-based)."""
+    return Axis(float(ax.celld[c0]), ax.steps[c0:c1].copy(), ax.nlip, ax.grid_type)
+
+
+def _box_index(v, level):
+    idx = 0
+    for l in range(level - 1, -1, -1):
+        idx = idx * 8 + ((v[0] >> l) & 1) + 2 * ((v[1] >> l) & 1) + 4 * ((v[2] >> l) & 1)
+    return idx
+
+
+def _box_vector(idx, level):
+    v = [0, 0, 0]
+    for l in range(level):
+        o = (idx >> (3 * l)) & 7
+        v[0] |= (o & 1) << l
+        v[1] |= ((o >> 1) & 1) << l
+        v[2] |= ((o >> 2) & 1) << l
+    return v
+
+
+def gbfmm_potential(axes, rho, maxlevel, lmax, tp, tw, start_level=2):
+    """GBFMMCoulomb3D_calculate_potential, cube part, as the GPU path computes
+    it (gbfmm_coulomb3d.F90:650-821, 933-1098, 1100-1159, 1336-1622;
+    gbfmm_parallel_info.F90:827-1470).  Returns (potential, q, L) with q and
+    L of shape ((lmax+1)^2, nbox_total), level maxlevel first."""
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    nb = 1 << maxlevel
+    N = (lmax + 1) ** 2
+    w = coulomb_weights(tw)
+    pot = np.zeros(rho.shape, order="F")
+    qmin = [a.points[0] for a in axes]
+    qmax = [a.points[-1] for a in axes]
+    s = [a.ncell // nb for a in axes]
+    npl = [a.nlip - 1 for a in axes]
+    for a in range(3):
+        if axes[a].ncell % nb:
+            raise ValueError("cells do not split into boxes")
+    offs = {maxlevel: 0}
+    for l in range(maxlevel - 1, start_level - 1, -1):
+        offs[l] = offs[l + 1] + 8 ** (l + 1)
+    ntot = sum(8 ** l for l in range(start_level, maxlevel + 1))
+    q = np.zeros((N, ntot))
+    L = np.zeros((N, ntot))
+
+    def centre(v, level):
+        return np.array([qmin[a] + (v[a] + 0.5) * ((qmax[a] - qmin[a]) / (1 << level))
+                         for a in range(3)])
+
+    # near field (no coda), output cut at shared box ends
+    ops = {}
+    for idx in range(nb ** 3):
+        v = _box_vector(idx, maxlevel)
+        sub_in, sub_out, lo_in, lo_out, cut = [], [], [], [], []
+        for a in range(3):
+            b = v[a]
+            c0, c1 = max(0, b - 1) * s[a], min(nb, b + 2) * s[a]
+            sub_in.append(sub_axis(axes[a], c0, c1))
+            sub_out.append(sub_axis(axes[a], b * s[a], (b + 1) * s[a]))
+            lo_in.append(c0 * npl[a])
+            lo_out.append(b * s[a] * npl[a])
+            cut.append(s[a] * npl[a] + (1 if b == nb - 1 else 0))
+        key = tuple(v)
+        mats = []
+        for a in range(3):
+            mk = (a, v[a])
+            if mk not in ops:
+                ops[mk] = exponentials(sub_in[a], sub_out[a], tp, a != 0)
+            mats.append(ops[mk])
+        nf = rho[lo_in[0]:lo_in[0] + sub_in[0].n, lo_in[1]:lo_in[1] + sub_in[1].n,
+                 lo_in[2]:lo_in[2] + sub_in[2].n]
+        box = transform_cube(mats[0], mats[1], mats[2], w, 0.0, nf, add_coda=False)
+        pot[lo_out[0]:lo_out[0] + cut[0], lo_out[1]:lo_out[1] + cut[1],
+            lo_out[2]:lo_out[2] + cut[2]] = box[:cut[0], :cut[1], :cut[2]]
+        # multipoles over the whole box sub-grid with its own weights
+        bx = [sub_out[a] for a in range(3)]
+        sub = np.asfortranarray(rho[lo_out[0]:lo_out[0] + bx[0].n, lo_out[1]:lo_out[1] + bx[1].n,
+                                    lo_out[2]:lo_out[2] + bx[2].n])
+        qq = np.zeros(N)
+        c = _f64(centre(v, maxlevel))
+        _load().dage_oracle_box_multipoles(
+            lmax, sub.ctypes.data_as(_dp), sub.shape[0], sub.shape[1],
+            _p(_f64(bx[0].points)), _p(_f64(bx[0].ints())), bx[0].n,
+            _p(_f64(bx[1].points)), _p(_f64(bx[1].ints())), bx[1].n,
+            _p(_f64(bx[2].points)), _p(_f64(bx[2].ints())), bx[2].n, _p(c), _p(qq))
+        q[:, offs[maxlevel] + idx] = qq
+    # upward pass (translate_multipoles)
+    for l in range(maxlevel - 1, start_level - 1, -1):
+        for child in range(8 ** (l + 1)):
+            parent = child // 8
+            W = fmm_translation_packed(lmax, centre(_box_vector(child, l + 1), l + 1)
+                                       - centre(_box_vector(parent, l), l))
+            q[:, offs[l] + parent] += fmm_translate(lmax, W, q[:, offs[l + 1] + child])
+    # far field per level and downward pass
+    for l in range(start_level, maxlevel + 1):
+        n = 1 << l
+        for t in range(8 ** l):
+            v = _box_vector(t, l)
+            if l > start_level:
+                parent = t // 8
+                W = fmm_translation_packed(lmax, centre(v, l) - centre(_box_vector(parent, l - 1), l - 1))
+                L[:, offs[l] + t] += fmm_translate(lmax, W, L[:, offs[l - 1] + parent], transpose=True)
+            p = [x >> 1 for x in v]
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        if dx == dy == dz == 0:
+                            continue
+                        pn = [p[0] + dx, p[1] + dy, p[2] + dz]
+                        if min(pn) < 0 or max(pn) >= n // 2:
+                            continue
+                        for o in range(8):
+                            cvec = [2 * pn[0] + (o & 1), 2 * pn[1] + ((o >> 1) & 1),
+                                    2 * pn[2] + ((o >> 2) & 1)]
+                            if max(abs(cvec[a] - v[a]) for a in range(3)) <= 1:
+                                continue
+                            T = fmm_interaction(lmax, centre(cvec, l) - centre(v, l))
+                            L[:, offs[l] + t] += T @ q[:, offs[l] + _box_index(cvec, l)]
+    # local expansions on each box's own points
+    for idx in range(nb ** 3):
+        v = _box_vector(idx, maxlevel)
+        lo, cut, pts = [], [], []
+        for a in range(3):
+            b = v[a]
+            lo.append(b * s[a] * npl[a])
+            cut.append(s[a] * npl[a] + (1 if b == nb - 1 else 0))
+            pts.append(_f64(axes[a].points[lo[a]:lo[a] + cut[a]]))
+        blk = np.zeros(cut, order="F")
+        c = _f64(centre(v, maxlevel))
+        _load().dage_oracle_eval_local_expansion(lmax, _p(_f64(L[:, offs[maxlevel] + idx])),
+                                                 _p(pts[0]), cut[0], _p(pts[1]), cut[1],
+                                                 _p(pts[2]), cut[2], _p(c),
+                                                 blk.ctypes.data_as(_dp), cut[0], cut[1])
+        pot[lo[0]:lo[0] + cut[0], lo[1]:lo[1] + cut[1], lo[2]:lo[2] + cut[2]] += blk
+    return pot, q, L
