@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=2)
     p.add_argument("--no-inject", dest="inject", action="store_false",
                    help="skip the bubbles-inject measurement")
+    p.add_argument("--no-gbfmm", dest="gbfmm", action="store_false",
+                   help="skip the GBFMM measurement")
     p.add_argument("--copy-calibration", action="store_true",
                    help="time a plain device copy of a 257^3 cube instead (the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -235,6 +237,8 @@ def main():
         obj.close()
     if args.inject:
         line["inject"] = bench_inject(ctx, args)
+    if args.gbfmm:
+        line["gbfmm"] = bench_gbfmm(ctx, args)
     print(json.dumps(line), flush=True)
     ctx.close()
 
@@ -300,6 +304,106 @@ def bench_inject(ctx, args):
                      "traffic": measured_traffic("bubbles_inject_kernel")},
         "fp64_valu_tflops": round(FLOP_PER_POINT_BUBBLE * natom * n ** 3 / (kernel_ms * 1e-3) / 1e12, 2),
         "max_rel_err_vs_oracle_slab": err,
+    }
+
+
+def bench_gbfmm(ctx, args):
+    """GBFMM Coulomb potential at the config-2 shape: 241^3 cube over
+    [-12,12]^3 (nlip 7, 40 cells), 4x4x4 boxes (maxlevel 2, 10 cells = 61
+    points per box), lmax 12, near field with GaussQuad(25, 25) (T = 50);
+    density = 16 Gaussians, exponents log-uniform in [0.3, 30], centres in
+    [-4,4]^3.  Compared with the analytic potential and with the direct
+    FP64 Coulomb apply of the same quadrature on the whole cube."""
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    rng = np.random.default_rng(0xDA6E0002)
+    ax = O.Axis.ncell_fixed(-12.0, 12.0, 7, 40)
+    n = ax.n
+    q = np.ones(16)
+    alpha = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 16))
+    centers = rng.uniform(-4.0, 4.0, (16, 3))
+    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    gq = dage_amd.GaussQuad(25, 25)
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=2, lmax=12, gauss=gq)
+    d_in = dage_amd.DeviceCube(ctx, rho.shape, rho)
+    d_out = dage_amd.DeviceCube(ctx, rho.shape)
+    for _ in range(max(1, args.warmup // 2)):
+        fmm.apply_device(d_in, d_out)
+    steps = max(2, args.steps // 5)
+    ctx.profile(True)
+    ctx.time_begin()
+    for _ in range(steps):
+        fmm.apply_device(d_in, d_out)
+    ms = ctx.time_end() / steps
+    kernels = {}
+    for k in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+              "box_multipoles_kernel", "m2m_kernel", "m2l_kernel", "l2l_kernel",
+              "local_expansion_kernel"):
+        kms, kn = ctx.profile_query(k)
+        if kn:
+            kernels[k] = {"ms_per_apply": round(kms / steps, 4), "launches_per_apply": kn // steps}
+    ctx.profile(False)
+    pot = d_out.download()
+    exact = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
+    err_exact = float(np.max(np.abs(pot - exact)) / np.max(np.abs(exact)))
+    # direct apply with the same quadrature (no coda, as the near field)
+    op = dage_amd.Coulomb3D(ctx, grid, grid, gq, coda=False)
+    op.transform_cuda_cube(d_in, d_out)
+    ctx.time_begin()
+    for _ in range(2):
+        op.transform_cuda_cube(d_in, d_out)
+    direct_ms = ctx.time_end() / 2
+    direct = d_out.download()
+    err_direct = float(np.max(np.abs(pot - direct)) / np.max(np.abs(direct)))
+    # the reference applies one sub-operator per box: sum over boxes of
+    # 2 T (nxo nxi nyi nzi + nxo nyo nyi nzi + nxo nyo nzo nzi); here the near
+    # fields of all boxes are one whole-cube apply with banded matrices,
+    # 6 n^4 T dense FLOPs (the MFMA work actually issued)
+    T = gq.get_npoints()
+    s = 10 * 6
+    nf = [min(4, b + 2) * s - max(0, b - 1) * s + 1 for b in range(4)]
+    ref_flops = 0.0
+    for bx in range(4):
+        for by in range(4):
+            for bz in range(4):
+                no = s + 1
+                ref_flops += 2 * T * (no * nf[bx] * nf[by] * nf[bz] + no * no * nf[by] * nf[bz]
+                                      + no * no * no * nf[bz])
+    flops = 6.0 * n ** 4 * T
+    # split-BF16 near field on the same input
+    fmm.apply_device(d_in, d_out, dage_amd.DAGE_PREC_BF16X6)
+    ctx.time_begin()
+    for _ in range(steps):
+        fmm.apply_device(d_in, d_out, dage_amd.DAGE_PREC_BF16X6)
+    mixed_ms = ctx.time_end() / steps
+    err_mixed = float(np.max(np.abs(d_out.download() - pot)) / np.max(np.abs(pot)))
+    gemm_ms = sum(kernels.get(k, {}).get("ms_per_apply", 0.0)
+                  for k in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode"))
+    mp_ms = kernels.get("box_multipoles_kernel", {}).get("ms_per_apply", 0.0)
+    le_ms = kernels.get("local_expansion_kernel", {}).get("ms_per_apply", 0.0)
+    for obj in (op, fmm, d_in, d_out, grid):
+        obj.close()
+    return {
+        "workload": "config 2 GBFMM: 241^3 cube over [-12,12]^3, 4x4x4 boxes, lmax 12, "
+                    "near field T=50, 16 Gaussians",
+        "ms_per_apply": round(ms, 3),
+        "Gpts_s": round(n ** 3 / (ms * 1e-3) / 1e9, 4),
+        "nearfield_dense_flops": flops,
+        "nearfield_per_box_reference_flops": ref_flops,
+        "nearfield_tflops": round(flops / (gemm_ms * 1e-3) / 1e12, 2) if gemm_ms else None,
+        "nearfield_mfma_frac": round(flops / (gemm_ms * 1e-3) / 1e12 / 78.6, 4) if gemm_ms else None,
+        "bf16x6_nearfield": {"ms_per_apply": round(mixed_ms, 3),
+                             "speedup_vs_fp64": round(ms / mixed_ms, 3),
+                             "max_rel_diff_vs_fp64_gbfmm": err_mixed},
+        "multipoles_GBps": round(8.0 * 64 * 61 ** 3 / (mp_ms * 1e-3) / 1e9, 1) if mp_ms else None,
+        "local_expansion_GBps": round(16.0 * n ** 3 / (le_ms * 1e-3) / 1e9, 1) if le_ms else None,
+        "kernels": kernels,
+        "direct_apply_ms": round(direct_ms, 3),
+        "max_rel_err_vs_analytic": err_exact,
+        "max_rel_diff_vs_direct_apply": err_direct,
     }
 
 

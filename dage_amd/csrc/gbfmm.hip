@@ -6,7 +6,14 @@
 //   near field  exact Coulomb operator (no coda) from the box and its nearest
 //               neighbours to the box's points, i.e. the operator built on
 //               the near-field sub-grid -> box sub-grid (Coulomb3D_get_suboperator,
-//               coulomb.F90:517-635), evaluated with the FP64 MFMA apply;
+//               coulomb.F90:517-635).  The near-field region of every point is
+//               the tensor product of per-axis ranges, so the near fields of
+//               all boxes together are ONE separable operator on the whole
+//               cube whose 1D matrices are banded: row xo holds the sub-
+//               operator row of the box owning xo, placed at that box's
+//               near-field columns.  The near field is therefore one
+//               whole-cube apply (FP64 MFMA, or split-BF16) with those
+//               assembled matrices instead of one apply per box;
 //   far field   box multipoles q_lm (lmax, Racah solid harmonics, box sub-grid
 //               weights), translated up the tree (M2M), converted to local
 //               expansions from each interaction list (M2L), translated down
@@ -43,13 +50,11 @@ struct dage_gbfmm_s {
     int nbox_total = 0;
     // per axis and box coordinate: box and near-field point ranges
     std::vector<int> box_lo[3], box_n[3], box_cut[3], nf_lo[3], nf_n[3];
-    std::vector<dage_grid3d> grids;   // near-field and box sub-grids
-    std::vector<dage_operator> ops;   // ops[b]: near field (b,b,b) -> box (b,b,b)
+    dage_operator nf_op = nullptr;    // whole-cube operator with the banded near-field matrices
     std::vector<int> boxvec;          // 3 per maxlevel box, by box index
-    double *d_nf = nullptr;           // gathered near-field sub-cube
-    dage::Workspace ws;
     // far field
-    double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr;
+    double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr, *d_m2l_part = nullptr;
+    int m2l_chunks = 8;
     int zsplit = 1;
     int *d_boxvec = nullptr;
     int *d_geo = nullptr;             // [axis][box_lo, box_n, box_cut][nb]
@@ -135,22 +140,102 @@ struct MultArgs {
     int points_ld;
     const double *centers;
     const double *coef;
-    int lmax, N, zsplit, nbox;
+    int zsplit, nbox;
     double *part;  // [zsplit][nbox][N]
 };
 
-// q_lm(box) = sum over the box sub-grid of rho * wx wy wz * S_lm(r - centre)
-// (GBFMMCoulomb3DMultipoleEvaluator, integrator.cu:1122-1196).  Block
-// (box, z-slab); each wavefront sums 64 points per lm with a butterfly and
-// lane lm%64 keeps the running total of lm, so the accumulators take 4
-// registers; the 4 wavefronts and the z-slabs are added in a fixed order.
-__global__ void __launch_bounds__(256) box_multipoles_kernel(MultArgs A) {
-    __shared__ double coef[3 * 256 + 32];
-    __shared__ double red[4][256];
+// The (l,m) of one lmax split into groups of whole m-columns (|m| = mm, both
+// signs) holding at most 64 values each, in the order the column recursion
+// produces them: group[mm] and the group sizes.
+template <int LMAX>
+struct MpPlan {
+    int group[LMAX + 1];
+    int size[8];
+    int ngroups;
+    constexpr MpPlan() : group(), size(), ngroups(0) {
+        int g = 0, fill = 0;
+        for (int mm = 0; mm <= LMAX; ++mm) {
+            const int c = (mm ? 2 : 1) * (LMAX + 1 - mm);
+            if (fill + c > 64) {
+                ++g;
+                fill = 0;
+            }
+            group[mm] = g;
+            fill += c;
+            size[g] = fill;
+        }
+        ngroups = g + 1;
+    }
+};
+
+// Accumulates val * S_lm(x,y,z) for the lm of group G into acc[0..size).
+// Written as template recursion so every accumulator index is a compile-
+// time constant (the accumulators stay in registers); the m-columns of
+// other groups are skipped and the diagonal chain S_{mm,+-mm} that seeds
+// each column is always advanced.  Same recursion and association as
+// solid_harmonics_by_column.  `coef` is read through an offset that is 0 but
+// changes per point batch, so the reads stay LDS reads inside the loop
+// instead of being hoisted into registers.
+template <int LMAX, int M, int L, int K>
+__device__ __forceinline__ void mp_column(double z, double r2, double val, double p1, double p2,
+                                          const double *coef, double (&acc)[64]) {
+    if constexpr (L <= LMAX) {
+        constexpr int N = (LMAX + 1) * (LMAX + 1);
+        constexpr int AM = M < 0 ? -M : M;
+        constexpr int lm = L * L + L + M;
+        double v = coef[lm] * (z * p1);
+        if constexpr (L - AM > 1) v = v - coef[N + lm] * (r2 * p2);
+        acc[K] += v * val;
+        mp_column<LMAX, M, L + 1, K + 1>(z, r2, val, v, p1, coef, acc);
+    }
+}
+
+template <int LMAX, int G, int MM, int K>
+__device__ __forceinline__ void mp_columns(double x, double y, double z, double r2, double val,
+                                           double dp, double dm, const double *coef,
+                                           double (&acc)[64]) {
+    constexpr MpPlan<LMAX> plan{};
+    if constexpr (MM <= LMAX) {
+        constexpr int N = (LMAX + 1) * (LMAX + 1);
+        if constexpr (plan.group[MM] <= G) {
+            double ndp = dp, ndm = dm;
+            if constexpr (MM == 1) {
+                ndp = x;
+                ndm = y;
+            } else if constexpr (MM > 1) {
+                const double f = coef[2 * N + MM];
+                ndp = f * (x * dp - y * dm);
+                ndm = f * (y * dp + x * dm);
+            }
+            if constexpr (plan.group[MM] == G) {
+                acc[K] += ndp * val;
+                mp_column<LMAX, MM, MM + 1, K + 1>(z, r2, val, ndp, 0.0, coef, acc);
+                if constexpr (MM > 0) {
+                    constexpr int K2 = K + (LMAX + 1 - MM);
+                    acc[K2] += ndm * val;
+                    mp_column<LMAX, -MM, MM + 1, K2 + 1>(z, r2, val, ndm, 0.0, coef, acc);
+                }
+                constexpr int C = (MM ? 2 : 1) * (LMAX + 1 - MM);
+                mp_columns<LMAX, G, MM + 1, K + C>(x, y, z, r2, val, ndp, ndm, coef, acc);
+            } else {
+                mp_columns<LMAX, G, MM + 1, K>(x, y, z, r2, val, ndp, ndm, coef, acc);
+            }
+        }
+    }
+}
+
+template <int LMAX, int G>
+__device__ __forceinline__ void multipole_point(double x, double y, double z, double val,
+                                                const double *coef, double (&acc)[64]) {
+    const double r2 = x * x + y * y + z * z;
+    mp_columns<LMAX, G, 0, 0>(x, y, z, r2, val, 1.0, 0.0, coef, acc);
+}
+
+template <int LMAX, int G>
+__device__ void box_multipoles_group(const MultArgs &A, const double *coef, double (*red)[64]) {
+    constexpr MpPlan<LMAX> plan{};
+    constexpr int N = (LMAX + 1) * (LMAX + 1);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ncoef = 2 * A.N + A.lmax + 1;
-    for (int i = tid; i < ncoef; i += 256) coef[i] = A.coef[i];
-    __syncthreads();
     const int box = blockIdx.x;
     const int bx = A.boxvec[3 * box], by = A.boxvec[3 * box + 1], bz = A.boxvec[3 * box + 2];
     const int nb = A.nb;
@@ -164,44 +249,102 @@ __global__ void __launch_bounds__(256) box_multipoles_kernel(MultArgs A) {
     const double *wz = A.wbox + (long)(2 * nb + bz) * A.wbox_ld;
     const double *px = A.points, *py = A.points + A.points_ld, *pz = A.points + 2 * A.points_ld;
     const double cx = A.centers[3 * box], cy = A.centers[3 * box + 1], cz = A.centers[3 * box + 2];
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    double acc[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) acc[i] = 0.0;
     const long total = (long)nx * ny * max(0, ze - zb);
     for (long base = 0; base < total; base += 256) {
         const long e = base + tid;
-        double val = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-        if (e < total) {
-            const int i = (int)(e % nx);
-            const long r = e / nx;
-            const int j = (int)(r % ny);
-            const int k = zb + (int)(r / ny);
-            val = A.rho[(x0 + i) + A.ldx * ((y0 + j) + A.ldy * (long)(z0 + k))] * wx[i] * wy[j] *
-                  wz[k];
-            dx = px[x0 + i] - cx;
-            dy = py[y0 + j] - cy;
-            dz = pz[z0 + k] - cz;
-        }
-        solid_harmonics_by_column(A.lmax, dx, dy, dz, coef, [&](int lm, double s) {
-            double v = s * val;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == (lm & 63)) {
-                switch (lm >> 6) {
-                case 0: acc0 += v; break;
-                case 1: acc1 += v; break;
-                case 2: acc2 += v; break;
-                default: acc3 += v; break;
-                }
-            }
-        });
+        if (e >= total) break;
+        const int i = (int)(e % nx);
+        const long r = e / nx;
+        const int j = (int)(r % ny);
+        const int kz = zb + (int)(r / ny);
+        const double val = A.rho[(x0 + i) + A.ldx * ((y0 + j) + A.ldy * (long)(z0 + kz))] *
+                           wx[i] * wy[j] * wz[kz];
+        multipole_point<LMAX, G>(px[x0 + i] - cx, py[y0 + j] - cy, pz[z0 + kz] - cz, val,
+                                 coef + (int)(base >> 62), acc);
     }
-    red[wave][lane] = acc0;
-    red[wave][64 + lane] = acc1;
-    red[wave][128 + lane] = acc2;
-    red[wave][192 + lane] = acc3;
+    // wavefront sums in a fixed butterfly order, then the 4 wavefronts in order
+#pragma unroll
+    for (int i = 0; i < plan.size[G]; ++i) {
+        double v = acc[i];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][i] = v;
+    }
     __syncthreads();
-    if (tid < A.N) {
+    if (tid < plan.size[G]) {
+        // lm of the tid-th value of group G (the packing of MpPlan, at run time)
+        int g = 0, fill = 0, lm = 0;
+        for (int mm = 0, k = 0; mm <= LMAX; ++mm) {
+            const int c = (mm ? 2 : 1) * (LMAX + 1 - mm);
+            if (fill + c > 64) {
+                ++g;
+                fill = 0;
+            }
+            fill += c;
+            if (g != G) continue;
+            for (int sg = 0; sg < (mm ? 2 : 1); ++sg)
+                for (int l = mm; l <= LMAX; ++l, ++k)
+                    if (k == tid) lm = l * l + l + (sg ? -mm : mm);
+        }
         const double s = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-        A.part[((long)blockIdx.y * A.nbox + box) * A.N + tid] = s;
+        A.part[((long)blockIdx.y * A.nbox + box) * N + lm] = s;
+    }
+}
+
+// q_lm(box) = sum over the box sub-grid of rho * wx wy wz * S_lm(r - centre)
+// (GBFMMCoulomb3DMultipoleEvaluator, integrator.cu:1122-1196).  Block
+// (box, z-slab, lm group): every thread keeps its group's <= 64 sums in
+// registers over its points and the sums are combined once at the end.
+template <int LMAX>
+__global__ void __launch_bounds__(256) box_multipoles_kernel(MultArgs A) {
+    constexpr MpPlan<LMAX> plan{};
+    constexpr int N = (LMAX + 1) * (LMAX + 1);
+    __shared__ double coef[2 * N + LMAX + 1];
+    __shared__ double red[4][64];
+    for (int i = threadIdx.x; i < 2 * N + LMAX + 1; i += 256) coef[i] = A.coef[i];
+    __syncthreads();
+    switch (blockIdx.z) {
+    case 0: box_multipoles_group<LMAX, 0>(A, coef, red); break;
+    case 1: if constexpr (plan.ngroups > 1) box_multipoles_group<LMAX, 1>(A, coef, red); break;
+    case 2: if constexpr (plan.ngroups > 2) box_multipoles_group<LMAX, 2>(A, coef, red); break;
+    case 3: if constexpr (plan.ngroups > 3) box_multipoles_group<LMAX, 3>(A, coef, red); break;
+    case 4: if constexpr (plan.ngroups > 4) box_multipoles_group<LMAX, 4>(A, coef, red); break;
+    default: break;
+    }
+}
+
+template <int LMAX>
+int launch_multipoles(const MultArgs &A, int nbox, hipStream_t s) {
+    constexpr MpPlan<LMAX> plan{};
+    static_assert(plan.ngroups <= 5, "multipole groups");
+    hipLaunchKernelGGL(box_multipoles_kernel<LMAX>, dim3(nbox, A.zsplit, plan.ngroups),
+                       dim3(256), 0, s, A);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int multipoles(int lmax, const MultArgs &A, int nbox, hipStream_t s) {
+    switch (lmax) {
+    case 0: return launch_multipoles<0>(A, nbox, s);
+    case 1: return launch_multipoles<1>(A, nbox, s);
+    case 2: return launch_multipoles<2>(A, nbox, s);
+    case 3: return launch_multipoles<3>(A, nbox, s);
+    case 4: return launch_multipoles<4>(A, nbox, s);
+    case 5: return launch_multipoles<5>(A, nbox, s);
+    case 6: return launch_multipoles<6>(A, nbox, s);
+    case 7: return launch_multipoles<7>(A, nbox, s);
+    case 8: return launch_multipoles<8>(A, nbox, s);
+    case 9: return launch_multipoles<9>(A, nbox, s);
+    case 10: return launch_multipoles<10>(A, nbox, s);
+    case 11: return launch_multipoles<11>(A, nbox, s);
+    case 12: return launch_multipoles<12>(A, nbox, s);
+    case 13: return launch_multipoles<13>(A, nbox, s);
+    case 14: return launch_multipoles<14>(A, nbox, s);
+    case 15: return launch_multipoles<15>(A, nbox, s);
+    default: set_error("lmax %d unsupported", lmax); return DAGE_ERR_UNSUPPORTED;
     }
 }
 
@@ -231,19 +374,35 @@ __global__ void __launch_bounds__(256) m2m_kernel(const double *W, const double 
 }
 
 // L(target) += sum_list T(offset) q(source)  (calculate_local_farfield_potential,
-// gbfmm_coulomb3d.F90:1461-1500).
+// gbfmm_coulomb3d.F90:1461-1500).  Block (target, chunk of the list): each
+// chunk's partial goes to part[chunk][target][lm]; m2l_sum_kernel adds the
+// chunks in order.
 __global__ void __launch_bounds__(256) m2l_kernel(const double *T, const int *start,
                                                   const int *src, const int *mat, const double *q,
-                                                  double *L, int N) {
-    const int t = blockIdx.x, i = threadIdx.x;
+                                                  double *part, int nbox, int N) {
+    const int t = blockIdx.x, c = blockIdx.y, nch = gridDim.y, i = threadIdx.x;
     if (i >= N) return;
+    const int e0 = start[t], e1 = start[t + 1];
+    const int per = (e1 - e0 + nch - 1) / nch;
+    const int eb = e0 + c * per, ee = min(e1, eb + per);
     double acc = 0.0;
-    for (int e = start[t]; e < start[t + 1]; ++e) {
+    for (int e = eb; e < ee; ++e) {
         const double *Te = T + (long)mat[e] * N * N;
         const double *qs = q + (long)src[e] * N;
         for (int s = 0; s < N; ++s) acc += Te[i + (long)N * s] * qs[s];
     }
-    L[(long)t * N + i] += acc;
+    part[((long)c * nbox + t) * N + i] = acc;
+}
+
+__global__ void __launch_bounds__(256) m2l_sum_kernel(const double *part, int nch, int nbox,
+                                                      int N, double *L) {
+    const long len = (long)nbox * N;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < len;
+         e += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < nch; ++c) s += part[(long)c * len + e];
+        L[e] += s;
+    }
 }
 
 // L(child) += W_octant^T L(parent)  (translate_potential, gbfmm_coulomb3d.F90:1562-1622).
@@ -305,20 +464,6 @@ __global__ void __launch_bounds__(256) local_expansion_kernel(LeArgs A) {
     }
 }
 
-__global__ void __launch_bounds__(256) gather_kernel(const double *src, long ldx, long ldy,
-                                                     int x0, int y0, int z0, int nx, int ny,
-                                                     int nz, double *dst) {
-    const long total = (long)nx * ny * nz;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-         e += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % nx);
-        const long r = e / nx;
-        const int j = (int)(r % ny);
-        const int k = (int)(r / ny);
-        dst[e] = src[(x0 + i) + ldx * ((y0 + j) + ldy * (long)(z0 + k))];
-    }
-}
-
 int upload(void **dst, const void *src, size_t bytes) {
     DAGE_HIP_CHECK(hipMalloc(dst, bytes ? bytes : 8));
     if (bytes) DAGE_HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
@@ -326,11 +471,8 @@ int upload(void **dst, const void *src, size_t bytes) {
 }
 
 void free_gbfmm(dage_gbfmm f) {
-    for (dage_operator op : f->ops)
-        if (op) dage_operator_destroy(op);
-    for (dage_grid3d g : f->grids)
-        if (g) dage_grid3d_destroy(g);
-    void *bufs[] = {f->d_nf, f->ws.p, f->d_q, f->d_L, f->d_part, f->d_boxvec, f->d_geo,
+    if (f->nf_op) dage_operator_destroy(f->nf_op);
+    void *bufs[] = {f->d_q, f->d_L, f->d_part, f->d_m2l_part, f->d_boxvec, f->d_geo,
                     f->d_wbox, f->d_points, f->d_centers, f->d_coef};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -365,40 +507,74 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
         }
         maxn = std::max(maxn, s * np1 + 1);
     }
-    // ---- near-field operators on sub-grids, one per box coordinate
-    size_t nf_max = 0;
-    for (int b = 0; b < nb; ++b) {
-        double q_nf[3], q_box[3];
-        int n_nf[3], n_box[3];
-        std::vector<double> st_nf[3], st_box[3];
-        for (int a = 0; a < 3; ++a) {
-            const Axis &ax = grid->axis[a];
-            const int s = f->cells_per_box[a];
-            const int c0 = std::max(0, b - 1) * s, c1 = std::min(nb, b + 2) * s;
-            q_nf[a] = ax.celld[c0];
-            n_nf[a] = c1 - c0;
-            st_nf[a].assign(ax.steps.begin() + c0, ax.steps.begin() + c1);
-            q_box[a] = ax.celld[b * s];
-            n_box[a] = s;
-            st_box[a].assign(ax.steps.begin() + b * s, ax.steps.begin() + (b + 1) * s);
+    // ---- near field: the sub-operator of each box coordinate (near-field
+    // sub-grid -> box sub-grid), assembled into banded whole-cube matrices:
+    // output row o (owned by box b under the cut rule) takes row o - box_lo
+    // of box b's sub-operator at columns nf_lo(b) + j.
+    {
+        const int T = ntp;
+        const int n0 = f->npt[0], n1 = f->npt[1], n2 = f->npt[2];
+        std::vector<double> FX((size_t)n0 * n0 * T, 0.0), FY((size_t)n1 * n1 * T, 0.0),
+            FZ((size_t)n2 * n2 * T, 0.0);
+        for (int b = 0; b < nb; ++b) {
+            double q_nf[3], q_box[3];
+            int n_nf[3], n_box[3];
+            std::vector<double> st_nf[3], st_box[3];
+            for (int a = 0; a < 3; ++a) {
+                const Axis &ax = grid->axis[a];
+                const int s = f->cells_per_box[a];
+                const int c0 = std::max(0, b - 1) * s, c1 = std::min(nb, b + 2) * s;
+                q_nf[a] = ax.celld[c0];
+                n_nf[a] = c1 - c0;
+                st_nf[a].assign(ax.steps.begin() + c0, ax.steps.begin() + c1);
+                q_box[a] = ax.celld[b * s];
+                n_box[a] = s;
+                st_box[a].assign(ax.steps.begin() + b * s, ax.steps.begin() + (b + 1) * s);
+            }
+            const int bn[3] = {f->box_n[0][b], f->box_n[1][b], f->box_n[2][b]};
+            const int nn[3] = {f->nf_n[0][b], f->nf_n[1][b], f->nf_n[2][b]};
+            std::vector<double> fx((size_t)bn[0] * nn[0] * T), fy((size_t)nn[1] * bn[1] * T),
+                fz((size_t)nn[2] * bn[2] * T);
+            dage_grid3d gnf = nullptr, gbox = nullptr;
+            dage_operator op = nullptr;
+            int rc = dage_grid3d_create(f->ctx, q_nf, n_nf, f->nlip, st_nf[0].data(),
+                                        st_nf[1].data(), st_nf[2].data(), grid->axis[0].grid_type,
+                                        &gnf);
+            if (rc == DAGE_OK)
+                rc = dage_grid3d_create(f->ctx, q_box, n_box, f->nlip, st_box[0].data(),
+                                        st_box[1].data(), st_box[2].data(),
+                                        grid->axis[0].grid_type, &gbox);
+            if (rc == DAGE_OK)
+                rc = dage_coulomb3d_create(f->ctx, gnf, gbox, tp, tw, ntp, 1.0, DAGE_OP_NO_CODA,
+                                           &op);
+            if (rc == DAGE_OK) rc = dage_operator_get_matrices(op, fx.data(), fy.data(), fz.data());
+            if (op) dage_operator_destroy(op);
+            if (gbox) dage_grid3d_destroy(gbox);
+            if (gnf) dage_grid3d_destroy(gnf);
+            if (rc != DAGE_OK) return rc;
+            const int cx = f->box_cut[0][b], cy = f->box_cut[1][b], cz = f->box_cut[2][b];
+            for (int t = 0; t < T; ++t) {
+                for (int xi = 0; xi < nn[0]; ++xi)
+                    for (int xo = 0; xo < cx; ++xo)
+                        FX[(size_t)(f->box_lo[0][b] + xo) +
+                           (size_t)n0 * ((f->nf_lo[0][b] + xi) + (size_t)n0 * t)] =
+                            fx[(size_t)xo + (size_t)bn[0] * (xi + (size_t)nn[0] * t)];
+                for (int yo = 0; yo < cy; ++yo)
+                    for (int yi = 0; yi < nn[1]; ++yi)
+                        FY[(size_t)(f->nf_lo[1][b] + yi) +
+                           (size_t)n1 * ((f->box_lo[1][b] + yo) + (size_t)n1 * t)] =
+                            fy[(size_t)yi + (size_t)nn[1] * (yo + (size_t)bn[1] * t)];
+                for (int zo = 0; zo < cz; ++zo)
+                    for (int zi = 0; zi < nn[2]; ++zi)
+                        FZ[(size_t)(f->nf_lo[2][b] + zi) +
+                           (size_t)n2 * ((f->box_lo[2][b] + zo) + (size_t)n2 * t)] =
+                            fz[(size_t)zi + (size_t)nn[2] * (zo + (size_t)bn[2] * t)];
+            }
         }
-        dage_grid3d gnf = nullptr, gbox = nullptr;
-        DAGE_TRY(dage_grid3d_create(f->ctx, q_nf, n_nf, f->nlip, st_nf[0].data(), st_nf[1].data(),
-                                    st_nf[2].data(), grid->axis[0].grid_type, &gnf));
-        f->grids.push_back(gnf);
-        DAGE_TRY(dage_grid3d_create(f->ctx, q_box, n_box, f->nlip, st_box[0].data(),
-                                    st_box[1].data(), st_box[2].data(), grid->axis[0].grid_type,
-                                    &gbox));
-        f->grids.push_back(gbox);
-        dage_operator op = nullptr;
-        DAGE_TRY(dage_coulomb3d_create(f->ctx, gnf, gbox, tp, tw, ntp, 1.0, DAGE_OP_NO_CODA, &op));
-        f->ops.push_back(op);
+        DAGE_TRY(dage_coulomb3d_create(f->ctx, grid, grid, tp, tw, ntp, 1.0, DAGE_OP_NO_CODA,
+                                       &f->nf_op));
+        DAGE_TRY(dage_operator_set_matrices(f->nf_op, FX.data(), FY.data(), FZ.data()));
     }
-    for (int bx = 0; bx < nb; ++bx)
-        for (int by = 0; by < nb; ++by)
-            for (int bz = 0; bz < nb; ++bz)
-                nf_max = std::max(nf_max, (size_t)f->nf_n[0][bx] * f->nf_n[1][by] * f->nf_n[2][bz]);
-    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_nf, 8 * nf_max));
 
     // ---- box numbering and centres (get_box_vector, get_box_center)
     const int nbox_max = nb * nb * nb;
@@ -467,6 +643,7 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_L, 8 * (size_t)N * f->nbox_total));
     f->zsplit = std::max(1, std::min(maxn, (2048 + nbox_max - 1) / nbox_max));
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_part, 8 * (size_t)N * nbox_max * f->zsplit));
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_part, 8 * (size_t)N * nbox_max * f->m2l_chunks));
 
     // ---- far-field matrices (init_interaction_matrices, get_local_farfield_indices)
     const FmmTables &tab = fmm_tables(lmax);
@@ -536,41 +713,17 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     return DAGE_OK;
 }
 
-int gbfmm_apply_f64(dage_gbfmm f, const double *rho, double *pot) {
+int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
     dage_ctx ctx = f->ctx;
     hipStream_t s = ctx->stream;
     const int nb = f->nb, N = f->N, L = f->maxlevel;
     const long ldx = f->npt[0], ldy = f->npt[1];
     hipEvent_t ev;
-    // near field: every box writes its own (cut) points
-    for (int box = 0; box < nb * nb * nb; ++box) {
-        const int bx = f->boxvec[3 * box], by = f->boxvec[3 * box + 1], bz = f->boxvec[3 * box + 2];
-        const int nx = f->nf_n[0][bx], ny = f->nf_n[1][by], nz = f->nf_n[2][bz];
-        const long tot = (long)nx * ny * nz;
-        prof_begin(ctx, "gather_kernel", &ev);
-        hipLaunchKernelGGL(gather_kernel, dim3((int)std::min<long>((tot + 255) / 256, 65536)),
-                           dim3(256), 0, s, rho, ldx, ldy, f->nf_lo[0][bx], f->nf_lo[1][by],
-                           f->nf_lo[2][bz], nx, ny, nz, f->d_nf);
-        DAGE_HIP_CHECK(hipGetLastError());
-        prof_end(ctx, "gather_kernel", ev);
-        ApplyMats m;
-        m.fx = f->ops[bx]->d_fx;
-        m.fy = f->ops[by]->d_fy;
-        m.gz = f->ops[bz]->d_gz;
-        m.w = f->ops[0]->d_w;
-        m.din[0] = nx; m.din[1] = ny; m.din[2] = nz;
-        m.dout[0] = f->box_n[0][bx]; m.dout[1] = f->box_n[1][by]; m.dout[2] = f->box_n[2][bz];
-        m.ntp = f->ops[0]->ntp;
-        OutView ov;
-        ov.p = pot + f->box_lo[0][bx] + ldx * (f->box_lo[1][by] + ldy * (long)f->box_lo[2][bz]);
-        ov.ldx = ldx;
-        ov.ldy = ldy;
-        ov.n[0] = f->box_cut[0][bx]; ov.n[1] = f->box_cut[1][by]; ov.n[2] = f->box_cut[2][bz];
-        DAGE_TRY(apply_f64_view(ctx, m, f->d_nf, ov, 0.0, &f->ws));
-    }
+    // near field of every box: one whole-cube apply with the banded matrices
+    if (precision == DAGE_PREC_FP64) DAGE_TRY(apply_f64(f->nf_op, rho, pot));
+    else DAGE_TRY(apply_mixed(f->nf_op, rho, pot, precision == DAGE_PREC_BF16X6 ? 6 : 3));
     // multipoles of the finest boxes
     const int nbox_max = nb * nb * nb;
-    DAGE_TRY(fill_zero(ctx, f->d_q, (size_t)N * f->nbox_total));
     DAGE_TRY(fill_zero(ctx, f->d_L, (size_t)N * f->nbox_total));
     MultArgs ma;
     ma.rho = rho; ma.ldx = ldx; ma.ldy = ldy;
@@ -578,11 +731,10 @@ int gbfmm_apply_f64(dage_gbfmm f, const double *rho, double *pot) {
     ma.wbox = f->d_wbox; ma.wbox_ld = f->wbox_ld;
     ma.points = f->d_points; ma.points_ld = f->points_ld;
     ma.centers = f->d_centers; ma.coef = f->d_coef;
-    ma.lmax = f->lmax; ma.N = N; ma.zsplit = f->zsplit; ma.nbox = nbox_max;
+    ma.zsplit = f->zsplit; ma.nbox = nbox_max;
     ma.part = f->d_part;
     prof_begin(ctx, "box_multipoles_kernel", &ev);
-    hipLaunchKernelGGL(box_multipoles_kernel, dim3(nbox_max, f->zsplit), dim3(256), 0, s, ma);
-    DAGE_HIP_CHECK(hipGetLastError());
+    DAGE_TRY(multipoles(f->lmax, ma, nbox_max, s));
     prof_end(ctx, "box_multipoles_kernel", ev);
     const long qlen = (long)N * nbox_max;
     hipLaunchKernelGGL(sum_parts_kernel, dim3((int)std::min<long>((qlen + 255) / 256, 65536)),
@@ -600,19 +752,23 @@ int gbfmm_apply_f64(dage_gbfmm f, const double *rho, double *pot) {
     // far field per level, translated down
     for (int l = f->start_level; l <= L; ++l) {
         const int nbox = 1 << (3 * l);
+        double *Ll = f->d_L + (size_t)N * f->level_offset[l];
         if (l > f->start_level) {
             prof_begin(ctx, "l2l_kernel", &ev);
             hipLaunchKernelGGL(l2l_kernel, dim3(nbox), dim3(256), 0, s, f->d_W[l],
-                               f->d_L + (size_t)N * f->level_offset[l - 1],
-                               f->d_L + (size_t)N * f->level_offset[l], N);
+                               f->d_L + (size_t)N * f->level_offset[l - 1], Ll, N);
             DAGE_HIP_CHECK(hipGetLastError());
             prof_end(ctx, "l2l_kernel", ev);
         }
         const int *lst = f->d_list[l];
         prof_begin(ctx, "m2l_kernel", &ev);
-        hipLaunchKernelGGL(m2l_kernel, dim3(nbox), dim3(256), 0, s, f->d_T[l], lst,
+        hipLaunchKernelGGL(m2l_kernel, dim3(nbox, f->m2l_chunks), dim3(256), 0, s, f->d_T[l], lst,
                            lst + nbox + 1, lst + nbox + 1 + f->list_len[l], f->d_q,
-                           f->d_L + (size_t)N * f->level_offset[l], N);
+                           f->d_m2l_part, nbox, N);
+        DAGE_HIP_CHECK(hipGetLastError());
+        const long len = (long)nbox * N;
+        hipLaunchKernelGGL(m2l_sum_kernel, dim3((int)std::min<long>((len + 255) / 256, 65536)),
+                           dim3(256), 0, s, f->d_m2l_part, f->m2l_chunks, nbox, N, Ll);
         DAGE_HIP_CHECK(hipGetLastError());
         prof_end(ctx, "m2l_kernel", ev);
     }
@@ -681,10 +837,11 @@ int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax, co
 
 int dage_gbfmm_apply_device(dage_gbfmm f, const double *rho, double *pot, int precision) {
     DAGE_REQUIRE(f && rho && pot, DAGE_ERR_ARG, "null argument");
-    DAGE_REQUIRE(precision == DAGE_PREC_FP64, DAGE_ERR_UNSUPPORTED,
-                 "GBFMM runs in FP64 (precision %d requested)", precision);
+    DAGE_REQUIRE(precision == DAGE_PREC_FP64 || precision == DAGE_PREC_BF16X6 ||
+                     precision == DAGE_PREC_BF16X3,
+                 DAGE_ERR_ARG, "unknown precision mode %d", precision);
     DAGE_TRY(use_device(f->ctx));
-    return gbfmm_apply_f64(f, rho, pot);
+    return gbfmm_apply(f, rho, pot, precision);
 }
 
 int dage_gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {

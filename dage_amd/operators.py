@@ -420,20 +420,21 @@ class GBFMMCoulomb3D:
         except Exception:
             pass
 
-    def apply(self, rho: np.ndarray) -> np.ndarray:
-        """Potential of the density cube rho (host arrays, Fortran order)."""
+    def apply(self, rho: np.ndarray, precision: int = DAGE_PREC_FP64) -> np.ndarray:
+        """Potential of the density cube rho (host arrays, Fortran order).
+        precision selects the near-field arithmetic; the far field is FP64."""
         shape = self.grid.get_shape()
         rho = np.asfortranarray(rho, dtype=np.float64)
         if rho.shape != shape:
             raise ValueError(f"density shape {rho.shape} != grid shape {shape}")
         pot = np.zeros(shape, order="F")
         call("dage_gbfmm_apply", self.handle, rho.ctypes.data_as(_dp), pot.ctypes.data_as(_dp),
-             DAGE_PREC_FP64)
+             int(precision))
         return pot
 
-    def apply_device(self, rho: "DeviceCube", pot: "DeviceCube"):
+    def apply_device(self, rho: "DeviceCube", pot: "DeviceCube", precision: int = DAGE_PREC_FP64):
         call("dage_gbfmm_apply_device", self.handle, ctypes.c_void_p(rho.ptr),
-             ctypes.c_void_p(pot.ptr), DAGE_PREC_FP64)
+             ctypes.c_void_p(pot.ptr), int(precision))
 
     def multipole_moments(self) -> np.ndarray:
         q = np.zeros(((self.lmax + 1) ** 2, self.nbox_total), order="F")

@@ -232,7 +232,9 @@ int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax,
                       dage_gbfmm *gbfmm);
 /* GBFMMCoulomb3D_calculate_potential, cube part (gbfmm_coulomb3d.F90:650-821):
  * pot = near field + local expansions, on the grid's points (device buffers,
- * asynchronous on the context stream).  precision: DAGE_PREC_FP64. */
+ * asynchronous on the context stream).  precision selects the near-field
+ * arithmetic (DAGE_PREC_*); multipoles, translations and local expansions
+ * are always FP64. */
 int dage_gbfmm_apply_device(dage_gbfmm gbfmm, const double *rho_dev,
                             double *pot_dev, int precision);
 /* Host-buffer variant (uploads, applies, downloads, synchronises). */

@@ -104,3 +104,19 @@ def test_gbfmm_rejects_bad_arguments(ctx):
         dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=1, lmax=4)
     with pytest.raises(dage_amd.DageError):
         dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=16)
+
+
+def test_gbfmm_mixed_near_field(ctx):
+    """Split-BF16 near field (bf16x6) against the FP64 GBFMM: 1e-7 of max|V|."""
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 7, 12)
+    rng = np.random.default_rng(31)
+    q_, a, c = random_gaussians(rng, 4, box=2.0)
+    rho = O.gauss_density(q_, a, c, ax.points, ax.points, ax.points)
+    g = grid_of(ctx, [ax] * 3)
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=8)
+    v64 = fmm.apply(rho)
+    v6 = fmm.apply(rho, precision=dage_amd.DAGE_PREC_BF16X6)
+    v3 = fmm.apply(rho, precision=dage_amd.DAGE_PREC_BF16X3)
+    print(f"GBFMM near field bf16x6 {rel_maxdiff(v6, v64):.2e}, bf16x3 {rel_maxdiff(v3, v64):.2e}")
+    assert rel_maxdiff(v6, v64) < 1e-7
+    assert rel_maxdiff(v3, v64) < 2e-5
