@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the FP64 tensor-quadrature Coulomb3D apply on one MI355X.
+"""Benchmarks of the operator-application hot path on one MI355X.
 
-Workload (the reference's config 1 shape, which fits one GPU): density = sum
-of 4 normalised Gaussians (exponents U[0.5,5] bohr^-2, centres U[-2,2]^3) on a
-97^3 cube over [-8,8]^3 built from 7-point LIP cells (16 cells, h = 1/6);
-40-point t-quadrature (20 Gauss-Legendre points on [0,2] + 20 logarithmic on
-[2,500]); coda on.  One step = one apply of the operator to the resident cube
-(Operator3D_transform_cube, src/bubbles/function3d.F90:4230-4349) with input
-and output already in HBM.
+Headline (config 2 shapes): one step = the density of 16 Gaussians
+(exponents log-uniform in [0.3, 30], centres in [-4,4]^3) plus 4 atoms'
+bubbles (Z in 1..8, 199-point radial grids, lmax 4, c_lm r^l exp(-Z r))
+injected onto a 241^3 cube over [-12,12]^3 (nlip 7, 40 cells), then its
+Coulomb potential by the grid-based FMM (4x4x4 boxes, lmax 12, near field
+with GaussQuad(25, 25), T = 50), all in FP64 with the cubes resident in HBM:
+  Bubbles_eval_3dgrid / bubbles_inject_cuda  (bubbles.F90:2148-2211)
+  GBFMMCoulomb3D_calculate_potential         (gbfmm_coulomb3d.F90:650-821)
+`value` = 241^3 output points per second of wall time per step.
 
-Prints one JSON line: throughput in Gpts/s (output points per second), the
-roofline entry of the dominant kernel (algorithmic FP64 FLOPs per launch over
-its event-timed average duration, against the 78.6 TF FP64 MFMA peak), the
-error against the CPU oracle and the analytic potential, and the CPU baseline
-(the oracle's single-threaded FP64 apply on this host).
+Also reported: config 1 (plain Coulomb3D apply, 97^3, T = 40, FP64 and
+split-BF16, CPU oracle timed beside it) and config 3 (32 Helmholtz
+right-hand sides, k^2 in [0.1, 4], cubes of 65/97/129/161 points at h 0.15,
+one graph launch per batch, autotuned tiles, against 32 separate FP64
+applies).
+
+Prints one JSON line.  The roofline entry is the dominant kernel of the
+headline step: algorithmic FLOPs per launch over its event-timed average
+duration, against the 78.6 TF FP64 MFMA peak; `traffic` is the HBM bytes per
+launch from the committed rocprofv3 counter summary.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -36,28 +42,27 @@ FLOP_PER_POINT_BUBBLE = 2 * 49 + 2 * 25 * 7 + 75 + 2 * 25
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-reps", type=int, default=2)
-    p.add_argument("--no-inject", dest="inject", action="store_false",
-                   help="skip the bubbles-inject measurement")
-    p.add_argument("--no-gbfmm", dest="gbfmm", action="store_false",
-                   help="skip the GBFMM measurement")
+    p.add_argument("--no-config1", dest="config1", action="store_false",
+                   help="skip the config-1 apply measurement")
+    p.add_argument("--no-config3", dest="config3", action="store_false",
+                   help="skip the config-3 batched Helmholtz measurement")
     p.add_argument("--copy-calibration", action="store_true",
                    help="time a plain device copy of a 257^3 cube instead (the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (profiles/hbm_traffic.json: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
-    factors from the copy calibration), or None."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 counter summary (profiles/hbm_traffic.json: FETCH_SIZE x 2.0 +
+    WRITE_SIZE x 1.0, factors from the copy calibration), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)["kernels"][kernel]["hbm_bytes"]
+            return json.load(f)["workloads"][workload]["kernels"][kernel]["hbm_bytes"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -85,180 +90,28 @@ def copy_calibration(args):
         obj.close()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the apply would shard).", file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration(args)
-        return
-
+def rel(a, b):
     import numpy as np
-
-    import dage_amd
-    import oracle as O
-
-    n_cell, nlip, T = 16, 7, 40
-    ax = O.Axis.ncell_fixed(-8.0, 8.0, nlip, n_cell)
-    N = ax.n
-    rng = np.random.default_rng(0xDA6E0001)
-    alpha = rng.uniform(0.5, 5.0, 4)
-    centers = rng.uniform(-2.0, 2.0, (4, 3))
-    q = np.ones(4)
-    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
-
-    # the library's own context stream carries the kernels and the timing events
-    ctx = dage_amd.Context(0)
-    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, nlip)
-    quad = dage_amd.GaussQuad(nlin=20, nlog=20, tlin=2.0, tlog=500.0)
-    op = dage_amd.Coulomb3D(ctx, grid, grid, quad)
-
-    d_in = dage_amd.DeviceCube(ctx, rho.shape, rho)
-    d_out = dage_amd.DeviceCube(ctx, rho.shape)
-
-    for _ in range(args.warmup):
-        op.transform_cuda_cube(d_in, d_out)
-    ctx.synchronize()
-
-    ctx.profile(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.time_begin()
-    for _ in range(args.steps):
-        op.transform_cuda_cube(d_in, d_out)
-    ms_total = ctx.time_end()
-    wall = time.perf_counter() - t0
-    ms_step = ms_total / args.steps
-
-    # mixed precision (split-BF16, 6 products) on the same operator and cube
-    d_mix = dage_amd.DeviceCube(ctx, rho.shape)
-    mixed = {}
-    for mode, label in ((dage_amd.DAGE_PREC_BF16X6, "bf16x6"), (dage_amd.DAGE_PREC_BF16X3, "bf16x3")):
-        for _ in range(args.warmup):
-            op.transform_cuda_cube(d_in, d_mix, precision=mode)
-        ctx.time_begin()
-        for _ in range(args.steps):
-            op.transform_cuda_cube(d_in, d_mix, precision=mode)
-        mms = ctx.time_end() / args.steps
-        mixed[label] = (mms, d_mix.download())
-
-    kernels = {}
-    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
-                 "gemm_mixed_xmode", "gemm_mixed_ymode", "gemm_mixed_zmode"):
-        tot, n = ctx.profile_query(name)
-        kernels[name] = (tot / max(n, 1), n // args.steps)
-    ctx.profile(False)
-
-    V = d_out.download()
-    pts = float(N) ** 3
-    gpts = pts / (ms_step * 1e-3) / 1e9
-    flop_mode = 2.0 * T * float(N) ** 4  # each mode product, per apply
-    flops_apply = 3.0 * flop_mode
-
-    # dominant kernel: the FP64 GEMM pass with the largest time per apply
-    per_apply = {k: v[0] * v[1] for k, v in kernels.items() if k.startswith("gemm_f64")}
-    dom = max(per_apply, key=per_apply.get)
-    dom_ms, dom_launches = kernels[dom]
-    dom_flops = flop_mode / max(dom_launches, 1)
-    achieved_tf = dom_flops / (dom_ms * 1e-3) / 1e12
-    roofline = {
-        "bound": "mfma",
-        "kernel": dom,
-        "achieved": round(achieved_tf, 3),
-        "peak": FP64_MFMA_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved_tf / FP64_MFMA_PEAK_TFLOPS, 4),
-        "traffic": measured_traffic(dom),
-        "avg_launch_ms": round(dom_ms, 4),
-        "algorithmic_flops_per_launch": dom_flops,
-    }
-
-    # parity against the oracle (same operator, CPU) and the analytic potential
-    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, quad.tp)
-    w = O.coulomb_weights(quad.tw)
-    cpu_times = []
-    reps = 1 if args.no_cpu_baseline else max(1, args.cpu_reps)
-    ref = None
-    for _ in range(reps):
-        c0 = time.perf_counter()
-        ref = O.transform_cube(fx, fy, fz, w, O.coda(500.0), rho)
-        cpu_times.append(time.perf_counter() - c0)
-    err_oracle = float(np.max(np.abs(V - ref)) / np.max(np.abs(ref)))
-    Va = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
-    err_analytic = float(np.max(np.abs(V - Va)) / np.max(np.abs(Va)))
-    cpu_s = min(cpu_times)
-    cpu_baseline = None
-    if not args.no_cpu_baseline:
-        cpu_baseline = {
-            "value": round(pts / cpu_s / 1e9, 6),
-            "unit": "Gpts/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"full 97^3 T=40 apply with the oracle's C restatement of "
-                      f"Operator3D_transform_cube (1 thread, best of {reps}: {cpu_s:.2f} s)",
-        }
-
-    line = {
-        "metric": METRIC,
-        "value": round(gpts, 4),
-        "unit": "Gpts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "config 1: plain Coulomb3D apply, FP64, 97^3 cube over [-8,8]^3, "
-                        "nlip 7, T=40 (20 lin [0,2] + 20 log [2,500]), 4 Gaussians, coda on",
-            "n": N, "T": T, "nlip": nlip, "precision": "fp64",
-        },
-        "roofline": roofline,
-        "cpu_baseline": cpu_baseline,
-        "apply_tflops": round(flops_apply / (ms_step * 1e-3) / 1e12, 3),
-        "apply_mfma_frac": round(flops_apply / (ms_step * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4),
-        "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kernels.items() if v[1] > 0},
-        "mixed": {label: {"ms_per_apply": round(mms, 4),
-                          "Gpts_s": round(pts / (mms * 1e-3) / 1e9, 4),
-                          "speedup_vs_fp64": round(ms_step / mms, 3),
-                          "dense_fp64_equiv_tflops": round(flops_apply / (mms * 1e-3) / 1e12, 2),
-                          "max_rel_err_vs_oracle": float(np.max(np.abs(vm - ref)) / np.max(np.abs(ref)))}
-                  for label, (mms, vm) in mixed.items()},
-        "max_rel_err_vs_oracle": err_oracle,
-        "max_rel_err_vs_analytic": err_analytic,
-        "wall_s_timed": round(wall, 4),
-    }
-    for obj in (d_in, d_out, d_mix, op, grid):
-        obj.close()
-    if args.inject:
-        line["inject"] = bench_inject(ctx, args)
-    if args.gbfmm:
-        line["gbfmm"] = bench_gbfmm(ctx, args)
-    print(json.dumps(line), flush=True)
-    ctx.close()
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def bench_inject(ctx, args):
-    """Bubbles inject at the config-2 shape: 241^3 cube over [-12,12]^3,
-    4 atoms (Z in 1..8, centres in [-4,4]^3), 199-point radial grids to 20
-    bohr, lmax 4, f_lm = c_lm r^l exp(-Z r).  One fused launch per step."""
+# ----------------------------------------------------------------- config 2
+def config2_inputs(ctx):
     import numpy as np
 
     import dage_amd
     import oracle as O
     rng = np.random.default_rng(0xDA6E0002)
     ax = O.Axis.ncell_fixed(-12.0, 12.0, 7, 40)
-    n = ax.n
+    q = np.ones(16)
+    alpha = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 16))
+    centers = rng.uniform(-4.0, 4.0, (16, 3))
+    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
     grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
     lmax, natom = 4, 4
     bub = dage_amd.Bubbles(ctx, natom, lmax, 0, 7)
     Z = rng.integers(1, 9, natom).astype(float)
-    centers = rng.uniform(-4.0, 4.0, (natom, 3))
+    acent = rng.uniform(-4.0, 4.0, (natom, 3))
     host = []
     for a in range(natom):
         st = dage_amd.radial_grid_steps(Z[a], 33, 7, 20.0)
@@ -268,143 +121,358 @@ def bench_inject(ctx, args):
         for l in range(lmax + 1):
             for m in range(-l, l + 1):
                 bf[:, l * l + l + m] = c[l * l + l + m] * r ** l * np.exp(-Z[a] * r)
-        bub.set_bubble(a, centers[a], Z[a], st, bf)
+        bub.set_bubble(a, acent[a], Z[a], st, bf)
         host.append((st, bf))
-    cube = dage_amd.DeviceCube(ctx, (n, n, n))
-    cube.set_to_zero()
-    for _ in range(max(1, args.warmup)):
-        bub.inject(grid, cube)
-    steps = max(3, args.steps // 2)
-    ctx.profile(True)
-    ctx.time_begin()
-    for _ in range(steps):
-        bub.inject(grid, cube)
-    ms = ctx.time_end() / steps
-    kms, kn = ctx.profile_query("bubbles_inject_kernel")
-    ctx.profile(False)
-    kernel_ms = kms / max(kn, 1)
-    # parity on a slab of the cube: z-planes 0..7 through the oracle
-    cube.set_to_zero()
-    bub.inject(grid, cube)
-    got = cube.download()
-    bset = O.BubbleSet(centers, [h[0] for h in host], [h[1] for h in host], lmax, 0, 7)
-    ref = O.bubbles_eval_grid(bset, ax.points, ax.points, ax.points[:8])
-    err = float(np.max(np.abs(got[:, :, :8] - ref)) / np.max(np.abs(ref)))
-    nbytes = 16.0 * n ** 3  # cube read-modify-write per launch
-    for obj in (cube, bub, grid):
-        obj.close()
-    return {
-        "workload": "config 2 bubbles: 241^3 cube over [-12,12]^3, 4 atoms, lmax 4, "
-                    "199-point radial grids, fused all-atom inject",
-        "ms_per_launch": round(kernel_ms, 4),
-        "Gpts_s": round(n ** 3 / (ms * 1e-3) / 1e9, 3),
-        "roofline": {"bound": "hbm", "achieved": round(nbytes / (kernel_ms * 1e-3) / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(nbytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("bubbles_inject_kernel")},
-        "fp64_valu_tflops": round(FLOP_PER_POINT_BUBBLE * natom * n ** 3 / (kernel_ms * 1e-3) / 1e12, 2),
-        "max_rel_err_vs_oracle_slab": err,
-    }
+    bset = O.BubbleSet(acent, [h[0] for h in host], [h[1] for h in host], lmax, 0, 7)
+    return ax, (q, alpha, centers), rho, grid, bub, bset
 
 
-def bench_gbfmm(ctx, args):
-    """GBFMM Coulomb potential at the config-2 shape: 241^3 cube over
-    [-12,12]^3 (nlip 7, 40 cells), 4x4x4 boxes (maxlevel 2, 10 cells = 61
-    points per box), lmax 12, near field with GaussQuad(25, 25) (T = 50);
-    density = 16 Gaussians, exponents log-uniform in [0.3, 30], centres in
-    [-4,4]^3.  Compared with the analytic potential and with the direct
-    FP64 Coulomb apply of the same quadrature on the whole cube."""
+def bench_config2(ctx, args):
     import numpy as np
 
     import dage_amd
     import oracle as O
-    rng = np.random.default_rng(0xDA6E0002)
-    ax = O.Axis.ncell_fixed(-12.0, 12.0, 7, 40)
+    ax, gauss, rho, grid, bub, bset = config2_inputs(ctx)
     n = ax.n
-    q = np.ones(16)
-    alpha = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 16))
-    centers = rng.uniform(-4.0, 4.0, (16, 3))
-    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
-    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
     gq = dage_amd.GaussQuad(25, 25)
     fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=2, lmax=12, gauss=gq)
+    d_rho = dage_amd.DeviceCube(ctx, rho.shape, rho)
+    d_tot = dage_amd.DeviceCube(ctx, rho.shape)
+    d_pot = dage_amd.DeviceCube(ctx, rho.shape)
+
+    def step(precision=dage_amd.DAGE_PREC_FP64):
+        d_tot.copy_from(d_rho)
+        bub.inject(grid, d_tot)
+        fmm.apply_device(d_tot, d_pot, precision)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.profile(True)
+    wall0 = time.perf_counter()
+    ctx.time_begin()
+    for _ in range(args.steps):
+        step()
+    ms = ctx.time_end() / args.steps
+    wall = time.perf_counter() - wall0
+    names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+             "bubbles_inject_kernel", "box_multipoles_kernel", "m2m_kernel", "m2l_kernel",
+             "l2l_kernel", "local_expansion_kernel")
+    kernels = {}
+    for k in names:
+        kms, kn = ctx.profile_query(k)
+        if kn:
+            kernels[k] = {"ms_per_launch": round(kms / kn, 4), "launches_per_step": kn // args.steps,
+                          "ms_per_step": round(kms / args.steps, 4)}
+    ctx.profile(False)
+    T = gq.get_npoints()
+    # dominant kernel: largest time per step; GEMM passes do 2 T n^4 dense
+    # FLOPs per apply (the banded near-field matrices are applied densely)
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dom]
+    if dom.startswith("gemm_f64"):
+        per_launch = 2.0 * T * float(n) ** 4 / dk["launches_per_step"]
+        achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
+                    "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                    "traffic": measured_traffic("config2", dom),
+                    "avg_launch_ms": dk["ms_per_launch"], "algorithmic_flops_per_launch": per_launch}
+    else:
+        per_launch = 16.0 * n ** 3
+        achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic("config2", dom),
+                    "avg_launch_ms": dk["ms_per_launch"], "algorithmic_bytes_per_launch": per_launch}
+    # split-BF16 near field in the same step
+    step(dage_amd.DAGE_PREC_BF16X6)
+    ctx.time_begin()
+    for _ in range(args.steps):
+        step(dage_amd.DAGE_PREC_BF16X6)
+    ms_mixed = ctx.time_end() / args.steps
+    pot_mixed = d_pot.download()
+    step()
+    pot = d_pot.download()
+    # parity: GBFMM of the Gaussians alone vs the analytic potential and vs
+    # the direct apply; the inject on 8 z-planes vs the oracle
+    fmm.apply_device(d_rho, d_pot)
+    pot_g = d_pot.download()
+    q, alpha, centers = gauss
+    exact = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
+    op = dage_amd.Coulomb3D(ctx, grid, grid, gq, coda=False)
+    op.transform_cuda_cube(d_rho, d_pot)
+    ctx.time_begin()
+    op.transform_cuda_cube(d_rho, d_pot)
+    direct_ms = ctx.time_end()
+    direct = d_pot.download()
+    d_tot.set_to_zero()
+    bub.inject(grid, d_tot)
+    inj = d_tot.download()
+    inj_ref = O.bubbles_eval_grid(bset, ax.points, ax.points, ax.points[:8])
+    out = {
+        "ms_per_step": ms, "wall_s": wall, "n": n, "T": T, "kernels": kernels,
+        "roofline": roofline,
+        "bf16x6_nearfield": {"ms_per_step": round(ms_mixed, 3), "speedup": round(ms / ms_mixed, 3),
+                             "max_rel_diff_vs_fp64": rel(pot_mixed, pot)},
+        "gbfmm_vs_analytic": rel(pot_g, exact),
+        "gbfmm_vs_direct_apply": rel(pot_g, direct),
+        "direct_apply_ms": round(direct_ms, 3),
+        "inject_vs_oracle_8_planes": rel(inj[:, :, :8], inj_ref),
+        "inject_valu_fp64_tflops": round(FLOP_PER_POINT_BUBBLE * 4 * n ** 3 /
+                                         (kernels["bubbles_inject_kernel"]["ms_per_launch"] * 1e-3)
+                                         / 1e12, 2),
+    }
+    # CPU baseline: the oracle's near field of one interior box (181^3 ->
+    # 61^3, all 50 t-points) and its inject on 8 z-planes, scaled by work to
+    # the whole step
+    if not args.no_cpu_baseline:
+        s = 10
+        sub_in = O.sub_axis(ax, 0, 3 * s)
+        sub_out = O.sub_axis(ax, s, 2 * s)
+        mats = O.coulomb3d_matrices([sub_in] * 3, [sub_out] * 3, gq.tp)
+        nf = rho[:sub_in.n, :sub_in.n, :sub_in.n]
+        c0 = time.perf_counter()
+        O.transform_cube(*mats, O.coulomb_weights(gq.tw), 0.0, nf, add_coda=False)
+        t_box = time.perf_counter() - c0
+        c0 = time.perf_counter()
+        O.bubbles_eval_grid(bset, ax.points, ax.points, ax.points[:8])
+        t_inj = time.perf_counter() - c0
+
+        def box_flops(nx, ny, nz, no=61):
+            return 2 * T * (no * nx * ny * nz + no * no * ny * nz + no ** 3 * nz)
+        nfs = [121, 181, 181, 121]
+        total = sum(box_flops(nfs[x], nfs[y], nfs[z])
+                    for x in range(4) for y in range(4) for z in range(4))
+        est = t_box * total / box_flops(181, 181, 181) + t_inj * n / 8
+        out["cpu_baseline"] = {
+            "value": round(n ** 3 / est / 1e9, 6), "unit": "Gpts/s", "cores": 1, "kind": "port",
+            "sample": f"oracle near field of one interior box (181^3 -> 61^3, T=50) {t_box:.2f} s "
+                      f"and oracle inject on 8 of 241 z-planes {t_inj:.2f} s, scaled by FLOPs "
+                      f"and planes to one step ({est:.1f} s); multipoles, translations and local "
+                      f"expansions (under 2% of the step on the GPU) not included"}
+    for obj in (op, fmm, bub, d_rho, d_tot, d_pot, grid):
+        obj.close()
+    return out
+
+
+# ----------------------------------------------------------------- config 1
+def bench_config1(ctx, args):
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    n_cell, nlip, T = 16, 7, 40
+    ax = O.Axis.ncell_fixed(-8.0, 8.0, nlip, n_cell)
+    N = ax.n
+    rng = np.random.default_rng(0xDA6E0001)
+    alpha = rng.uniform(0.5, 5.0, 4)
+    centers = rng.uniform(-2.0, 2.0, (4, 3))
+    q = np.ones(4)
+    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, nlip)
+    quad = dage_amd.GaussQuad(nlin=20, nlog=20, tlin=2.0, tlog=500.0)
+    op = dage_amd.Coulomb3D(ctx, grid, grid, quad)
     d_in = dage_amd.DeviceCube(ctx, rho.shape, rho)
     d_out = dage_amd.DeviceCube(ctx, rho.shape)
-    for _ in range(max(1, args.warmup // 2)):
-        fmm.apply_device(d_in, d_out)
-    steps = max(2, args.steps // 5)
+    steps = max(args.steps, 10)
+    for _ in range(args.warmup):
+        op.transform_cuda_cube(d_in, d_out)
     ctx.profile(True)
     ctx.time_begin()
     for _ in range(steps):
-        fmm.apply_device(d_in, d_out)
+        op.transform_cuda_cube(d_in, d_out)
     ms = ctx.time_end() / steps
     kernels = {}
-    for k in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
-              "box_multipoles_kernel", "m2m_kernel", "m2l_kernel", "l2l_kernel",
-              "local_expansion_kernel"):
-        kms, kn = ctx.profile_query(k)
-        if kn:
-            kernels[k] = {"ms_per_apply": round(kms / steps, 4), "launches_per_apply": kn // steps}
+    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel"):
+        tot, k = ctx.profile_query(name)
+        if k:
+            kernels[name] = {"ms_per_launch": round(tot / k, 4), "launches_per_apply": k // steps}
     ctx.profile(False)
-    pot = d_out.download()
-    exact = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
-    err_exact = float(np.max(np.abs(pot - exact)) / np.max(np.abs(exact)))
-    # direct apply with the same quadrature (no coda, as the near field)
-    op = dage_amd.Coulomb3D(ctx, grid, grid, gq, coda=False)
-    op.transform_cuda_cube(d_in, d_out)
-    ctx.time_begin()
-    for _ in range(2):
-        op.transform_cuda_cube(d_in, d_out)
-    direct_ms = ctx.time_end() / 2
-    direct = d_out.download()
-    err_direct = float(np.max(np.abs(pot - direct)) / np.max(np.abs(direct)))
-    # the reference applies one sub-operator per box: sum over boxes of
-    # 2 T (nxo nxi nyi nzi + nxo nyo nyi nzi + nxo nyo nzo nzi); here the near
-    # fields of all boxes are one whole-cube apply with banded matrices,
-    # 6 n^4 T dense FLOPs (the MFMA work actually issued)
+    V = d_out.download()
+    mixed = {}
+    d_mix = dage_amd.DeviceCube(ctx, rho.shape)
+    for mode, label in ((dage_amd.DAGE_PREC_BF16X6, "bf16x6"), (dage_amd.DAGE_PREC_BF16X3, "bf16x3")):
+        op.transform_cuda_cube(d_in, d_mix, precision=mode)
+        ctx.time_begin()
+        for _ in range(steps):
+            op.transform_cuda_cube(d_in, d_mix, precision=mode)
+        mms = ctx.time_end() / steps
+        mixed[label] = (mms, d_mix.download())
+    flops = 6.0 * T * float(N) ** 4
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, quad.tp)
+    c0 = time.perf_counter()
+    ref = O.transform_cube(fx, fy, fz, O.coulomb_weights(quad.tw), O.coda(500.0), rho)
+    cpu_s = time.perf_counter() - c0
+    Va = O.gauss_potential(q, alpha, centers, ax.points, ax.points, ax.points)
+    out = {
+        "workload": "config 1: plain Coulomb3D apply, FP64, 97^3 over [-8,8]^3, nlip 7, T=40, "
+                    "4 Gaussians, coda on",
+        "ms_per_apply": round(ms, 4),
+        "Gpts_s": round(N ** 3 / (ms * 1e-3) / 1e9, 4),
+        "tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+        "mfma_frac": round(flops / (ms * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4),
+        "kernels": kernels,
+        "traffic": {k: measured_traffic("config1", k) for k in kernels},
+        "max_rel_err_vs_oracle": rel(V, ref),
+        "max_rel_err_vs_analytic": rel(V, Va),
+        "mixed": {label: {"ms_per_apply": round(mms, 4), "speedup_vs_fp64": round(ms / mms, 3),
+                          "max_rel_err_vs_oracle": rel(vm, ref)}
+                  for label, (mms, vm) in mixed.items()},
+        "cpu_baseline": {"value": round(N ** 3 / cpu_s / 1e9, 6), "unit": "Gpts/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"full 97^3 T=40 apply, oracle C restatement of "
+                                   f"Operator3D_transform_cube, 1 thread: {cpu_s:.2f} s"},
+    }
+    for obj in (d_in, d_out, d_mix, op, grid):
+        obj.close()
+    return out
+
+
+# ----------------------------------------------------------------- config 3
+def bench_config3(ctx, args):
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    rng = np.random.default_rng(0xDA6E0003)
+    nrhs = 32
+    sizes = rng.choice([65, 97, 129, 161], nrhs)
+    k2 = rng.uniform(0.1, 4.0, nrhs)
+    energies = -0.5 * k2
+    gq = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
     T = gq.get_npoints()
-    s = 10 * 6
-    nf = [min(4, b + 2) * s - max(0, b - 1) * s + 1 for b in range(4)]
-    ref_flops = 0.0
-    for bx in range(4):
-        for by in range(4):
-            for bz in range(4):
-                no = s + 1
-                ref_flops += 2 * T * (no * nf[bx] * nf[by] * nf[bz] + no * no * nf[by] * nf[bz]
-                                      + no * no * no * nf[bz])
-    flops = 6.0 * n ** 4 * T
-    # split-BF16 near field on the same input
-    fmm.apply_device(d_in, d_out, dage_amd.DAGE_PREC_BF16X6)
+    axes, grids = {}, {}
+    for nn in sorted(set(int(s) for s in sizes)):
+        ncell = (nn - 1) // 4
+        half = ncell * 4 * 0.15 / 2
+        axes[nn] = O.Axis(-half, np.full(ncell, 0.15), 5)
+        grids[nn] = dage_amd.Grid3D(ctx, [-half] * 3, [axes[nn].steps] * 3, 5)
+    rhos, gparams = [], []
+    for s in sizes:
+        ax = axes[int(s)]
+        a = rng.uniform(0.5, 5.0, 4)
+        c = rng.uniform(-2.0, 2.0, (4, 3))
+        gparams.append((a, c))
+        rhos.append(O.gauss_density(np.ones(4), a, c, ax.points, ax.points, ax.points))
+    pts = float(sum(int(s) ** 3 for s in sizes))
+    flops = float(sum(6.0 * T * int(s) ** 4 for s in sizes))
+    gl = [grids[int(s)] for s in sizes]
+    # plain: one FP64 apply per orbital with set_energy in between
+    ops = {nn: dage_amd.Helmholtz3D(ctx, grids[nn], grids[nn], gq) for nn in grids}
+    d_in = [dage_amd.DeviceCube(ctx, r.shape, r) for r in rhos]
+    d_out = [dage_amd.DeviceCube(ctx, r.shape) for r in rhos]
+
+    def plain():
+        for i, s in enumerate(sizes):
+            op = ops[int(s)]
+            op.set_energy(energies[i])
+            op.transform_cuda_cube(d_in[i], d_out[i])
+
+    plain()
+    reps = 2
     ctx.time_begin()
-    for _ in range(steps):
-        fmm.apply_device(d_in, d_out, dage_amd.DAGE_PREC_BF16X6)
-    mixed_ms = ctx.time_end() / steps
-    err_mixed = float(np.max(np.abs(d_out.download() - pot)) / np.max(np.abs(pot)))
-    gemm_ms = sum(kernels.get(k, {}).get("ms_per_apply", 0.0)
-                  for k in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode"))
-    mp_ms = kernels.get("box_multipoles_kernel", {}).get("ms_per_apply", 0.0)
-    le_ms = kernels.get("local_expansion_kernel", {}).get("ms_per_apply", 0.0)
-    for obj in (op, fmm, d_in, d_out, grid):
+    for _ in range(reps):
+        plain()
+    plain_ms = ctx.time_end() / reps
+    plain_out = [d.download() for d in d_out]
+    res = {}
+    for label, prec in (("fp64", dage_amd.DAGE_PREC_FP64), ("bf16x6", dage_amd.DAGE_PREC_BF16X6)):
+        b = dage_amd.HelmholtzBatch(ctx, gl, energies, gq, precision=prec)
+        t0 = time.perf_counter()
+        tiles = b.tune(reps=1)
+        tune_s = time.perf_counter() - t0
+        for i, r in enumerate(rhos):
+            b.upload(i, r)
+        b.apply()
+        ctx.time_begin()
+        for _ in range(reps):
+            b.apply()
+        ms = ctx.time_end() / reps
+        outs = [b.download(i) for i in range(nrhs)]
+        res[label] = {
+            "ms_per_batch": round(ms, 3),
+            "Gpts_s": round(pts / (ms * 1e-3) / 1e9, 4),
+            "dense_fp64_equiv_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+            "speedup_vs_plain_fp64": round(plain_ms / ms, 3),
+            "max_rel_diff_vs_plain_fp64": max(rel(o, p) for o, p in zip(outs, plain_out)),
+            "tuned_tiles": tiles.tolist(),
+            "tune_s": round(tune_s, 2),
+        }
+        b.close()
+    # closed-form check on the first two 65^3 orbitals
+    checks = []
+    for i, s in enumerate(sizes):
+        if int(s) == 65 and len(checks) < 2:
+            ax = axes[65]
+            a, c = gparams[i]
+            ex = O.yukawa_gauss_potential(np.ones(4), a, c, np.sqrt(k2[i]), ax.points, ax.points,
+                                          ax.points)
+            checks.append(rel(plain_out[i], ex))
+    for obj in list(ops.values()) + d_in + d_out + list(grids.values()):
         obj.close()
     return {
-        "workload": "config 2 GBFMM: 241^3 cube over [-12,12]^3, 4x4x4 boxes, lmax 12, "
-                    "near field T=50, 16 Gaussians",
-        "ms_per_apply": round(ms, 3),
-        "Gpts_s": round(n ** 3 / (ms * 1e-3) / 1e9, 4),
-        "nearfield_dense_flops": flops,
-        "nearfield_per_box_reference_flops": ref_flops,
-        "nearfield_tflops": round(flops / (gemm_ms * 1e-3) / 1e12, 2) if gemm_ms else None,
-        "nearfield_mfma_frac": round(flops / (gemm_ms * 1e-3) / 1e12 / 78.6, 4) if gemm_ms else None,
-        "bf16x6_nearfield": {"ms_per_apply": round(mixed_ms, 3),
-                             "speedup_vs_fp64": round(ms / mixed_ms, 3),
-                             "max_rel_diff_vs_fp64_gbfmm": err_mixed},
-        "multipoles_GBps": round(8.0 * 64 * 61 ** 3 / (mp_ms * 1e-3) / 1e9, 1) if mp_ms else None,
-        "local_expansion_GBps": round(16.0 * n ** 3 / (le_ms * 1e-3) / 1e9, 1) if le_ms else None,
-        "kernels": kernels,
-        "direct_apply_ms": round(direct_ms, 3),
-        "max_rel_err_vs_analytic": err_exact,
-        "max_rel_diff_vs_direct_apply": err_direct,
+        "workload": "config 3: 32 Helmholtz right-hand sides, k^2 U[0.1,4], cubes "
+                    "U{65,97,129,161}^3 at h 0.15 (nlip 5), T=40, 4 Gaussians each",
+        "sizes": [int(s) for s in sizes],
+        "points": pts,
+        "dense_fp64_flops": flops,
+        "plain_fp64_ms": round(plain_ms, 3),
+        "plain_fp64_Gpts_s": round(pts / (plain_ms * 1e-3) / 1e9, 4),
+        "batched": res,
+        "plain_vs_yukawa_closed_form": checks,
     }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the apply would shard).", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args)
+        return
+    import dage_amd
+    ctx = dage_amd.Context(0)
+    c2 = bench_config2(ctx, args)
+    n = c2["n"]
+    line = {
+        "metric": METRIC,
+        "value": round(n ** 3 / (c2["ms_per_step"] * 1e-3) / 1e9, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(c2["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "config 2: 241^3 cube over [-12,12]^3 (nlip 7, 40 cells); 16 Gaussians "
+                        "+ 4 atoms' bubbles (lmax 4, 199-point radial grids) injected, then the "
+                        "GBFMM Coulomb potential (4x4x4 boxes, lmax 12, near field T=50), FP64",
+            "n": n, "T": c2["T"], "maxlevel": 2, "lmax": 12, "atoms": 4,
+        },
+        "roofline": c2["roofline"],
+        "cpu_baseline": c2.get("cpu_baseline"),
+        "kernels": c2["kernels"],
+        "bf16x6_nearfield": c2["bf16x6_nearfield"],
+        "gbfmm_vs_analytic": c2["gbfmm_vs_analytic"],
+        "gbfmm_vs_direct_apply": c2["gbfmm_vs_direct_apply"],
+        "direct_apply_ms": c2["direct_apply_ms"],
+        "inject_vs_oracle_8_planes": c2["inject_vs_oracle_8_planes"],
+        "inject_valu_fp64_tflops": c2["inject_valu_fp64_tflops"],
+        "wall_s_timed": round(c2["wall_s"], 4),
+    }
+    if args.config1:
+        line["config1"] = bench_config1(ctx, args)
+    if args.config3:
+        line["config3"] = bench_config3(ctx, args)
+    print(json.dumps(line), flush=True)
+    ctx.close()
 
 
 if __name__ == "__main__":

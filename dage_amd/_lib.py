@@ -84,6 +84,25 @@ _SIGNATURES = {
     "dage_gbfmm_destroy": [_c_p],
     "dage_fmm_interaction_matrix": [_int, _dp, _dp],
     "dage_fmm_translation_matrix": [_int, _dp, _dp],
+    "dage_helmholtz_batch_create": [_c_p, _int, ctypes.POINTER(_c_p), _dp, _dp, _dp, _int, _dbl,
+                                    _int, _int, ctypes.POINTER(_c_p)],
+    "dage_batch_input": [_c_p, _int, ctypes.POINTER(_c_p)],
+    "dage_batch_output": [_c_p, _int, ctypes.POINTER(_c_p)],
+    "dage_batch_upload": [_c_p, _int, _dp],
+    "dage_batch_download": [_c_p, _int, _dp],
+    "dage_batch_set_energies": [_c_p, _dp],
+    "dage_batch_apply": [_c_p],
+    "dage_batch_apply_eager": [_c_p],
+    "dage_batch_tune": [_c_p, _int, _ip],
+    "dage_batch_group_count": [_c_p, _ip],
+    "dage_batch_destroy": [_c_p],
+    "dage_vec_axpby": [_c_p, ctypes.c_long, _dbl, _c_p, _dbl, _c_p],
+    "dage_vec_mul": [_c_p, ctypes.c_long, _c_p, _c_p, _c_p],
+    "dage_vec_square_acc": [_c_p, ctypes.c_long, _dbl, _c_p, _c_p],
+    "dage_vec_lda_x": [_c_p, ctypes.c_long, _c_p, _c_p, _c_p],
+    "dage_vec_lincomb": [_c_p, ctypes.c_long, _int, ctypes.POINTER(_c_p), _dp, _int,
+                         ctypes.POINTER(_c_p)],
+    "dage_cube_gram": [_c_p, _int, ctypes.POINTER(_c_p), _int, ctypes.POINTER(_c_p), _dp],
 }
 _RESTYPES = {"dage_last_error": ctypes.c_char_p}
 

@@ -36,9 +36,12 @@ struct Gemm {
     long sAm0, sAm1, sAk;     // A(m,k)
     long sBk, sBn;            // B(k,n)
     long sCm0, sCm1, sCn;     // C(m,n)
-    long bA, bB, bC;          // strides per blockIdx.z
-    const double *alpha;      // alpha[blockIdx.z] (null: 1)
-    int Ksplit;               // >0: blockIdx.z selects a K range of this size
+    long bA, bB, bC;          // strides per batch index z_lo
+    int Z0;                   // >0: blockIdx.z = z_lo + Z0*z_hi (z_hi: right-hand side)
+    long bA1, bB1, bC1;       // strides per z_hi
+    const double *alpha;      // alpha[z_lo + aS1*z_hi] (null: 1)
+    long aS1;
+    int Ksplit;               // >0: z_lo selects a K range of this size
     int accumulate;           // C += result instead of C = result
 };
 
@@ -74,17 +77,20 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     double *C = g.C;
     double alpha = 1.0;
     int kbeg = 0, kend = g.K;
+    const int zlo = g.Z0 > 0 ? (int)blockIdx.z % g.Z0 : (int)blockIdx.z;
+    const int zhi = g.Z0 > 0 ? (int)blockIdx.z / g.Z0 : 0;
+    A += (long)zhi * g.bA1;
+    B += (long)zhi * g.bB1;
+    C += (long)zlo * g.bC + (long)zhi * g.bC1;
     if (g.Ksplit > 0) {
-        kbeg = blockIdx.z * g.Ksplit;
+        kbeg = zlo * g.Ksplit;
         kend = min(g.K, kbeg + g.Ksplit);
         A += (long)kbeg * g.sAk;
         B += (long)kbeg * g.sBk;
-        C += (long)blockIdx.z * g.bC;
     } else {
-        A += (long)blockIdx.z * g.bA;
-        B += (long)blockIdx.z * g.bB;
-        C += (long)blockIdx.z * g.bC;
-        if (g.alpha) alpha = g.alpha[blockIdx.z];
+        A += (long)zlo * g.bA;
+        B += (long)zlo * g.bB;
+        if (g.alpha) alpha = g.alpha[zlo + g.aS1 * zhi];
     }
     const int K = kend - kbeg;
 
@@ -208,19 +214,6 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     }
 }
 
-// out[i] = sum_s part[s*len + i] + coda*in[i]  (in may be null)
-__global__ void __launch_bounds__(256) splitk_reduce_kernel(const double *part, int splits,
-                                                            long len, const double *in,
-                                                            double coda, double *out) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len;
-         i += (long)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int k = 0; k < splits; ++k) s += part[(long)k * len + i];
-        if (in) s += coda * in[i];
-        out[i] = s;
-    }
-}
-
 // Fz (nzi, nzo, T) -> Gz((zi + nzi*t), zo)
 __global__ void __launch_bounds__(256) relayout_fz_kernel(const double *fz, double *gz, int nzi,
                                                           int nzo, int T) {
@@ -277,10 +270,10 @@ int launch_nb(const Gemm &g, int nb, int zdim, hipStream_t s) {
 // The three passes have fixed operand layouts: x mode reads k-contiguous A and
 // n-contiguous B; y and z modes read m-contiguous A and k-contiguous B.
 template <int PASS>
-int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim) {
+int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim, int nb_override = 0) {
     // the k-contiguous A loader steps rows linearly: one-level row index only
     DAGE_REQUIRE(PASS != 0 || g.M0 >= g.M, DAGE_ERR_STATE, "x-mode GEMM needs a one-level M");
-    const int nb = pick_nb(g.N);
+    const int nb = (nb_override >= 1 && nb_override <= 8) ? nb_override : pick_nb(g.N);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
     int rc;
@@ -297,10 +290,10 @@ struct Plan {
     size_t v_elems, u_elems, p_elems;
 };
 
-Plan make_plan(const ApplyMats &m) {
+Plan make_plan(const ApplyMats &m, int B = 1) {
     const long nyi = m.din[1], nzi = m.din[2];
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
-    size_t budget = 4ull << 30;  // bytes for V + U
+    size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
     const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
     Plan p;
@@ -310,32 +303,28 @@ Plan make_plan(const ApplyMats &m) {
     const int nb = pick_nb((int)nzo);
     const int ntiles = (int)((nzo + 16 * nb - 1) / (16 * nb));
     const long K3 = nzi * p.tchunk;
-    int splits = std::max(1, (1024 + mtiles * ntiles - 1) / (mtiles * ntiles));
+    int splits = std::max(1, (1024 + mtiles * ntiles * B - 1) / (mtiles * ntiles * B));
     splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 64));
     p.splits = splits;
-    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk;
-    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk;
-    p.p_elems = (size_t)M3 * nzo * splits;
+    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk * B;
+    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk * B;
+    p.p_elems = (size_t)M3 * nzo * splits * B;
     return p;
 }
 
-// out(x,y,z) = sum_s part[s*len + (x + nxo*(y + nyo*z))] over the written
-// extent of a strided output view.
-__global__ void __launch_bounds__(256) splitk_reduce_view_kernel(const double *part, int splits,
-                                                                 long len, int nxo, int nyo,
-                                                                 int nx, int ny, int nz,
-                                                                 double *out, long ldx, long ldy) {
-    const long total = (long)nx * ny * nz;
+// out[b*len + i] = sum_s part[(b*splits + s)*len + i] + coda*in[b*len + i]
+__global__ void __launch_bounds__(256) splitk_reduce_batch_kernel(const double *part, int splits,
+                                                                  long len, int B,
+                                                                  const double *in, double coda,
+                                                                  double *out) {
+    const long total = len * B;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
          e += (long)gridDim.x * blockDim.x) {
-        const int x = (int)(e % nx);
-        const long r = e / nx;
-        const int y = (int)(r % ny);
-        const int z = (int)(r / ny);
-        const long src = x + (long)nxo * (y + (long)nyo * z);
+        const long b = e / len, i = e - b * len;
         double s = 0.0;
-        for (int k = 0; k < splits; ++k) s += part[(long)k * len + src];
-        out[x + ldx * (y + ldy * z)] = s;
+        for (int k = 0; k < splits; ++k) s += part[(b * splits + k) * len + i];
+        if (in) s += coda * in[e];
+        out[e] = s;
     }
 }
 
@@ -371,17 +360,21 @@ int apply_f64_workspace(dage_operator op, size_t *bytes) {
     return DAGE_OK;
 }
 
-int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const OutView &ov,
-                   double coda, Workspace *ws) {
+int apply_f64_batch_workspace(const ApplyMats &m, int B, size_t *bytes) {
+    const Plan p = make_plan(m, B);
+    *bytes = 8 * (p.v_elems + p.u_elems + p.p_elems);
+    return DAGE_OK;
+}
+
+int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *out,
+                    const BatchSpec &bs, Workspace *ws) {
     const long nxi = m.din[0], nyi = m.din[1], nzi = m.din[2];
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
-    const int T = m.ntp;
-    DAGE_REQUIRE(ov.n[0] <= nxo && ov.n[1] <= nyo && ov.n[2] <= nzo && ov.ldx >= ov.n[0] &&
-                     ov.ldy >= ov.n[1],
-                 DAGE_ERR_ARG, "output view exceeds the operator's output grid");
-    const Plan plan = make_plan(m);
+    const int T = m.ntp, B = bs.B;
+    const Plan plan = make_plan(m, B);
     const size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
     if (need > ws->bytes) {
+        DAGE_REQUIRE(!bs.no_alloc, DAGE_ERR_STATE, "workspace of %zu bytes not reserved", need);
         if (ws->p) DAGE_HIP_CHECK(hipFree(ws->p));
         ws->p = nullptr;
         ws->bytes = 0;
@@ -391,12 +384,16 @@ int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const Out
     double *V = ws->p;
     double *U = V + plan.v_elems;
     double *P = U + plan.u_elems;
-    hipStream_t s = ctx->stream;
+    const double *W = bs.w ? bs.w : m.w;
+    const long wS = bs.w ? T : 0;  // weights row per right-hand side
+    const long nin = nxi * nyi * nzi, nout = nxo * nyo * nzo;
     const long M3 = nxo * nyo;
+    const int tcmax = plan.tchunk;
+    const long Vt = nxo * nyi * nzi, Ut = nxo * nyo * nzi;  // per (t, rhs)
 
-    for (int t0 = 0; t0 < T; t0 += plan.tchunk) {
-        const int tc = std::min(plan.tchunk, T - t0);
-        // pass 1: V_t^T((yi,zi), xo) = in^T((yi,zi), xi) . Fx_t^T(xi, xo)
+    for (int t0 = 0; t0 < T; t0 += tcmax) {
+        const int tc = std::min(tcmax, T - t0);
+        // pass 1, batch (t, b): V_{b,t}((yi,zi), xo) = in_b((yi,zi), xi) . Fx_t^T(xi, xo)
         Gemm g1{};
         g1.A = in;
         g1.B = m.fx + (long)t0 * nxo * nxi;
@@ -405,10 +402,11 @@ int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const Out
         g1.M0 = g1.M; g1.sAm0 = nxi; g1.sAm1 = 0; g1.sAk = 1;
         g1.sBk = nxo; g1.sBn = 1;
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1;
-        g1.bA = 0; g1.bB = nxo * nxi; g1.bC = nxo * nyi * nzi;
-        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc));
+        g1.bA = 0; g1.bB = nxo * nxi; g1.bC = Vt;
+        g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
+        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc * B, bs.nb[0]));
 
-        // pass 2: U((xo,yo),(zi,t)) = w_t V_t((xo,zi), yi) . Fy_t(yi, yo)
+        // pass 2, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
         g2.A = V;
         g2.B = m.fy + (long)t0 * nyi * nyo;
@@ -417,11 +415,13 @@ int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const Out
         g2.M0 = (int)nxo; g2.sAm0 = 1; g2.sAm1 = nxo * nyi; g2.sAk = nxo;
         g2.sBk = 1; g2.sBn = nyi;
         g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo;
-        g2.bA = nxo * nyi * nzi; g2.bB = nyi * nyo; g2.bC = nxo * nyo * nzi;
-        g2.alpha = m.w + t0;
-        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc));
+        g2.bA = Vt; g2.bB = nyi * nyo; g2.bC = Ut;
+        g2.Z0 = tc; g2.bA1 = Vt * tc; g2.bB1 = 0; g2.bC1 = Ut * tc;
+        g2.alpha = W + t0;
+        g2.aS1 = wS;
+        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc * B, bs.nb[1]));
 
-        // pass 3: P_s((xo,yo), zo) (+)= U((xo,yo), k) . Gz(k, zo), k in split s
+        // pass 3, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
         const int K3 = (int)(nzi * tc);
         const int ks = (K3 + plan.splits - 1) / plan.splits;
         const int splits = (K3 + ks - 1) / ks;
@@ -434,48 +434,34 @@ int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const Out
         g3.sBk = 1; g3.sBn = (long)nzi * T;
         g3.sCm0 = 1; g3.sCm1 = 0; g3.sCn = M3;
         g3.bC = M3 * nzo;
+        g3.Z0 = splits; g3.bA1 = Ut * tc; g3.bB1 = 0; g3.bC1 = M3 * nzo * plan.splits;
         g3.Ksplit = ks;
         g3.accumulate = (t0 > 0);
         if (t0 == 0 && splits < plan.splits)
-            DAGE_TRY(fill_zero(ctx, P + (size_t)splits * M3 * nzo,
-                               (size_t)(plan.splits - splits) * M3 * nzo));
-        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits));
+            for (int b = 0; b < B; ++b)
+                DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + splits) * M3 * nzo,
+                                   (size_t)(plan.splits - splits) * M3 * nzo));
+        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits * B, bs.nb[2]));
     }
     const long len = M3 * nzo;
-    const bool whole = ov.n[0] == nxo && ov.n[1] == nyo && ov.n[2] == nzo && ov.ldx == nxo &&
-                       ov.ldy == nyo;
+    const long total = len * B;
+    const int grid = (int)std::min<long>((total + 255) / 256, 65536);
     hipEvent_t ev;
-    if (whole) {
-        const int grid = (int)std::min<long>((len + 255) / 256, 65536);
-        prof_begin(ctx, "splitk_reduce_kernel", &ev);
-        hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid), dim3(256), 0, s, P, plan.splits,
-                           len, coda != 0.0 ? in : nullptr, coda, ov.p);
-        DAGE_HIP_CHECK(hipGetLastError());
-        prof_end(ctx, "splitk_reduce_kernel", ev);
-    } else {
-        DAGE_REQUIRE(coda == 0.0, DAGE_ERR_ARG, "the coda term needs a whole-cube output");
-        const long n = (long)ov.n[0] * ov.n[1] * ov.n[2];
-        const int grid = (int)std::min<long>((n + 255) / 256, 65536);
-        prof_begin(ctx, "splitk_reduce_view_kernel", &ev);
-        hipLaunchKernelGGL(splitk_reduce_view_kernel, dim3(grid), dim3(256), 0, s, P,
-                           plan.splits, len, (int)nxo, (int)nyo, ov.n[0], ov.n[1], ov.n[2], ov.p,
-                           ov.ldx, ov.ldy);
-        DAGE_HIP_CHECK(hipGetLastError());
-        prof_end(ctx, "splitk_reduce_view_kernel", ev);
-    }
+    prof_begin(ctx, "splitk_reduce_kernel", &ev);
+    hipLaunchKernelGGL(splitk_reduce_batch_kernel, dim3(grid), dim3(256), 0, ctx->stream, P,
+                       plan.splits, len, B, bs.coda != 0.0 ? in : nullptr, bs.coda, out);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "splitk_reduce_kernel", ev);
+    (void)nout;
     return DAGE_OK;
 }
 
 int apply_f64(dage_operator op, const double *in, double *out) {
-    const ApplyMats m = operator_mats(op);
-    OutView ov;
-    ov.p = out;
-    ov.ldx = op->dout[0];
-    ov.ldy = op->dout[1];
-    for (int i = 0; i < 3; ++i) ov.n[i] = op->dout[i];
+    BatchSpec bs;
     const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
+    bs.coda = coda ? op->coda : 0.0;
     Workspace ws{op->d_v, op->ws_bytes};
-    const int rc = apply_f64_view(op->ctx, m, in, ov, coda ? op->coda : 0.0, &ws);
+    const int rc = apply_f64_batch(op->ctx, operator_mats(op), in, out, bs, &ws);
     op->d_v = ws.p;
     op->ws_bytes = ws.bytes;
     return rc;

@@ -83,8 +83,11 @@ struct SGemm {
     void *C;                  // C(m,n) = C[mlo*sCm0 + mhi*sCm1 + n*sCn]
     int N;
     long sCm0, sCm1, sCn, bC;
-    const double *alpha;      // per batch (null: 1)
-    int Ksplit;               // > 0: blockIdx.z selects K steps [z*Ksplit, (z+1)*Ksplit) of 32
+    int Z0;                   // > 0: blockIdx.z = z_lo + Z0*z_hi (z_hi: right-hand side)
+    long bA1, bC1;            // strides per z_hi (the B planes are shared)
+    const double *alpha;      // alpha[z_lo + aS1*z_hi] (null: 1)
+    long aS1;
+    int Ksplit;               // > 0: z_lo selects K steps [z_lo*Ksplit, (z_lo+1)*Ksplit) of 32
     int accumulate;
 };
 
@@ -109,15 +112,17 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     TC *C = static_cast<TC *>(g.C);
     double alpha = 1.0;
     int kbeg = 0, kend = g.K;
+    const int zlo = g.Z0 > 0 ? (int)blockIdx.z % g.Z0 : (int)blockIdx.z;
+    const int zhi = g.Z0 > 0 ? (int)blockIdx.z / g.Z0 : 0;
+    A += (long)zhi * g.bA1;
+    C += (long)zlo * g.bC + (long)zhi * g.bC1;
     if (g.Ksplit > 0) {
-        kbeg = blockIdx.z * g.Ksplit * kKS;
+        kbeg = zlo * g.Ksplit * kKS;
         kend = min(g.K, kbeg + g.Ksplit * kKS);
-        C += (long)blockIdx.z * g.bC;
     } else {
-        A += (long)blockIdx.z * g.bA;
-        Bp += (long)blockIdx.z * g.bB;
-        C += (long)blockIdx.z * g.bC;
-        if (g.alpha) alpha = g.alpha[blockIdx.z];
+        A += (long)zlo * g.bA;
+        Bp += (long)zlo * g.bB;
+        if (g.alpha) alpha = g.alpha[zlo + g.aS1 * zhi];
     }
     const int nsteps = (kend - kbeg + kKS - 1) / kKS;
 
@@ -236,19 +241,6 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     }
 }
 
-// out[i] = sum_s part[s*len + i] + coda*in[i]
-__global__ void __launch_bounds__(256) mixed_reduce_kernel(const double *part, int splits, long len,
-                                                           const double *in, double coda,
-                                                           double *out) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len;
-         i += (long)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int k = 0; k < splits; ++k) s += part[(long)k * len + i];
-        if (in) s += coda * in[i];
-        out[i] = s;
-    }
-}
-
 int pick_nb_mixed(int n) {
     int best = 8;
     double best_cost = 1e30;
@@ -265,6 +257,13 @@ int pick_nb_mixed(int n) {
 }
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Columns of a plane padded so that every NB in 1..8 reads inside it.
+int pad_all_nb(int n) {
+    int p = 0;
+    for (int nb = 1; nb <= 8; ++nb) p = std::max(p, round_up(n, 16 * nb));
+    return p;
+}
 
 constexpr int kMWs = 2;
 
@@ -315,9 +314,9 @@ int prepare_planes(dage_operator op) {
     L.nbx = pick_nb_mixed(nxo);
     L.nby = pick_nb_mixed(nyo);
     L.nbz = pick_nb_mixed(nzo);
-    L.npx = round_up(nxo, 16 * L.nbx);
-    L.npy = round_up(nyo, 16 * L.nby);
-    L.npz = round_up(nzo, 16 * L.nbz);
+    L.npx = pad_all_nb(nxo);  // any column-block count may be launched on these planes
+    L.npy = pad_all_nb(nyo);
+    L.npz = pad_all_nb(nzo);
     L.kpx = round_up(nxi, kKS);
     L.kpy = round_up(nyi, kKS);
     L.kpz = round_up(nzi * T, kKS);
@@ -347,10 +346,10 @@ struct MPlan {
     size_t v_elems, u_elems, p_elems;  // FP32, FP32, FP64
 };
 
-MPlan make_mplan(dage_operator op) {
+MPlan make_mplan(dage_operator op, int B) {
     const long nyi = op->din[1], nzi = op->din[2];
     const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
-    size_t budget = 4ull << 30;
+    size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
     const size_t per_t = 4ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
     MPlan p;
@@ -368,39 +367,63 @@ MPlan make_mplan(dage_operator op) {
     const int nb = pick_nb_mixed((int)nzo);
     const int ntiles = (int)((nzo + 16 * nb - 1) / (16 * nb));
     const long K3 = nzi * p.tchunk;
-    int splits = std::max(1, (1024 + mtiles * ntiles - 1) / (mtiles * ntiles));
+    int splits = std::max(1, (1024 + mtiles * ntiles * B - 1) / (mtiles * ntiles * B));
     splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 128));
     p.splits = splits;
-    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk;
-    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk;
-    p.p_elems = (size_t)M3 * nzo * splits;
+    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk * B;
+    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk * B;
+    p.p_elems = (size_t)M3 * nzo * splits * B;
     return p;
 }
 
+// out[b*len + i] = sum_s part[(b*splits + s)*len + i] + coda*in[b*len + i]
+__global__ void __launch_bounds__(256) mixed_reduce_kernel(const double *part, int splits,
+                                                           long len, int B, const double *in,
+                                                           double coda, double *out) {
+    const long total = len * B;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long b = e / len, i = e - b * len;
+        double s = 0.0;
+        for (int k = 0; k < splits; ++k) s += part[(b * splits + k) * len + i];
+        if (in) s += coda * in[e];
+        out[e] = s;
+    }
+}
+
 template <int NPAIR>
-int apply_mixed_t(dage_operator op, const double *in, double *out) {
+int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSpec &bs,
+                  Workspace *ws) {
     const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
     const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
-    const int T = op->ntp;
+    const int T = op->ntp, B = bs.B;
     DAGE_TRY(prepare_planes(op));
     const MixedLayout &L = op->mixed;
+    const int nbx = bs.nb[0] ? bs.nb[0] : L.nbx, nby = bs.nb[1] ? bs.nb[1] : L.nby,
+              nbz = bs.nb[2] ? bs.nb[2] : L.nbz;
     const __bf16 *P16 = reinterpret_cast<const __bf16 *>(op->d_planes);
-    const MPlan plan = make_mplan(op);
+    const MPlan plan = make_mplan(op, B);
     const size_t need = 4 * (plan.v_elems + plan.u_elems) + 8 * plan.p_elems + 256;
-    if (need > op->ws_bytes) {
-        if (op->d_v) DAGE_HIP_CHECK(hipFree(op->d_v));
-        op->d_v = nullptr;
-        op->ws_bytes = 0;
-        DAGE_HIP_CHECK(hipMalloc((void **)&op->d_v, need));
-        op->ws_bytes = need;
+    if (need > ws->bytes) {
+        DAGE_REQUIRE(!bs.no_alloc, DAGE_ERR_STATE, "workspace of %zu bytes not reserved", need);
+        if (ws->p) DAGE_HIP_CHECK(hipFree(ws->p));
+        ws->p = nullptr;
+        ws->bytes = 0;
+        DAGE_HIP_CHECK(hipMalloc((void **)&ws->p, need));
+        ws->bytes = need;
     }
-    double *P = op->d_v;  // FP64 partials first (8-byte aligned)
+    double *P = ws->p;  // FP64 partials first (8-byte aligned)
     float *V = reinterpret_cast<float *>(P + plan.p_elems);
     float *U = V + plan.v_elems;
+    const double *W = bs.w ? bs.w : op->d_w;
+    const long wS = bs.w ? T : 0;
+    const long nin = nxi * nyi * nzi;
     const long M3 = nxo * nyo;
-    for (int t0 = 0; t0 < T; t0 += plan.tchunk) {
-        const int tc = std::min(plan.tchunk, T - t0);
-        // x mode: V_t((yi,zi), xo) = in((yi,zi), xi) . Fx_t^T(xi, xo)
+    const int tcmax = plan.tchunk;
+    const long Vt = nxo * nyi * nzi, Ut = nxo * nyo * nzi;
+    for (int t0 = 0; t0 < T; t0 += tcmax) {
+        const int tc = std::min(tcmax, T - t0);
+        // x mode, batch (t, b): V_{b,t}((yi,zi), xo) = in_b((yi,zi), xi) . Fx_t^T(xi, xo)
         SGemm g1{};
         g1.A = in;
         g1.M = (int)(nyi * nzi); g1.K = (int)nxi; g1.M0 = g1.M;
@@ -409,23 +432,26 @@ int apply_mixed_t(dage_operator op, const double *in, double *out) {
         g1.Np = L.npx; g1.Kp = L.kpx;
         g1.sBq = (long)T * L.npx * L.kpx; g1.bB = (long)L.npx * L.kpx;
         g1.C = V; g1.N = (int)nxo;
-        g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1; g1.bC = nxo * nyi * nzi;
+        g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1; g1.bC = Vt;
+        g1.Z0 = tc; g1.bA1 = nin; g1.bC1 = Vt * tc;
         DAGE_TRY((launch_split<NPAIR, double, float, true, 0>(op->ctx, "gemm_mixed_xmode", g1,
-                                                              L.nbx, tc)));
-        // y mode: U((xo,yo),(zi,t)) = w_t V_t((xo,zi), yi) . Fy_t(yi, yo)
+                                                              nbx, tc * B)));
+        // y mode, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         SGemm g2{};
         g2.A = V;
         g2.M = (int)(nxo * nzi); g2.K = (int)nyi; g2.M0 = (int)nxo;
-        g2.sAm0 = 1; g2.sAm1 = nxo * nyi; g2.sAk = nxo; g2.bA = nxo * nyi * nzi;
+        g2.sAm0 = 1; g2.sAm1 = nxo * nyi; g2.sAk = nxo; g2.bA = Vt;
         g2.Bp = P16 + L.off_y + (size_t)t0 * L.npy * L.kpy;
         g2.Np = L.npy; g2.Kp = L.kpy;
         g2.sBq = (long)T * L.npy * L.kpy; g2.bB = (long)L.npy * L.kpy;
         g2.C = U; g2.N = (int)nyo;
-        g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo; g2.bC = nxo * nyo * nzi;
-        g2.alpha = op->d_w + t0;
+        g2.sCm0 = 1; g2.sCm1 = nxo * nyo; g2.sCn = nxo; g2.bC = Ut;
+        g2.Z0 = tc; g2.bA1 = Vt * tc; g2.bC1 = Ut * tc;
+        g2.alpha = W + t0;
+        g2.aS1 = wS;
         DAGE_TRY((launch_split<NPAIR, float, float, false, 1>(op->ctx, "gemm_mixed_ymode", g2,
-                                                              L.nby, tc)));
-        // z mode: P_s((xo,yo), zo) (+)= U((xo,yo), k) . Gz(k, zo), k = zi + nzi*(t - t0)
+                                                              nby, tc * B)));
+        // z mode, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
         const int K3 = (int)(nzi * tc);
         const int steps = (K3 + kKS - 1) / kKS;
         const int per = (steps + plan.splits - 1) / plan.splits;
@@ -439,21 +465,23 @@ int apply_mixed_t(dage_operator op, const double *in, double *out) {
         g3.sBq = (long)L.npz * L.kpz; g3.bB = 0;
         g3.C = P; g3.N = (int)nzo;
         g3.sCm0 = 1; g3.sCm1 = 0; g3.sCn = M3; g3.bC = M3 * nzo;
+        g3.Z0 = splits; g3.bA1 = Ut * tc; g3.bC1 = M3 * nzo * plan.splits;
         g3.Ksplit = per;
         g3.accumulate = (t0 > 0);
         if (t0 == 0 && splits < plan.splits)
-            DAGE_TRY(fill_zero(op->ctx, P + (size_t)splits * M3 * nzo,
-                               (size_t)(plan.splits - splits) * M3 * nzo));
+            for (int b = 0; b < B; ++b)
+                DAGE_TRY(fill_zero(op->ctx, P + ((size_t)b * plan.splits + splits) * M3 * nzo,
+                                   (size_t)(plan.splits - splits) * M3 * nzo));
         DAGE_TRY((launch_split<NPAIR, float, double, false, 2>(op->ctx, "gemm_mixed_zmode", g3,
-                                                               L.nbz, splits)));
+                                                               nbz, splits * B)));
     }
-    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
     const long len = M3 * nzo;
-    const int grid = (int)std::min<long>((len + 255) / 256, 65536);
+    const long total = len * B;
+    const int grid = (int)std::min<long>((total + 255) / 256, 65536);
     hipEvent_t ev;
     prof_begin(op->ctx, "mixed_reduce_kernel", &ev);
     hipLaunchKernelGGL(mixed_reduce_kernel, dim3(grid), dim3(256), 0, op->ctx->stream, P,
-                       plan.splits, len, coda ? in : nullptr, op->coda, out);
+                       plan.splits, len, B, bs.coda != 0.0 ? in : nullptr, bs.coda, out);
     DAGE_HIP_CHECK(hipGetLastError());
     prof_end(op->ctx, "mixed_reduce_kernel", ev);
     return DAGE_OK;
@@ -462,10 +490,30 @@ int apply_mixed_t(dage_operator op, const double *in, double *out) {
 }  // namespace
 
 int apply_mixed(dage_operator op, const double *in, double *out, int npair) {
-    if (npair == 6) return apply_mixed_t<6>(op, in, out);
-    if (npair == 3) return apply_mixed_t<3>(op, in, out);
+    BatchSpec bs;
+    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
+    bs.coda = coda ? op->coda : 0.0;
+    Workspace ws{op->d_v, op->ws_bytes};
+    const int rc = apply_mixed_batch(op, in, out, npair, bs, &ws);
+    op->d_v = ws.p;
+    op->ws_bytes = ws.bytes;
+    return rc;
+}
+
+int apply_mixed_batch(dage_operator op, const double *in, double *out, int npair,
+                      const BatchSpec &bs, Workspace *ws) {
+    if (npair == 6) return apply_mixed_t<6>(op, in, out, bs, ws);
+    if (npair == 3) return apply_mixed_t<3>(op, in, out, bs, ws);
     set_error("unsupported split-BF16 product count %d", npair);
     return DAGE_ERR_UNSUPPORTED;
 }
+
+int apply_mixed_batch_workspace(dage_operator op, int B, size_t *bytes) {
+    const MPlan p = make_mplan(op, B);
+    *bytes = 4 * (p.v_elems + p.u_elems) + 8 * p.p_elems + 256;
+    return DAGE_OK;
+}
+
+int prepare_mixed(dage_operator op) { return prepare_planes(op); }
 
 }  // namespace dage

@@ -193,26 +193,37 @@ struct ApplyMats {
     int din[3] = {0, 0, 0}, dout[3] = {0, 0, 0};
     int ntp = 0;
 };
-// Strided destination: element (x,y,z) at p[x + ldx*(y + ldy*z)] for
-// x < n[0], y < n[1], z < n[2] (n <= the operator's output shape).
-struct OutView {
-    double *p = nullptr;
-    long ldx = 0, ldy = 0;
-    int n[3] = {0, 0, 0};
-};
 // Device scratch owned by the caller, grown on demand.
 struct Workspace {
     double *p = nullptr;
     size_t bytes = 0;
 };
 ApplyMats operator_mats(dage_operator op);
-// FP64 apply of contiguous `in` (din) into an output view; coda != 0 adds
-// coda*in and needs the whole, contiguous output.
-int apply_f64_view(dage_ctx ctx, const ApplyMats &m, const double *in, const OutView &out,
-                   double coda, Workspace *ws);
+// A batch of right-hand sides on one operator's grids: `in` and `out` hold B
+// cubes back to back; w (B x T, row b at w + b*T) replaces the operator's
+// weights when given; coda != 0 adds coda*in (in and out on the same grid);
+// nb[pass] overrides the GEMM column-block count (0: heuristic); no_alloc
+// makes a too-small workspace an error instead of growing it.
+struct BatchSpec {
+    int B = 1;
+    const double *w = nullptr;
+    double coda = 0.0;
+    int nb[3] = {0, 0, 0};
+    bool no_alloc = false;
+};
+int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *out,
+                    const BatchSpec &bs, Workspace *ws);
+int apply_f64_batch_workspace(const ApplyMats &m, int B, size_t *bytes);
 int apply_f64_workspace(dage_operator op, size_t *bytes);
 // Mixed-precision apply (apply_mixed.hip): npair = 6 (bf16x6) or 3 (bf16x3).
 int apply_mixed(dage_operator op, const double *in_dev, double *out_dev, int npair);
+// Batch of right-hand sides on one operator (same conventions as apply_f64_batch).
+int apply_mixed_batch(dage_operator op, const double *in, double *out, int npair,
+                      const BatchSpec &bs, Workspace *ws);
+int apply_mixed_batch_workspace(dage_operator op, int B, size_t *bytes);
+// Builds the operator's split-BF16 matrix planes now (they are otherwise built
+// on the first mixed apply).
+int prepare_mixed(dage_operator op);
 // FMM matrices (fmm_host.cpp).  Sparse term lists per matrix entry (row +
 // N*col): interaction entries combine I_{L,n}(D), translation entries S_{L,n}(d).
 struct FmmTables {

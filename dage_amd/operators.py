@@ -10,6 +10,7 @@ Names and argument meaning follow the reference:
   Helmholtz3D.set_energy                        src/bubbles/helmholtz3d.F90:411-444
   GBFMMCoulomb3D(grid, maxlevel, lmax, gauss)   src/gbfmm/gbfmm_coulomb3d.F90:306-506
   InteractionMatrix / TranslationMatrix         src/bubbles/multipole.F90:33-560
+  HelmholtzBatch (per-orbital set_energy+apply)  src/bubbles/scf_cycle.F90:2927-3054
 
 Cubes are numpy arrays in Fortran order (x fastest) for the host path and
 DeviceCube objects (the library's CudaCube, same layout) for the device path.
@@ -461,3 +462,79 @@ def translation_matrix(lmax: int, dR) -> np.ndarray:
     W = np.zeros((N, N), order="F")
     call("dage_fmm_translation_matrix", lmax, _ptr(_f64(dR)), W.ctypes.data_as(_dp))
     return W
+
+
+class HelmholtzBatch:
+    """Helmholtz applies of many orbitals at once, one hipGraph launch per
+    apply.  grids[i] is orbital i's grid (input = output), energies[i] its
+    energy (Helmholtz3D_set_energy semantics); orbitals on identical grids
+    share matrices."""
+
+    def __init__(self, ctx: Context, grids, energies, gauss: GaussQuad,
+                 precision: int = DAGE_PREC_FP64, coda: bool = True):
+        self.ctx, self.grids = ctx, list(grids)
+        n = len(self.grids)
+        arr = (ctypes.c_void_p * n)(*[g.handle.value if isinstance(g.handle, ctypes.c_void_p)
+                                      else g.handle for g in self.grids])
+        e = _f64(energies)
+        if len(e) != n:
+            raise ValueError("one energy per grid")
+        h = ctypes.c_void_p()
+        call("dage_helmholtz_batch_create", ctx.handle, n, arr, _ptr(e), _ptr(gauss.tp),
+             _ptr(gauss.tw), gauss.get_npoints(), float(gauss.get_tlog()),
+             0 if coda else DAGE_OP_NO_CODA, int(precision), ctypes.byref(h))
+        self.handle = h
+        self.n = n
+        g = ctypes.c_int()
+        call("dage_batch_group_count", h, ctypes.byref(g))
+        self.ngroups = g.value
+
+    def close(self):
+        if self.handle:
+            _lib.load().dage_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, i: int, cube: np.ndarray):
+        a = np.asfortranarray(cube, dtype=np.float64)
+        if a.shape != self.grids[i].get_shape():
+            raise ValueError(f"cube {i} shape {a.shape} != grid shape {self.grids[i].get_shape()}")
+        call("dage_batch_upload", self.handle, i, a.ctypes.data_as(_dp))
+
+    def download(self, i: int) -> np.ndarray:
+        out = np.zeros(self.grids[i].get_shape(), order="F")
+        call("dage_batch_download", self.handle, i, out.ctypes.data_as(_dp))
+        return out
+
+    def input_ptr(self, i: int) -> int:
+        p = ctypes.c_void_p()
+        call("dage_batch_input", self.handle, i, ctypes.byref(p))
+        return p.value
+
+    def output_ptr(self, i: int) -> int:
+        p = ctypes.c_void_p()
+        call("dage_batch_output", self.handle, i, ctypes.byref(p))
+        return p.value
+
+    def set_energies(self, energies):
+        e = _f64(energies)
+        if len(e) != self.n:
+            raise ValueError("one energy per orbital")
+        call("dage_batch_set_energies", self.handle, _ptr(e))
+
+    def apply(self):
+        """All orbitals: one graph launch (asynchronous on the context stream)."""
+        call("dage_batch_apply", self.handle)
+
+    def apply_eager(self):
+        call("dage_batch_apply_eager", self.handle)
+
+    def tune(self, reps: int = 2) -> np.ndarray:
+        tiles = (ctypes.c_int * (3 * self.ngroups))()
+        call("dage_batch_tune", self.handle, int(reps), tiles)
+        return np.array(tiles[:], dtype=int).reshape(self.ngroups, 3)

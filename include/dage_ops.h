@@ -65,6 +65,7 @@ typedef struct dage_operator_s *dage_operator;
 typedef struct dage_cube_s *dage_cube;
 typedef struct dage_bubbles_s *dage_bubbles;
 typedef struct dage_gbfmm_s *dage_gbfmm;
+typedef struct dage_batch_s *dage_batch;
 
 /* ----------------------------------------------------------------- library */
 int dage_abi_version(void);
@@ -256,6 +257,67 @@ int dage_fmm_interaction_matrix(int lmax, const double dR[3], double *T);
 /* TranslationMatrix%eval (multipole.F90:444-470) unpacked: W, (lmax+1)^2
  * square column-major, unit lower triangular, q(new) = W(old - new) q(old). */
 int dage_fmm_translation_matrix(int lmax, const double dR[3], double *W);
+
+/* ---------------------------------------------------------- batched applies
+ * Helmholtz applies of many orbitals at once: the per-orbital loop of
+ * HelmholtzSCFCycle_update_orbitals_worker (src/bubbles/scf_cycle.F90:
+ * 2927-3054) that sets each orbital's energy (Helmholtz3D_set_energy,
+ * helmholtz3d.F90:411-444) and applies the operator once per orbital.
+ * Right-hand side i has its own grid (in = out grid; cube sizes may differ)
+ * and energy E_i; orbitals on identical grids share matrices.  The batch owns
+ * one device input and output cube per right-hand side (dage_batch_input /
+ * _output) and captures all of its work into one hipGraph, replayed by
+ * dage_batch_apply with a single graph launch on the context stream.
+ * precision: DAGE_PREC_*; flags: DAGE_OP_NO_CODA. */
+int dage_helmholtz_batch_create(dage_ctx ctx, int nrhs, const dage_grid3d *grids,
+                                const double *energies, const double *tp,
+                                const double *tw, int ntp, double tlog,
+                                int flags, int precision, dage_batch *batch);
+int dage_batch_input(dage_batch batch, int i, double **cube_dev);
+int dage_batch_output(dage_batch batch, int i, double **cube_dev);
+/* Synchronous host copies into input cube i / out of output cube i. */
+int dage_batch_upload(dage_batch batch, int i, const double *host);
+int dage_batch_download(dage_batch batch, int i, double *host);
+/* New energies (device weights are rewritten; the graph is kept). */
+int dage_batch_set_energies(dage_batch batch, const double *energies);
+/* out_i = G(E_i) in_i for every i: one hipGraphLaunch (asynchronous). */
+int dage_batch_apply(dage_batch batch);
+/* The same work launched kernel by kernel (for profiling). */
+int dage_batch_apply_eager(dage_batch batch);
+/* Autotuner: times every GEMM column-block count for each pass of each grid
+ * group (reps timed runs each), keeps the fastest and re-captures the graph.
+ * tiles (optional, 3 per group) receives the choices.  Overwrites outputs. */
+int dage_batch_tune(dage_batch batch, int reps, int *tiles);
+int dage_batch_group_count(dage_batch batch, int *ngroups);
+int dage_batch_destroy(dage_batch batch);
+
+/* -------------------------------------------------------------- cube algebra
+ * The cube part of the Function3D arithmetic of the Helmholtz SCF cycle
+ * (src/bubbles/scf_cycle.F90:2927-3054: product_in_place_REAL64,
+ * add_in_place, subtract_in_place; electron density :2333-2388; overlap
+ * matrix :2399-2424; orbital rotations :936-968).  Device buffers of n
+ * doubles; asynchronous on the context stream unless stated. */
+/* y = alpha*x + beta*y */
+int dage_vec_axpby(dage_ctx ctx, long n, double alpha, const double *x,
+                   double beta, double *y);
+/* out = x*y (elementwise; out may alias x or y) */
+int dage_vec_mul(dage_ctx ctx, long n, const double *x, const double *y,
+                 double *out);
+/* rho += occ*x^2 */
+int dage_vec_square_acc(dage_ctx ctx, long n, double occ, const double *x,
+                        double *rho);
+/* Slater exchange of the density: vx = -(3 rho/pi)^(1/3) and (ex may be NULL)
+ * ex = -(3/4)(3/pi)^(1/3) rho^(4/3) (negative rho counts as 0). */
+int dage_vec_lda_x(dage_ctx ctx, long n, const double *rho, double *vx,
+                   double *ex);
+/* out[j] = sum_i coef[i + nin*j] in[i], j < nout (no output may alias an
+ * input); synchronises. */
+int dage_vec_lincomb(dage_ctx ctx, long n, int nin, const double *const *in,
+                     const double *coef, int nout, double *const *out);
+/* S[i + nf*j] = sum_p wx wy wz f_i(p) g_j(p) with the grid's LIP weights
+ * (overlap matrix); deterministic; synchronises and writes S on the host. */
+int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
+                   const double *const *g, double *S);
 
 #ifdef __cplusplus
 }

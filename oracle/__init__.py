@@ -516,3 +516,33 @@ def gbfmm_potential(axes, rho, maxlevel, lmax, tp, tw, start_level=2):
                                                  blk.ctypes.data_as(_dp), cut[0], cut[1])
         pot[lo[0]:lo[0] + cut[0], lo[1]:lo[1] + cut[1], lo[2]:lo[2] + cut[2]] += blk
     return pot, q, L
+
+
+def yukawa_gauss_potential(q, alpha, centers, k, x, y, z):
+    """Closed form of the Helmholtz (Yukawa, exp(-k r)/r) potential of a sum of
+    normalised Gaussians q (alpha/pi)^1.5 exp(-alpha |r-R|^2):
+        phi(r) = e^{k^2/4a}/(2r) [e^{-kr} erfc(k/2sqrt(a) - sqrt(a) r)
+                                  - e^{kr} erfc(k/2sqrt(a) + sqrt(a) r)],
+    evaluated with erfcx for stability; phi(0) from the series limit.  This is
+    the mathematics the Helmholtz3D operator approximates (helmholtz3d.F90:
+    411-444), not a reference routine."""
+    from scipy.special import erfc, erfcx
+    X, Y, Z = np.meshgrid(_f64(x), _f64(y), _f64(z), indexing="ij")
+    out = np.zeros(X.shape)
+    c = np.asarray(centers, float).reshape(-1, 3)
+    for qi, a, R in zip(np.asarray(q, float), np.asarray(alpha, float), c):
+        r = np.sqrt((X - R[0]) ** 2 + (Y - R[1]) ** 2 + (Z - R[2]) ** 2)
+        sa = np.sqrt(a)
+        A = k / (2 * sa)
+        B = sa * r
+        d1 = A - B
+        t1 = np.where(d1 >= 0, np.exp(-B * B) * erfcx(np.maximum(d1, 0)),
+                      np.exp(A * A - 2 * A * B) * erfc(d1))
+        t2 = np.exp(-B * B) * erfcx(A + B)
+        small = r < 1e-6
+        rs = np.where(small, 1.0, r)
+        phi = (t1 - t2) / (2 * rs)
+        # r -> 0: phi(0) = 2 sqrt(a/pi) - k e^{k^2/4a} erfc(k/(2 sqrt a))
+        phi0 = 2 * sa / np.sqrt(np.pi) - k * erfcx(A)
+        out += qi * np.where(small, phi0, phi)
+    return np.asfortranarray(out)
