@@ -103,6 +103,21 @@ _SIGNATURES = {
     "dage_vec_lincomb": [_c_p, ctypes.c_long, _int, ctypes.POINTER(_c_p), _dp, _int,
                          ctypes.POINTER(_c_p)],
     "dage_cube_gram": [_c_p, _int, ctypes.POINTER(_c_p), _int, ctypes.POINTER(_c_p), _dp],
+    "dage_cube_gradient": [_c_p, _int, _c_p, _c_p],
+    "dage_cube_stiffness": [_c_p, _int, _c_p, _c_p],
+    "dage_conv3d_forward": [_c_p, _int, _int, _int, _int, _int, _c_p, _c_p, _c_p, _int, _c_p],
+    "dage_conv3d_backward_data": [_c_p, _int, _int, _int, _int, _int, _c_p, _c_p, _c_p],
+    "dage_conv3d_backward_filter": [_c_p, _int, _int, _int, _int, _int, _c_p, _c_p, _c_p, _c_p],
+    "dage_relu_backward": [_c_p, ctypes.c_long, _c_p, _c_p],
+    "dage_adam_step": [_c_p, _int, _c_p, _c_p, _c_p, _c_p, ctypes.c_float, ctypes.c_float,
+                       ctypes.c_float, ctypes.c_float, _int, _c_p],
+    "dage_weights_to_bf16": [_c_p, _int, _c_p, _c_p],
+    "dage_mse_loss": [_c_p, ctypes.c_long, _c_p, _c_p, _c_p, _dp],
+    "dage_cvt_f64_f32": [_c_p, ctypes.c_long, _c_p, _dbl, _c_p],
+    "dage_axpy_f32_f64": [_c_p, ctypes.c_long, _dbl, _c_p, _c_p],
+    "dage_device_alloc": [_c_p, ctypes.c_size_t, ctypes.POINTER(_c_p)],
+    "dage_device_free": [_c_p, _c_p],
+    "dage_memcpy": [_c_p, _c_p, _c_p, ctypes.c_size_t, _int],
 }
 _RESTYPES = {"dage_last_error": ctypes.c_char_p}
 

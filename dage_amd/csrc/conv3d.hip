@@ -1,0 +1,433 @@
+// 3x3x3 convolution layers for the learned orbital-update preconditioner:
+// BF16 weights, FP32 activations and FP32 accumulation, zero padding,
+// forward / backward-data / backward-filter, plus the Adam step, the MSE
+// loss and FP64 <-> FP32 conversions between SCF cubes and activations.
+//
+// Layout: activations [c][z][y][x] (x fastest: the cube layout, one cube per
+// channel); weights [cout][cin][27] with tap = (dx+1) + 3 (dy+1) + 9 (dz+1);
+// BF16 weights are the upper 16 bits of the FP32 master weights rounded to
+// nearest even.  Every reduction (filter gradient, loss) is a fixed-order
+// two-stage sum, so training is deterministic.
+//
+// Roofline: these are small convolutions (<= 16 channels), VALU-bound in
+// FP32 (forward 2 * cin * cout * 27 FLOPs per voxel); the backward filter
+// stages, per output x-row and per dz, the 3 neighbouring input rows of every
+// channel in LDS, and every thread owns one (cout, cin) pair's 27 taps.
+#include <cstdint>
+#include <vector>
+
+#include "dage_internal.hpp"
+
+namespace dage {
+namespace {
+
+constexpr int kMaxC = 16;
+
+int use_device(dage_ctx ctx) {
+    DAGE_REQUIRE(ctx != nullptr, DAGE_ERR_ARG, "null context handle");
+    DAGE_HIP_CHECK(hipSetDevice(ctx->device));
+    return DAGE_OK;
+}
+
+int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 65536); }
+
+__device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ inline uint16_t f32_to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);                    // nearest even
+    return (uint16_t)(r >> 16);
+}
+
+template <int COUT>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(const float *in, int cin,
+                                                       const uint16_t *w, const float *bias,
+                                                       int relu, float *out, int nx, int ny,
+                                                       int nz) {
+    __shared__ float ws[COUT * kMaxC * 27];
+    const int nw = COUT * cin * 27;
+    for (int i = threadIdx.x; i < nw; i += 256) ws[i] = bf16_to_f32(w[i]);
+    __syncthreads();
+    const long vol = (long)nx * ny * nz;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < vol;
+         p += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(p % nx);
+        const long r = p / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        float acc[COUT];
+#pragma unroll
+        for (int o = 0; o < COUT; ++o) acc[o] = bias[o];
+        for (int c = 0; c < cin; ++c) {
+            const float *ic = in + (long)c * vol;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+                const int xx = x + dx, yy = y + dy, zz = z + dz;
+                const bool ok = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+                const float v = ok ? ic[xx + (long)nx * (yy + (long)ny * zz)] : 0.0f;
+#pragma unroll
+                for (int o = 0; o < COUT; ++o) acc[o] += ws[(o * cin + c) * 27 + t] * v;
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < COUT; ++o)
+            out[(long)o * vol + p] = (relu && acc[o] < 0.0f) ? 0.0f : acc[o];
+    }
+}
+
+// gin[c][p] = sum_o sum_t W[o][c][t] gout[o][p - off(t)]
+__global__ void __launch_bounds__(256) conv_bwd_data_kernel(const float *gout, int cout, int cin,
+                                                            const uint16_t *w, float *gin, int nx,
+                                                            int ny, int nz) {
+    __shared__ float ws[kMaxC * kMaxC * 27];
+    const int nw = cout * cin * 27;
+    for (int i = threadIdx.x; i < nw; i += 256) ws[i] = bf16_to_f32(w[i]);
+    __syncthreads();
+    const long vol = (long)nx * ny * nz;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < vol;
+         p += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(p % nx);
+        const long r = p / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        float acc[kMaxC];
+#pragma unroll
+        for (int c = 0; c < kMaxC; ++c) acc[c] = 0.0f;
+        for (int o = 0; o < cout; ++o) {
+            const float *go = gout + (long)o * vol;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+                const int xx = x - dx, yy = y - dy, zz = z - dz;
+                const bool ok = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+                const float g = ok ? go[xx + (long)nx * (yy + (long)ny * zz)] : 0.0f;
+#pragma unroll
+                for (int c = 0; c < kMaxC; ++c)
+                    if (c < cin) acc[c] += ws[(o * cin + c) * 27 + t] * g;
+            }
+        }
+        for (int c = 0; c < cin; ++c) gin[(long)c * vol + p] = acc[c];
+    }
+}
+
+// Filter gradient, stage 1: block b takes x-rows (y, z) in [b*rows, ...);
+// thread t = o*cin + c accumulates gW[o][c][0..26] and (c == 0) gb[o].  Per
+// row and per dz the 3 input rows (y-1..y+1) of every channel are staged in
+// LDS with a one-point halo in x.
+__global__ void __launch_bounds__(256) conv_bwd_filter_kernel(const float *in, const float *gout,
+                                                              int cin, int cout, int nx, int ny,
+                                                              int nz, int rows_per_block,
+                                                              float *part) {
+    extern __shared__ float sm[];
+    const int W = nx + 2;
+    float *sin_ = sm;                  // [cin][3][W]
+    float *sgo = sm + cin * 3 * W;     // [cout][nx]
+    const int tid = threadIdx.x;
+    const int o = tid / cin, c = tid % cin;
+    const bool active = tid < cin * cout;
+    float acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = 0.0f;
+    float accb = 0.0f;
+    const long vol = (long)nx * ny * nz;
+    const long nrows = (long)ny * nz;
+    const long r0 = (long)blockIdx.x * rows_per_block;
+    const long r1 = r0 + rows_per_block < nrows ? r0 + rows_per_block : nrows;
+    for (long row = r0; row < r1; ++row) {
+        const int y = (int)(row % ny), z = (int)(row / ny);
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) {
+            __syncthreads();
+            const int zz = z + kz - 1;
+            for (int e = tid; e < cin * 3 * W; e += 256) {
+                const int cc = e / (3 * W);
+                const int rem = e - cc * 3 * W;
+                const int ky = rem / W, xx = rem % W - 1;
+                const int yy = y + ky - 1;
+                const bool ok = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+                sin_[e] = ok ? in[(long)cc * vol + xx + (long)nx * (yy + (long)ny * zz)] : 0.0f;
+            }
+            if (kz == 0)
+                for (int e = tid; e < cout * nx; e += 256) {
+                    const int oo = e / nx, xx = e % nx;
+                    sgo[e] = gout[(long)oo * vol + xx + (long)nx * (y + (long)ny * z)];
+                }
+            __syncthreads();
+            if (active) {
+                const float *go = sgo + o * nx;
+                const float *ic = sin_ + c * 3 * W;
+                for (int x = 0; x < nx; ++x) {
+                    const float g = go[x];
+                    if (kz == 0 && c == 0) accb += g;
+#pragma unroll
+                    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                        for (int dx = 0; dx < 3; ++dx)
+                            acc[9 * kz + 3 * ky + dx] += g * ic[ky * W + x + dx];
+                }
+            }
+        }
+    }
+    if (active) {
+        float *pp = part + (long)blockIdx.x * (cout * cin * 27 + cout);
+#pragma unroll
+        for (int t = 0; t < 27; ++t) pp[(o * cin + c) * 27 + t] = acc[t];
+        if (c == 0) pp[cout * cin * 27 + o] = accb;
+    }
+}
+
+__global__ void __launch_bounds__(256) sum_blocks_kernel(const float *part, int nblocks, int len,
+                                                         float *out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int b = 0; b < nblocks; ++b) s += part[(long)b * len + i];
+        out[i] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) relu_bwd_kernel(long n, const float *out, float *g) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        if (out[i] <= 0.0f) g[i] = 0.0f;
+}
+
+__global__ void __launch_bounds__(256) adam_kernel(int n, float *w, const float *g, float *m,
+                                                   float *v, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2,
+                                                   uint16_t *wbf) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        wbf[i] = f32_to_bf16(wi);
+    }
+}
+
+__global__ void __launch_bounds__(256) to_bf16_kernel(int n, const float *w, uint16_t *wbf) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        wbf[i] = f32_to_bf16(w[i]);
+}
+
+// grad = scale*(pred - target); part[block] = sum over the block's range of (pred - target)^2
+__global__ void __launch_bounds__(256) mse_kernel(long n, const float *pred, const float *target,
+                                                  float scale, float *grad, double *part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const float d = pred[i] - target[i];
+        grad[i] = scale * d;
+        s += (double)d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) cvt_f64_f32_kernel(long n, const double *x, double scale,
+                                                          float *y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)(scale * x[i]);
+}
+
+__global__ void __launch_bounds__(256) axpy_f32_f64_kernel(long n, double a, const float *x,
+                                                           double *y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        y[i] = y[i] + a * (double)x[i];
+}
+
+int check_dims(int nx, int ny, int nz, int cin, int cout) {
+    DAGE_REQUIRE(nx > 0 && ny > 0 && nz > 0, DAGE_ERR_ARG, "empty volume");
+    DAGE_REQUIRE(cin >= 1 && cin <= kMaxC && cout >= 1 && cout <= kMaxC, DAGE_ERR_UNSUPPORTED,
+                 "channels %d -> %d outside 1..%d", cin, cout, kMaxC);
+    return DAGE_OK;
+}
+
+}  // namespace
+}  // namespace dage
+
+using namespace dage;
+
+extern "C" {
+
+int dage_conv3d_forward(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout, const float *in,
+                        const uint16_t *w, const float *bias, int relu, float *out) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_TRY(check_dims(nx, ny, nz, cin, cout));
+    DAGE_REQUIRE(in && w && bias && out && in != out, DAGE_ERR_ARG, "bad argument");
+    DAGE_REQUIRE(cout == 1 || cout == kMaxC, DAGE_ERR_UNSUPPORTED,
+                 "forward supports 1 or %d output channels", kMaxC);
+    const long vol = (long)nx * ny * nz;
+    if (cout == 1)
+        hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, in,
+                           cin, w, bias, relu, out, nx, ny, nz);
+    else
+        hipLaunchKernelGGL(conv_fwd_kernel<kMaxC>, dim3(grid_of(vol)), dim3(256), 0, ctx->stream,
+                           in, cin, w, bias, relu, out, nx, ny, nz);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_conv3d_backward_data(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
+                              const float *gout, const uint16_t *w, float *gin) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_TRY(check_dims(nx, ny, nz, cin, cout));
+    DAGE_REQUIRE(gout && w && gin && gout != gin, DAGE_ERR_ARG, "bad argument");
+    const long vol = (long)nx * ny * nz;
+    hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, gout,
+                       cout, cin, w, gin, nx, ny, nz);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_conv3d_backward_filter(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
+                                const float *in, const float *gout, float *gw, float *gb) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_TRY(check_dims(nx, ny, nz, cin, cout));
+    DAGE_REQUIRE(in && gout && gw && gb, DAGE_ERR_ARG, "bad argument");
+    const size_t smem = sizeof(float) * ((size_t)cin * 3 * (nx + 2) + (size_t)cout * nx);
+    DAGE_REQUIRE(smem <= 64 * 1024, DAGE_ERR_UNSUPPORTED, "x extent %d too long for one block",
+                 nx);
+    const long nrows = (long)ny * nz;
+    const int rows = (int)std::max<long>(1, (nrows + 1023) / 1024);
+    const int nblocks = (int)((nrows + rows - 1) / rows);
+    const int len = cout * cin * 27 + cout;
+    float *part = nullptr;
+    DAGE_HIP_CHECK(hipMalloc((void **)&part, sizeof(float) * (size_t)nblocks * len));
+    hipLaunchKernelGGL(conv_bwd_filter_kernel, dim3(nblocks), dim3(256), smem, ctx->stream, in,
+                       gout, cin, cout, nx, ny, nz, rows, part);
+    int rc = DAGE_OK;
+    if (hipGetLastError() != hipSuccess) {
+        set_error("filter gradient launch failed");
+        rc = DAGE_ERR_HIP;
+    }
+    float *sums = nullptr;
+    if (rc == DAGE_OK && hipMalloc((void **)&sums, sizeof(float) * len) != hipSuccess) {
+        set_error("device allocation failed");
+        rc = DAGE_ERR_HIP;
+    }
+    if (rc == DAGE_OK) {
+        hipLaunchKernelGGL(sum_blocks_kernel, dim3((len + 255) / 256), dim3(256), 0, ctx->stream,
+                           part, nblocks, len, sums);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(gw, sums, sizeof(float) * (len - cout), hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(gb, sums + (len - cout), sizeof(float) * cout, hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            set_error("filter gradient reduction failed");
+            rc = DAGE_ERR_HIP;
+        }
+    }
+    (void)hipFree(part);
+    if (sums) (void)hipFree(sums);
+    return rc;
+}
+
+int dage_relu_backward(dage_ctx ctx, long n, const float *out, float *grad) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && out && grad, DAGE_ERR_ARG, "bad argument");
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, out, grad);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_adam_step(dage_ctx ctx, int n, float *w, const float *g, float *m, float *v, float lr,
+                   float beta1, float beta2, float eps, int step, uint16_t *w_bf16) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && w && g && m && v && w_bf16 && step >= 1, DAGE_ERR_ARG, "bad argument");
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, w, g, m, v,
+                       lr, beta1, beta2, eps, c1, c2, w_bf16);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_weights_to_bf16(dage_ctx ctx, int n, const float *w, uint16_t *w_bf16) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && w && w_bf16, DAGE_ERR_ARG, "bad argument");
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, w, w_bf16);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_mse_loss(dage_ctx ctx, long n, const float *pred, const float *target, float *grad,
+                  double *loss) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && pred && target && grad && loss, DAGE_ERR_ARG, "bad argument");
+    const int nb = std::min<long>(1024, (n + 255) / 256);
+    double *part = nullptr;
+    DAGE_HIP_CHECK(hipMalloc((void **)&part, sizeof(double) * nb));
+    hipLaunchKernelGGL(mse_kernel, dim3(nb), dim3(256), 0, ctx->stream, n, pred, target,
+                       (float)(2.0 / (double)n), grad, part);
+    std::vector<double> h(nb);
+    int rc = DAGE_OK;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("loss failed");
+        rc = DAGE_ERR_HIP;
+    }
+    double s = 0.0;
+    for (double v : h) s += v;
+    *loss = s / (double)n;
+    (void)hipFree(part);
+    return rc;
+}
+
+int dage_cvt_f64_f32(dage_ctx ctx, long n, const double *x, double scale, float *y) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && x && y, DAGE_ERR_ARG, "bad argument");
+    hipLaunchKernelGGL(cvt_f64_f32_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, x,
+                       scale, y);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_axpy_f32_f64(dage_ctx ctx, long n, double a, const float *x, double *y) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(n > 0 && x && y, DAGE_ERR_ARG, "bad argument");
+    hipLaunchKernelGGL(axpy_f32_f64_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, a, x,
+                       y);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int dage_device_alloc(dage_ctx ctx, size_t bytes, void **ptr) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(ptr != nullptr && bytes > 0, DAGE_ERR_ARG, "bad argument");
+    DAGE_HIP_CHECK(hipMalloc(ptr, bytes));
+    return DAGE_OK;
+}
+
+int dage_device_free(dage_ctx ctx, void *ptr) {
+    DAGE_TRY(use_device(ctx));
+    if (ptr) DAGE_HIP_CHECK(hipFree(ptr));
+    return DAGE_OK;
+}
+
+int dage_memcpy(dage_ctx ctx, void *dst, const void *src, size_t bytes, int kind) {
+    DAGE_TRY(use_device(ctx));
+    DAGE_REQUIRE(dst && src, DAGE_ERR_ARG, "bad argument");
+    DAGE_REQUIRE(kind >= 0 && kind <= 2, DAGE_ERR_ARG, "kind 0 (h2d), 1 (d2h) or 2 (d2d)");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                      : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    DAGE_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return DAGE_OK;
+}
+
+}  // extern "C"

@@ -94,6 +94,99 @@ __global__ void __launch_bounds__(256) gram_kernel(const double *const *f, const
     if (threadIdx.x == 0) part[(long)pair * nz + z] = red[0] * wz[z];
 }
 
+// df = d f / d(axis) from the cells' Lagrange interpolants: at a point
+// inside a cell the derivative of that cell's interpolant, at a point shared
+// by two cells the mean of both.  D[j*nlip + k] = L_k'(node j) for unit node
+// spacing; the cell's step is recovered from its end points.
+__global__ void __launch_bounds__(256) gradient_kernel(const double *f, double *df, int nx, int ny,
+                                                       int nz, int axis, const double *pts,
+                                                       int nlip, const double *D) {
+    const long total = (long)nx * ny * nz;
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const long stride = axis == 0 ? 1 : (axis == 1 ? (long)nx : (long)nx * ny);
+    const int np1 = nlip - 1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % nx);
+        const long r = e / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        const int i = axis == 0 ? x : (axis == 1 ? y : z);
+        const long base = e - (long)i * stride;  // the line's first point
+        double sum = 0.0;
+        int cnt = 0;
+        for (int side = 0; side < 2; ++side) {
+            // side 0: the cell on the left of a shared point / the cell
+            // containing an interior point; side 1: the cell on the right
+            int c;
+            if (side == 0) {
+                if (i % np1 == 0) {
+                    if (i == 0) continue;
+                    c = i / np1 - 1;
+                } else {
+                    c = i / np1;
+                }
+            } else {
+                if (i % np1 != 0 || i == n - 1) continue;
+                c = i / np1;
+            }
+            const int j = i - c * np1;
+            const double h = (pts[(c + 1) * np1] - pts[c * np1]) / np1;
+            double d = 0.0;
+            for (int k = 0; k < nlip; ++k) d += D[j * nlip + k] * f[base + (long)(c * np1 + k) * stride];
+            sum += d / h;
+            ++cnt;
+        }
+        df[e] = sum / cnt;
+    }
+}
+
+// out = (K f) / w along `axis`: K the assembled stiffness matrix of the
+// cells' Lagrange interpolants, (K f)_i = sum over the cells c holding i of
+// sum_k Kc[j*nlip + k] f(c, k) / h_c with Kc[a*nlip + b] = int L_a' L_b' over
+// a cell of unit node spacing; w the point's LIP weight along the axis.  Then
+// sum_p wx wy wz g (out) = sum_lines w_perp int g' f' (the interpolants'
+// exact kinetic integral along that axis).
+__global__ void __launch_bounds__(256) stiffness_kernel(const double *f, double *out, int nx,
+                                                        int ny, int nz, int axis,
+                                                        const double *pts, const double *wts,
+                                                        int nlip, const double *Kc) {
+    const long total = (long)nx * ny * nz;
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const long stride = axis == 0 ? 1 : (axis == 1 ? (long)nx : (long)nx * ny);
+    const int np1 = nlip - 1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % nx);
+        const long r = e / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        const int i = axis == 0 ? x : (axis == 1 ? y : z);
+        const long base = e - (long)i * stride;
+        double sum = 0.0;
+        for (int side = 0; side < 2; ++side) {
+            int c;
+            if (side == 0) {
+                if (i % np1 == 0) {
+                    if (i == 0) continue;
+                    c = i / np1 - 1;
+                } else {
+                    c = i / np1;
+                }
+            } else {
+                if (i % np1 != 0 || i == n - 1) continue;
+                c = i / np1;
+            }
+            const int j = i - c * np1;
+            const double h = (pts[(c + 1) * np1] - pts[c * np1]) / np1;
+            double d = 0.0;
+            for (int k = 0; k < nlip; ++k) d += Kc[j * nlip + k] * f[base + (long)(c * np1 + k) * stride];
+            sum += d / h;
+        }
+        out[e] = sum / wts[i];
+    }
+}
+
 template <typename T>
 int to_device(const T *host, size_t n, T **dev) {
     DAGE_HIP_CHECK(hipMalloc((void **)dev, sizeof(T) * (n ? n : 1)));
@@ -173,6 +266,103 @@ int dage_vec_lincomb(dage_ctx ctx, long n, int nin, const double *const *in, con
     (void)hipFree(d_in);
     (void)hipFree(d_out);
     (void)hipFree(d_coef);
+    return rc;
+}
+
+int dage_cube_gradient(dage_grid3d grid, int axis, const double *f, double *df) {
+    DAGE_REQUIRE(grid && f && df && axis >= 0 && axis < 3, DAGE_ERR_ARG, "bad argument");
+    DAGE_REQUIRE(f != df, DAGE_ERR_ARG, "gradient output aliases its input");
+    dage_ctx ctx = grid->ctx;
+    DAGE_TRY(use_device(ctx));
+    const Axis &ax = grid->axis[axis];
+    const int nlip = ax.nlip;
+    // node positions in units of the cell step (0..nlip-1 for equidistant cells)
+    std::vector<double> node(nlip), wb(nlip), D((size_t)nlip * nlip, 0.0);
+    for (int k = 0; k < nlip; ++k)
+        node[k] = (ax.points[k] - ax.points[0]) / ax.steps[0];
+    for (int k = 0; k < nlip; ++k) {
+        double p = 1.0;
+        for (int m = 0; m < nlip; ++m)
+            if (m != k) p *= node[k] - node[m];
+        wb[k] = 1.0 / p;
+    }
+    for (int j = 0; j < nlip; ++j) {
+        double diag = 0.0;
+        for (int k = 0; k < nlip; ++k)
+            if (k != j) {
+                D[j * nlip + k] = (wb[k] / wb[j]) / (node[j] - node[k]);
+                diag -= D[j * nlip + k];
+            }
+        D[j * nlip + j] = diag;
+    }
+    // cell step = (last - first)/(nlip-1) must equal the node spacing above
+    // for every cell: the node template is shared (grid_type 1 equidistant)
+    DAGE_REQUIRE(ax.grid_type == 1, DAGE_ERR_UNSUPPORTED, "gradient needs equidistant cell nodes");
+    double *d_D = nullptr;
+    DAGE_TRY(to_device(D.data(), D.size(), &d_D));
+    const long total = (long)grid->shape(0) * grid->shape(1) * grid->shape(2);
+    hipLaunchKernelGGL(gradient_kernel, dim3(grid_of(total)), dim3(256), 0, ctx->stream, f, df,
+                       grid->shape(0), grid->shape(1), grid->shape(2), axis,
+                       grid->d_points[axis], nlip, d_D);
+    int rc = DAGE_OK;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("gradient failed");
+        rc = DAGE_ERR_HIP;
+    }
+    (void)hipFree(d_D);
+    return rc;
+}
+
+int dage_cube_stiffness(dage_grid3d grid, int axis, const double *f, double *out) {
+    DAGE_REQUIRE(grid && f && out && axis >= 0 && axis < 3, DAGE_ERR_ARG, "bad argument");
+    DAGE_REQUIRE(f != out, DAGE_ERR_ARG, "stiffness output aliases its input");
+    dage_ctx ctx = grid->ctx;
+    DAGE_TRY(use_device(ctx));
+    const Axis &ax = grid->axis[axis];
+    DAGE_REQUIRE(ax.grid_type == 1, DAGE_ERR_UNSUPPORTED, "stiffness needs equidistant cell nodes");
+    const int nlip = ax.nlip;
+    // derivative matrix at the nodes 0..nlip-1 (barycentric form) and the
+    // nlip-point Gauss-Legendre rule on [0, nlip-1], exact for L_a' L_b'
+    std::vector<double> wb(nlip), Kc((size_t)nlip * nlip, 0.0);
+    for (int k = 0; k < nlip; ++k) {
+        double p = 1.0;
+        for (int m = 0; m < nlip; ++m)
+            if (m != k) p *= (double)(k - m);
+        wb[k] = 1.0 / p;
+    }
+    std::vector<double> gx(nlip), gw(nlip);
+    gauleg(nlip, gx.data(), gw.data());  // on [-1, 1]
+    const double half = 0.5 * (nlip - 1);
+    for (int q = 0; q < nlip; ++q) {
+        const double xi = half * (gx[q] + 1.0), wq = half * gw[q];
+        // L_a'(xi) = L_a(xi) * sum_{m != a} 1/(xi - m), evaluated by products
+        std::vector<double> dL(nlip, 0.0);
+        for (int a = 0; a < nlip; ++a) {
+            double s = 0.0;
+            for (int m = 0; m < nlip; ++m) {
+                if (m == a) continue;
+                double p = wb[a];
+                for (int mm = 0; mm < nlip; ++mm)
+                    if (mm != a && mm != m) p *= xi - mm;
+                s += p;
+            }
+            dL[a] = s;
+        }
+        for (int a = 0; a < nlip; ++a)
+            for (int b = 0; b < nlip; ++b) Kc[a * nlip + b] += wq * dL[a] * dL[b];
+    }
+    double *d_K = nullptr;
+    DAGE_TRY(to_device(Kc.data(), Kc.size(), &d_K));
+    const long total = (long)grid->shape(0) * grid->shape(1) * grid->shape(2);
+    hipLaunchKernelGGL(stiffness_kernel, dim3(grid_of(total)), dim3(256), 0, ctx->stream, f, out,
+                       grid->shape(0), grid->shape(1), grid->shape(2), axis,
+                       grid->d_points[axis], grid->d_weights[axis], nlip, d_K);
+    int rc = DAGE_OK;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("stiffness apply failed");
+        rc = DAGE_ERR_HIP;
+    }
+    (void)hipFree(d_K);
     return rc;
 }
 

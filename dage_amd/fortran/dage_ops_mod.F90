@@ -15,6 +15,8 @@ module dage_ops_m
     integer(c_int), parameter, public :: DAGE_OK = 0
     integer(c_int), parameter, public :: DAGE_PREC_FP64 = 0
     integer(c_int), parameter, public :: DAGE_OP_NO_CODA = 1
+    integer(c_int), parameter :: DAGE_PREC_BF16X6 = 1
+    integer(c_int), parameter :: DAGE_PREC_BF16X3 = 2
 
     public :: dage_gauss_quad, dage_ctx_create, dage_ctx_destroy, dage_ctx_synchronize
     public :: dage_grid3d_create, dage_grid3d_destroy, dage_grid3d_shape
@@ -22,6 +24,11 @@ module dage_ops_m
     public :: dage_coulomb3d_create, dage_operator_set_weights, dage_operator_set_energy
     public :: dage_operator_apply, dage_operator_destroy, dage_last_error_f
     public :: dage_bubbles_create, dage_bubbles_set, dage_bubbles_destroy
+    public :: dage_gbfmm_create, dage_gbfmm_apply, dage_gbfmm_box_count
+    public :: dage_gbfmm_get_multipoles, dage_gbfmm_destroy
+    public :: dage_helmholtz_batch_create, dage_batch_upload, dage_batch_download
+    public :: dage_batch_set_energies, dage_batch_apply, dage_batch_tune, dage_batch_destroy
+    public :: DAGE_PREC_BF16X6, DAGE_PREC_BF16X3
     public :: DageOperator3D, dage_check
 
     interface
@@ -144,6 +151,93 @@ module dage_ops_m
         integer(c_int) function dage_bubbles_destroy(bubbles) bind(C, name="dage_bubbles_destroy")
             import :: c_int, c_ptr
             type(c_ptr), value :: bubbles
+        end function
+
+        ! GBFMMCoulomb3D (gbfmm_coulomb3d.F90:306-1330)
+        integer(c_int) function dage_gbfmm_create(ctx, grid, maxlevel, lmax, tp, tw, ntp, fmm) &
+                bind(C, name="dage_gbfmm_create")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: ctx, grid
+            integer(c_int), value :: maxlevel, lmax, ntp
+            real(c_double)        :: tp(*), tw(*)
+            type(c_ptr)           :: fmm
+        end function
+
+        integer(c_int) function dage_gbfmm_apply(fmm, rho, pot, precision) &
+                bind(C, name="dage_gbfmm_apply")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: fmm
+            real(c_double)        :: rho(*), pot(*)
+            integer(c_int), value :: precision
+        end function
+
+        integer(c_int) function dage_gbfmm_box_count(fmm, nbox) bind(C, name="dage_gbfmm_box_count")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: fmm
+            integer(c_int)     :: nbox
+        end function
+
+        integer(c_int) function dage_gbfmm_get_multipoles(fmm, q) &
+                bind(C, name="dage_gbfmm_get_multipoles")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: fmm
+            real(c_double)     :: q(*)
+        end function
+
+        integer(c_int) function dage_gbfmm_destroy(fmm) bind(C, name="dage_gbfmm_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: fmm
+        end function
+
+        ! per-orbital Helmholtz applies as one batch (scf_cycle.F90:2927-3054)
+        integer(c_int) function dage_helmholtz_batch_create(ctx, nrhs, grids, energies, tp, tw, &
+                ntp, tlog, flags, precision, batch) bind(C, name="dage_helmholtz_batch_create")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: ctx
+            integer(c_int), value :: nrhs, ntp, flags, precision
+            type(c_ptr)           :: grids(*)
+            real(c_double)        :: energies(*), tp(*), tw(*)
+            real(c_double), value :: tlog
+            type(c_ptr)           :: batch
+        end function
+
+        integer(c_int) function dage_batch_upload(batch, i, cube) bind(C, name="dage_batch_upload")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: batch
+            integer(c_int), value :: i
+            real(c_double)        :: cube(*)
+        end function
+
+        integer(c_int) function dage_batch_download(batch, i, cube) &
+                bind(C, name="dage_batch_download")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: batch
+            integer(c_int), value :: i
+            real(c_double)        :: cube(*)
+        end function
+
+        integer(c_int) function dage_batch_set_energies(batch, energies) &
+                bind(C, name="dage_batch_set_energies")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: batch
+            real(c_double)     :: energies(*)
+        end function
+
+        integer(c_int) function dage_batch_apply(batch) bind(C, name="dage_batch_apply")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+        end function
+
+        integer(c_int) function dage_batch_tune(batch, reps, tiles) bind(C, name="dage_batch_tune")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: batch
+            integer(c_int), value :: reps
+            integer(c_int)        :: tiles(*)
+        end function
+
+        integer(c_int) function dage_batch_destroy(batch) bind(C, name="dage_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
         end function
 
         type(c_ptr) function dage_last_error() bind(C, name="dage_last_error")

@@ -314,10 +314,62 @@ int dage_vec_lda_x(dage_ctx ctx, long n, const double *rho, double *vx,
  * input); synchronises. */
 int dage_vec_lincomb(dage_ctx ctx, long n, int nin, const double *const *in,
                      const double *coef, int nout, double *const *out);
+/* df = d f / d(axis) on the grid's points (axis 0..2) from the cells' LIP
+ * interpolants (mean of both cells at shared cell ends); equidistant cell
+ * nodes (grid_type 1); synchronises. */
+int dage_cube_gradient(dage_grid3d grid, int axis, const double *f,
+                       double *df);
+/* out = (K f)/w along axis, K the assembled stiffness matrix of the cells'
+ * Lagrange interpolants (int L_a' L_b', exact) and w the LIP weights, so that
+ * the Gram of g with out is sum_lines w_perp int g' f' along that axis (the
+ * interpolants' exact kinetic integral); synchronises. */
+int dage_cube_stiffness(dage_grid3d grid, int axis, const double *f,
+                        double *out);
 /* S[i + nf*j] = sum_p wx wy wz f_i(p) g_j(p) with the grid's LIP weights
  * (overlap matrix); deterministic; synchronises and writes S on the host. */
 int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
                    const double *const *g, double *S);
+
+/* ----------------------------------------- learned preconditioner layers
+ * No reference counterpart (the learned orbital-update preconditioner is new
+ * in this build).  3x3x3 convolutions with zero padding: activations FP32
+ * [c][z][y][x], weights BF16 (uint16 bit patterns) [cout][cin][27] with tap
+ * (dx+1) + 3(dy+1) + 9(dz+1), FP32 accumulation; 1 <= cin, cout <= 16
+ * (forward: cout 1 or 16).  Device buffers; asynchronous unless stated. */
+int dage_conv3d_forward(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
+                        const float *in, const unsigned short *w_bf16,
+                        const float *bias, int relu, float *out);
+int dage_conv3d_backward_data(dage_ctx ctx, int nx, int ny, int nz, int cin,
+                              int cout, const float *grad_out,
+                              const unsigned short *w_bf16, float *grad_in);
+/* Deterministic filter and bias gradients; synchronises. */
+int dage_conv3d_backward_filter(dage_ctx ctx, int nx, int ny, int nz, int cin,
+                                int cout, const float *in, const float *grad_out,
+                                float *grad_w, float *grad_b);
+/* grad = 0 where out <= 0 */
+int dage_relu_backward(dage_ctx ctx, long n, const float *out, float *grad);
+/* Adam on FP32 master weights (bias-corrected with `step` >= 1), BF16 copy
+ * refreshed. */
+int dage_adam_step(dage_ctx ctx, int n, float *w, const float *grad, float *m,
+                   float *v, float lr, float beta1, float beta2, float eps,
+                   int step, unsigned short *w_bf16);
+int dage_weights_to_bf16(dage_ctx ctx, int n, const float *w,
+                         unsigned short *w_bf16);
+/* grad = 2 (pred - target)/n, *loss = mean (pred - target)^2; synchronises. */
+int dage_mse_loss(dage_ctx ctx, long n, const float *pred, const float *target,
+                  float *grad, double *loss);
+/* y = (float)(scale x) ; y += a x */
+int dage_cvt_f64_f32(dage_ctx ctx, long n, const double *x, double scale,
+                     float *y);
+int dage_axpy_f32_f64(dage_ctx ctx, long n, double a, const float *x,
+                      double *y);
+/* Raw device memory for callers without their own allocator (the Python
+ * mirror): allocate, free, and synchronous copies (kind 0 host->device,
+ * 1 device->host, 2 device->device). */
+int dage_device_alloc(dage_ctx ctx, size_t bytes, void **ptr);
+int dage_device_free(dage_ctx ctx, void *ptr);
+int dage_memcpy(dage_ctx ctx, void *dst, const void *src, size_t bytes,
+                int kind);
 
 #ifdef __cplusplus
 }

@@ -546,3 +546,49 @@ def yukawa_gauss_potential(q, alpha, centers, k, x, y, z):
         phi0 = 2 * sa / np.sqrt(np.pi) - k * erfcx(A)
         out += qi * np.where(small, phi0, phi)
     return np.asfortranarray(out)
+
+
+# ------------------------------------------------- learned-preconditioner layers
+def bf16_round(w):
+    """float32 -> nearest-even bfloat16 values (as float64)."""
+    u = np.asarray(w, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _taps():
+    for t in range(27):
+        yield t, t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1
+
+
+def _shift(a, dx, dy, dz):
+    """b[z, y, x] = a[z + dz, y + dy, x + dx] with zeros outside (a: [z][y][x])."""
+    b = np.zeros_like(a)
+    nz, ny, nx = a.shape
+    zs, ys, xs = (slice(max(0, -d), n - max(0, d)) for d, n in ((dz, nz), (dy, ny), (dx, nx)))
+    zd, yd, xd = (slice(max(0, d), n - max(0, -d)) for d, n in ((dz, nz), (dy, ny), (dx, nx)))
+    b[zs, ys, xs] = a[zd, yd, xd]
+    return b
+
+
+def conv3d_forward(x, w, b, relu):
+    """x [cin][z][y][x] float64, w [cout][cin][27] (already bf16-rounded), b [cout]."""
+    cout = w.shape[0]
+    out = np.zeros((cout,) + x.shape[1:])
+    for t, dx, dy, dz in _taps():
+        xs = np.stack([_shift(x[c], dx, dy, dz) for c in range(x.shape[0])])
+        out += np.einsum("oc,czyx->ozyx", w[:, :, t], xs)
+    out += b[:, None, None, None]
+    return np.maximum(out, 0.0) if relu else out
+
+
+def conv3d_backward(x, w, gout):
+    """Gradients of sum(gout * conv(x)) w.r.t. x, w and b (no activation)."""
+    gx = np.zeros_like(x)
+    gw = np.zeros_like(w)
+    for t, dx, dy, dz in _taps():
+        xs = np.stack([_shift(x[c], dx, dy, dz) for c in range(x.shape[0])])
+        gw[:, :, t] = np.einsum("ozyx,czyx->oc", gout, xs)
+        gs = np.stack([_shift(gout[o], -dx, -dy, -dz) for o in range(gout.shape[0])])
+        gx += np.einsum("oc,ozyx->czyx", w[:, :, t], gs)
+    return gx, gw, gout.sum(axis=(1, 2, 3))

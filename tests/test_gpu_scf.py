@@ -1,0 +1,146 @@
+"""Cube algebra, LIP gradients and the Helmholtz SCF (rows A27-A28) on the GPU.
+
+Pins: elementwise ops and the Gram reduction against numpy on the same
+cubes (1e-13); the LIP gradient against the analytic derivative of a
+Gaussian (interpolation error of the 5-point cells); the stiffness (exact
+kinetic integral of the cell interpolants) against int |grad f|^2 of a
+Gaussian; the one-electron SCF
+of a smeared nucleus against the radial eigenvalue computed by a dense 1D
+finite-difference eigensolver (independent discretisation); convergence by
+the reference rule (|dE|, |d eps| < 1e-6) for a two-centre molecule with
+Hartree and Slater exchange; the convergence rule itself is a CPU test
+(test_scf_cpu.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dage_amd
+import dage_amd.scf as scf
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def grid_of(ctx, ax):
+    return dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, ax.nlip)
+
+
+def test_cube_ops_and_gram(ctx):
+    ax = O.Axis.ncell_fixed(-3.0, 3.0, 5, 6)
+    g = grid_of(ctx, ax)
+    rng = np.random.default_rng(2)
+    shape = (ax.n,) * 3
+    a, b = (np.asfortranarray(rng.standard_normal(shape)) for _ in range(2))
+    da, db, dc = dage_amd.DeviceCube(ctx, shape, a), dage_amd.DeviceCube(ctx, shape, b), \
+        dage_amd.DeviceCube(ctx, shape)
+    n = a.size
+    dage_amd._lib.call("dage_vec_axpby", ctx.handle, n, 2.0, da.ptr, -0.5, db.ptr)
+    assert np.max(np.abs(db.download() - (2 * a - 0.5 * b))) < 1e-13
+    db.upload(b)
+    dage_amd._lib.call("dage_vec_mul", ctx.handle, n, da.ptr, db.ptr, dc.ptr)
+    assert np.array_equal(dc.download(), a * b)
+    rho = np.abs(a)
+    dr = dage_amd.DeviceCube(ctx, shape, rho)
+    dage_amd._lib.call("dage_vec_lda_x", ctx.handle, n, dr.ptr, dc.ptr, db.ptr)
+    assert np.max(np.abs(dc.download() + (3 * rho / math.pi) ** (1 / 3))) < 1e-13
+    assert np.max(np.abs(db.download() + 0.75 * (3 / math.pi) ** (1 / 3) * rho ** (4 / 3))) < 1e-12
+    w = ax.ints()
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    fs = [np.asfortranarray(rng.standard_normal(shape)) for _ in range(3)]
+    dfs = [dage_amd.DeviceCube(ctx, shape, f) for f in fs]
+    S = np.zeros(9)
+    import ctypes
+    arr = (ctypes.c_void_p * 3)(*[d.ptr for d in dfs])
+    dage_amd._lib.call("dage_cube_gram", g.handle, 3, arr, 3, arr,
+                       S.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    ref = np.array([[np.sum(W * fi * fj) for fj in fs] for fi in fs])
+    assert np.max(np.abs(S.reshape(3, 3, order="F") - ref)) < 1e-11 * np.max(np.abs(ref))
+    outs = [dage_amd.DeviceCube(ctx, shape) for _ in range(2)]
+    C = np.asfortranarray(rng.standard_normal((3, 2)))
+    oarr = (ctypes.c_void_p * 2)(*[o.ptr for o in outs])
+    dage_amd._lib.call("dage_vec_lincomb", ctx.handle, n, 3, arr,
+                       C.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 2, oarr)
+    for j in range(2):
+        assert np.max(np.abs(outs[j].download() - sum(C[i, j] * fs[i] for i in range(3)))) < 1e-12
+
+
+def test_gradient_of_gaussian(ctx):
+    ax = O.Axis.ncell_fixed(-5.0, 5.0, 5, 16)
+    g = grid_of(ctx, ax)
+    x = ax.points
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.exp(-0.7 * ((X - 0.3) ** 2 + (Y + 0.2) ** 2 + Z ** 2))
+    d_f = dage_amd.DeviceCube(ctx, f.shape, np.asfortranarray(f))
+    d_g = dage_amd.DeviceCube(ctx, f.shape)
+    for axis, D in enumerate((X - 0.3, Y + 0.2, Z)):
+        dage_amd._lib.call("dage_cube_gradient", g.handle, axis, d_f.ptr, d_g.ptr)
+        exact = -1.4 * D * f
+        assert np.max(np.abs(d_g.download() - exact)) < 2e-3, axis
+
+
+def test_stiffness_kinetic_integral(ctx):
+    """sum_axes Gram(f, K_a f / w_a) = int |grad f|^2 = 3 a (pi / 2a)^1.5 for
+    f = exp(-a r^2) (the cells' interpolants integrated exactly)."""
+    import ctypes
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 5, 24)  # h = 0.125
+    g = grid_of(ctx, ax)
+    x = ax.points
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    a = 0.8
+    f = np.asfortranarray(np.exp(-a * (X ** 2 + Y ** 2 + Z ** 2)))
+    d_f = dage_amd.DeviceCube(ctx, f.shape, f)
+    d_k = dage_amd.DeviceCube(ctx, f.shape)
+    total = 0.0
+    for axis in range(3):
+        dage_amd._lib.call("dage_cube_stiffness", g.handle, axis, d_f.ptr, d_k.ptr)
+        total += g.integrate(d_f, d_k)
+    exact = 3 * a * (math.pi / (2 * a)) ** 1.5
+    assert abs(total - exact) < 1e-6 * exact
+
+
+def radial_eigenvalue(Z, beta, rmax=30.0, n=40000):
+    """Lowest s eigenvalue of -1/2 u'' - Z erf(sqrt(beta) r)/r u by dense FD."""
+    from scipy.linalg import eigh_tridiagonal
+    from scipy.special import erf
+    h = rmax / (n + 1)
+    r = h * np.arange(1, n + 1)
+    V = -Z * erf(math.sqrt(beta) * r) / r
+    d = 1.0 / h ** 2 + V
+    e = np.full(n - 1, -0.5 / h ** 2)
+    w = eigh_tridiagonal(d, e, select="i", select_range=(0, 0), eigvals_only=True)
+    return float(w[0])
+
+
+def test_one_electron_scf_matches_radial_eigenvalue(ctx):
+    ax = O.Axis.ncell_fixed(-9.0, 9.0, 5, 18)  # h = 0.25
+    g = grid_of(ctx, ax)
+    mol = scf.Molecule(np.array([1.0]), np.zeros((1, 3)))
+    run = scf.HelmholtzSCF(ctx, g, mol, nocc=1, beta=4.0, exchange=False, hartree=False)
+    run.initial_guess()
+    res = run.run(max_iter=60)
+    ref = radial_eigenvalue(1.0, 4.0)
+    print(f"eps {run.eps[0]:.8f} radial {ref:.8f} iterations {res.iterations}")
+    assert res.converged
+    assert abs(run.eps[0] - ref) < 5e-5
+    # one electron without e-e terms: E = 2 eps + E_nn (occupation 2, no Hartree)
+    assert abs(res.energies[-1] - 2 * run.eps[0]) < 1e-6
+    run.close()
+
+
+def test_two_centre_scf_converges(ctx):
+    ax = O.Axis.ncell_fixed(-7.0, 7.0, 5, 14)  # h = 0.25
+    g = grid_of(ctx, ax)
+    mol = scf.Molecule(np.array([1.0, 1.0]), np.array([[0.0, 0.0, -0.7], [0.0, 0.0, 0.7]]))
+    run = scf.HelmholtzSCF(ctx, g, mol, nocc=1, beta=4.0)
+    run.initial_guess()
+    res = run.run(max_iter=80)
+    print(f"H2-like: E {res.energies[-1]:.8f} eps {run.eps[0]:.6f} in {res.iterations} iterations")
+    assert res.converged
+    assert run.eps[0] < 0
+    # the converged orbital is a fixed point of the Helmholtz step
+    run.build_potential()
+    run.helmholtz_step()
+    assert math.sqrt(run.integrate(run.delta[0], run.delta[0])) < 1e-4
+    run.close()
