@@ -427,8 +427,8 @@ def bench_config3(ctx, args):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the apply would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; "
+              "run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args)

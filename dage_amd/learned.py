@@ -15,7 +15,7 @@ defined up to a rotation of the occupied space).  Forward and backward run in
 the library's HIP kernels (dage_conv3d_*), the optimiser is Adam
 (dage_adam_step); the last layer starts at zero, so an untrained network
 proposes exactly the plain step.  In the SCF the plain proposal stays the
-fallback and the judge (HelmholtzSCF.run keeps whichever has the lower
+fallback and the arbiter (HelmholtzSCF.run keeps whichever has the lower
 occupied Ritz eigenvalue sum).
 """
 from __future__ import annotations

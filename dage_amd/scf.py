@@ -12,7 +12,7 @@ Mirrors the reference's HelmholtzSCFCycle / PowerMethod loop
   iteration > 1, |E_k - E_{k-1}| < 1e-6 and every |eps_i,k - eps_i,k-1| < 1e-6
   (SCFOptimizer_is_converged, scf.F90:99-136).
 
-The model Hamiltonian (the reference's orbitals carry bubbles; this build
+The model Hamiltonian (the reference's orbitals carry bubbles; this module
 has the cube part only): H = -1/2 laplacian + V_nuc + J[rho] + v_x[rho] with
 nuclei smeared as normalised Gaussians of exponent `beta` (so the orbitals
 need no cusp), J the Coulomb3D potential of rho = 2 sum |psi_i|^2 and v_x the

@@ -331,8 +331,8 @@ int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
                    const double *const *g, double *S);
 
 /* ----------------------------------------- learned preconditioner layers
- * No reference counterpart (the learned orbital-update preconditioner is new
- * in this build).  3x3x3 convolutions with zero padding: activations FP32
+ * No reference counterpart (the reference has no learned orbital-update
+ * preconditioner).  3x3x3 convolutions with zero padding: activations FP32
  * [c][z][y][x], weights BF16 (uint16 bit patterns) [cout][cin][27] with tap
  * (dx+1) + 3(dy+1) + 9(dz+1), FP32 accumulation; 1 <= cin, cout <= 16
  * (forward: cout 1 or 16).  Device buffers; asynchronous unless stated. */

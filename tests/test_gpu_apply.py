@@ -7,7 +7,7 @@ Tolerances:
     device erf/exp differ from glibc's by an ulp, which moves such entries
     by ~1e-9 absolute -- measured and reported by the test)
   * end to end (GPU matrices + GPU apply) on Gaussian densities: 1e-10 of
-    max|V| (the north-star parity bar)
+    max|V| (the FP64 parity bar)
   * reference goldens (tests/gauss_test.F90 TEST 1-3) through the GPU: 1e-13
 """
 import math

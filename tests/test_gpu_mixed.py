@@ -1,7 +1,7 @@
 """Mixed-precision (split-BF16) apply against the oracle and the FP64 path.
 
 bf16x6: 3-term bf16 splits, 6 MFMA products, FP32 sums of 32 products
-promoted to FP64, FP32 intermediates.  <= 1e-7 of max|V|
+promoted to FP64, FP32 intermediates.  Accuracy bar: <= 1e-7 of max|V|
 on the configs' Gaussian densities (CPU emulation of the same arithmetic:
 5.2e-8 at 97^3, T=40).  bf16x3 is the faster 2-term variant (~2e-6).
 """
