@@ -49,6 +49,10 @@ def parse():
                    help="skip the config-1 apply measurement")
     p.add_argument("--no-config3", dest="config3", action="store_false",
                    help="skip the config-3 batched Helmholtz measurement")
+    p.add_argument("--no-config4", dest="config4", action="store_false",
+                   help="skip the config-4 SCF (plain vs learned) measurement")
+    p.add_argument("--scf-max-iter", type=int, default=600)
+    p.add_argument("--scf-train-iter", type=int, default=150)
     p.add_argument("--copy-calibration", action="store_true",
                    help="time a plain device copy of a 257^3 cube instead (the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -424,6 +428,88 @@ def bench_config3(ctx, args):
     }
 
 
+# ----------------------------------------------------------------- config 4
+def bench_config4(ctx, args):
+    """Helmholtz SCF time to 1e-6 Ha, plain vs learned preconditioner.
+
+    129^3 cube over [-9.6, 9.6]^3 (nlip 5, h 0.15), 10 atoms (Z uniform in
+    {1,6,7,8}, in a 6-bohr sphere, >= 1.8 bohr apart, nuclei smeared with
+    beta 4), 12 doubly occupied orbitals, Hartree + Slater exchange.  The
+    network is trained once, offline, on the plain iterations of another
+    molecule from the same generator; the timed runs are the plain and the
+    learned SCF of a molecule it has not seen, from the same initial guess,
+    each to the convergence rule |dE| < 1e-6 Ha and all |d eps| < 1e-6."""
+    import numpy as np
+
+    import dage_amd
+    import dage_amd.learned as L
+    import dage_amd.scf as scf
+    grid = dage_amd.Grid3D(ctx, [-9.6] * 3, [np.full(32, 0.15)] * 3, 5)
+    train_mol = scf.random_molecule(np.random.default_rng(1), 10)
+    test_mol = scf.random_molecule(np.random.default_rng(0), 10)
+    nocc, maxit = 12, args.scf_max_iter
+
+    class EveryK(L.SampleRecorder):
+        def __init__(self, s, k):
+            super().__init__(s)
+            self.k, self.calls = k, 0
+
+        def __call__(self, psi, delta, out):
+            self.calls += 1
+            if self.calls % self.k == 1:
+                return super().__call__(psi, delta, out)
+            for p_, d_, o_ in zip(psi, delta, out):
+                o_.copy_from(p_)
+                self.scf._axpby(self.scf.w, d_, 1.0, o_)
+
+    t0 = time.perf_counter()
+    tr = scf.HelmholtzSCF(ctx, grid, train_mol, nocc=nocc, beta=4.0)
+    tr.initial_guess()
+    rec = EveryK(tr, 4)
+    r_train = tr.run(max_iter=args.scf_train_iter, update=rec)
+    model = L.ConvPreconditioner(ctx, tr.shape, seed=0, lr=2e-3)
+    samples = rec.samples(model, tr.psi)
+    rec.close()
+    tr.close()
+    c0 = time.perf_counter()
+    losses = L.train(model, samples, epochs=2)
+    ctx.synchronize()
+    train_s = time.perf_counter() - c0
+    for xb, tb in samples:
+        xb.close()
+        tb.close()
+    offline_s = time.perf_counter() - t0
+    runs = {}
+    for mode in ("plain", "learned"):
+        S = scf.HelmholtzSCF(ctx, grid, test_mol, nocc=nocc, beta=4.0)
+        S.initial_guess()
+        upd = None if mode == "plain" else (lambda p_, d_, o_, S=S: model.propose(S, p_, d_, o_))
+        r = S.run(max_iter=maxit, update=upd)
+        runs[mode] = {"converged": r.converged, "iterations": r.iterations,
+                      "wall_s": round(r.wall_s, 3),
+                      "ms_per_iteration": round(1e3 * r.wall_s / r.iterations, 2),
+                      "final_energy": r.energies[-1],
+                      "last_dE": abs(r.energies[-1] - r.energies[-2]),
+                      "learned_proposals_kept": int(sum(r.learned_used))}
+        S.close()
+    model.close()
+    grid.close()
+    p_, l_ = runs["plain"], runs["learned"]
+    return {
+        "workload": "config 4: Helmholtz SCF, 129^3 over [-9.6,9.6]^3 (nlip 5, h 0.15), 10 atoms "
+                    "Z~U{1,6,7,8} smeared (beta 4), 12 orbitals, Hartree + Slater exchange, FP64 "
+                    "batched Helmholtz applies; time to |dE| < 1e-6 Ha and |d eps| < 1e-6",
+        "Z": [float(z) for z in test_mol.Z],
+        "plain": p_, "learned": l_,
+        "wall_speedup_learned_vs_plain": round(p_["wall_s"] / l_["wall_s"], 3),
+        "energy_difference": abs(p_["final_energy"] - l_["final_energy"]),
+        "training": {"molecule_Z": [float(z) for z in train_mol.Z],
+                     "plain_iterations": r_train.iterations, "samples": len(samples),
+                     "epochs": 2, "loss_per_epoch": [round(x, 4) for x in losses],
+                     "train_s": round(train_s, 2), "offline_total_s": round(offline_s, 2)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -471,6 +557,8 @@ def main():
         line["config1"] = bench_config1(ctx, args)
     if args.config3:
         line["config3"] = bench_config3(ctx, args)
+    if args.config4:
+        line["config4"] = bench_config4(ctx, args)
     print(json.dumps(line), flush=True)
     ctx.close()
 

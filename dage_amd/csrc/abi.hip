@@ -114,6 +114,8 @@ void free_operator_buffers(dage_operator op) {
     }
     if (op->d_planes) (void)hipFree(op->d_planes);
     op->d_planes = nullptr;
+    if (op->d_mask) (void)hipFree(op->d_mask);
+    op->d_mask = nullptr;
     op->planes_valid = false;
     op->ws_bytes = 0;
 }
@@ -482,6 +484,7 @@ int dage_coulomb3d_create(dage_ctx ctx, dage_grid3d gin, dage_grid3d gout, const
     if (rc == DAGE_OK) rc = upload_weights(op);
     if (rc == DAGE_OK) rc = generate_matrices(op);
     if (rc == DAGE_OK) rc = relayout_fz(op);
+    if (rc == DAGE_OK) rc = build_masks(op);
     if (rc == DAGE_OK) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
@@ -542,6 +545,7 @@ int dage_operator_set_matrices(dage_operator op, const double *fx, const double 
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fz, fz, 8 * (size_t)op->din[2] * op->dout[2] * T,
                                   hipMemcpyHostToDevice, s));
     DAGE_TRY(relayout_fz(op));
+    DAGE_TRY(build_masks(op));
     op->planes_valid = false;
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
     return DAGE_OK;

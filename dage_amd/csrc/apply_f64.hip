@@ -17,7 +17,9 @@
 #include "dage_internal.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 namespace dage {
 namespace {
@@ -43,7 +45,14 @@ struct Gemm {
     long aS1;
     int Ksplit;               // >0: z_lo selects a K range of this size
     int accumulate;           // C += result instead of C = result
+    // Optional row masks of B: bit c of mask[k] is set when B(k, n) != 0 for
+    // some n in [16c, 16c + 16).  Rows of batch index z_lo start at
+    // mask + z_lo*mS (mS = 0 with Ksplit: the K range start is used).
+    const uint64_t *mask;
+    long mS;
 };
+
+constexpr int kMaxPanels = 2048;  // K panels a block can list (K <= 32768)
 
 // C tile of (64*MW) x (16*NB) per block; wavefront w owns rows
 // [w*16*MW, (w+1)*16*MW) and all 16*NB columns.  MFMA f64 16x16x4 operand
@@ -93,6 +102,8 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         if (g.alpha) alpha = g.alpha[zlo + g.aS1 * zhi];
     }
     const int K = kend - kbeg;
+    const uint64_t *mrow = nullptr;
+    if (g.mask) mrow = g.mask + (g.Ksplit > 0 ? (long)kbeg : (long)zlo * g.mS);
 
     // Loader geometry: every thread loads A_PER elements of the A panel and NB
     // of the B panel at fixed (m, k) / (k, n) steps from one base position, so
@@ -173,20 +184,52 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
                 acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     };
 
-    const int npanels = (K + kBK - 1) / kBK;
+    // K panels whose B rows are zero on all of this block's columns add
+    // exactly zero and are skipped.  Wavefront 0 lists the others in
+    // ascending order (ballot + prefix count), so the summation order is the
+    // same as without masks.
+    __shared__ unsigned short plist[kMaxPanels];
+    __shared__ int plist_n;
+    int npanels = (K + kBK - 1) / kBK;
+    const bool listed = mrow != nullptr && npanels <= kMaxPanels;
+    if (listed) {
+        if (wave == 0) {
+            const uint64_t cols = ((NB >= 64) ? ~0ull : ((1ull << NB) - 1ull)) << (n0 / 16);
+            int cnt = 0;
+            for (int base = 0; base < npanels; base += 64) {
+                const int p = base + lane;
+                bool need = false;
+                if (p < npanels) {
+                    const int k0 = p * kBK;
+                    const int kn = min(kBK, K - k0);
+                    uint64_t any = 0;
+                    for (int j = 0; j < kn; ++j) any |= mrow[k0 + j];
+                    need = (any & cols) != 0;
+                }
+                const uint64_t bal = __ballot(need);
+                if (need) plist[cnt + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)p;
+                cnt += __popcll(bal);
+            }
+            if (lane == 0) plist_n = cnt;
+        }
+        __syncthreads();
+        npanels = plist_n;
+    }
+    auto panel_k = [&](int p) { return (listed ? (int)plist[p] : p) * kBK; };
+
     if (npanels > 0) {
-        load_panel(0);
+        load_panel(panel_k(0));
         store_panel(0);
         __syncthreads();
     }
     for (int p = 0; p < npanels; ++p) {
         const int buf = p & 1;
         const bool more = p + 1 < npanels;
-        if (more) load_panel((p + 1) * kBK);
+        if (more) load_panel(panel_k(p + 1));
         const double *as = As[buf];
         const double *bs = Bs[buf];
-        // uniform bound: the last panel skips its all-padding k-steps
-        const int ksteps = more ? kBK / 4 : (K - p * kBK + 3) / 4;
+        // uniform bound: a panel at the end of K skips its all-padding k-steps
+        const int ksteps = min(kBK / 4, (K - panel_k(p) + 3) / 4);
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks)
             if (ks < ksteps) kstep(as, bs, ks);
@@ -225,6 +268,20 @@ __global__ void __launch_bounds__(256) relayout_fz_kernel(const double *fz, doub
         const int zo = (int)(r % nzo);
         const int t = (int)(r / nzo);
         gz[(long)zi + (long)nzi * t + (long)nzi * T * zo] = fz[e];
+    }
+}
+
+// Row masks: one thread per B row of a pass; bit c when the row has a nonzero
+// entry among output columns [16c, 16c + 16).  Element (row r, column n) of a
+// pass's B matrix is at b[r*rs + n*cs] (r local to one t for x and y).
+__global__ void __launch_bounds__(256) row_mask_kernel(const double *b, long rows, int n, long rs,
+                                                       long cs, uint64_t *mask) {
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows;
+         r += (long)gridDim.x * blockDim.x) {
+        uint64_t m = 0;
+        for (int c = 0; c < n; ++c)
+            if (b[r * rs + (long)c * cs] != 0.0) m |= 1ull << (c >> 4);
+        mask[r] = m;
     }
 }
 
@@ -340,6 +397,98 @@ int relayout_fz(dage_operator op) {
     return DAGE_OK;
 }
 
+namespace {
+
+// Relative cost of one K panel of one block: NB column groups of MFMA work
+// plus a fixed share (operand loads, barrier) worth kPanelCost groups.
+constexpr double kPanelCost = 1.0;
+
+// Estimated work of a pass whose B rows (nseg segments of klen rows, panels
+// aligned to each segment) have the given masks, output axis n points, at
+// column-block count nb; all panels count when dense.
+double pass_cost(const std::vector<uint64_t> &m, long nseg, long klen, int n, int nb, bool dense) {
+    double cost = 0.0;
+    for (int n0 = 0; n0 < n; n0 += 16 * nb) {
+        const uint64_t cols = ((1ull << nb) - 1ull) << (n0 / 16);
+        long need = 0;
+        for (long sgi = 0; sgi < nseg; ++sgi)
+            for (long k0 = 0; k0 < klen; k0 += 16) {
+                uint64_t any = 0;
+                for (long k = k0; k < std::min(klen, k0 + 16); ++k) any |= m[sgi * klen + k];
+                need += dense || (any & cols) != 0;
+            }
+        cost += (double)need * (nb + kPanelCost);
+    }
+    return cost;
+}
+
+// Column-block count for a pass (least estimated work) and whether the
+// masks pay: listing a block's panels costs one dependent load before its
+// first panel, so they are used only when they cut the work by 7 % or more.
+int pick_nb_masked(const std::vector<uint64_t> &m, long nseg, long klen, int n, bool *use) {
+    int best = 1;
+    double best_cost = 1e300, best_dense = 1e300;
+    for (int nb = 8; nb >= 1; --nb) {
+        const double c = pass_cost(m, nseg, klen, n, nb, false);
+        if (c < best_cost * (1.0 - 1e-9)) { best_cost = c; best = nb; }
+        best_dense = std::min(best_dense, pass_cost(m, nseg, klen, n, nb, true));
+    }
+    *use = best_cost < 0.93 * best_dense;
+    return *use ? best : pick_nb(n);
+}
+
+}  // namespace
+
+int build_masks(dage_operator op) {
+    const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
+    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
+    const long T = op->ntp;
+    const long rx = T * nxi, ry = T * nyi, rz = nzi * T;
+    if (!op->d_mask) DAGE_HIP_CHECK(hipMalloc((void **)&op->d_mask, 8 * (rx + ry + rz)));
+    hipStream_t s = op->ctx->stream;
+    auto grid = [](long rows) { return (int)std::min<long>((rows + 255) / 256, 4096); };
+    // x: Fx (nxo, nxi, T), row (t, xi) = t*nxi + xi, column xo
+    hipLaunchKernelGGL(row_mask_kernel, dim3(grid(rx)), dim3(256), 0, s, op->d_fx, rx, (int)nxo,
+                       nxo, 1L, op->d_mask);
+    DAGE_HIP_CHECK(hipGetLastError());
+    // y: Fy (nyi, nyo, T); row t*nyi + yi sits at yi + nyi*nyo*t, column stride nyi
+    for (long t = 0; t < T; ++t) {
+        hipLaunchKernelGGL(row_mask_kernel, dim3(grid(nyi)), dim3(256), 0, s,
+                           op->d_fy + t * nyi * nyo, nyi, (int)nyo, 1L, nyi,
+                           op->d_mask + rx + t * nyi);
+        DAGE_HIP_CHECK(hipGetLastError());
+    }
+    // z: Gz ((zi + nzi*t), zo), column stride nzi*T
+    hipLaunchKernelGGL(row_mask_kernel, dim3(grid(rz)), dim3(256), 0, s, op->d_gz, rz, (int)nzo,
+                       1L, rz, op->d_mask + rx + ry);
+    DAGE_HIP_CHECK(hipGetLastError());
+    std::vector<uint64_t> h(rx + ry + rz);
+    DAGE_HIP_CHECK(hipMemcpyAsync(h.data(), op->d_mask, 8 * h.size(), hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    const long n[3] = {nxo, nyo, nzo};
+    const long klen[3] = {nxi, nyi, rz}, nseg[3] = {T, T, 1}, off[3] = {0, rx, rx + ry};
+    for (int p = 0; p < 3; ++p) {
+        op->nb_pick[p] = 0;
+        op->mask_used[p] = false;
+        if (n[p] > 1024) continue;
+        std::vector<uint64_t> mp(h.begin() + off[p], h.begin() + off[p] + nseg[p] * klen[p]);
+        bool use = false;
+        op->nb_pick[p] = pick_nb_masked(mp, nseg[p], klen[p], (int)n[p], &use);
+        op->mask_used[p] = use;
+    }
+    // tuning knob "x,y,z": fixed column-block counts, masks on wherever defined
+    if (const char *e = std::getenv("DAGE_APPLY_NB")) {
+        int v[3] = {0, 0, 0};
+        if (std::sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+            for (int p = 0; p < 3; ++p)
+                if (v[p] >= 1 && v[p] <= 8) {
+                    op->nb_pick[p] = v[p];
+                    op->mask_used[p] = n[p] <= 1024;
+                }
+    }
+    return DAGE_OK;
+}
+
 ApplyMats operator_mats(dage_operator op) {
     ApplyMats m;
     m.fx = op->d_fx;
@@ -351,6 +500,14 @@ ApplyMats operator_mats(dage_operator op) {
         m.dout[i] = op->dout[i];
     }
     m.ntp = op->ntp;
+    if (op->d_mask) {
+        const long rx = (long)op->ntp * op->din[0], ry = (long)op->ntp * op->din[1];
+        const uint64_t *base[3] = {op->d_mask, op->d_mask + rx, op->d_mask + rx + ry};
+        for (int p = 0; p < 3; ++p) {
+            m.mask[p] = op->mask_used[p] ? base[p] : nullptr;
+            m.nb_pick[p] = op->nb_pick[p];
+        }
+    }
     return m;
 }
 
@@ -390,6 +547,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const long M3 = nxo * nyo;
     const int tcmax = plan.tchunk;
     const long Vt = nxo * nyi * nzi, Ut = nxo * nyo * nzi;  // per (t, rhs)
+    auto nb_for = [&](int p) { return bs.nb[p] > 0 ? bs.nb[p] : m.nb_pick[p]; };
 
     for (int t0 = 0; t0 < T; t0 += tcmax) {
         const int tc = std::min(tcmax, T - t0);
@@ -404,7 +562,8 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1;
         g1.bA = 0; g1.bB = nxo * nxi; g1.bC = Vt;
         g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
-        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc * B, bs.nb[0]));
+        if (m.mask[0]) { g1.mask = m.mask[0] + (long)t0 * nxi; g1.mS = nxi; }
+        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc * B, nb_for(0)));
 
         // pass 2, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
@@ -419,7 +578,8 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g2.Z0 = tc; g2.bA1 = Vt * tc; g2.bB1 = 0; g2.bC1 = Ut * tc;
         g2.alpha = W + t0;
         g2.aS1 = wS;
-        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc * B, bs.nb[1]));
+        if (m.mask[1]) { g2.mask = m.mask[1] + (long)t0 * nyi; g2.mS = nyi; }
+        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc * B, nb_for(1)));
 
         // pass 3, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
         const int K3 = (int)(nzi * tc);
@@ -437,11 +597,12 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g3.Z0 = splits; g3.bA1 = Ut * tc; g3.bB1 = 0; g3.bC1 = M3 * nzo * plan.splits;
         g3.Ksplit = ks;
         g3.accumulate = (t0 > 0);
+        if (m.mask[2]) g3.mask = m.mask[2] + (long)nzi * t0;
         if (t0 == 0 && splits < plan.splits)
             for (int b = 0; b < B; ++b)
                 DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + splits) * M3 * nzo,
                                    (size_t)(plan.splits - splits) * M3 * nzo));
-        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits * B, bs.nb[2]));
+        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits * B, nb_for(2)));
     }
     const long len = M3 * nzo;
     const long total = len * B;

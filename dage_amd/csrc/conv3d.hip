@@ -9,11 +9,24 @@
 // nearest even.  Every reduction (filter gradient, loss) is a fixed-order
 // two-stage sum, so training is deterministic.
 //
-// Roofline: these are small convolutions (<= 16 channels), VALU-bound in
-// FP32 (forward 2 * cin * cout * 27 FLOPs per voxel); the backward filter
-// stages, per output x-row and per dz, the 3 neighbouring input rows of every
-// channel in LDS, and every thread owns one (cout, cin) pair's 27 taps.
+// The 16 -> 16 layers (forward, and backward-data as the same convolution
+// with transposed, mirrored weights) run on bf16 MFMA: every FP32 activation
+// is split into three bf16 terms (24 bits, so the products are as exact as
+// FP32 ones) and multiplied with the bf16 weights, FP32 accumulation.  A
+// block streams along z through its (16*NS) x 4 column of the volume: one
+// input plane (with a one-point halo) sits in LDS, pre-split, and feeds the
+// three output planes it touches, whose accumulators stay in registers; the
+// weights stay in registers as MFMA operands for the whole launch.
+// Algorithmic work: 2*16*16*27 = 13824 FLOPs per voxel (dense, as FP32);
+// HBM roofline 128 bytes per voxel (16 channels in and out, FP32).
+//
+// Layers with 1 or 2 channels on a side run as FP32 VALU kernels; the
+// backward filter stages, per output x-row and per dz, the 3 neighbouring
+// input rows of every channel in LDS, and every thread owns one (cout, cin)
+// pair's 27 taps.
+#include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "dage_internal.hpp"
@@ -110,6 +123,217 @@ __global__ void __launch_bounds__(256) conv_bwd_data_kernel(const float *gout, i
         }
         for (int c = 0; c < cin; ++c) gin[(long)c * vol + p] = acc[c];
     }
+}
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCY = 4;  // output y-rows per block: one per wavefront
+
+__device__ inline __bf16 bf16_bits(uint16_t b) { return __builtin_bit_cast(__bf16, b); }
+
+// 16 -> 16 channel 3x3x3 convolution on bf16 MFMA (v_mfma_f32_16x16x32_bf16;
+// lane l holds A(l%16, 8(l/16)..+7) and B(8(l/16)..+7, l%16); result
+// register r of lane l is D(4(l/16)+r, l%16)).  D rows are 16 x-points of
+// one output row, columns the 16 output channels; K = 32 runs over two taps
+// (dx, dy) of one dz times 16 input channels, 5 K-steps per dz (the tenth
+// tap slot is zero).  Block (bx, by, bz) owns x in [16*NS*bx, +16*NS),
+// y in [4*by, +4) (wavefront w: row 4*by + w) and z in [bz*zchunk, +zchunk).
+// out = act(sum + bias); bias may be null (zero).
+template <int NS>
+__global__ void __launch_bounds__(256) conv16_mfma_kernel(const float *in, const uint16_t *w,
+                                                          const float *bias, int relu, float *out,
+                                                          int nx, int ny, int nz, int zchunk) {
+    constexpr int PX = 16 * NS + 2;  // halo plane extent in x and y
+    constexpr int PY = kCY + 2;
+    constexpr int NPT = PX * PY;
+    constexpr int PTS = 48;          // bf16 per plane point: 3 terms x 16 channels
+    constexpr int LOADS = (NPT + 255) / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 plane[2][NPT * PTS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = lane & 15, kg = lane >> 4;
+    const int x0 = blockIdx.x * 16 * NS, y0 = blockIdx.y * kCY, z0 = blockIdx.z * zchunk;
+    const int z1 = min(nz, z0 + zchunk);
+    const long pl = (long)nx * ny, vol = pl * nz;
+
+    // B operands: bw[dzi][j] = W[o = lane%16][c = 8(kg&1)+i][tau + 9 dzi], tau = 2j + kg/2
+    bf8 bw[3][5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int tau = 2 * j + (kg >> 1);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = 8 * (kg & 1) + i;
+                bw[d][j][i] = bf16_bits(tau < 9 ? w[(lrow * 16 + c) * 27 + tau + 9 * d] : 0);
+            }
+        }
+    const float bo = bias ? bias[lrow] : 0.0f;
+
+    f4 acc[3][NS];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[q][s] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // plane loads: thread point e = tid + 256 r, all 16 channels
+    float raw[LOADS][16];
+    auto load_plane = [&](int zi) {
+#pragma unroll
+        for (int r = 0; r < LOADS; ++r) {
+            const int e = tid + 256 * r;
+            const int ly = e / PX, lx = e - (e / PX) * PX;
+            const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+            const bool ok = e < NPT && x >= 0 && x < nx && y >= 0 && y < ny && zi >= 0 && zi < nz;
+            const long base = ok ? (long)zi * pl + (long)y * nx + x : 0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const float v = in[base + c * vol];
+                raw[r][c] = ok ? v : 0.0f;
+            }
+        }
+    };
+    auto store_plane = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < LOADS; ++r) {
+            const int e = tid + 256 * r;
+            if (e >= NPT) continue;
+            bf8 t[3][2];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const float x = raw[r][c];
+                const __bf16 h = (__bf16)x;
+                const float r1 = x - (float)h;
+                const __bf16 m = (__bf16)r1;
+                const float r2 = r1 - (float)m;
+                t[0][c >> 3][c & 7] = h;
+                t[1][c >> 3][c & 7] = m;
+                t[2][c >> 3][c & 7] = (__bf16)r2;
+            }
+            bf8 *dst = reinterpret_cast<bf8 *>(&plane[buf][e * PTS]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                dst[2 * q] = t[q][0];
+                dst[2 * q + 1] = t[q][1];
+            }
+        }
+    };
+    auto epilogue = [&](f4 (&a)[NS], int zo) {
+        const int y = y0 + wave;
+        if (y >= ny) return;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 16 * s + 4 * kg + r;
+                if (x >= nx) continue;
+                float v = a[s][r] + bo;
+                if (relu && v < 0.0f) v = 0.0f;
+                out[lrow * vol + (long)zo * pl + (long)y * nx + x] = v;
+            }
+    };
+
+    const int nplanes = z1 - z0 + 2;  // input planes z0-1 .. z1
+    load_plane(z0 - 1);
+    store_plane(0);
+    __syncthreads();
+
+    // plane q = zi - (z0 - 1) feeds outputs zi+1, zi, zi-1 in slots (R+1)%3,
+    // R, (R+2)%3 with R = q % 3; output zi-1 is complete afterwards.
+    auto step = [&](int q, auto rc) {
+        constexpr int R = decltype(rc)::value;
+        constexpr int SN = (R + 1) % 3, SC = R, SP = (R + 2) % 3;
+        if (q >= nplanes) return;
+        const int zi = z0 - 1 + q;
+        const int buf = q & 1;
+        const bool more = q + 1 < nplanes;
+        if (more) load_plane(zi + 1);
+        const bool dn = zi + 1 >= z0 && zi + 1 < z1;  // output zi+1 (weights dz = -1)
+        const bool dc = zi >= z0 && zi < z1;          // output zi    (dz = 0)
+        const bool dp = zi - 1 >= z0 && zi - 1 < z1;  // output zi-1  (dz = +1)
+        if (zi >= 0 && zi < nz) {
+            const __bf16 *pb = plane[buf];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    const int tau = min(2 * j + (kg >> 1), 8);  // slot 9 has zero weights
+                    const int ly = wave + tau / 3, lx = 16 * s + lrow + tau % 3;
+                    const bf8 *src =
+                        reinterpret_cast<const bf8 *>(pb + (ly * PX + lx) * PTS + 8 * (kg & 1));
+                    const bf8 a0 = src[0], a1 = src[2], a2 = src[4];
+                    if (dn) {
+                        acc[SN][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bw[0][j], acc[SN][s], 0, 0, 0);
+                        acc[SN][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[0][j], acc[SN][s], 0, 0, 0);
+                        acc[SN][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[0][j], acc[SN][s], 0, 0, 0);
+                    }
+                    if (dc) {
+                        acc[SC][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bw[1][j], acc[SC][s], 0, 0, 0);
+                        acc[SC][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[1][j], acc[SC][s], 0, 0, 0);
+                        acc[SC][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[1][j], acc[SC][s], 0, 0, 0);
+                    }
+                    if (dp) {
+                        acc[SP][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bw[2][j], acc[SP][s], 0, 0, 0);
+                        acc[SP][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[2][j], acc[SP][s], 0, 0, 0);
+                        acc[SP][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[2][j], acc[SP][s], 0, 0, 0);
+                    }
+                }
+        }
+        if (dp) epilogue(acc[SP], zi - 1);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[SP][s] = f4{0.f, 0.f, 0.f, 0.f};
+        if (more) store_plane(buf ^ 1);
+        __syncthreads();
+    };
+    for (int q = 0; q < nplanes; q += 3) {
+        step(q, std::integral_constant<int, 0>{});
+        step(q + 1, std::integral_constant<int, 1>{});
+        step(q + 2, std::integral_constant<int, 2>{});
+    }
+}
+
+// W'[c][o][26 - t] = W[o][c][t] (bf16 bits): backward-data weights
+__global__ void __launch_bounds__(256) transpose_mirror_kernel(const uint16_t *w, int cout,
+                                                               int cin, uint16_t *wt) {
+    const int n = cout * cin * 27;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int t = e % 27, c = (e / 27) % cin, o = e / (27 * cin);
+        wt[(c * cout + o) * 27 + 26 - t] = w[e];
+    }
+}
+
+// x-subtiles per block for an x extent: least padded width, halo included
+int conv_ns(int nx) {
+    int best = 1;
+    double best_cost = 1e30;
+    for (int ns = 1; ns <= 4; ++ns) {
+        const int bw = 16 * ns;
+        const double cost = (double)((nx + bw - 1) / bw) * (bw + 2);
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = ns; }
+    }
+    return best;
+}
+
+int launch_conv16(dage_ctx ctx, int nx, int ny, int nz, const float *in, const uint16_t *w,
+                  const float *bias, int relu, float *out) {
+    const int ns = conv_ns(nx);
+    const int bx = (nx + 16 * ns - 1) / (16 * ns), by = (ny + kCY - 1) / kCY;
+    const int want = std::max(1, (2048 + bx * by - 1) / (bx * by));  // z-chunks for ~2048 blocks
+    const int zchunk = std::max(2, (nz + want - 1) / want);
+    const dim3 grid(bx, by, (nz + zchunk - 1) / zchunk);
+    hipEvent_t ev;
+    prof_begin(ctx, "conv16_mfma_kernel", &ev);
+    switch (ns) {
+    case 1: hipLaunchKernelGGL(conv16_mfma_kernel<1>, grid, dim3(256), 0, ctx->stream, in, w, bias, relu, out, nx, ny, nz, zchunk); break;
+    case 2: hipLaunchKernelGGL(conv16_mfma_kernel<2>, grid, dim3(256), 0, ctx->stream, in, w, bias, relu, out, nx, ny, nz, zchunk); break;
+    case 3: hipLaunchKernelGGL(conv16_mfma_kernel<3>, grid, dim3(256), 0, ctx->stream, in, w, bias, relu, out, nx, ny, nz, zchunk); break;
+    default: hipLaunchKernelGGL(conv16_mfma_kernel<4>, grid, dim3(256), 0, ctx->stream, in, w, bias, relu, out, nx, ny, nz, zchunk); break;
+    }
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "conv16_mfma_kernel", ev);
+    return DAGE_OK;
 }
 
 // Filter gradient, stage 1: block b takes x-rows (y, z) in [b*rows, ...);
@@ -270,6 +494,8 @@ int dage_conv3d_forward(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
     DAGE_REQUIRE(cout == 1 || cout == kMaxC, DAGE_ERR_UNSUPPORTED,
                  "forward supports 1 or %d output channels", kMaxC);
     const long vol = (long)nx * ny * nz;
+    if (cin == kMaxC && cout == kMaxC)
+        return launch_conv16(ctx, nx, ny, nz, in, w, bias, relu, out);
     if (cout == 1)
         hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, in,
                            cin, w, bias, relu, out, nx, ny, nz);
@@ -286,6 +512,21 @@ int dage_conv3d_backward_data(dage_ctx ctx, int nx, int ny, int nz, int cin, int
     DAGE_TRY(check_dims(nx, ny, nz, cin, cout));
     DAGE_REQUIRE(gout && w && gin && gout != gin, DAGE_ERR_ARG, "bad argument");
     const long vol = (long)nx * ny * nz;
+    if (cin == kMaxC && cout == kMaxC) {
+        uint16_t *wt = nullptr;
+        DAGE_HIP_CHECK(hipMalloc((void **)&wt, sizeof(uint16_t) * kMaxC * kMaxC * 27));
+        hipLaunchKernelGGL(transpose_mirror_kernel, dim3(27), dim3(256), 0, ctx->stream, w, cout,
+                           cin, wt);
+        int rc = hipGetLastError() == hipSuccess ? DAGE_OK : DAGE_ERR_HIP;
+        if (rc != DAGE_OK) set_error("weight transpose launch failed");
+        if (rc == DAGE_OK) rc = launch_conv16(ctx, nx, ny, nz, gout, wt, nullptr, 0, gin);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == DAGE_OK) {
+            set_error("backward data failed");
+            rc = DAGE_ERR_HIP;
+        }
+        (void)hipFree(wt);
+        return rc;
+    }
     hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, gout,
                        cout, cin, w, gin, nx, ny, nz);
     DAGE_HIP_CHECK(hipGetLastError());

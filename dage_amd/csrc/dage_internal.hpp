@@ -165,6 +165,9 @@ struct dage_operator_s {
     double *d_u = nullptr;   // y-mode intermediate, ((xo,yo), (zi,t))
     double *d_in = nullptr, *d_out = nullptr;  // staging for host applies
     size_t ws_bytes = 0;
+    uint64_t *d_mask = nullptr;  // B-row masks of the three passes (build_masks)
+    int nb_pick[3] = {0, 0, 0};  // column-block counts chosen from the masks
+    bool mask_used[3] = {false, false, false};  // masks cut the pass's work enough
     void *d_planes = nullptr;   // split-BF16 matrix planes (built on first mixed apply)
     bool planes_valid = false;  // cleared when the matrices change
     MixedLayout mixed;
@@ -188,10 +191,16 @@ int generate_matrices(dage_operator op);
 int apply_f64(dage_operator op, const double *in_dev, double *out_dev);
 // One operator's matrices in the library's device layouts (Fx (nxo,nxi,T),
 // Fy (nyi,nyo,T), Gz ((zi + nzi*t), zo)) and weights.
+// mask (optional): per B row of each pass, bit c set when the row has a
+// nonzero in output columns [16c, 16c + 16): x rows (t, xi) at t*nxi + xi,
+// then y rows (t, yi), then z rows zi + nzi*t; null for a pass whose output
+// axis exceeds 1024 points.
 struct ApplyMats {
     const double *fx = nullptr, *fy = nullptr, *gz = nullptr, *w = nullptr;
     int din[3] = {0, 0, 0}, dout[3] = {0, 0, 0};
     int ntp = 0;
+    const uint64_t *mask[3] = {nullptr, nullptr, nullptr};
+    int nb_pick[3] = {0, 0, 0};
 };
 // Device scratch owned by the caller, grown on demand.
 struct Workspace {
@@ -244,6 +253,9 @@ int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);
 // Relayout Fz (nzi, nzo, T) -> Gz ((zi + nzi*t), zo).
 int relayout_fz(dage_operator op);
+// Row masks of the operator's matrices and the column-block counts they make
+// cheapest; run after every change of the matrices (synchronises).
+int build_masks(dage_operator op);
 // Bubbles (inject.hip, abi.hip).
 int bubbles_sync_device(dage_bubbles bub);
 int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multiplier,

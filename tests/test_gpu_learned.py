@@ -69,6 +69,39 @@ def test_layers_match_oracle(ctx):
     model.close()
 
 
+@pytest.mark.parametrize("shape", [(9, 7, 6), (20, 6, 5), (40, 5, 7), (50, 9, 11)])
+def test_conv16_shapes(ctx, shape):
+    """16 -> 16 layers (bf16 MFMA over three-term activation splits): forward
+    with bias and ReLU and backward-data, at x extents that select 1-4
+    x-subtiles per block and several z-chunks, within 2e-5 of the maxima of
+    the oracle's float64 results on the same BF16 weights."""
+    import ctypes
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * 131 + nz)
+    x = rng.standard_normal((16, nz, ny, nx)).astype(np.float32)
+    w = (rng.standard_normal((16, 16, 27)) * 0.1).astype(np.float32)
+    b = (rng.standard_normal(16) * 0.1).astype(np.float32)
+    wr = O.bf16_round(w)
+    bits = (wr.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    bufs = [L._Buf(ctx, a.nbytes) for a in (x, bits, b, x)]
+    for buf, a in zip(bufs, (x, bits, b)):
+        buf.upload(a)
+    dx_, dw, db, dy_ = (ctypes.c_void_p(q.ptr) for q in bufs)
+    dage_amd._lib.call("dage_conv3d_forward", ctx.handle, nx, ny, nz, 16, 16, dx_, dw, db, 1, dy_)
+    got = bufs[3].download(np.float32, x.size).reshape(x.shape)
+    ref = O.conv3d_forward(x.astype(np.float64), wr, b.astype(np.float64), True)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    # backward data of sum(g * conv(x)): g = the forward output as a test field
+    dage_amd._lib.call("dage_conv3d_backward_data", ctx.handle, nx, ny, nz, 16, 16, dy_, dw, dx_)
+    gx = bufs[0].download(np.float32, x.size).reshape(x.shape)
+    gref, _, _ = O.conv3d_backward(x.astype(np.float64), wr, got.astype(np.float64))
+    gerr = np.max(np.abs(gx - gref)) / np.max(np.abs(gref))
+    print(f"conv16 {shape}: forward {err:.2e}, backward data {gerr:.2e}")
+    assert err < 2e-5 and gerr < 2e-5
+    for buf in bufs:
+        buf.close()
+
+
 def test_training_reduces_loss(ctx):
     """Fit a fixed 3x3x3 filter of the second input channel (a target the
     network can represent) from 8 random volumes."""
