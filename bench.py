@@ -12,10 +12,12 @@ with GaussQuad(25, 25), T = 50), all in FP64 with the cubes resident in HBM:
 `value` = 241^3 output points per second of wall time per step.
 
 Also reported: config 1 (plain Coulomb3D apply, 97^3, T = 40, FP64 and
-split-BF16, CPU oracle timed beside it) and config 3 (32 Helmholtz
+split-BF16, CPU oracle timed beside it), config 3 (32 Helmholtz
 right-hand sides, k^2 in [0.1, 4], cubes of 65/97/129/161 points at h 0.15,
 one graph launch per batch, autotuned tiles, against 32 separate FP64
-applies).
+applies) and config 4 (Helmholtz SCF of a 10-atom molecule on a 129^3 cube,
+12 orbitals: iterations and wall time to 1e-6 Ha, plain vs the learned
+preconditioner trained beforehand on another molecule).
 
 Prints one JSON line.  The roofline entry is the dominant kernel of the
 headline step: algorithmic FLOPs per launch over its event-timed average

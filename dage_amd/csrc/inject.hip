@@ -76,12 +76,19 @@ __device__ inline int radial_icell(const double *celld, int ncell, double qmax, 
     return 0;
 }
 
-template <int LMAX>
+// NLIP > 0: compile-time cell size (unrolled Horner and dot products);
+// NLIP == 0: A.nlip at run time.  The reference divides every S_lm by r^l
+// (RealSphericalHarmonics_eval, harmonics.F90:1360-1370); here each l's sum
+// over m is scaled once by (1/r)^l, which moves the result by rounding only
+// (1e-16 relative per term).
+template <int LMAX, int NLIP>
 __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
     const int ix = blockIdx.x * 64 + threadIdx.x;
     const int iy = blockIdx.y * 4 + threadIdx.y;
     const int iz = blockIdx.z;
     if (ix >= A.nx || iy >= A.ny) return;
+    constexpr int NMAX = NLIP > 0 ? NLIP : kMaxNlip;
+    const int nlip = NLIP > 0 ? NLIP : A.nlip;
     const double px = A.gx[ix], py = A.gy[iy], pz = A.gz[iz];
     constexpr double kEps = 2.220446049250313e-16;  // epsilon(0.d0)
     double total = 0.0;
@@ -92,38 +99,58 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
         const int ic = radial_icell(B.celld, B.ncell, B.qmax, r);
         if (ic == 0) continue;
         const double crd = (r - B.celld[ic - 1]) / B.steps[ic - 1] + A.first;
-        double polys[kMaxNlip];
+        double polys[NMAX];
 #pragma unroll
-        for (int j = 0; j < kMaxNlip; ++j) {
-            if (j >= A.nlip) break;
+        for (int j = 0; j < NMAX; ++j) {
+            if (j >= nlip) break;
             double v = A.lip[j];
-            for (int q = 1; q < A.nlip; ++q) v = v * crd + A.lip[j + A.nlip * q];
+#pragma unroll
+            for (int q = 1; q < NMAX; ++q) {
+                if (q >= nlip) break;
+                v = v * crd + A.lip[j + nlip * q];
+            }
             polys[j] = v;
         }
         double S[(LMAX + 1) * (LMAX + 1)];
         solid_harmonics<LMAX>(dx, dy, dz, S);
-        const double *f = B.f + (long)(ic - 1) * (A.nlip - 1);
+        const double *f = B.f + (long)(ic - 1) * (nlip - 1);
+        const double inv = (r > kEps) ? 1.0 / r : 1.0;  // r <= eps: r^l taken as 1
         double res = 0.0;
-        double rl = 1.0;
+        double ipow = 1.0;
 #pragma unroll
         for (int l = 0; l <= LMAX; ++l) {
             if (l >= A.lmin) {
-                const double den = (r > kEps) ? rl : 1.0;
+                double sl = 0.0;
 #pragma unroll
                 for (int m = -l; m <= l; ++m) {
                     const int lm = l * l + l + m;
                     const double *fl = f + (long)lm * B.npts;
                     double fv = 0.0;
-                    for (int j = 0; j < A.nlip; ++j) fv += fl[j] * polys[j];
-                    res += fv * (S[lm] / den);
+#pragma unroll
+                    for (int j = 0; j < NMAX; ++j) {
+                        if (j >= nlip) break;
+                        fv += fl[j] * polys[j];
+                    }
+                    sl += fv * S[lm];
                 }
+                res += sl * ipow;
             }
-            rl *= r;
+            ipow *= inv;
         }
         total += res;
     }
     const long id = ix + (long)A.nx * (iy + (long)A.ny * iz);
     A.cube[id] = A.cube[id] + A.multiplier * total;
+}
+
+template <int LMAX>
+void launch_inject(const InjectArgs &A, dim3 g, dim3 block, hipStream_t s) {
+    if (A.nlip == 7)
+        hipLaunchKernelGGL((bubbles_inject_kernel<LMAX, 7>), g, block, 0, s, A);
+    else if (A.nlip == 5)
+        hipLaunchKernelGGL((bubbles_inject_kernel<LMAX, 5>), g, block, 0, s, A);
+    else
+        hipLaunchKernelGGL((bubbles_inject_kernel<LMAX, 0>), g, block, 0, s, A);
 }
 
 }  // namespace
@@ -175,13 +202,13 @@ int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multipli
     hipEvent_t ev;
     prof_begin(ctx, "bubbles_inject_kernel", &ev);
     switch (bub->lmax) {
-    case 0: hipLaunchKernelGGL(bubbles_inject_kernel<0>, g, block, 0, ctx->stream, A); break;
-    case 1: hipLaunchKernelGGL(bubbles_inject_kernel<1>, g, block, 0, ctx->stream, A); break;
-    case 2: hipLaunchKernelGGL(bubbles_inject_kernel<2>, g, block, 0, ctx->stream, A); break;
-    case 3: hipLaunchKernelGGL(bubbles_inject_kernel<3>, g, block, 0, ctx->stream, A); break;
-    case 4: hipLaunchKernelGGL(bubbles_inject_kernel<4>, g, block, 0, ctx->stream, A); break;
-    case 5: hipLaunchKernelGGL(bubbles_inject_kernel<5>, g, block, 0, ctx->stream, A); break;
-    case 6: hipLaunchKernelGGL(bubbles_inject_kernel<6>, g, block, 0, ctx->stream, A); break;
+    case 0: launch_inject<0>(A, g, block, ctx->stream); break;
+    case 1: launch_inject<1>(A, g, block, ctx->stream); break;
+    case 2: launch_inject<2>(A, g, block, ctx->stream); break;
+    case 3: launch_inject<3>(A, g, block, ctx->stream); break;
+    case 4: launch_inject<4>(A, g, block, ctx->stream); break;
+    case 5: launch_inject<5>(A, g, block, ctx->stream); break;
+    case 6: launch_inject<6>(A, g, block, ctx->stream); break;
     default:
         set_error("bubbles lmax %d outside 0..%d", bub->lmax, kMaxBubbleLmax);
         return DAGE_ERR_UNSUPPORTED;
