@@ -71,27 +71,65 @@ __global__ void __launch_bounds__(256) lincomb_kernel(long n, int nin, const dou
 }
 
 // part[(pair * nz + z)] = sum over the z-plane of wx wy f_i g_j (times wz)
+// Gram tiles: block (z, tile) sums w(x,y) f_i g_j over plane z for the 8 x 8
+// pairs (i0 + a, j0 + b) of its tile, reading each function's plane once;
+// every thread keeps the 64 sums of its points in registers, then a fixed
+// shuffle tree per wavefront and a fixed sum over the 4 wavefronts
+// (deterministic).  part[(i + nf*j)*nz + z] = wz[z] * plane sum.
+constexpr int kGT = 8;
+
 __global__ void __launch_bounds__(256) gram_kernel(const double *const *f, const double *const *g,
-                                                   int nf, const double *wx, const double *wy,
-                                                   const double *wz, int nx, int ny, int nz,
-                                                   double *part) {
-    __shared__ double red[256];
-    const int pair = blockIdx.y, z = blockIdx.x;
-    const int i = pair % nf, j = pair / nf;
-    const double *fi = f[i] + (long)nx * ny * z;
-    const double *gj = g[j] + (long)nx * ny * z;
-    double s = 0.0;
+                                                   int nf, int ng, const double *wx,
+                                                   const double *wy, const double *wz, int nx,
+                                                   int ny, int nz, double *part) {
+    __shared__ double red[4][kGT * kGT];
+    const int z = blockIdx.x;
+    const int tilesf = (nf + kGT - 1) / kGT;
+    const int i0 = (blockIdx.y % tilesf) * kGT, j0 = (blockIdx.y / tilesf) * kGT;
+    const long off = (long)nx * ny * z;
+    const double *fp[kGT], *gp[kGT];
+#pragma unroll
+    for (int a = 0; a < kGT; ++a) {
+        fp[a] = f[min(i0 + a, nf - 1)] + off;  // out-of-range rows read a valid cube, unused
+        gp[a] = g[min(j0 + a, ng - 1)] + off;
+    }
+    double acc[kGT][kGT];
+#pragma unroll
+    for (int a = 0; a < kGT; ++a)
+#pragma unroll
+        for (int b = 0; b < kGT; ++b) acc[a][b] = 0.0;
     for (int e = threadIdx.x; e < nx * ny; e += 256) {
         const int x = e % nx, y = e / nx;
-        s += (wx[x] * wy[y]) * (fi[e] * gj[e]);
+        const double w = wx[x] * wy[y];
+        double fv[kGT], gv[kGT];
+#pragma unroll
+        for (int a = 0; a < kGT; ++a) {
+            fv[a] = w * fp[a][e];
+            gv[a] = gp[a][e];
+        }
+#pragma unroll
+        for (int a = 0; a < kGT; ++a)
+#pragma unroll
+            for (int b = 0; b < kGT; ++b) acc[a][b] += fv[a] * gv[b];
     }
-    red[threadIdx.x] = s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < kGT; ++a)
+#pragma unroll
+        for (int b = 0; b < kGT; ++b) {
+            double v = acc[a][b];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) red[wave][a * kGT + b] = v;
+        }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x < kGT * kGT) {
+        const int a = threadIdx.x / kGT, b = threadIdx.x % kGT;
+        const double v = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                         (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        if (i0 + a < nf && j0 + b < ng)
+            part[((long)(i0 + a) + (long)nf * (j0 + b)) * nz + z] = v * wz[z];
     }
-    if (threadIdx.x == 0) part[(long)pair * nz + z] = red[0] * wz[z];
 }
 
 // df = d f / d(axis) from the cells' Lagrange interpolants: at a point
@@ -373,7 +411,8 @@ int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
     DAGE_TRY(use_device(ctx));
     const int nx = grid->shape(0), ny = grid->shape(1), nz = grid->shape(2);
     const int npair = nf * ng;
-    DAGE_REQUIRE(npair <= 65535, DAGE_ERR_ARG, "too many pairs (%d)", npair);
+    const int ntile = ((nf + kGT - 1) / kGT) * ((ng + kGT - 1) / kGT);
+    DAGE_REQUIRE(ntile <= 65535, DAGE_ERR_ARG, "too many pairs (%d)", npair);
     const double **d_f = nullptr, **d_g = nullptr;
     double *d_part = nullptr;
     int rc = to_device(f, nf, &d_f);
@@ -384,9 +423,9 @@ int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
     }
     std::vector<double> part((size_t)npair * nz);
     if (rc == DAGE_OK) {
-        hipLaunchKernelGGL(gram_kernel, dim3(nz, npair), dim3(256), 0, ctx->stream, d_f, d_g, nf,
-                           grid->d_weights[0], grid->d_weights[1], grid->d_weights[2], nx, ny, nz,
-                           d_part);
+        hipLaunchKernelGGL(gram_kernel, dim3(nz, ntile), dim3(256), 0, ctx->stream, d_f, d_g, nf,
+                           ng, grid->d_weights[0], grid->d_weights[1], grid->d_weights[2], nx, ny,
+                           nz, d_part);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(part.data(), d_part, 8 * part.size(), hipMemcpyDeviceToHost,
                            ctx->stream) != hipSuccess ||

@@ -190,31 +190,46 @@ class HelmholtzSCF:
             e += self.integrate(self.ex)
         return e + self.mol.nuclear_repulsion()
 
+    def _scratch(self, k):
+        """Gradient and work cubes for k functions (grown on demand)."""
+        while len(self.work) < k:
+            self.work.append(DeviceCube(self.ctx, self.shape))
+            for g in self.grad:
+                g.append(DeviceCube(self.ctx, self.shape))
+
     def ritz(self, phi):
         """Ritz vectors of H (with the current V) in span(phi): returns
-        (eigenvalues, coefficients (k x k), kinetic matrix)."""
+        (eigenvalues, coefficients (k x m), kinetic matrix), m <= k.  The
+        overlap is inverted by canonical orthogonalisation: directions with
+        overlap eigenvalue below 1e-10 of the largest are dropped."""
+        self._scratch(len(phi))
         S = self._gram(phi, phi)
-        T = np.zeros_like(S)
-        for a in range(3):
-            # exact kinetic integral of the cell interpolants along axis a
-            for i, f in enumerate(phi):
+        # exact kinetic integrals of the cell interpolants: the three axes'
+        # stiffness results are summed first (the Gram is linear in them)
+        k = len(phi)
+        for i, f in enumerate(phi):
+            for a in range(3):
                 call("dage_cube_stiffness", self.grid.handle, a, _vp(f), _vp(self.grad[a][i]))
-            T += 0.5 * self._gram(phi, self.grad[a][:len(phi)])
+            self._axpby(1.0, self.grad[1][i], 1.0, self.grad[0][i])
+            self._axpby(1.0, self.grad[2][i], 1.0, self.grad[0][i])
+        T = 0.5 * self._gram(phi, self.grad[0][:k])
         for i, f in enumerate(phi):
             self._mul(self.V, f, self.work[i])
         Vm = self._gram(phi, self.work[:len(phi)])
         H = T + Vm
         H = 0.5 * (H + H.T)
         S = 0.5 * (S + S.T)
-        # generalized symmetric eigenproblem through S^-1/2
         s, U = np.linalg.eigh(S)
-        X = U @ np.diag(1.0 / np.sqrt(s)) @ U.T
-        e, C = np.linalg.eigh(X @ H @ X)
+        keep = s > 1e-10 * s[-1]
+        X = U[:, keep] / np.sqrt(s[keep])
+        e, C = np.linalg.eigh(X.T @ H @ X)
         return e, X @ C, T
 
     def _adopt(self, phi, e, C, T):
-        """psi <- phi C (orthonormal Ritz vectors, signs kept close to phi)."""
-        k = len(phi)
+        """psi <- the lowest nocc Ritz vectors phi C (orthonormal, signs kept
+        close to phi)."""
+        k = self.nocc
+        C = C[:, :k].copy()
         for j in range(k):
             if C[j, j] < 0:
                 C[:, j] = -C[:, j]
@@ -269,7 +284,12 @@ class HelmholtzSCF:
             # delta = -1/(2 pi) G - psi
             self._axpby(-1.0 / (2.0 * math.pi), out, -1.0, self.delta[i])
 
-    def run(self, max_iter=100, e_tol=1e-6, eig_tol=1e-6, update=None, log=None) -> SCFResult:
+    def run(self, max_iter=100, e_tol=1e-6, eig_tol=1e-6, update=None, log=None,
+            combine="choose") -> SCFResult:
+        """`combine` says how an `update` proposal meets the plain one:
+        "choose" keeps whichever set has the lower occupied Ritz eigenvalue
+        sum; "span" takes the lowest nocc Ritz vectors of the span of both
+        (never above either, by the variational principle)."""
         res = SCFResult()
         t0 = time.perf_counter()
         for it in range(1, max_iter + 1):
@@ -290,15 +310,22 @@ class HelmholtzSCF:
             for i in range(self.nocc):
                 self.phi[i].copy_from(self.psi[i])
                 self._axpby(self.w, self.delta[i], 1.0, self.phi[i])
-            e, C, T = self.ritz(self.phi)
-            chosen = (self.phi, e, C, T)
             used = False
-            if update is not None:
+            if update is not None and combine == "span":
                 update(self.psi, self.delta, self.alt)
-                e2, C2, T2 = self.ritz(self.alt)
-                if np.sum(e2[:self.nocc]) < np.sum(e[:self.nocc]):
-                    chosen = (self.alt, e2, C2, T2)
-                    used = True
+                both = self.phi + self.alt
+                e2, C2, T2 = self.ritz(both)
+                chosen = (both, e2, C2, T2)
+                used = True
+            else:
+                e, C, T = self.ritz(self.phi)
+                chosen = (self.phi, e, C, T)
+                if update is not None:
+                    update(self.psi, self.delta, self.alt)
+                    e2, C2, T2 = self.ritz(self.alt)
+                    if np.sum(e2[:self.nocc]) < np.sum(e[:self.nocc]):
+                        chosen = (self.alt, e2, C2, T2)
+                        used = True
             res.learned_used.append(used)
             self._adopt(*chosen)
         res.wall_s = time.perf_counter() - t0
