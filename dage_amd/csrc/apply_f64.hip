@@ -78,19 +78,8 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    // XCD-aware tile order: workgroups go round-robin to the 8 XCDs (h % 8),
-    // each with its own L2.  The bijection below hands XCD c a contiguous
-    // range of tiles, and consecutive tiles share the A panel (column tiles
-    // of one row tile), so those re-reads hit the same L2.
-    int m0, n0;
-    {
-        const int total = gridDim.x, h = blockIdx.x;
-        const int c = h & 7, k = h >> 3;
-        const int tile = c * (total >> 3) + min(c, total & 7) + k;
-        const int ntn = (g.N + BN - 1) / BN;
-        m0 = (tile / ntn) * BM;
-        n0 = (tile % ntn) * BN;
-    }
+    const int m0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
 
     const double *A = g.A;
     const double *B = g.B;
@@ -320,9 +309,7 @@ void launch_t(const Gemm &g, dim3 grid, hipStream_t s) {
 template <bool AM, bool BK, int PASS>
 int launch_nb(const Gemm &g, int nb, int zdim, hipStream_t s) {
     const int BM = 64 * kMW;
-    const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + 16 * nb - 1) / (16 * nb));
-    DAGE_REQUIRE(tiles < (1L << 31), DAGE_ERR_ARG, "GEMM of %ld tiles too large", tiles);
-    const dim3 grid((unsigned)tiles, 1, zdim);
+    const dim3 grid((g.M + BM - 1) / BM, (g.N + 16 * nb - 1) / (16 * nb), zdim);
     switch (nb) {
     case 1: launch_t<1, AM, BK, PASS>(g, grid, s); break;
     case 2: launch_t<2, AM, BK, PASS>(g, grid, s); break;

@@ -104,17 +104,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     const int wave = tid >> 6;
     const int lrow = lane & 15;
     const int kg = lane >> 4;  // k group of 8
-    // XCD-aware tile order (as in the FP64 apply): XCD c = h % 8 gets a
-    // contiguous range of tiles, column tiles of one row tile adjacent.
-    int m0, n0;
-    {
-        const int total = gridDim.x, h = blockIdx.x;
-        const int c = h & 7, k = h >> 3;
-        const int tile = c * (total >> 3) + min(c, total & 7) + k;
-        const int ntn = (g.N + BN - 1) / BN;
-        m0 = (tile / ntn) * BM;
-        n0 = (tile % ntn) * BN;
-    }
+    const int m0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
 
     const TA *A = static_cast<const TA *>(g.A);
     const __bf16 *Bp = g.Bp;
@@ -274,7 +265,9 @@ int pad_all_nb(int n) {
     return p;
 }
 
-constexpr int kMWs = 2;
+// 64-row block tiles: 144-148 VGPRs, so two wavefronts per SIMD (128-row
+// tiles need 256 VGPRs + 64 AGPRs, one wavefront, and ran 1.47x slower at 97^3)
+constexpr int kMWs = 1;
 
 template <int NB, int NPAIR, typename TA, typename TC, bool AK, int PASS>
 void launch_s(const SGemm &g, dim3 grid, hipStream_t s) {
@@ -285,9 +278,7 @@ void launch_s(const SGemm &g, dim3 grid, hipStream_t s) {
 template <int NPAIR, typename TA, typename TC, bool AK, int PASS>
 int launch_split(dage_ctx ctx, const char *name, const SGemm &g, int nb, int zdim) {
     const int BM = 64 * kMWs;
-    const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + 16 * nb - 1) / (16 * nb));
-    DAGE_REQUIRE(tiles < (1L << 31), DAGE_ERR_ARG, "GEMM of %ld tiles too large", tiles);
-    const dim3 grid((unsigned)tiles, 1, zdim);
+    const dim3 grid((g.M + BM - 1) / BM, (g.N + 16 * nb - 1) / (16 * nb), zdim);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
     switch (nb) {

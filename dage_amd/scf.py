@@ -90,6 +90,15 @@ def is_converged(energies, eigen_values, e_tol=1e-6, eig_tol=1e-6) -> bool:
     return bool(np.all(d <= eig_tol))
 
 
+def backoff_skips(rejected: int, backoff) -> bool:
+    """Whether the learned update is skipped this iteration: after n = backoff[0]
+    proposals in a row lost to the plain step, only every backoff[1]-th
+    iteration evaluates it (counting from the n-th loss) until one wins."""
+    if backoff is None or rejected < backoff[0]:
+        return False
+    return (rejected - backoff[0]) % backoff[1] != 0
+
+
 @dataclass
 class SCFResult:
     energies: list = field(default_factory=list)
@@ -285,13 +294,17 @@ class HelmholtzSCF:
             self._axpby(-1.0 / (2.0 * math.pi), out, -1.0, self.delta[i])
 
     def run(self, max_iter=100, e_tol=1e-6, eig_tol=1e-6, update=None, log=None,
-            combine="choose") -> SCFResult:
+            combine="choose", backoff=None) -> SCFResult:
         """`combine` says how an `update` proposal meets the plain one:
         "choose" keeps whichever set has the lower occupied Ritz eigenvalue
         sum; "span" takes the lowest nocc Ritz vectors of the span of both
-        (never above either, by the variational principle)."""
+        (never above either, by the variational principle).
+        `backoff` = (n, period), "choose" only: after n proposals in a row
+        lost to the plain step, the update is evaluated only every period-th
+        iteration until one wins again (the plain step is taken meanwhile)."""
         res = SCFResult()
         t0 = time.perf_counter()
+        rejected = 0
         for it in range(1, max_iter + 1):
             self.build_potential()
             E = self.total_energy()
@@ -320,12 +333,17 @@ class HelmholtzSCF:
             else:
                 e, C, T = self.ritz(self.phi)
                 chosen = (self.phi, e, C, T)
-                if update is not None:
+                if update is not None and backoff_skips(rejected, backoff):
+                    rejected += 1
+                elif update is not None:
                     update(self.psi, self.delta, self.alt)
                     e2, C2, T2 = self.ritz(self.alt)
                     if np.sum(e2[:self.nocc]) < np.sum(e[:self.nocc]):
                         chosen = (self.alt, e2, C2, T2)
                         used = True
+                        rejected = 0
+                    else:
+                        rejected += 1
             res.learned_used.append(used)
             self._adopt(*chosen)
         res.wall_s = time.perf_counter() - t0
