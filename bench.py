@@ -25,6 +25,7 @@ duration, against the 78.6 TF FP64 MFMA peak; `traffic` is the HBM bytes per
 launch from the committed rocprofv3 counter summary.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -53,6 +54,8 @@ def parse():
                    help="skip the config-3 batched Helmholtz measurement")
     p.add_argument("--no-config4", dest="config4", action="store_false",
                    help="skip the config-4 SCF (plain vs learned) measurement")
+    p.add_argument("--config5", action="store_true",
+                   help="also measure config-5 shapes (385^3 GBFMM, 64-orbital sweep; ~1 min)")
     p.add_argument("--scf-max-iter", type=int, default=1500,
                    help="iteration cap of the timed plain and learned SCF runs")
     p.add_argument("--scf-train-iter", type=int, default=600,
@@ -533,6 +536,95 @@ def bench_config4(ctx, args):
     }
 
 
+# ----------------------------------------------------------------- config 5
+def bench_config5(ctx, args):
+    """Config-5 shapes (opt-in, --config5): a 385^3 cube over [-19.2,19.2]^3
+    (nlip 7, 64 cells, h 0.1; 57.07 M points, 456.5 MB per cube).  One GBFMM
+    Coulomb apply with 8x8x8 boxes (lmax 12, near field T = 50), FP64 and
+    bf16x6, checked against the analytic potential of 8 Gaussians on 8
+    z-planes; one Helmholtz sweep over 64 orbitals (k^2 uniform in
+    [0.05, 6], T = 40) as one batch graph in FP64 and bf16x6 against 64
+    plain FP64 applies.  Not the full config-5 SCF (hours on one GPU)."""
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    from dage_amd._lib import call
+    rng = np.random.default_rng(0xDA6E0005)
+    ax = O.Axis.ncell_fixed(-19.2, 19.2, 7, 64)
+    n = ax.n
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    q = np.ones(8)
+    alpha = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 8))
+    centers = rng.uniform(-6.0, 6.0, (8, 3))
+    rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
+    d_rho = dage_amd.DeviceCube(ctx, rho.shape, rho)
+    del rho
+    d_pot = dage_amd.DeviceCube(ctx, (n, n, n))
+    out = {"workload": "config-5 shapes: 385^3 over [-19.2,19.2]^3 (nlip 7, h 0.1); GBFMM 8x8x8 "
+                       "boxes lmax 12 T=50; 64-orbital Helmholtz sweep k^2 U[0.05,6] T=40",
+           "n": n}
+    gq = dage_amd.GaussQuad(25, 25)
+    c0 = time.perf_counter()
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=3, lmax=12, gauss=gq)
+    out["gbfmm_setup_s"] = round(time.perf_counter() - c0, 2)
+    pots = {}
+    for label, prec in (("fp64", dage_amd.DAGE_PREC_FP64), ("bf16x6", dage_amd.DAGE_PREC_BF16X6)):
+        fmm.apply_device(d_rho, d_pot, prec)
+        ctx.time_begin()
+        for _ in range(2):
+            fmm.apply_device(d_rho, d_pot, prec)
+        ms = ctx.time_end() / 2
+        pots[label] = d_pot.download()
+        out[f"gbfmm_{label}_ms"] = round(ms, 2)
+        out[f"gbfmm_{label}_Gpts_s"] = round(n ** 3 / (ms * 1e-3) / 1e9, 4)
+    zs = ax.points[n // 2 - 4:n // 2 + 4]
+    exact = O.gauss_potential(q, alpha, centers, ax.points, ax.points, zs)
+    out["gbfmm_vs_analytic_8_planes"] = rel(pots["fp64"][:, :, n // 2 - 4:n // 2 + 4], exact)
+    out["gbfmm_bf16x6_vs_fp64"] = rel(pots["bf16x6"], pots["fp64"])
+    del pots, exact
+    fmm.close()
+    progress("config5: GBFMM done")
+    # 64-orbital Helmholtz sweep
+    nrhs = 64
+    k2 = rng.uniform(0.05, 6.0, nrhs)
+    energies = -0.5 * k2
+    hq = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
+    T = hq.get_npoints()
+    flops = 6.0 * T * float(n) ** 4 * nrhs
+    op = dage_amd.Helmholtz3D(ctx, grid, grid, hq)
+    op.set_energy(energies[0])
+    op.transform_cuda_cube(d_rho, d_pot)
+    ctx.time_begin()
+    for i in range(nrhs):
+        op.set_energy(energies[i])
+        op.transform_cuda_cube(d_rho, d_pot)
+    plain_ms = ctx.time_end()
+    last_plain = d_pot.download()
+    op.close()
+    out["sweep_plain_fp64_ms"] = round(plain_ms, 1)
+    sweep = {}
+    for label, prec in (("fp64", dage_amd.DAGE_PREC_FP64), ("bf16x6", dage_amd.DAGE_PREC_BF16X6)):
+        b = dage_amd.HelmholtzBatch(ctx, [grid] * nrhs, energies, hq, precision=prec)
+        for i in range(nrhs):
+            call("dage_memcpy", ctx.handle, ctypes.c_void_p(b.input_ptr(i)),
+                 ctypes.c_void_p(d_rho.ptr), ctypes.c_size_t(8 * n ** 3), 2)
+        b.apply()
+        ctx.time_begin()
+        b.apply()
+        ms = ctx.time_end()
+        sweep[label] = {"ms_per_sweep": round(ms, 1),
+                        "dense_fp64_equiv_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                        "speedup_vs_plain_fp64": round(plain_ms / ms, 3),
+                        "last_orbital_vs_plain": rel(b.download(nrhs - 1), last_plain)}
+        b.close()
+        progress(f"config5: {label} sweep {ms:.0f} ms")
+    out["sweep_batched"] = sweep
+    for obj in (d_rho, d_pot, grid):
+        obj.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -585,6 +677,8 @@ def main():
         progress("config3 done")
     if args.config4:
         line["config4"] = bench_config4(ctx, args)
+    if args.config5:
+        line["config5"] = bench_config5(ctx, args)
     print(json.dumps(line), flush=True)
     ctx.close()
 

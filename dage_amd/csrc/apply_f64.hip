@@ -352,9 +352,11 @@ Plan make_plan(const ApplyMats &m, int B = 1) {
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
     size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
+    const size_t total_cap = 64ull << 30;  // and for all B right-hand sides together
     const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
     Plan p;
-    p.tchunk = (int)std::max<size_t>(1, std::min<size_t>(m.ntp, budget / per_t));
+    p.tchunk = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)m.ntp, budget / per_t, total_cap / (per_t * (size_t)B)}));
     const long M3 = nxo * nyo;
     const int mtiles = (int)((M3 + 64 * kMW - 1) / (64 * kMW));
     const int nb = pick_nb((int)nzo);

@@ -89,10 +89,21 @@ struct SGemm {
     long aS1;
     int Ksplit;               // > 0: z_lo selects K steps [z_lo*Ksplit, (z_lo+1)*Ksplit) of 32
     int accumulate;
+    // Optional B row masks (bit c: a nonzero among columns [16c, 16c+16));
+    // rows of batch index z_lo start at mask + z_lo*mS, or at the K range
+    // start with Ksplit.  K steps that are zero on the block's columns are
+    // skipped (they would add exactly zero).
+    const uint64_t *mask;
+    long mS;
 };
 
+constexpr int kMaxSteps = 2048;  // K steps a block can list (K <= 65536)
+
 // Block: 4 wavefronts stacked along M, wavefront tile (16*MW) x (16*NB).
-template <int MW, int NB, int NPAIR, typename TA, typename TC, bool A_KCONTIG, int PASS>
+// MASKED: the block lists its nonzero K steps from g.mask (the unmasked
+// instantiation carries neither the list nor its LDS).
+template <int MW, int NB, int NPAIR, typename TA, typename TC, bool A_KCONTIG, int PASS,
+          bool MASKED>
 __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     constexpr int BM = 64 * MW;
     constexpr int BN = 16 * NB;
@@ -124,7 +135,37 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
         Bp += (long)zlo * g.bB;
         if (g.alpha) alpha = g.alpha[zlo + g.aS1 * zhi];
     }
-    const int nsteps = (kend - kbeg + kKS - 1) / kKS;
+    int nsteps = (kend - kbeg + kKS - 1) / kKS;
+    __shared__ unsigned short slist[MASKED ? kMaxSteps : 1];
+    __shared__ int slist_n;
+    const uint64_t *mrow = nullptr;
+    if (MASKED) mrow = g.mask + (g.Ksplit > 0 ? (long)kbeg : (long)zlo * g.mS);
+    const bool listed = MASKED && nsteps <= kMaxSteps;
+    if (listed) {
+        if (wave == 0) {
+            const uint64_t cols = ((NB >= 64) ? ~0ull : ((1ull << NB) - 1ull)) << (n0 / 16);
+            const int klen = kend - kbeg;
+            int cnt = 0;
+            for (int base = 0; base < nsteps; base += 64) {
+                const int st = base + lane;
+                bool need = false;
+                if (st < nsteps) {
+                    const int k0 = st * kKS;
+                    const int kn = min(kKS, klen - k0);
+                    uint64_t any = 0;
+                    for (int j = 0; j < kn; ++j) any |= mrow[k0 + j];
+                    need = (any & cols) != 0;
+                }
+                const uint64_t bal = __ballot(need);
+                if (need) slist[cnt + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)st;
+                cnt += __popcll(bal);
+            }
+            if (lane == 0) slist_n = cnt;
+        }
+        __syncthreads();
+        nsteps = slist_n;
+    }
+    auto step_of = [&](int s) { return listed ? (int)slist[s] : s; };
 
     long arow[MW];
     bool rok[MW];
@@ -170,8 +211,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
         for (int j = 0; j < NB; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
     if (nsteps > 0) {
-        copy_b(0, 0);
-        load_a(0);
+        copy_b(step_of(0), 0);
+        load_a(step_of(0));
         __syncthreads();
     }
     for (int s = 0; s < nsteps; ++s) {
@@ -188,8 +229,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
                 a[2][i][j] = l;
             }
         if (s + 1 < nsteps) {
-            copy_b(s + 1, buf ^ 1);
-            load_a(s + 1);
+            copy_b(step_of(s + 1), buf ^ 1);
+            load_a(step_of(s + 1));
         }
         f4 c[MW][NB];
 #pragma unroll
@@ -271,8 +312,12 @@ constexpr int kMWs = 1;
 
 template <int NB, int NPAIR, typename TA, typename TC, bool AK, int PASS>
 void launch_s(const SGemm &g, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS>), grid,
-                       dim3(kThreads), 0, s, g);
+    if (g.mask)
+        hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, true>), grid,
+                           dim3(kThreads), 0, s, g);
+    else
+        hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, false>), grid,
+                           dim3(kThreads), 0, s, g);
 }
 
 template <int NPAIR, typename TA, typename TC, bool AK, int PASS>
@@ -321,7 +366,9 @@ int prepare_planes(dage_operator op) {
     L.npz = pad_all_nb(nzo);
     L.kpx = round_up(nxi, kKS);
     L.kpy = round_up(nyi, kKS);
-    L.kpz = round_up(nzi * T, kKS);
+    // one extra zero K step: a t-chunk starts at a multiple of 8 rows, so its
+    // last 32-row step can reach up to 24 rows past round_up(nzi*T, 32)
+    L.kpz = round_up(nzi * T, kKS) + kKS;
     const size_t px = (size_t)T * L.npx * L.kpx, py = (size_t)T * L.npy * L.kpy,
                  pz = (size_t)L.npz * L.kpz;
     L.off_x = 0;
@@ -353,13 +400,16 @@ MPlan make_mplan(dage_operator op, int B) {
     const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
     size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
+    const size_t total_cap = 64ull << 30;  // and for all B right-hand sides together
     const size_t per_t = 4ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
     MPlan p;
-    int tc = (int)std::max<size_t>(1, std::min<size_t>(op->ntp, budget / per_t));
-    // z-pass K steps of 32 must not straddle t chunks: nzi*tchunk % 32 == 0
-    // unless one chunk covers all t
+    int tc = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)op->ntp, budget / per_t, total_cap / (per_t * (size_t)B)}));
+    // the z pass reads its B planes from row nzi*t0 in 16-byte (8 x bf16)
+    // pieces, so nzi*tchunk % 8 == 0 unless one chunk covers all t (a K step
+    // reaching past the chunk's rows multiplies them by zeroed A values)
     if (tc < op->ntp) {
-        const int unit = 32 / (int)std::__gcd<long>(nzi, 32);
+        const int unit = 8 / (int)std::__gcd<long>(nzi, 8);
         tc = std::max(unit, tc / unit * unit);
         tc = std::min(tc, op->ntp);
     }
@@ -401,8 +451,13 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
     const int T = op->ntp, B = bs.B;
     DAGE_TRY(prepare_planes(op));
     const MixedLayout &L = op->mixed;
-    const int nbx = bs.nb[0] ? bs.nb[0] : L.nbx, nby = bs.nb[1] ? bs.nb[1] : L.nby,
-              nbz = bs.nb[2] ? bs.nb[2] : L.nbz;
+    // passes whose zero-panel masks pay use them, with the column-block count
+    // the mask cost model chose
+    auto nb_of = [&](int p, int dflt) {
+        return bs.nb[p] ? bs.nb[p] : (op->mask_used[p] ? op->nb_pick[p] : dflt);
+    };
+    const int nbx = nb_of(0, L.nbx), nby = nb_of(1, L.nby), nbz = nb_of(2, L.nbz);
+    const long mrx = (long)T * nxi, mry = (long)T * nyi;
     const __bf16 *P16 = reinterpret_cast<const __bf16 *>(op->d_planes);
     const MPlan plan = make_mplan(op, B);
     const size_t need = 4 * (plan.v_elems + plan.u_elems) + 8 * plan.p_elems + 256;
@@ -436,6 +491,7 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g1.C = V; g1.N = (int)nxo;
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1; g1.bC = Vt;
         g1.Z0 = tc; g1.bA1 = nin; g1.bC1 = Vt * tc;
+        if (op->mask_used[0]) { g1.mask = op->d_mask + (long)t0 * nxi; g1.mS = nxi; }
         DAGE_TRY((launch_split<NPAIR, double, float, true, 0>(op->ctx, "gemm_mixed_xmode", g1,
                                                               nbx, tc * B)));
         // y mode, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
@@ -451,6 +507,7 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g2.Z0 = tc; g2.bA1 = Vt * tc; g2.bC1 = Ut * tc;
         g2.alpha = W + t0;
         g2.aS1 = wS;
+        if (op->mask_used[1]) { g2.mask = op->d_mask + mrx + (long)t0 * nyi; g2.mS = nyi; }
         DAGE_TRY((launch_split<NPAIR, float, float, false, 1>(op->ctx, "gemm_mixed_ymode", g2,
                                                               nby, tc * B)));
         // z mode, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
@@ -462,13 +519,14 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g3.A = U;
         g3.M = (int)M3; g3.K = K3; g3.M0 = g3.M;
         g3.sAm0 = 1; g3.sAm1 = 0; g3.sAk = M3; g3.bA = 0;
-        g3.Bp = P16 + L.off_z + (size_t)nzi * t0;  // nzi*t0 is a multiple of 32 (see make_mplan)
+        g3.Bp = P16 + L.off_z + (size_t)nzi * t0;  // nzi*t0 is a multiple of 8 (see make_mplan)
         g3.Np = L.npz; g3.Kp = L.kpz;
         g3.sBq = (long)L.npz * L.kpz; g3.bB = 0;
         g3.C = P; g3.N = (int)nzo;
         g3.sCm0 = 1; g3.sCm1 = 0; g3.sCn = M3; g3.bC = M3 * nzo;
         g3.Z0 = splits; g3.bA1 = Ut * tc; g3.bC1 = M3 * nzo * plan.splits;
         g3.Ksplit = per;
+        if (op->mask_used[2]) g3.mask = op->d_mask + mrx + mry + (long)nzi * t0;
         g3.accumulate = (t0 > 0);
         if (t0 == 0 && splits < plan.splits)
             for (int b = 0; b < B; ++b)

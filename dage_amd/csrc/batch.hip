@@ -15,6 +15,7 @@
 // group and re-captures the graph with the fastest.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "dage_internal.hpp"
@@ -26,6 +27,7 @@ struct dage_batch_s {
         double *d_in = nullptr, *d_out = nullptr, *d_w = nullptr;
         long vol = 0;              // points per cube
         int nb[3] = {0, 0, 0};     // tuned column blocks (0: heuristic)
+        int slice = 0;             // right-hand sides per apply call (workspace cap)
     };
     dage_ctx ctx = nullptr;
     int nrhs = 0;
@@ -77,22 +79,54 @@ int upload_weights(dage_batch b) {
     return DAGE_OK;
 }
 
-// All groups, one after the other, on the context stream.
-int run_batch(dage_batch b, bool no_alloc) {
-    for (auto &g : b->groups) {
+int group_workspace(dage_batch b, dage_batch_s::Group &g, int B, size_t *n) {
+    if (b->precision == DAGE_PREC_FP64) return apply_f64_batch_workspace(operator_mats(g.op), B, n);
+    return apply_mixed_batch_workspace(g.op, B, n);
+}
+
+// Right-hand sides per apply call: all of the group's, halved until the
+// workspace fits the cap (the t-chunking has a floor: one t-point in FP64,
+// up to 8 in split-BF16, so very large groups run in slices).
+int pick_slice(dage_batch b, dage_batch_s::Group &g) {
+    size_t cap = 64ull << 30;
+    if (const char *e = std::getenv("DAGE_BATCH_WORKSPACE_MB")) cap = (size_t)atol(e) << 20;
+    int s = (int)g.members.size();
+    while (s > 1) {
+        size_t n = 0;
+        DAGE_TRY(group_workspace(b, g, s, &n));
+        if (n <= cap) break;
+        s = (s + 1) / 2;
+    }
+    g.slice = s;
+    return DAGE_OK;
+}
+
+// One group's applies, slice by slice; nb: column-block counts (0: heuristic).
+int run_group(dage_batch b, dage_batch_s::Group &g, const int nb[3], bool no_alloc) {
+    const int B = (int)g.members.size(), T = (int)b->tp.size();
+    const int S = g.slice > 0 ? g.slice : B;
+    for (int s0 = 0; s0 < B; s0 += S) {
         BatchSpec bs;
-        bs.B = (int)g.members.size();
-        bs.w = g.d_w;
+        bs.B = std::min(S, B - s0);
+        bs.w = g.d_w + (size_t)s0 * T;
         const bool coda = (g.op->coda != 0.0) && !(b->flags & DAGE_OP_NO_CODA);
         bs.coda = coda ? g.op->coda : 0.0;
-        for (int i = 0; i < 3; ++i) bs.nb[i] = g.nb[i];
+        for (int i = 0; i < 3; ++i) bs.nb[i] = nb[i];
         bs.no_alloc = no_alloc;
+        const double *in = g.d_in + (size_t)s0 * g.vol;
+        double *out = g.d_out + (size_t)s0 * g.vol;
         if (b->precision == DAGE_PREC_FP64)
-            DAGE_TRY(apply_f64_batch(b->ctx, operator_mats(g.op), g.d_in, g.d_out, bs, &b->ws));
+            DAGE_TRY(apply_f64_batch(b->ctx, operator_mats(g.op), in, out, bs, &b->ws));
         else
-            DAGE_TRY(apply_mixed_batch(g.op, g.d_in, g.d_out,
-                                       b->precision == DAGE_PREC_BF16X6 ? 6 : 3, bs, &b->ws));
+            DAGE_TRY(apply_mixed_batch(g.op, in, out, b->precision == DAGE_PREC_BF16X6 ? 6 : 3,
+                                       bs, &b->ws));
     }
+    return DAGE_OK;
+}
+
+// All groups, one after the other, on the context stream.
+int run_batch(dage_batch b, bool no_alloc) {
+    for (auto &g : b->groups) DAGE_TRY(run_group(b, g, g.nb, no_alloc));
     return DAGE_OK;
 }
 
@@ -100,9 +134,8 @@ int reserve_workspace(dage_batch b) {
     size_t need = 0;
     for (auto &g : b->groups) {
         size_t n = 0;
-        const int B = (int)g.members.size();
-        if (b->precision == DAGE_PREC_FP64) DAGE_TRY(apply_f64_batch_workspace(operator_mats(g.op), B, &n));
-        else DAGE_TRY(apply_mixed_batch_workspace(g.op, B, &n));
+        DAGE_TRY(pick_slice(b, g));
+        DAGE_TRY(group_workspace(b, g, g.slice, &n));
         need = std::max(need, n);
     }
     if (need > b->ws.bytes) {
@@ -310,11 +343,6 @@ int dage_batch_tune(dage_batch b, int reps, int *tiles) {
         int pick[3] = {0, 0, 0};
         for (int nb = 1; nb <= 8; ++nb) {
             for (int i = 0; i < 3; ++i) g.nb[i] = nb;
-            BatchSpec bs;
-            bs.B = (int)g.members.size();
-            bs.w = g.d_w;
-            bs.nb[0] = bs.nb[1] = bs.nb[2] = nb;
-            bs.no_alloc = true;
             ctx->profiling = true;
             ctx->profile.clear();
             int rc = DAGE_OK;
@@ -324,11 +352,7 @@ int dage_batch_tune(dage_batch b, int reps, int *tiles) {
                     (void)prof_resolve(ctx);
                     ctx->profile.clear();
                 }
-                rc = (kind == 0) ? apply_f64_batch(ctx, operator_mats(g.op), g.d_in, g.d_out, bs,
-                                                   &b->ws)
-                                 : apply_mixed_batch(g.op, g.d_in, g.d_out,
-                                                     b->precision == DAGE_PREC_BF16X6 ? 6 : 3, bs,
-                                                     &b->ws);
+                rc = run_group(b, g, g.nb, true);
             }
             DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
             (void)prof_resolve(ctx);

@@ -139,3 +139,25 @@ def test_batch_errors(ctx):
     with pytest.raises(dage_amd.DageError):  # the C-ABI checks the orbital index
         dage_amd._lib.call("dage_batch_download", b.handle, 1,
                            host.ctypes.data_as(dage_amd._lib._dp))
+
+
+@pytest.mark.parametrize("precision,tol", [(dage_amd.DAGE_PREC_FP64, 1e-13),
+                                           (dage_amd.DAGE_PREC_BF16X6, 2e-7)])
+def test_batch_slices_under_workspace_cap(ctx, monkeypatch, precision, tol):
+    """A group whose workspace would exceed the cap runs in slices of
+    right-hand sides inside the same graph; the result matches the unsliced
+    batch (the z pass's K split depends on the slice size, so only to
+    rounding)."""
+    sizes = [25] * 7
+    energies = [-0.3, -1.2, -0.05, -2.0, -0.5, -0.8, -1.5]
+    whole, _, _, _, _ = make_batch(ctx, sizes, energies, precision=precision, seed=4)
+    whole.apply()
+    ref = [whole.download(i) for i in range(len(sizes))]
+    whole.close()
+    monkeypatch.setenv("DAGE_BATCH_WORKSPACE_MB", "1")
+    sliced, _, _, _, _ = make_batch(ctx, sizes, energies, precision=precision, seed=4)
+    for _ in range(2):  # replays of the captured graph
+        sliced.apply()
+        for i in range(len(sizes)):
+            assert rel_maxdiff(sliced.download(i), ref[i]) < tol, i
+    sliced.close()

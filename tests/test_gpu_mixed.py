@@ -87,3 +87,44 @@ def test_mixed_follows_set_matrices(ctx):
     op.set_matrices(2.0 * fx, fy, fz)
     v2 = op.transform_cube(rho, precision=BF16X6)
     assert rel_maxdiff(v2, 2.0 * v1) < 1e-7
+
+
+def test_mixed_block_sparse_matrices(ctx, monkeypatch):
+    """Split-BF16 passes skip 32-deep K steps that are zero on a block's
+    columns: bit-identical across column-block counts, and within the bf16x6
+    bar of the FP64 apply of the same banded, panel-zeroed matrices."""
+    rng = np.random.default_rng(78)
+    axes = [O.Axis.ncell_fixed(-4.0, 4.0, 5, 15), O.Axis.ncell_fixed(-3.0, 3.0, 5, 11),
+            O.Axis.ncell_fixed(-5.0, 5.0, 5, 18)]
+    n = [a.n for a in axes]
+    q = dage_amd.GaussQuad(nlin=20, nlog=20, tlin=2.0, tlog=500.0)
+    T = q.get_npoints()
+    mats = []
+    for a in range(3):
+        i, j = np.meshgrid(np.arange(n[a]), np.arange(n[a]), indexing="ij")
+        F = np.exp(-0.05 * (i - j) ** 2)[:, :, None] * rng.uniform(0.5, 1.5, (1, 1, T))
+        for t in range(T):
+            bw = int(rng.integers(2, n[a]))
+            F[np.abs(i - j) > bw, t] = 0.0
+            if t % 3 == 0:
+                k0 = 32 * int(rng.integers(0, (n[a] + 31) // 32))
+                F[:, k0:k0 + 32, t] = 0.0
+        mats.append(np.asfortranarray(F))
+    fx = mats[0]
+    fy = np.asfortranarray(mats[1].transpose(1, 0, 2))
+    fz = np.asfortranarray(mats[2].transpose(1, 0, 2))
+    cube = np.asfortranarray(1.0 + rng.uniform(0.0, 1.0, n))
+    g = dage_amd.Grid3D(ctx, [a.qmin for a in axes], [a.steps for a in axes], 5)
+    outs = []
+    for nb in ("1,1,1", "3,5,2", "8,8,8"):
+        monkeypatch.setenv("DAGE_APPLY_NB", nb)
+        op = dage_amd.Coulomb3D(ctx, g, g, q, coda=False)
+        op.set_matrices(fx, fy, fz)
+        if nb == "1,1,1":
+            ref = op.transform_cube(cube)
+        outs.append(op.transform_cube(cube, precision=BF16X6))
+        op.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    err = rel_maxdiff(outs[0], ref)
+    print(f"block-sparse bf16x6 vs FP64: {err:.3e}")
+    assert err < 2e-7  # toy shapes: the bar of test_gpu_batch.py (1.1e-7 measured here)
