@@ -1,6 +1,7 @@
 // C-ABI entry points (include/dage_ops.h).  Argument checking, handle
 // lifetime, host<->device staging and profiling live here; the numerics are in
 // host_math.cpp, matrices.hip, apply_f64.hip and reduce.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -29,6 +30,25 @@ namespace {
 std::mutex g_ref_mutex;
 }
 
+int ctx_scratch(dage_ctx ctx, size_t bytes, void **p) {
+    if (ctx->scratch_stream && ctx->scratch_stream != ctx->stream)
+        DAGE_HIP_CHECK(hipStreamSynchronize(ctx->scratch_stream));
+    ctx->scratch_stream = ctx->stream;
+    if (bytes > ctx->scratch_bytes) {
+        if (ctx->scratch) {
+            DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->scratch);
+        }
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        const size_t n = std::max(bytes, (size_t)1 << 20);
+        DAGE_HIP_CHECK(hipMalloc(&ctx->scratch, n));
+        ctx->scratch_bytes = n;
+    }
+    *p = ctx->scratch;
+    return DAGE_OK;
+}
+
 void ctx_retain(dage_ctx ctx) {
     std::lock_guard<std::mutex> lock(g_ref_mutex);
     ++ctx->refs;
@@ -51,6 +71,7 @@ void ctx_release(dage_ctx ctx) {
     }
     if (ctx->timer_start) (void)hipEventDestroy(ctx->timer_start);
     if (ctx->timer_stop) (void)hipEventDestroy(ctx->timer_stop);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -405,6 +426,10 @@ int dage_grid3d_destroy(dage_grid3d g) {
         if (p) (void)hipFree(p);
     for (auto &p : g->d_points)
         if (p) (void)hipFree(p);
+    for (int a = 0; a < 3; ++a) {
+        if (g->d_deriv[a]) (void)hipFree(g->d_deriv[a]);
+        if (g->d_stiff[a]) (void)hipFree(g->d_stiff[a]);
+    }
     delete g;
     ctx_release(ctx);
     return DAGE_OK;

@@ -513,19 +513,13 @@ int dage_conv3d_backward_data(dage_ctx ctx, int nx, int ny, int nz, int cin, int
     DAGE_REQUIRE(gout && w && gin && gout != gin, DAGE_ERR_ARG, "bad argument");
     const long vol = (long)nx * ny * nz;
     if (cin == kMaxC && cout == kMaxC) {
-        uint16_t *wt = nullptr;
-        DAGE_HIP_CHECK(hipMalloc((void **)&wt, sizeof(uint16_t) * kMaxC * kMaxC * 27));
+        void *scr = nullptr;  // transposed, mirrored weights
+        DAGE_TRY(ctx_scratch(ctx, sizeof(uint16_t) * kMaxC * kMaxC * 27, &scr));
+        uint16_t *wt = static_cast<uint16_t *>(scr);
         hipLaunchKernelGGL(transpose_mirror_kernel, dim3(27), dim3(256), 0, ctx->stream, w, cout,
                            cin, wt);
-        int rc = hipGetLastError() == hipSuccess ? DAGE_OK : DAGE_ERR_HIP;
-        if (rc != DAGE_OK) set_error("weight transpose launch failed");
-        if (rc == DAGE_OK) rc = launch_conv16(ctx, nx, ny, nz, gout, wt, nullptr, 0, gin);
-        if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == DAGE_OK) {
-            set_error("backward data failed");
-            rc = DAGE_ERR_HIP;
-        }
-        (void)hipFree(wt);
-        return rc;
+        DAGE_HIP_CHECK(hipGetLastError());
+        return launch_conv16(ctx, nx, ny, nz, gout, wt, nullptr, 0, gin);
     }
     hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, gout,
                        cout, cin, w, gin, nx, ny, nz);
@@ -545,36 +539,22 @@ int dage_conv3d_backward_filter(dage_ctx ctx, int nx, int ny, int nz, int cin, i
     const int rows = (int)std::max<long>(1, (nrows + 1023) / 1024);
     const int nblocks = (int)((nrows + rows - 1) / rows);
     const int len = cout * cin * 27 + cout;
-    float *part = nullptr;
-    DAGE_HIP_CHECK(hipMalloc((void **)&part, sizeof(float) * (size_t)nblocks * len));
+    const size_t bpart = sizeof(float) * (size_t)nblocks * len;
+    void *scr = nullptr;
+    DAGE_TRY(ctx_scratch(ctx, (bpart + 255) / 256 * 256 + sizeof(float) * len, &scr));
+    float *part = static_cast<float *>(scr);
+    float *sums = reinterpret_cast<float *>(static_cast<char *>(scr) + (bpart + 255) / 256 * 256);
     hipLaunchKernelGGL(conv_bwd_filter_kernel, dim3(nblocks), dim3(256), smem, ctx->stream, in,
                        gout, cin, cout, nx, ny, nz, rows, part);
-    int rc = DAGE_OK;
-    if (hipGetLastError() != hipSuccess) {
-        set_error("filter gradient launch failed");
-        rc = DAGE_ERR_HIP;
-    }
-    float *sums = nullptr;
-    if (rc == DAGE_OK && hipMalloc((void **)&sums, sizeof(float) * len) != hipSuccess) {
-        set_error("device allocation failed");
-        rc = DAGE_ERR_HIP;
-    }
-    if (rc == DAGE_OK) {
-        hipLaunchKernelGGL(sum_blocks_kernel, dim3((len + 255) / 256), dim3(256), 0, ctx->stream,
-                           part, nblocks, len, sums);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(gw, sums, sizeof(float) * (len - cout), hipMemcpyDeviceToDevice,
-                           ctx->stream) != hipSuccess ||
-            hipMemcpyAsync(gb, sums + (len - cout), sizeof(float) * cout, hipMemcpyDeviceToDevice,
-                           ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            set_error("filter gradient reduction failed");
-            rc = DAGE_ERR_HIP;
-        }
-    }
-    (void)hipFree(part);
-    if (sums) (void)hipFree(sums);
-    return rc;
+    DAGE_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3((len + 255) / 256), dim3(256), 0, ctx->stream,
+                       part, nblocks, len, sums);
+    DAGE_HIP_CHECK(hipGetLastError());
+    DAGE_HIP_CHECK(hipMemcpyAsync(gw, sums, sizeof(float) * (len - cout), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpyAsync(gb, sums + (len - cout), sizeof(float) * cout,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    return DAGE_OK;
 }
 
 int dage_relu_backward(dage_ctx ctx, long n, const float *out, float *grad) {
@@ -609,24 +589,20 @@ int dage_mse_loss(dage_ctx ctx, long n, const float *pred, const float *target, 
     DAGE_TRY(use_device(ctx));
     DAGE_REQUIRE(n > 0 && pred && target && grad && loss, DAGE_ERR_ARG, "bad argument");
     const int nb = std::min<long>(1024, (n + 255) / 256);
-    double *part = nullptr;
-    DAGE_HIP_CHECK(hipMalloc((void **)&part, sizeof(double) * nb));
+    void *scr = nullptr;
+    DAGE_TRY(ctx_scratch(ctx, sizeof(double) * nb, &scr));
+    double *part = static_cast<double *>(scr);
     hipLaunchKernelGGL(mse_kernel, dim3(nb), dim3(256), 0, ctx->stream, n, pred, target,
                        (float)(2.0 / (double)n), grad, part);
+    DAGE_HIP_CHECK(hipGetLastError());
     std::vector<double> h(nb);
-    int rc = DAGE_OK;
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        set_error("loss failed");
-        rc = DAGE_ERR_HIP;
-    }
+    DAGE_HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     double s = 0.0;
     for (double v : h) s += v;
     *loss = s / (double)n;
-    (void)hipFree(part);
-    return rc;
+    return DAGE_OK;
 }
 
 int dage_cvt_f64_f32(dage_ctx ctx, long n, const double *x, double scale, float *y) {

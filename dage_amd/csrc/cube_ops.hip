@@ -8,6 +8,7 @@
 // kernels (HBM-bound, grid-stride, x fastest) and one deterministic Gram-
 // matrix reduction with the grid's LIP weights (FP64: overlaps and energies
 // stay FP64).
+#include <initializer_list>
 #include <vector>
 
 #include "dage_internal.hpp"
@@ -225,10 +226,91 @@ __global__ void __launch_bounds__(256) stiffness_kernel(const double *f, double 
     }
 }
 
-template <typename T>
-int to_device(const T *host, size_t n, T **dev) {
-    DAGE_HIP_CHECK(hipMalloc((void **)dev, sizeof(T) * (n ? n : 1)));
-    DAGE_HIP_CHECK(hipMemcpy(*dev, host, sizeof(T) * n, hipMemcpyHostToDevice));
+// Carves 256-byte-aligned pieces out of the context scratch.
+struct Carver {
+    char *base = nullptr;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t n) {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += (sizeof(T) * (n ? n : 1) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+size_t carve_size(std::initializer_list<size_t> bytes) {
+    size_t s = 0;
+    for (size_t b : bytes) s += (b + 255) / 256 * 256;
+    return s;
+}
+
+// D[j*nlip + k] = L_k'(node j) for unit node spacing (barycentric form).
+std::vector<double> deriv_table(const Axis &ax) {
+    const int nlip = ax.nlip;
+    std::vector<double> node(nlip), wb(nlip), D((size_t)nlip * nlip, 0.0);
+    for (int k = 0; k < nlip; ++k) node[k] = (ax.points[k] - ax.points[0]) / ax.steps[0];
+    for (int k = 0; k < nlip; ++k) {
+        double p = 1.0;
+        for (int m = 0; m < nlip; ++m)
+            if (m != k) p *= node[k] - node[m];
+        wb[k] = 1.0 / p;
+    }
+    for (int j = 0; j < nlip; ++j) {
+        double diag = 0.0;
+        for (int k = 0; k < nlip; ++k)
+            if (k != j) {
+                D[j * nlip + k] = (wb[k] / wb[j]) / (node[j] - node[k]);
+                diag -= D[j * nlip + k];
+            }
+        D[j * nlip + j] = diag;
+    }
+    return D;
+}
+
+// Kc[a*nlip + b] = int_0^{nlip-1} L_a' L_b' (nodes 0..nlip-1), by the
+// nlip-point Gauss-Legendre rule (exact: the integrand has degree 2 nlip - 4).
+std::vector<double> stiffness_table(int nlip) {
+    std::vector<double> wb(nlip), Kc((size_t)nlip * nlip, 0.0);
+    for (int k = 0; k < nlip; ++k) {
+        double p = 1.0;
+        for (int m = 0; m < nlip; ++m)
+            if (m != k) p *= (double)(k - m);
+        wb[k] = 1.0 / p;
+    }
+    std::vector<double> gx(nlip), gw(nlip);
+    gauleg(nlip, gx.data(), gw.data());  // on [-1, 1]
+    const double half = 0.5 * (nlip - 1);
+    for (int q = 0; q < nlip; ++q) {
+        const double xi = half * (gx[q] + 1.0), wq = half * gw[q];
+        // L_a'(xi) = sum over m != a of wb[a] * prod_{mm != a, m} (xi - mm)
+        std::vector<double> dL(nlip, 0.0);
+        for (int a = 0; a < nlip; ++a) {
+            double sum = 0.0;
+            for (int m = 0; m < nlip; ++m) {
+                if (m == a) continue;
+                double p = wb[a];
+                for (int mm = 0; mm < nlip; ++mm)
+                    if (mm != a && mm != m) p *= xi - mm;
+                sum += p;
+            }
+            dL[a] = sum;
+        }
+        for (int a = 0; a < nlip; ++a)
+            for (int b = 0; b < nlip; ++b) Kc[a * nlip + b] += wq * dL[a] * dL[b];
+    }
+    return Kc;
+}
+
+// The grid's device copy of an axis table, built on first use.
+int grid_table(dage_grid3d grid, int axis, bool stiff, const double **out) {
+    double *&slot = stiff ? grid->d_stiff[axis] : grid->d_deriv[axis];
+    if (!slot) {
+        const Axis &ax = grid->axis[axis];
+        const std::vector<double> t = stiff ? stiffness_table(ax.nlip) : deriv_table(ax);
+        DAGE_HIP_CHECK(hipMalloc((void **)&slot, sizeof(double) * t.size()));
+        DAGE_HIP_CHECK(hipMemcpy(slot, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    }
+    *out = slot;
     return DAGE_OK;
 }
 
@@ -287,24 +369,23 @@ int dage_vec_lincomb(dage_ctx ctx, long n, int nin, const double *const *in, con
     for (int j = 0; j < nout; ++j)
         for (int i = 0; i < nin; ++i)
             DAGE_REQUIRE(out[j] != in[i], DAGE_ERR_ARG, "output %d aliases input %d", j, i);
-    const double **d_in = nullptr;
-    double **d_out = nullptr;
-    double *d_coef = nullptr;
-    int rc = to_device(in, nin, &d_in);
-    if (rc == DAGE_OK) rc = to_device(out, nout, &d_out);
-    if (rc == DAGE_OK) rc = to_device(coef, (size_t)nin * nout, &d_coef);
-    if (rc == DAGE_OK) {
-        hipLaunchKernelGGL(lincomb_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, nin,
-                           d_in, d_coef, nout, d_out);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            set_error("linear combination failed");
-            rc = DAGE_ERR_HIP;
-        }
-    }
-    (void)hipFree(d_in);
-    (void)hipFree(d_out);
-    (void)hipFree(d_coef);
-    return rc;
+    const size_t bi = sizeof(void *) * nin, bo = sizeof(void *) * nout,
+                 bc = sizeof(double) * nin * nout;
+    void *scr = nullptr;
+    DAGE_TRY(ctx_scratch(ctx, carve_size({bi, bo, bc}), &scr));
+    Carver cv{static_cast<char *>(scr)};
+    const double **d_in = cv.take<const double *>(nin);
+    double **d_out = cv.take<double *>(nout);
+    double *d_coef = cv.take<double>((size_t)nin * nout);
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_in, in, bi, hipMemcpyHostToDevice, ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_out, out, bo, hipMemcpyHostToDevice, ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_coef, coef, bc, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_of(n)), dim3(256), 0, ctx->stream, n, nin, d_in,
+                       d_coef, nout, d_out);
+    DAGE_HIP_CHECK(hipGetLastError());
+    // the host arrays are the caller's: finish before returning
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return DAGE_OK;
 }
 
 int dage_cube_gradient(dage_grid3d grid, int axis, const double *f, double *df) {
@@ -312,43 +393,17 @@ int dage_cube_gradient(dage_grid3d grid, int axis, const double *f, double *df) 
     DAGE_REQUIRE(f != df, DAGE_ERR_ARG, "gradient output aliases its input");
     dage_ctx ctx = grid->ctx;
     DAGE_TRY(use_device(ctx));
-    const Axis &ax = grid->axis[axis];
-    const int nlip = ax.nlip;
-    // node positions in units of the cell step (0..nlip-1 for equidistant cells)
-    std::vector<double> node(nlip), wb(nlip), D((size_t)nlip * nlip, 0.0);
-    for (int k = 0; k < nlip; ++k)
-        node[k] = (ax.points[k] - ax.points[0]) / ax.steps[0];
-    for (int k = 0; k < nlip; ++k) {
-        double p = 1.0;
-        for (int m = 0; m < nlip; ++m)
-            if (m != k) p *= node[k] - node[m];
-        wb[k] = 1.0 / p;
-    }
-    for (int j = 0; j < nlip; ++j) {
-        double diag = 0.0;
-        for (int k = 0; k < nlip; ++k)
-            if (k != j) {
-                D[j * nlip + k] = (wb[k] / wb[j]) / (node[j] - node[k]);
-                diag -= D[j * nlip + k];
-            }
-        D[j * nlip + j] = diag;
-    }
-    // cell step = (last - first)/(nlip-1) must equal the node spacing above
-    // for every cell: the node template is shared (grid_type 1 equidistant)
-    DAGE_REQUIRE(ax.grid_type == 1, DAGE_ERR_UNSUPPORTED, "gradient needs equidistant cell nodes");
-    double *d_D = nullptr;
-    DAGE_TRY(to_device(D.data(), D.size(), &d_D));
+    // the node template is shared by every cell: equidistant nodes only
+    DAGE_REQUIRE(grid->axis[axis].grid_type == 1, DAGE_ERR_UNSUPPORTED,
+                 "gradient needs equidistant cell nodes");
+    const double *d_D = nullptr;
+    DAGE_TRY(grid_table(grid, axis, false, &d_D));
     const long total = (long)grid->shape(0) * grid->shape(1) * grid->shape(2);
     hipLaunchKernelGGL(gradient_kernel, dim3(grid_of(total)), dim3(256), 0, ctx->stream, f, df,
                        grid->shape(0), grid->shape(1), grid->shape(2), axis,
-                       grid->d_points[axis], nlip, d_D);
-    int rc = DAGE_OK;
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        set_error("gradient failed");
-        rc = DAGE_ERR_HIP;
-    }
-    (void)hipFree(d_D);
-    return rc;
+                       grid->d_points[axis], grid->axis[axis].nlip, d_D);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
 }
 
 int dage_cube_stiffness(dage_grid3d grid, int axis, const double *f, double *out) {
@@ -358,50 +413,14 @@ int dage_cube_stiffness(dage_grid3d grid, int axis, const double *f, double *out
     DAGE_TRY(use_device(ctx));
     const Axis &ax = grid->axis[axis];
     DAGE_REQUIRE(ax.grid_type == 1, DAGE_ERR_UNSUPPORTED, "stiffness needs equidistant cell nodes");
-    const int nlip = ax.nlip;
-    // derivative matrix at the nodes 0..nlip-1 (barycentric form) and the
-    // nlip-point Gauss-Legendre rule on [0, nlip-1], exact for L_a' L_b'
-    std::vector<double> wb(nlip), Kc((size_t)nlip * nlip, 0.0);
-    for (int k = 0; k < nlip; ++k) {
-        double p = 1.0;
-        for (int m = 0; m < nlip; ++m)
-            if (m != k) p *= (double)(k - m);
-        wb[k] = 1.0 / p;
-    }
-    std::vector<double> gx(nlip), gw(nlip);
-    gauleg(nlip, gx.data(), gw.data());  // on [-1, 1]
-    const double half = 0.5 * (nlip - 1);
-    for (int q = 0; q < nlip; ++q) {
-        const double xi = half * (gx[q] + 1.0), wq = half * gw[q];
-        // L_a'(xi) = L_a(xi) * sum_{m != a} 1/(xi - m), evaluated by products
-        std::vector<double> dL(nlip, 0.0);
-        for (int a = 0; a < nlip; ++a) {
-            double s = 0.0;
-            for (int m = 0; m < nlip; ++m) {
-                if (m == a) continue;
-                double p = wb[a];
-                for (int mm = 0; mm < nlip; ++mm)
-                    if (mm != a && mm != m) p *= xi - mm;
-                s += p;
-            }
-            dL[a] = s;
-        }
-        for (int a = 0; a < nlip; ++a)
-            for (int b = 0; b < nlip; ++b) Kc[a * nlip + b] += wq * dL[a] * dL[b];
-    }
-    double *d_K = nullptr;
-    DAGE_TRY(to_device(Kc.data(), Kc.size(), &d_K));
+    const double *d_K = nullptr;
+    DAGE_TRY(grid_table(grid, axis, true, &d_K));
     const long total = (long)grid->shape(0) * grid->shape(1) * grid->shape(2);
     hipLaunchKernelGGL(stiffness_kernel, dim3(grid_of(total)), dim3(256), 0, ctx->stream, f, out,
                        grid->shape(0), grid->shape(1), grid->shape(2), axis,
-                       grid->d_points[axis], grid->d_weights[axis], nlip, d_K);
-    int rc = DAGE_OK;
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        set_error("stiffness apply failed");
-        rc = DAGE_ERR_HIP;
-    }
-    (void)hipFree(d_K);
-    return rc;
+                       grid->d_points[axis], grid->d_weights[axis], ax.nlip, d_K);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
 }
 
 int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
@@ -413,37 +432,29 @@ int dage_cube_gram(dage_grid3d grid, int nf, const double *const *f, int ng,
     const int npair = nf * ng;
     const int ntile = ((nf + kGT - 1) / kGT) * ((ng + kGT - 1) / kGT);
     DAGE_REQUIRE(ntile <= 65535, DAGE_ERR_ARG, "too many pairs (%d)", npair);
-    const double **d_f = nullptr, **d_g = nullptr;
-    double *d_part = nullptr;
-    int rc = to_device(f, nf, &d_f);
-    if (rc == DAGE_OK) rc = to_device(g, ng, &d_g);
-    if (rc == DAGE_OK && hipMalloc((void **)&d_part, 8 * (size_t)npair * nz) != hipSuccess) {
-        set_error("device allocation failed");
-        rc = DAGE_ERR_HIP;
-    }
+    const size_t bf = sizeof(void *) * nf, bg = sizeof(void *) * ng,
+                 bp = sizeof(double) * (size_t)npair * nz;
+    void *scr = nullptr;
+    DAGE_TRY(ctx_scratch(ctx, carve_size({bf, bg, bp}), &scr));
+    Carver cv{static_cast<char *>(scr)};
+    const double **d_f = cv.take<const double *>(nf);
+    const double **d_g = cv.take<const double *>(ng);
+    double *d_part = cv.take<double>((size_t)npair * nz);
     std::vector<double> part((size_t)npair * nz);
-    if (rc == DAGE_OK) {
-        hipLaunchKernelGGL(gram_kernel, dim3(nz, ntile), dim3(256), 0, ctx->stream, d_f, d_g, nf,
-                           ng, grid->d_weights[0], grid->d_weights[1], grid->d_weights[2], nx, ny,
-                           nz, d_part);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(part.data(), d_part, 8 * part.size(), hipMemcpyDeviceToHost,
-                           ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            set_error("Gram matrix failed");
-            rc = DAGE_ERR_HIP;
-        }
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_f, f, bf, hipMemcpyHostToDevice, ctx->stream));
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_g, g, bg, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(gram_kernel, dim3(nz, ntile), dim3(256), 0, ctx->stream, d_f, d_g, nf, ng,
+                       grid->d_weights[0], grid->d_weights[1], grid->d_weights[2], nx, ny, nz,
+                       d_part);
+    DAGE_HIP_CHECK(hipGetLastError());
+    DAGE_HIP_CHECK(hipMemcpyAsync(part.data(), d_part, bp, hipMemcpyDeviceToHost, ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < npair; ++p) {
+        double sum = 0.0;
+        for (int z = 0; z < nz; ++z) sum += part[(size_t)p * nz + z];
+        S[p] = sum;
     }
-    if (rc == DAGE_OK)
-        for (int p = 0; p < npair; ++p) {
-            double s = 0.0;
-            for (int z = 0; z < nz; ++z) s += part[(size_t)p * nz + z];
-            S[p] = s;
-        }
-    (void)hipFree(d_f);
-    (void)hipFree(d_g);
-    (void)hipFree(d_part);
-    return rc;
+    return DAGE_OK;
 }
 
 }  // extern "C"

@@ -93,6 +93,10 @@ struct dage_ctx_s {
     std::vector<Pending> pending;
     std::map<std::string, dage::ProfileEntry> profile;
     hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+    // grow-only device scratch for per-call temporaries (dage::ctx_scratch)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipStream_t scratch_stream = nullptr;
 };
 
 struct dage_cube_s {
@@ -107,6 +111,10 @@ struct dage_grid3d_s {
     dage::Axis axis[3];
     double *d_weights[3] = {nullptr, nullptr, nullptr};  // LIP weights (device)
     double *d_points[3] = {nullptr, nullptr, nullptr};   // grid points (device)
+    // per-axis cell tables built on first use: LIP derivative matrix at the
+    // nodes and exact stiffness matrix (nlip x nlip each)
+    double *d_deriv[3] = {nullptr, nullptr, nullptr};
+    double *d_stiff[3] = {nullptr, nullptr, nullptr};
     int shape(int a) const { return axis[a].n(); }
 };
 
@@ -178,6 +186,11 @@ namespace dage {
 // Context reference counting (grids and operators keep their context alive).
 void ctx_retain(dage_ctx ctx);
 void ctx_release(dage_ctx ctx);
+// Device scratch of at least `bytes` for temporaries of one library call.
+// Uses are ordered by the context stream (copies into it must be enqueued
+// on ctx->stream); growing it, or a change of the context stream since the
+// last use, first waits for the previous users.
+int ctx_scratch(dage_ctx ctx, size_t bytes, void **p);
 
 // Event-bracketed launch helpers (record only when profiling is on).
 void prof_begin(dage_ctx ctx, const char *name, hipEvent_t *start);

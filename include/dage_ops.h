@@ -316,13 +316,14 @@ int dage_vec_lincomb(dage_ctx ctx, long n, int nin, const double *const *in,
                      const double *coef, int nout, double *const *out);
 /* df = d f / d(axis) on the grid's points (axis 0..2) from the cells' LIP
  * interpolants (mean of both cells at shared cell ends); equidistant cell
- * nodes (grid_type 1); synchronises. */
+ * nodes (grid_type 1); asynchronous on the context stream. */
 int dage_cube_gradient(dage_grid3d grid, int axis, const double *f,
                        double *df);
 /* out = (K f)/w along axis, K the assembled stiffness matrix of the cells'
  * Lagrange interpolants (int L_a' L_b', exact) and w the LIP weights, so that
  * the Gram of g with out is sum_lines w_perp int g' f' along that axis (the
- * interpolants' exact kinetic integral); synchronises. */
+ * interpolants' exact kinetic integral); asynchronous on the context
+ * stream. */
 int dage_cube_stiffness(dage_grid3d grid, int axis, const double *f,
                         double *out);
 /* S[i + nf*j] = sum_p wx wy wz f_i(p) g_j(p) with the grid's LIP weights
@@ -342,7 +343,7 @@ int dage_conv3d_forward(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
 int dage_conv3d_backward_data(dage_ctx ctx, int nx, int ny, int nz, int cin,
                               int cout, const float *grad_out,
                               const unsigned short *w_bf16, float *grad_in);
-/* Deterministic filter and bias gradients; synchronises. */
+/* Deterministic filter and bias gradients (fixed-order two-stage sums). */
 int dage_conv3d_backward_filter(dage_ctx ctx, int nx, int ny, int nz, int cin,
                                 int cout, const float *in, const float *grad_out,
                                 float *grad_w, float *grad_b);
