@@ -39,3 +39,36 @@ def test_fortran_drop_in_reproduces_reference_golden():
     e = float(re.search(r"selfint=\s*(\S+)", text).group(1))
     ref = 0.7979283927253593  # tests/gauss_test.F90:89-96 of the reference
     assert abs(e - ref) < 1e-13 * ref, (e, ref)
+
+
+@needs_binary
+@pytest.mark.gpu
+def test_fortran_gbfmm_drop_in(ctx):
+    """GBFMMCoulomb3D from Fortran (dage_gbfmm_create / _apply / _box_count):
+    the alpha = 1 Gaussian's self-interaction with the FMM potential on a 49^3
+    grid, 4x4x4 boxes, lmax 6 -- equal to the same call through the Python
+    mirror, and within the quadrature error of the analytic sqrt(2/pi)."""
+    import math
+
+    import numpy as np
+
+    import dage_amd
+    text = _run("gbfmm")
+    m = re.search(r"gbfmm_selfint=\s*(\S+)\s+boxes=(\d+)", text)
+    e_f, nbox = float(m.group(1)), int(m.group(2))
+    assert nbox == 64
+    ax = O.Axis(-6.0, np.full(8, 0.25), 7)
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    gq = dage_amd.GaussQuad(nlin=10, nlog=8, tlin=2.0, tlog=500.0)
+    X, Y, Z = np.meshgrid(ax.points, ax.points, ax.points, indexing="ij")
+    rho = np.asfortranarray(math.pi ** -1.5 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2)))
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=6, gauss=gq)
+    pot = fmm.apply(rho)
+    w = ax.ints()
+    e_py = float(np.einsum("i,j,k,ijk->", w, w, w, rho * pot))
+    print(f"Fortran GBFMM self-interaction {e_f:.15f}, Python {e_py:.15f}, "
+          f"analytic {math.sqrt(2 / math.pi):.15f}")
+    assert abs(e_f - e_py) < 1e-13 * abs(e_py)
+    assert abs(e_f - math.sqrt(2 / math.pi)) < 1e-3
+    fmm.close()
+    g.close()
