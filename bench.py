@@ -174,8 +174,9 @@ def bench_config2(ctx, args):
     ms = ctx.time_end() / args.steps
     wall = time.perf_counter() - wall0
     names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
-             "bubbles_inject_kernel", "box_multipoles_kernel", "m2m_kernel", "m2l_kernel",
-             "l2l_kernel", "local_expansion_kernel")
+             "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
+             "m2m_kernel", "m2l_kernel", "l2l_kernel", "local_expansion_kernel",
+             "local_expansion_gemm")
     kernels = {}
     for k in names:
         kms, kn = ctx.profile_query(k)

@@ -619,6 +619,33 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     return DAGE_OK;
 }
 
+int gemm_colmajor(dage_ctx ctx, const char *name, int M, int N, int K, const double *A, long lda,
+                  const double *B, long ldb, double *C, int splits, double *part) {
+    Gemm g{};
+    g.A = A;
+    g.B = B;
+    g.M = M; g.N = N; g.K = K;
+    g.M0 = M; g.sAm0 = 1; g.sAm1 = 0; g.sAk = lda;
+    g.sBk = 1; g.sBn = ldb;
+    g.sCm0 = 1; g.sCm1 = 0; g.sCn = M;
+    if (splits <= 1) {
+        g.C = C;
+        return launch_gemm<3>(ctx, name, g, 1);
+    }
+    const int ks = ((K + splits - 1) / splits + kBK - 1) / kBK * kBK;  // whole K panels
+    const int nsplit = (K + ks - 1) / ks;
+    g.C = part;
+    g.bC = (long)M * N;
+    g.Ksplit = ks;
+    DAGE_TRY(launch_gemm<3>(ctx, name, g, nsplit));
+    const long len = (long)M * N;
+    const int grid = (int)std::min<long>((len + 255) / 256, 65536);
+    hipLaunchKernelGGL(splitk_reduce_batch_kernel, dim3(grid), dim3(256), 0, ctx->stream, part,
+                       nsplit, len, 1, (const double *)nullptr, 0.0, C);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
 int apply_f64(dage_operator op, const double *in, double *out) {
     BatchSpec bs;
     const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);

@@ -266,6 +266,11 @@ int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);
 // Relayout Fz (nzi, nzo, T) -> Gz ((zi + nzi*t), zo).
 int relayout_fz(dage_operator op);
+// C = A B with the FP64 MFMA GEMM, column-major: A(m,k) = A[m + k*lda],
+// B(k,n) = B[k + n*ldb], C(m,n) = C[m + n*M].  splits > 1 splits K into
+// that many partial products (part: splits*M*N doubles) summed in order.
+int gemm_colmajor(dage_ctx ctx, const char *name, int M, int N, int K, const double *A, long lda,
+                  const double *B, long ldb, double *C, int splits, double *part);
 // Row masks of the operator's matrices and the column-block counts they make
 // cheapest; run after every change of the matrices (synchronises).
 int build_masks(dage_operator op);

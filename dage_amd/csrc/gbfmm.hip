@@ -68,6 +68,18 @@ struct dage_gbfmm_s {
     std::vector<int *> d_list;        // per level: CSR start (nbox+1), src, mat
     std::vector<int> list_len;
     std::vector<double *> d_W;        // per child level: 8 translation matrices
+    // Uniform boxes (equal point counts, relative positions and sub-grid
+    // weights): multipoles and local expansions as FP64 MFMA GEMMs with one
+    // shared matrix of solid harmonics over a box's points.
+    bool uniform = false;
+    int pb[3] = {0, 0, 0};            // points per box and axis
+    long P = 0;                       // points per box
+    double *d_S = nullptr;            // [lm][p]  S_lm(point p - centre)
+    double *d_Sw = nullptr;           // [p][lm]  S_lm * wx wy wz
+    double *d_R = nullptr;            // [box][p] density at the box's points
+    double *d_Pbox = nullptr;         // [box][p] far-field potential at the box's points
+    double *d_gpart = nullptr;        // split-K partials of the multipole GEMM
+    int msplits = 1;
 };
 
 namespace dage {
@@ -470,10 +482,70 @@ int upload(void **dst, const void *src, size_t bytes) {
     return DAGE_OK;
 }
 
+// S[lm*P + p] = S_lm(rel(p)), Sw[p*N + lm] = S_lm(rel(p)) w(p) for the points
+// p = i + pb0 (j + pb1 k) of a box (relative positions and weights per axis).
+__global__ void __launch_bounds__(256) box_harmonics_kernel(const double *rel, const double *w,
+                                                            int pb0, int pb1, int pb2, int ld,
+                                                            int lmax, const double *coef,
+                                                            double *S, double *Sw) {
+    const long P = (long)pb0 * pb1 * pb2;
+    const int N = (lmax + 1) * (lmax + 1);
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P;
+         p += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(p % pb0), j = (int)((p / pb0) % pb1), k = (int)(p / ((long)pb0 * pb1));
+        const double wp = w[i] * w[ld + j] * w[2 * ld + k];
+        solid_harmonics_by_column(lmax, rel[i], rel[ld + j], rel[2 * ld + k], coef,
+                                  [&](int lm, double v) {
+                                      S[(long)lm * P + p] = v;
+                                      Sw[p * N + lm] = v * wp;
+                                  });
+    }
+}
+
+// R[b*P + p] = rho at point p of box b (gather into the GEMM operand)
+__global__ void __launch_bounds__(256) box_gather_kernel(const double *rho, long ldx, long ldy,
+                                                         const int *boxvec, const int *geo, int nb,
+                                                         int pb0, int pb1, int pb2, int nbox,
+                                                         double *R) {
+    const long P = (long)pb0 * pb1 * pb2, total = P * nbox;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(e / P);
+        const long p = e - (long)b * P;
+        const int i = (int)(p % pb0), j = (int)((p / pb0) % pb1), k = (int)(p / ((long)pb0 * pb1));
+        const int x = geo[boxvec[3 * b]] + i;
+        const int y = geo[3 * nb + boxvec[3 * b + 1]] + j;
+        const int z = geo[6 * nb + boxvec[3 * b + 2]] + k;
+        R[e] = rho[x + ldx * (y + ldy * (long)z)];
+    }
+}
+
+// pot += Pbox over each box's own points (the cut rule: box_cut per axis)
+__global__ void __launch_bounds__(256) box_scatter_kernel(const double *Pbox, long ldx, long ldy,
+                                                          const int *boxvec, const int *geo,
+                                                          int nb, int pb0, int pb1, int pb2,
+                                                          int nbox, double *pot) {
+    const long P = (long)pb0 * pb1 * pb2, total = P * nbox;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(e / P);
+        const long p = e - (long)b * P;
+        const int i = (int)(p % pb0), j = (int)((p / pb0) % pb1), k = (int)(p / ((long)pb0 * pb1));
+        const int bx = boxvec[3 * b], by = boxvec[3 * b + 1], bz = boxvec[3 * b + 2];
+        if (i >= geo[2 * nb + bx] || j >= geo[3 * nb + 2 * nb + by] ||
+            k >= geo[6 * nb + 2 * nb + bz])
+            continue;
+        const int x = geo[bx] + i, y = geo[3 * nb + by] + j, z = geo[6 * nb + bz] + k;
+        double *dst = pot + x + ldx * (y + ldy * (long)z);
+        *dst = *dst + Pbox[e];
+    }
+}
+
 void free_gbfmm(dage_gbfmm f) {
     if (f->nf_op) dage_operator_destroy(f->nf_op);
-    void *bufs[] = {f->d_q, f->d_L, f->d_part, f->d_m2l_part, f->d_boxvec, f->d_geo,
-                    f->d_wbox, f->d_points, f->d_centers, f->d_coef};
+    void *bufs[] = {f->d_q,      f->d_L,      f->d_part,   f->d_m2l_part, f->d_boxvec,
+                    f->d_geo,    f->d_wbox,   f->d_points, f->d_centers,  f->d_coef,
+                    f->d_S,      f->d_Sw,     f->d_R,      f->d_Pbox,     f->d_gpart};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (double *p : f->d_T)
@@ -645,6 +717,75 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_part, 8 * (size_t)N * nbox_max * f->zsplit));
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_part, 8 * (size_t)N * nbox_max * f->m2l_chunks));
 
+    // ---- uniform boxes: the shared solid-harmonic matrix of the GEMM route
+    {
+        bool uni = true;
+        std::vector<double> rel((size_t)3 * maxn, 0.0), w0((size_t)3 * maxn, 0.0);
+        for (int a = 0; a < 3 && uni; ++a) {
+            const int n0 = f->box_n[a][0];
+            const double ext = f->qmax[a] - f->qmin[a];
+            // centre of box coordinate b along axis a (boxes of one slab share it)
+            auto centre = [&](int b) {
+                int v[3] = {0, 0, 0};
+                v[a] = b;
+                return centers[3 * box_index(v[0], v[1], v[2], L) + a];
+            };
+            for (int i = 0; i < n0; ++i) {
+                rel[(size_t)a * maxn + i] = grid->axis[a].points[f->box_lo[a][0] + i] - centre(0);
+                w0[(size_t)a * maxn + i] = wbox[(size_t)(a * nb) * maxn + i];
+            }
+            for (int b = 1; b < nb && uni; ++b) {
+                if (f->box_n[a][b] != n0) uni = false;
+                for (int i = 0; i < n0 && uni; ++i) {
+                    const double r = grid->axis[a].points[f->box_lo[a][b] + i] - centre(b);
+                    const double wv = wbox[(size_t)(a * nb + b) * maxn + i];
+                    if (std::fabs(r - rel[(size_t)a * maxn + i]) > 1e-12 * ext ||
+                        std::fabs(wv - w0[(size_t)a * maxn + i]) >
+                            1e-13 * std::fabs(w0[(size_t)a * maxn + i]))
+                        uni = false;
+                }
+            }
+            f->pb[a] = n0;
+        }
+        f->uniform = uni;
+        if (uni) {
+            f->P = (long)f->pb[0] * f->pb[1] * f->pb[2];
+            double *d_rel = nullptr, *d_w = nullptr;
+            DAGE_TRY(upload((void **)&d_rel, rel.data(), 8 * rel.size()));
+            int rc = upload((void **)&d_w, w0.data(), 8 * w0.size());
+            const size_t ps = (size_t)f->P;
+            if (rc == DAGE_OK && (hipMalloc((void **)&f->d_S, 8 * ps * N) != hipSuccess ||
+                                  hipMalloc((void **)&f->d_Sw, 8 * ps * N) != hipSuccess ||
+                                  hipMalloc((void **)&f->d_R, 8 * ps * nbox_max) != hipSuccess ||
+                                  hipMalloc((void **)&f->d_Pbox, 8 * ps * nbox_max) != hipSuccess)) {
+                set_error("GBFMM box-harmonics allocation failed");
+                rc = DAGE_ERR_HIP;
+            }
+            // multipole GEMM: K = P split so that ~1024 workgroups run
+            const int mt = (N + 63) / 64, nt = (nbox_max + 63) / 64;
+            f->msplits = (int)std::max<long>(1, std::min<long>(f->P / 64, 1024 / (mt * nt)));
+            if (rc == DAGE_OK && hipMalloc((void **)&f->d_gpart,
+                                           8 * (size_t)f->msplits * N * nbox_max) != hipSuccess) {
+                set_error("GBFMM partials allocation failed");
+                rc = DAGE_ERR_HIP;
+            }
+            if (rc == DAGE_OK) {
+                const int grid_sz = (int)std::min<long>((f->P + 255) / 256, 65536);
+                hipLaunchKernelGGL(box_harmonics_kernel, dim3(grid_sz), dim3(256), 0, f->ctx->stream,
+                                   d_rel, d_w, f->pb[0], f->pb[1], f->pb[2], maxn, lmax,
+                                   f->d_coef, f->d_S, f->d_Sw);
+                if (hipGetLastError() != hipSuccess ||
+                    hipStreamSynchronize(f->ctx->stream) != hipSuccess) {
+                    set_error("box harmonics failed");
+                    rc = DAGE_ERR_HIP;
+                }
+            }
+            if (d_rel) (void)hipFree(d_rel);
+            if (d_w) (void)hipFree(d_w);
+            DAGE_TRY(rc);
+        }
+    }
+
     // ---- far-field matrices (init_interaction_matrices, get_local_farfield_indices)
     const FmmTables &tab = fmm_tables(lmax);
     f->d_T.assign(L + 1, nullptr);
@@ -733,13 +874,28 @@ int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
     ma.centers = f->d_centers; ma.coef = f->d_coef;
     ma.zsplit = f->zsplit; ma.nbox = nbox_max;
     ma.part = f->d_part;
-    prof_begin(ctx, "box_multipoles_kernel", &ev);
-    DAGE_TRY(multipoles(f->lmax, ma, nbox_max, s));
-    prof_end(ctx, "box_multipoles_kernel", ev);
-    const long qlen = (long)N * nbox_max;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((int)std::min<long>((qlen + 255) / 256, 65536)),
-                       dim3(256), 0, s, f->d_part, f->zsplit, qlen, f->d_q);
-    DAGE_HIP_CHECK(hipGetLastError());
+    if (f->uniform) {
+        // q(:, box) = Sw^T-layout GEMM over the box's points: q = (S w) R
+        const long tot = f->P * nbox_max;
+        const int gs = (int)std::min<long>((tot + 255) / 256, 65536);
+        hipEvent_t ev2;
+        prof_begin(ctx, "box_multipoles_gemm", &ev2);
+        hipLaunchKernelGGL(box_gather_kernel, dim3(gs), dim3(256), 0, s, rho, ldx, ldy,
+                           f->d_boxvec, f->d_geo, nb, f->pb[0], f->pb[1], f->pb[2], nbox_max,
+                           f->d_R);
+        DAGE_HIP_CHECK(hipGetLastError());
+        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_multipoles", N, nbox_max, (int)f->P, f->d_Sw, N,
+                               f->d_R, f->P, f->d_q, f->msplits, f->d_gpart));
+        prof_end(ctx, "box_multipoles_gemm", ev2);
+    } else {
+        prof_begin(ctx, "box_multipoles_kernel", &ev);
+        DAGE_TRY(multipoles(f->lmax, ma, nbox_max, s));
+        prof_end(ctx, "box_multipoles_kernel", ev);
+        const long qlen = (long)N * nbox_max;
+        hipLaunchKernelGGL(sum_parts_kernel, dim3((int)std::min<long>((qlen + 255) / 256, 65536)),
+                           dim3(256), 0, s, f->d_part, f->zsplit, qlen, f->d_q);
+        DAGE_HIP_CHECK(hipGetLastError());
+    }
     // upward pass
     for (int l = L - 1; l >= f->start_level; --l) {
         prof_begin(ctx, "m2m_kernel", &ev);
@@ -780,6 +936,20 @@ int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
     la.centers = f->d_centers; la.coef = f->d_coef;
     la.L = f->d_L;  // level maxlevel has offset 0
     la.lmax = f->lmax; la.N = N;
+    if (f->uniform) {
+        // potential at the box's points = S^T L, added over the points each box owns
+        prof_begin(ctx, "local_expansion_gemm", &ev);
+        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_local_expansion", (int)f->P, nbox_max, N, f->d_S,
+                               f->P, f->d_L, N, f->d_Pbox, 1, nullptr));
+        const long tot = f->P * nbox_max;
+        const int gs = (int)std::min<long>((tot + 255) / 256, 65536);
+        hipLaunchKernelGGL(box_scatter_kernel, dim3(gs), dim3(256), 0, s, f->d_Pbox, ldx, ldy,
+                           f->d_boxvec, f->d_geo, nb, f->pb[0], f->pb[1], f->pb[2], nbox_max,
+                           pot);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "local_expansion_gemm", ev);
+        return DAGE_OK;
+    }
     int maxz = 0;
     for (int b = 0; b < nb; ++b) maxz = std::max(maxz, f->box_cut[2][b]);
     prof_begin(ctx, "local_expansion_kernel", &ev);

@@ -44,6 +44,26 @@ def test_gbfmm_matches_oracle(ctx, maxlevel, nlip, ncell, lmax):
     assert colrel(fmm.local_expansions(), Lref) < 1e-11
 
 
+def test_gbfmm_unequal_cells_vs_oracle(ctx):
+    """Cells of unequal size: boxes differ in their points' positions, so the
+    multipoles and local expansions run box by box (the equal-box route puts
+    them on one shared harmonics matrix); both routes against the oracle."""
+    rng = np.random.default_rng(41)
+    nlip, ncell = 5, 8
+    ax = O.Axis(-4.0, rng.uniform(0.8, 1.2, ncell) * 8.0 / (ncell * (nlip - 1)), nlip)
+    q_, a, c = random_gaussians(rng, 3, box=2.0)
+    rho = O.gauss_density(q_, a, c, ax.points, ax.points, ax.points)
+    gq = dage_amd.GaussQuad(6, 6)
+    g = grid_of(ctx, [ax] * 3)
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=6, gauss=gq)
+    pot = fmm.apply(rho)
+    ref, qref, Lref = O.gbfmm_potential([ax] * 3, rho, 2, 6, gq.tp, gq.tw)
+    assert rel_maxdiff(pot, ref) < 1e-10
+    assert colrel(fmm.multipole_moments(), qref) < 1e-11
+    assert colrel(fmm.local_expansions(), Lref) < 1e-11
+    fmm.close()
+
+
 def test_gbfmm_uneven_axes(ctx):
     axes = [O.Axis.ncell_fixed(-4.0, 4.0, 5, 8), O.Axis.ncell_fixed(-3.0, 6.0, 5, 12),
             O.Axis(-2.5, np.array([0.3, 0.25, 0.35, 0.3]), 5)]
