@@ -175,7 +175,7 @@ def bench_config2(ctx, args):
     wall = time.perf_counter() - wall0
     names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
-             "m2m_kernel", "m2l_kernel", "l2l_kernel", "local_expansion_kernel",
+             "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm")
     kernels = {}
     for k in names:

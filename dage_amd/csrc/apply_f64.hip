@@ -646,6 +646,21 @@ int gemm_colmajor(dage_ctx ctx, const char *name, int M, int N, int K, const dou
     return DAGE_OK;
 }
 
+int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, const double *A,
+                          long lda, long strideA, const double *B, long ldb, long strideB,
+                          double *C, long strideC, int batch) {
+    Gemm g{};
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.M = M; g.N = N; g.K = K;
+    g.M0 = M; g.sAm0 = 1; g.sAm1 = 0; g.sAk = lda;
+    g.sBk = 1; g.sBn = ldb;
+    g.sCm0 = 1; g.sCm1 = 0; g.sCn = M;
+    g.bA = strideA; g.bB = strideB; g.bC = strideC;
+    return launch_gemm<3>(ctx, name, g, batch);
+}
+
 int apply_f64(dage_operator op, const double *in, double *out) {
     BatchSpec bs;
     const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);

@@ -271,6 +271,11 @@ int relayout_fz(dage_operator op);
 // that many partial products (part: splits*M*N doubles) summed in order.
 int gemm_colmajor(dage_ctx ctx, const char *name, int M, int N, int K, const double *A, long lda,
                   const double *B, long ldb, double *C, int splits, double *part);
+// batch independent products C_z = A_z B_z (z < batch), same layouts, with
+// A_z = A + z*strideA etc.
+int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, const double *A,
+                          long lda, long strideA, const double *B, long ldb, long strideB,
+                          double *C, long strideC, int batch);
 // Row masks of the operator's matrices and the column-block counts they make
 // cheapest; run after every change of the matrices (synchronises).
 int build_masks(dage_operator op);

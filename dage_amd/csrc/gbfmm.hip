@@ -53,8 +53,7 @@ struct dage_gbfmm_s {
     dage_operator nf_op = nullptr;    // whole-cube operator with the banded near-field matrices
     std::vector<int> boxvec;          // 3 per maxlevel box, by box index
     // far field
-    double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr, *d_m2l_part = nullptr;
-    int m2l_chunks = 8;
+    double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr;
     int zsplit = 1;
     int *d_boxvec = nullptr;
     int *d_geo = nullptr;             // [axis][box_lo, box_n, box_cut][nb]
@@ -65,9 +64,15 @@ struct dage_gbfmm_s {
     double *d_centers = nullptr;      // 3 per maxlevel box
     double *d_coef = nullptr;         // a[lm], b[lm], f[l] of the solid-harmonic recursion
     std::vector<double *> d_T;        // per level: M2L matrices
-    std::vector<int *> d_list;        // per level: CSR start (nbox+1), src, mat
+    std::vector<int *> d_list;        // per level: CSR start of each target's pairs (nbox+1)
     std::vector<int> list_len;
     std::vector<double *> d_W;        // per child level: 8 translation matrices
+    // M2L as one batched GEMM per level: source columns grouped by box
+    // offset (matrix), slots padded to the largest group
+    std::vector<int> m2l_nmat, m2l_smax;
+    std::vector<int *> d_m2l_gsrc;    // per level: [matrix][slot] source column (-1: none)
+    std::vector<int *> d_m2l_pos;     // per level: per pair (CSR order) its [matrix][slot]
+    double *d_m2l_G = nullptr, *d_m2l_out = nullptr;
     // Uniform boxes (equal point counts, relative positions and sub-grid
     // weights): multipoles and local expansions as FP64 MFMA GEMMs with one
     // shared matrix of solid harmonics over a box's points.
@@ -386,35 +391,28 @@ __global__ void __launch_bounds__(256) m2m_kernel(const double *W, const double 
 }
 
 // L(target) += sum_list T(offset) q(source)  (calculate_local_farfield_potential,
-// gbfmm_coulomb3d.F90:1461-1500).  Block (target, chunk of the list): each
-// chunk's partial goes to part[chunk][target][lm]; m2l_sum_kernel adds the
-// chunks in order.
-__global__ void __launch_bounds__(256) m2l_kernel(const double *T, const int *start,
-                                                  const int *src, const int *mat, const double *q,
-                                                  double *part, int nbox, int N) {
-    const int t = blockIdx.x, c = blockIdx.y, nch = gridDim.y, i = threadIdx.x;
-    if (i >= N) return;
-    const int e0 = start[t], e1 = start[t + 1];
-    const int per = (e1 - e0 + nch - 1) / nch;
-    const int eb = e0 + c * per, ee = min(e1, eb + per);
-    double acc = 0.0;
-    for (int e = eb; e < ee; ++e) {
-        const double *Te = T + (long)mat[e] * N * N;
-        const double *qs = q + (long)src[e] * N;
-        for (int s = 0; s < N; ++s) acc += Te[i + (long)N * s] * qs[s];
+// gbfmm_coulomb3d.F90:1461-1500), as one batched GEMM over the box offsets:
+// gather the sources' multipoles per offset, multiply, add back per target.
+// G[col][lm] = q[gsrc[col]][lm] (0 for empty slots): the M2L GEMM operand
+__global__ void __launch_bounds__(256) m2l_gather_kernel(const double *q, const int *gsrc,
+                                                         long ncol, int N, double *G) {
+    const long total = ncol * N;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long col = e / N;
+        const int src = gsrc[col];
+        G[e] = src >= 0 ? q[(long)src * N + (e - col * N)] : 0.0;
     }
-    part[((long)c * nbox + t) * N + i] = acc;
 }
 
-__global__ void __launch_bounds__(256) m2l_sum_kernel(const double *part, int nch, int nbox,
-                                                      int N, double *L) {
-    const long len = (long)nbox * N;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < len;
-         e += (long)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < nch; ++c) s += part[(long)c * len + e];
-        L[e] += s;
-    }
+// L[t] += the target's GEMM columns, in interaction-list order (deterministic)
+__global__ void __launch_bounds__(256) m2l_scatter_kernel(const double *out, const int *start,
+                                                          const int *pos, int N, double *L) {
+    const int t = blockIdx.x, i = threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int e = start[t]; e < start[t + 1]; ++e) acc += out[(long)pos[e] * N + i];
+    L[(long)t * N + i] += acc;
 }
 
 // L(child) += W_octant^T L(parent)  (translate_potential, gbfmm_coulomb3d.F90:1562-1622).
@@ -543,7 +541,7 @@ __global__ void __launch_bounds__(256) box_scatter_kernel(const double *Pbox, lo
 
 void free_gbfmm(dage_gbfmm f) {
     if (f->nf_op) dage_operator_destroy(f->nf_op);
-    void *bufs[] = {f->d_q,      f->d_L,      f->d_part,   f->d_m2l_part, f->d_boxvec,
+    void *bufs[] = {f->d_q,      f->d_L,      f->d_part,   f->d_boxvec,
                     f->d_geo,    f->d_wbox,   f->d_points, f->d_centers,  f->d_coef,
                     f->d_S,      f->d_Sw,     f->d_R,      f->d_Pbox,     f->d_gpart};
     for (void *p : bufs)
@@ -554,6 +552,12 @@ void free_gbfmm(dage_gbfmm f) {
         if (p) (void)hipFree(p);
     for (double *p : f->d_W)
         if (p) (void)hipFree(p);
+    for (int *p : f->d_m2l_gsrc)
+        if (p) (void)hipFree(p);
+    for (int *p : f->d_m2l_pos)
+        if (p) (void)hipFree(p);
+    if (f->d_m2l_G) (void)hipFree(f->d_m2l_G);
+    if (f->d_m2l_out) (void)hipFree(f->d_m2l_out);
     if (f->ctx) ctx_release(f->ctx);
     delete f;
 }
@@ -715,7 +719,6 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_L, 8 * (size_t)N * f->nbox_total));
     f->zsplit = std::max(1, std::min(maxn, (2048 + nbox_max - 1) / nbox_max));
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_part, 8 * (size_t)N * nbox_max * f->zsplit));
-    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_part, 8 * (size_t)N * nbox_max * f->m2l_chunks));
 
     // ---- uniform boxes: the shared solid-harmonic matrix of the GEMM route
     {
@@ -792,6 +795,11 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     f->d_list.assign(L + 1, nullptr);
     f->list_len.assign(L + 1, 0);
     f->d_W.assign(L + 1, nullptr);
+    f->m2l_nmat.assign(L + 1, 0);
+    f->m2l_smax.assign(L + 1, 0);
+    f->d_m2l_gsrc.assign(L + 1, nullptr);
+    f->d_m2l_pos.assign(L + 1, nullptr);
+    size_t m2l_cols = 1;
     std::vector<double> M((size_t)N * N);
     for (int l = f->start_level; l <= L; ++l) {
         const int n = 1 << l, nbox = n * n * n;
@@ -835,12 +843,26 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
             fmm_interaction(tab, D, &Tall[(size_t)kv.second * N * N]);
         }
         DAGE_TRY(upload((void **)&f->d_T[l], Tall.data(), 8 * Tall.size()));
-        std::vector<int> lst;
-        lst.insert(lst.end(), start.begin(), start.end());
-        lst.insert(lst.end(), src.begin(), src.end());
-        lst.insert(lst.end(), mat.begin(), mat.end());
         f->list_len[l] = (int)src.size();
-        DAGE_TRY(upload((void **)&f->d_list[l], lst.data(), 4 * lst.size()));
+        DAGE_TRY(upload((void **)&f->d_list[l], start.data(), 4 * start.size()));
+        {   // group the pairs by matrix: slot s of matrix o = its s-th pair in list order
+            const int nmat = std::max<int>(1, (int)mat_id.size());
+            std::vector<int> cnt(nmat, 0);
+            for (int m : mat) ++cnt[m];
+            const int smax = std::max(1, *std::max_element(cnt.begin(), cnt.end()));
+            std::vector<int> gsrc((size_t)nmat * smax, -1), pos(std::max<size_t>(1, src.size()));
+            std::vector<int> next(nmat, 0);
+            for (size_t e = 0; e < src.size(); ++e) {
+                const int o = mat[e], slot = next[o]++;
+                gsrc[(size_t)o * smax + slot] = src[e];
+                pos[e] = o * smax + slot;
+            }
+            f->m2l_nmat[l] = nmat;
+            f->m2l_smax[l] = smax;
+            DAGE_TRY(upload((void **)&f->d_m2l_gsrc[l], gsrc.data(), 4 * gsrc.size()));
+            DAGE_TRY(upload((void **)&f->d_m2l_pos[l], pos.data(), 4 * pos.size()));
+            m2l_cols = std::max(m2l_cols, (size_t)nmat * smax);
+        }
         if (l > f->start_level) {  // child level l: W(child - parent) per octant
             std::vector<double> Wall((size_t)8 * N * N);
             for (int o = 0; o < 8; ++o) {
@@ -851,6 +873,8 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
             DAGE_TRY(upload((void **)&f->d_W[l], Wall.data(), 8 * Wall.size()));
         }
     }
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_G, 8 * (size_t)N * m2l_cols));
+    DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_out, 8 * (size_t)N * m2l_cols));
     return DAGE_OK;
 }
 
@@ -916,17 +940,23 @@ int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
             DAGE_HIP_CHECK(hipGetLastError());
             prof_end(ctx, "l2l_kernel", ev);
         }
-        const int *lst = f->d_list[l];
-        prof_begin(ctx, "m2l_kernel", &ev);
-        hipLaunchKernelGGL(m2l_kernel, dim3(nbox, f->m2l_chunks), dim3(256), 0, s, f->d_T[l], lst,
-                           lst + nbox + 1, lst + nbox + 1 + f->list_len[l], f->d_q,
-                           f->d_m2l_part, nbox, N);
+        if (f->list_len[l] == 0) continue;
+        // one GEMM per box offset: T(offset) times the multipoles of every
+        // source at that offset, then each target adds its columns in list order
+        const int nmat = f->m2l_nmat[l], smax = f->m2l_smax[l];
+        const long ncol = (long)nmat * smax;
+        prof_begin(ctx, "m2l_gemm", &ev);
+        hipLaunchKernelGGL(m2l_gather_kernel,
+                           dim3((int)std::min<long>((ncol * N + 255) / 256, 65536)), dim3(256), 0,
+                           s, f->d_q, f->d_m2l_gsrc[l], ncol, N, f->d_m2l_G);
         DAGE_HIP_CHECK(hipGetLastError());
-        const long len = (long)nbox * N;
-        hipLaunchKernelGGL(m2l_sum_kernel, dim3((int)std::min<long>((len + 255) / 256, 65536)),
-                           dim3(256), 0, s, f->d_m2l_part, f->m2l_chunks, nbox, N, Ll);
+        DAGE_TRY(gemm_colmajor_batched(ctx, "gemm_f64_m2l", N, smax, N, f->d_T[l], N,
+                                       (long)N * N, f->d_m2l_G, N, (long)N * smax,
+                                       f->d_m2l_out, (long)N * smax, nmat));
+        hipLaunchKernelGGL(m2l_scatter_kernel, dim3(nbox), dim3(256), 0, s, f->d_m2l_out,
+                           f->d_list[l], f->d_m2l_pos[l], N, Ll);
         DAGE_HIP_CHECK(hipGetLastError());
-        prof_end(ctx, "m2l_kernel", ev);
+        prof_end(ctx, "m2l_gemm", ev);
     }
     // local expansions on the finest boxes' points
     LeArgs la;
