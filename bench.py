@@ -451,6 +451,7 @@ def bench_config4(ctx, args):
     p on the heaviest) rotated to the Ritz vectors of the nuclear Hamiltonian;
     the learned run evaluates the network until three of its proposals in a
     row lose to the plain step, then every 8th iteration until one wins.
+    Both runs autotune their 12-orbital Helmholtz batch graph once first.
 
     129^3 cube over [-9.6, 9.6]^3 (nlip 5, h 0.15), 10 atoms (Z uniform in
     {1,6,7,8}, in a 6-bohr sphere, >= 1.8 bohr apart, nuclei smeared with
@@ -502,7 +503,7 @@ def bench_config4(ctx, args):
     offline_s = time.perf_counter() - t0
     runs = {}
     for mode in ("plain", "learned"):
-        S = scf.HelmholtzSCF(ctx, grid, test_mol, nocc=nocc, beta=4.0)
+        S = scf.HelmholtzSCF(ctx, grid, test_mol, nocc=nocc, beta=4.0, tune=True)
         S.initial_guess(exponents=test_mol.Z.copy())
         upd = None if mode == "plain" else (lambda p_, d_, o_, S=S: model.propose(S, p_, d_, o_))
         r = S.run(max_iter=maxit, update=upd, backoff=(3, 8))

@@ -114,7 +114,9 @@ class HelmholtzSCF:
     def __init__(self, ctx: Context, grid: Grid3D, molecule: Molecule, nocc: int,
                  gauss: GaussQuad | None = None, beta: float = 4.0, exchange: bool = True,
                  update_weight: float = 1.0, external_potential: np.ndarray | None = None,
-                 hartree: bool = True):
+                 hartree: bool = True, tune: bool = False):
+        """tune: run the batch autotuner once (GEMM column blocks per pass) on
+        zeroed inputs before the SCF starts."""
         from scipy.special import erf
         self.ctx, self.grid, self.mol, self.nocc = ctx, grid, molecule, nocc
         self.gauss = gauss if gauss is not None else GaussQuad(20, 20, tlin=2.0, tlog=500.0)
@@ -146,6 +148,11 @@ class HelmholtzSCF:
         self.coul = Coulomb3D(ctx, grid, grid, self.gauss)
         self.batch = HelmholtzBatch(ctx, [grid] * nocc, [-0.5] * nocc, self.gauss,
                                     precision=DAGE_PREC_FP64)
+        if tune:
+            zero = np.zeros(self.shape, order="F")
+            for i in range(nocc):
+                self.batch.upload(i, zero)
+            self.batch.tune(reps=1)
         self.eps = np.zeros(nocc)
         self.kin_diag = np.zeros(nocc)
 

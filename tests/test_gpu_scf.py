@@ -163,3 +163,21 @@ def test_two_centre_scf_converges(ctx):
     assert abs(c1) < 1e-3 and r1 < 1e-3
     assert res2.converged and r2 < 0.1 * r1
     run.close()
+
+
+def test_tuned_batch_gives_the_same_iterations(ctx):
+    # the autotuner only changes GEMM column blocks; every K sum keeps its
+    # order, so ten SCF iterations agree to rounding of the Gram solves
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 5, 12)
+    g = grid_of(ctx, ax)
+    mol = scf.Molecule(np.array([3.0, 1.0]), np.array([[0.0, 0.0, -0.8], [0.0, 0.0, 0.8]]))
+    energies = []
+    for tune in (False, True):
+        run = scf.HelmholtzSCF(ctx, g, mol, nocc=2, beta=4.0, tune=tune)
+        run.initial_guess()
+        res = run.run(max_iter=10)
+        energies.append(np.array(res.energies))
+        run.close()
+    print(f"untuned {energies[0][-1]:.12f} tuned {energies[1][-1]:.12f}")
+    assert len(energies[0]) == len(energies[1])
+    assert np.max(np.abs(energies[0] - energies[1])) < 1e-10
