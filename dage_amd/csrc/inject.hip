@@ -4,9 +4,9 @@
 // Bubbles::inject it dispatches to, src/bubbles/bubbles_cuda.cu:3090-3340).
 //
 // One fused launch for all bubbles and all (l,m): one thread per cube point,
-// x fastest across the 64 lanes of a wavefront so the read-modify-write of the
-// cube is coalesced.  Per bubble the thread finds the radial cell by the
-// reference's bisection (grid.F90:1212-1246), evaluates the nlip cell
+// a wavefront per 4 x 4 x 4 patch of points.  Per bubble the thread finds
+// the radial cell by the reference's bisection (grid.F90:1212-1246),
+// evaluates the nlip cell
 // polynomials once by Horner (lip.F90:551-566), and reuses them for every
 // (l,m); the Racah solid harmonics come from the recursion of
 // harmonics.F90:586-659, fully unrolled for the compile-time lmax.
@@ -81,24 +81,38 @@ __device__ inline int radial_icell(const double *celld, int ncell, double qmax, 
 // (RealSphericalHarmonics_eval, harmonics.F90:1360-1370); here each l's sum
 // over m is scaled once by (1/r)^l, which moves the result by rounding only
 // (1e-16 relative per term).
+//
+// A wavefront covers a 4 x 4 x 4 patch of points (lane l: x = l & 3,
+// y = (l >> 2) & 3, z = l >> 4), a block four patches along x.  The points
+// of one patch almost always fall in one or two radial cells of a bubble,
+// so the wavefront loops over the distinct cells present (ballot, lowest
+// lane first): the cell index is wavefront-uniform inside the loop and the
+// 25 x nlip radial coefficients of the cell are read once per wavefront
+// through scalar loads instead of once per lane.  Each point still sees
+// exactly the arithmetic of its own cell, in the same order.
 template <int LMAX, int NLIP>
 __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
-    const int ix = blockIdx.x * 64 + threadIdx.x;
-    const int iy = blockIdx.y * 4 + threadIdx.y;
-    const int iz = blockIdx.z;
-    if (ix >= A.nx || iy >= A.ny) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = blockIdx.x * 16 + wave * 4 + (lane & 3);
+    const int iy = blockIdx.y * 4 + ((lane >> 2) & 3);
+    const int iz = blockIdx.z * 4 + (lane >> 4);
+    // no early exit: every lane takes part in the wavefront-uniform cell loop
+    const bool inside = ix < A.nx && iy < A.ny && iz < A.nz;
     constexpr int NMAX = NLIP > 0 ? NLIP : kMaxNlip;
     const int nlip = NLIP > 0 ? NLIP : A.nlip;
-    const double px = A.gx[ix], py = A.gy[iy], pz = A.gz[iz];
+    const double px = A.gx[inside ? ix : 0], py = A.gy[inside ? iy : 0],
+                 pz = A.gz[inside ? iz : 0];
     constexpr double kEps = 2.220446049250313e-16;  // epsilon(0.d0)
     double total = 0.0;
     for (int b = 0; b < A.nbub; ++b) {
         const BubbleDev B = A.bubs[b];
         const double dx = px - B.cx, dy = py - B.cy, dz = pz - B.cz;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const int ic = radial_icell(B.celld, B.ncell, B.qmax, r);
-        if (ic == 0) continue;
-        const double crd = (r - B.celld[ic - 1]) / B.steps[ic - 1] + A.first;
+        const int ic = inside ? radial_icell(B.celld, B.ncell, B.qmax, r) : 0;
+        unsigned long long todo = __ballot(ic != 0);
+        if (todo == 0ull) continue;
+        const int icl = ic > 0 ? ic - 1 : 0;
+        const double crd = (r - B.celld[icl]) / B.steps[icl] + A.first;
         double polys[NMAX];
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
@@ -113,34 +127,47 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
         }
         double S[(LMAX + 1) * (LMAX + 1)];
         solid_harmonics<LMAX>(dx, dy, dz, S);
-        const double *f = B.f + (long)(ic - 1) * (nlip - 1);
         const double inv = (r > kEps) ? 1.0 / r : 1.0;  // r <= eps: r^l taken as 1
-        double res = 0.0;
-        double ipow = 1.0;
+        while (todo != 0ull) {
+            const int leader = __builtin_ctzll(todo);
+            const int c = __builtin_amdgcn_readlane(ic, leader);  // uniform cell
+            const bool mine = ic == c;
+            todo &= ~__ballot(mine);
+            if (!mine) continue;
+            // re-read from the first active lane so the address stays scalar
+            // (inside the branch the compiler may otherwise substitute the
+            // per-lane ic for c)
+            const int cs = __builtin_amdgcn_readfirstlane(c);
+            const double *f = B.f + (long)(cs - 1) * (nlip - 1);
+            double res = 0.0;
+            double ipow = 1.0;
 #pragma unroll
-        for (int l = 0; l <= LMAX; ++l) {
-            if (l >= A.lmin) {
-                double sl = 0.0;
+            for (int l = 0; l <= LMAX; ++l) {
+                if (l >= A.lmin) {
+                    double sl = 0.0;
 #pragma unroll
-                for (int m = -l; m <= l; ++m) {
-                    const int lm = l * l + l + m;
-                    const double *fl = f + (long)lm * B.npts;
-                    double fv = 0.0;
+                    for (int m = -l; m <= l; ++m) {
+                        const int lm = l * l + l + m;
+                        const double *fl = f + (long)lm * B.npts;
+                        double fv = 0.0;
 #pragma unroll
-                    for (int j = 0; j < NMAX; ++j) {
-                        if (j >= nlip) break;
-                        fv += fl[j] * polys[j];
+                        for (int j = 0; j < NMAX; ++j) {
+                            if (j >= nlip) break;
+                            fv += fl[j] * polys[j];
+                        }
+                        sl += fv * S[lm];
                     }
-                    sl += fv * S[lm];
+                    res += sl * ipow;
                 }
-                res += sl * ipow;
+                ipow *= inv;
             }
-            ipow *= inv;
+            total += res;
         }
-        total += res;
     }
-    const long id = ix + (long)A.nx * (iy + (long)A.ny * iz);
-    A.cube[id] = A.cube[id] + A.multiplier * total;
+    if (inside) {
+        const long id = ix + (long)A.nx * (iy + (long)A.ny * iz);
+        A.cube[id] = A.cube[id] + A.multiplier * total;
+    }
 }
 
 template <int LMAX>
@@ -197,8 +224,8 @@ int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multipli
     DAGE_REQUIRE(lip_coeffs(bub->nlip, 1, A.lip) == 0, DAGE_ERR_ARG, "unsupported bubble nlip");
     A.multiplier = multiplier;
     A.cube = cube;
-    const dim3 block(64, 4, 1);
-    const dim3 g((A.nx + 63) / 64, (A.ny + 3) / 4, A.nz);
+    const dim3 block(256, 1, 1);  // four 4x4x4 patches along x
+    const dim3 g((A.nx + 15) / 16, (A.ny + 3) / 4, (A.nz + 3) / 4);
     hipEvent_t ev;
     prof_begin(ctx, "bubbles_inject_kernel", &ev);
     switch (bub->lmax) {
