@@ -68,6 +68,8 @@ _SIGNATURES = {
     "dage_operator_apply": [_c_p, _dp, _dp, _int],
     "dage_operator_apply_device": [_c_p, _c_p, _c_p, _int],
     "dage_operator_workspace_bytes": [_c_p, ctypes.POINTER(ctypes.c_size_t)],
+    "dage_operator_get_tiling": [_c_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int)],
     "dage_operator_destroy": [_c_p],
     "dage_cube_integrate_device": [_c_p, _c_p, _c_p, _dp],
     "dage_bubbles_create": [_c_p, _int, _int, _int, _int, ctypes.POINTER(_c_p)],

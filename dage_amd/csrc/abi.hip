@@ -137,6 +137,8 @@ void free_operator_buffers(dage_operator op) {
     op->d_planes = nullptr;
     if (op->d_mask) (void)hipFree(op->d_mask);
     op->d_mask = nullptr;
+    if (op->d_tmask) (void)hipFree(op->d_tmask);
+    op->d_tmask = nullptr;
     op->planes_valid = false;
     op->ws_bytes = 0;
 }
@@ -622,6 +624,16 @@ int dage_operator_apply(dage_operator op, const double *in, double *out, int pre
 int dage_operator_workspace_bytes(dage_operator op, size_t *bytes) {
     DAGE_REQUIRE(op && bytes, DAGE_ERR_ARG, "null argument");
     return apply_f64_workspace(op, bytes);
+}
+
+int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip[3]) {
+    DAGE_REQUIRE(op && nb && pitch && skip, DAGE_ERR_ARG, "null argument");
+    for (int p = 0; p < 3; ++p) {
+        pitch[p] = op->pitch[p];
+        nb[p] = op->pitch[p] > 0 ? op->nbp[p] : op->nb_pick[p];
+        skip[p] = op->mask_used[p] ? 1 : 0;
+    }
+    return DAGE_OK;
 }
 
 int dage_operator_destroy(dage_operator op) {

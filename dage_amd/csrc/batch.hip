@@ -11,8 +11,9 @@
 // a hipGraph and replayed with a single hipGraphLaunch; energies live in
 // device memory, so changing them needs no re-capture.  The batch owns its
 // input and output cubes (contiguous per group) so the graph's pointers are
-// fixed.  dage_batch_tune times every GEMM column-block count per pass and
-// group and re-captures the graph with the fastest.
+// fixed.  dage_batch_tune times every GEMM column-block count (and the
+// operators' own tiling) per pass and group and re-captures the graph with
+// the fastest.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -341,7 +342,7 @@ int dage_batch_tune(dage_batch b, int reps, int *tiles) {
     for (auto &g : b->groups) {
         double best[3] = {1e30, 1e30, 1e30};
         int pick[3] = {0, 0, 0};
-        for (int nb = 1; nb <= 8; ++nb) {
+        for (int nb = 0; nb <= 8; ++nb) {  // 0: the operator's own tiling
             for (int i = 0; i < 3; ++i) g.nb[i] = nb;
             ctx->profiling = true;
             ctx->profile.clear();

@@ -176,6 +176,11 @@ struct dage_operator_s {
     uint64_t *d_mask = nullptr;  // B-row masks of the three passes (build_masks)
     int nb_pick[3] = {0, 0, 0};  // column-block counts chosen from the masks
     bool mask_used[3] = {false, false, false};  // masks cut the pass's work enough
+    // FP64 passes may instead tile the output columns at a pitch that follows
+    // the matrices' zero blocks (pitch[p] > 0, column blocks nbp[p]); d_tmask
+    // then holds per row one bit per such tile, same row order as d_mask
+    uint64_t *d_tmask = nullptr;
+    int pitch[3] = {0, 0, 0}, nbp[3] = {0, 0, 0};
     void *d_planes = nullptr;   // split-BF16 matrix planes (built on first mixed apply)
     bool planes_valid = false;  // cleared when the matrices change
     MixedLayout mixed;
@@ -214,6 +219,9 @@ struct ApplyMats {
     int ntp = 0;
     const uint64_t *mask[3] = {nullptr, nullptr, nullptr};
     int nb_pick[3] = {0, 0, 0};
+    // pitch tiling (pitch[p] > 0): tile masks, same row order, bit j = tile j
+    const uint64_t *tmask[3] = {nullptr, nullptr, nullptr};
+    int pitch[3] = {0, 0, 0}, nbp[3] = {0, 0, 0};
 };
 // Device scratch owned by the caller, grown on demand.
 struct Workspace {

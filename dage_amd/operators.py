@@ -313,6 +313,13 @@ class Operator3D:
         call("dage_operator_workspace_bytes", self.handle, ctypes.byref(n))
         return n.value
 
+    def tiling(self) -> dict:
+        """Column tiling of the FP64 passes (x, y, z): column blocks per tile,
+        tile pitch (0: 16*nb columns) and whether zero K panels are skipped."""
+        nb, pitch, skip = (ctypes.c_int * 3)(), (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        call("dage_operator_get_tiling", self.handle, nb, pitch, skip)
+        return {"nb": list(nb), "pitch": list(pitch), "skip": [bool(v) for v in skip]}
+
 
 class Coulomb3D(Operator3D):
     """Coulomb operator 1/r (Coulomb3D_init_explicit_sub, coulomb.F90:109-176)."""

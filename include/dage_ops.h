@@ -180,6 +180,12 @@ int dage_operator_apply_device(dage_operator op, const double *cubein_dev,
                                double *cubeout_dev, int precision);
 /* Bytes of device workspace the operator holds for its intermediates. */
 int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
+/* No reference counterpart (diagnostic): the output-column tiling the FP64
+ * passes x, y, z use, chosen from the matrices' zero blocks.  nb[p]: 16-wide
+ * column blocks per tile (0: picked from the output width at each apply);
+ * pitch[p]: tile pitch in points, 0 for tiles of 16*nb[p] columns;
+ * skip[p]: 1 when all-zero K panels are skipped. */
+int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip[3]);
 int dage_operator_destroy(dage_operator op);
 
 /* ----------------------------------------------------------------- bubbles
@@ -284,9 +290,10 @@ int dage_batch_set_energies(dage_batch batch, const double *energies);
 int dage_batch_apply(dage_batch batch);
 /* The same work launched kernel by kernel (for profiling). */
 int dage_batch_apply_eager(dage_batch batch);
-/* Autotuner: times every GEMM column-block count for each pass of each grid
- * group (reps timed runs each), keeps the fastest and re-captures the graph.
- * tiles (optional, 3 per group) receives the choices.  Overwrites outputs. */
+/* Autotuner: times every GEMM column-block count 1..8 and the operator's own
+ * tiling (0) for each pass of each grid group (reps timed runs each), keeps
+ * the fastest and re-captures the graph.  tiles (optional, 3 per group)
+ * receives the choices.  Overwrites outputs. */
 int dage_batch_tune(dage_batch batch, int reps, int *tiles);
 int dage_batch_group_count(dage_batch batch, int *ngroups);
 int dage_batch_destroy(dage_batch batch);

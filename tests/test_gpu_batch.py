@@ -91,7 +91,7 @@ def test_mixed_batch_and_tuning(ctx):
     b64.apply()
     ref = [b64.download(i) for i in range(len(sizes))]
     tiles = b64.tune(reps=1)
-    assert tiles.shape == (3, 3) and tiles.min() >= 1 and tiles.max() <= 8
+    assert tiles.shape == (3, 3) and tiles.min() >= 0 and tiles.max() <= 8  # 0: own tiling
     for i in range(len(sizes)):
         b64.upload(i, rhos[i])  # tuning overwrote the outputs, not the inputs
     b64.apply()
