@@ -95,6 +95,10 @@ struct SGemm {
     // skipped (they would add exactly zero).
     const uint64_t *mask;
     long mS;
+    // > 0: column tile j is [j*pitch, (j+1)*pitch), the last one running to N,
+    // and bit j of a mask row stands for tile j (the planes are padded past
+    // N by at least 16*NB - pitch columns, so whole 16*NB tiles stay in them)
+    int pitch;
 };
 
 constexpr int kMaxSteps = 2048;  // K steps a block can list (K <= 65536)
@@ -116,7 +120,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     const int lrow = lane & 15;
     const int kg = lane >> 4;  // k group of 8
     const int m0 = blockIdx.x * BM;
-    const int n0 = blockIdx.y * BN;
+    const int n0 = g.pitch > 0 ? blockIdx.y * g.pitch : blockIdx.y * BN;
+    const int nend = g.pitch > 0 && blockIdx.y + 1 < gridDim.y ? n0 + g.pitch : g.N;
 
     const TA *A = static_cast<const TA *>(g.A);
     const __bf16 *Bp = g.Bp;
@@ -143,7 +148,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     const bool listed = MASKED && nsteps <= kMaxSteps;
     if (listed) {
         if (wave == 0) {
-            const uint64_t cols = ((NB >= 64) ? ~0ull : ((1ull << NB) - 1ull)) << (n0 / 16);
+            const uint64_t cols = g.pitch > 0 ? 1ull << blockIdx.y
+                                              : ((1ull << NB) - 1ull) << (n0 / 16);
             const int klen = kend - kbeg;
             int cnt = 0;
             for (int base = 0; base < nsteps; base += 64) {
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 const int gn = n0 + j * 16 + lrow;
-                if (gn >= g.N) continue;
+                if (gn >= nend) continue;
                 TC *dst = C + crow + (long)gn * g.sCn;
                 const double v = alpha * acc[i][j][r];
                 *dst = g.accumulate ? (TC)((double)*dst + v) : (TC)v;
@@ -323,7 +329,11 @@ void launch_s(const SGemm &g, dim3 grid, hipStream_t s) {
 template <int NPAIR, typename TA, typename TC, bool AK, int PASS>
 int launch_split(dage_ctx ctx, const char *name, const SGemm &g, int nb, int zdim) {
     const int BM = 64 * kMWs;
-    const dim3 grid((g.M + BM - 1) / BM, (g.N + 16 * nb - 1) / (16 * nb), zdim);
+    const int ntiles = g.pitch > 0 ? std::max(1, g.N / g.pitch) : (g.N + 16 * nb - 1) / (16 * nb);
+    DAGE_REQUIRE(g.pitch <= 0 || (ntiles <= 64 && g.pitch <= 16 * nb && g.pitch >= 16 * nb - 15 &&
+                                  g.N - (ntiles - 1) * g.pitch <= 16 * nb),
+                 DAGE_ERR_STATE, "column pitch %d does not fit %d-column tiles", g.pitch, 16 * nb);
+    const dim3 grid((g.M + BM - 1) / BM, ntiles, zdim);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
     switch (nb) {
@@ -454,10 +464,22 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
     // passes whose zero-panel masks pay use them, with the column-block count
     // the mask cost model chose
     auto nb_of = [&](int p, int dflt) {
+        if (!bs.nb[p] && op->pitch[p] > 0) return op->nbp[p];
         return bs.nb[p] ? bs.nb[p] : (op->mask_used[p] ? op->nb_pick[p] : dflt);
     };
     const int nbx = nb_of(0, L.nbx), nby = nb_of(1, L.nby), nbz = nb_of(2, L.nbz);
     const long mrx = (long)T * nxi, mry = (long)T * nyi;
+    // masks of pass p starting at row r: the pitch tiling's when it is used
+    // (no caller column-block count), else the 16-column ones if they pay
+    auto set_mask = [&](SGemm &g, int p, long r, long mS) {
+        if (!bs.nb[p] && op->pitch[p] > 0) {
+            g.pitch = op->pitch[p];
+            g.mask = op->d_tmask + r;
+        } else if (op->mask_used[p]) {
+            g.mask = op->d_mask + r;
+        }
+        if (g.mask) g.mS = mS;
+    };
     const __bf16 *P16 = reinterpret_cast<const __bf16 *>(op->d_planes);
     const MPlan plan = make_mplan(op, B);
     const size_t need = 4 * (plan.v_elems + plan.u_elems) + 8 * plan.p_elems + 256;
@@ -491,7 +513,7 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g1.C = V; g1.N = (int)nxo;
         g1.sCm0 = nxo; g1.sCm1 = 0; g1.sCn = 1; g1.bC = Vt;
         g1.Z0 = tc; g1.bA1 = nin; g1.bC1 = Vt * tc;
-        if (op->mask_used[0]) { g1.mask = op->d_mask + (long)t0 * nxi; g1.mS = nxi; }
+        set_mask(g1, 0, (long)t0 * nxi, nxi);
         DAGE_TRY((launch_split<NPAIR, double, float, true, 0>(op->ctx, "gemm_mixed_xmode", g1,
                                                               nbx, tc * B)));
         // y mode, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
@@ -507,7 +529,7 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g2.Z0 = tc; g2.bA1 = Vt * tc; g2.bC1 = Ut * tc;
         g2.alpha = W + t0;
         g2.aS1 = wS;
-        if (op->mask_used[1]) { g2.mask = op->d_mask + mrx + (long)t0 * nyi; g2.mS = nyi; }
+        set_mask(g2, 1, mrx + (long)t0 * nyi, nyi);
         DAGE_TRY((launch_split<NPAIR, float, float, false, 1>(op->ctx, "gemm_mixed_ymode", g2,
                                                               nby, tc * B)));
         // z mode, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
@@ -526,7 +548,7 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
         g3.sCm0 = 1; g3.sCm1 = 0; g3.sCn = M3; g3.bC = M3 * nzo;
         g3.Z0 = splits; g3.bA1 = Ut * tc; g3.bC1 = M3 * nzo * plan.splits;
         g3.Ksplit = per;
-        if (op->mask_used[2]) g3.mask = op->d_mask + mrx + mry + (long)nzi * t0;
+        set_mask(g3, 2, mrx + mry + (long)nzi * t0, 0);
         g3.accumulate = (t0 > 0);
         if (t0 == 0 && splits < plan.splits)
             for (int b = 0; b < B; ++b)

@@ -128,3 +128,43 @@ def test_mixed_block_sparse_matrices(ctx, monkeypatch):
     err = rel_maxdiff(outs[0], ref)
     print(f"block-sparse bf16x6 vs FP64: {err:.3e}")
     assert err < 2e-7  # toy shapes: the bar of test_gpu_batch.py (1.1e-7 measured here)
+
+
+def test_mixed_pitch_tiles(ctx, monkeypatch):
+    """Box-banded matrices (nonzero only between neighbouring 30-point boxes):
+    the split-BF16 passes use the operator's box-following column tiles; the
+    K-step order inside a tile is unchanged, so the result is bit-identical
+    to 16-column tiles, and within the bf16x6 bar of the FP64 apply."""
+    rng = np.random.default_rng(79)
+    ax = O.Axis.ncell_fixed(-4.0, 4.0, 7, 20)  # 121 points = 4 boxes of 30 (+1)
+    n = ax.n
+    q = dage_amd.GaussQuad(nlin=20, nlog=20, tlin=2.0, tlog=500.0)
+    T = q.get_npoints()
+    box = np.minimum(np.arange(n) // 30, 3)
+    near = np.abs(box[:, None] - box[None, :]) <= 1
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    mats = []
+    for a in range(3):
+        F = np.exp(-0.01 * (i - j) ** 2)[:, :, None] * rng.uniform(0.5, 1.5, (1, 1, T))
+        F[~near, :] = 0.0
+        mats.append(np.asfortranarray(F))
+    fx = mats[0]
+    fy = np.asfortranarray(mats[1].transpose(1, 0, 2))
+    fz = np.asfortranarray(mats[2].transpose(1, 0, 2))
+    cube = np.asfortranarray(1.0 + rng.uniform(0.0, 1.0, (n,) * 3))
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    op = dage_amd.Coulomb3D(ctx, g, g, q, coda=False)
+    op.set_matrices(fx, fy, fz)
+    assert all(p > 0 for p in op.tiling()["pitch"][:2])
+    ref = op.transform_cube(cube)
+    got = op.transform_cube(cube, precision=BF16X6)
+    op.close()
+    monkeypatch.setenv("DAGE_APPLY_NB", "2,2,2")
+    op = dage_amd.Coulomb3D(ctx, g, g, q, coda=False)
+    op.set_matrices(fx, fy, fz)
+    grid16 = op.transform_cube(cube, precision=BF16X6)
+    op.close()
+    assert np.array_equal(got, grid16)
+    err = rel_maxdiff(got, ref)
+    print(f"box-banded bf16x6 vs FP64: {err:.3e}")
+    assert err < 2e-7
