@@ -243,6 +243,36 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     }
 
     // Epilogue.  Register r of lane l holds D(4*r + l/16, l%16).
+    if constexpr (PASS == 1 && MW == 1 && NB <= 4) {
+        // y pass (its output rows are the contiguous axis and it writes the
+        // largest array, U): each wavefront's 16 x 16*NB tile goes through
+        // LDS (the operand buffers, free after the loop's last barrier:
+        // waves 0-1 in As, 2-3 in Bs, 16 rows of LDB each) so that stores
+        // run along rows, 16 consecutive rows (128 B) per column instead of
+        // 4 (32 B).  The x pass's output is contiguous along columns already.
+        static_assert(32 * LDB <= 2 * kBK * LDA && 32 * LDB <= 2 * kBK * LDB,
+                      "two wavefront tiles fit each operand buffer");
+        double *R = (wave < 2 ? &As[0][0] : &Bs[0][0]) + (wave & 1) * 16 * LDB;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) R[(4 * r + lk) * LDB + j * 16 + lrow] = acc[0][j][r];
+        __syncthreads();
+        const int gm = m0 + wrow + lrow;
+        if (gm < g.M) {
+            const long crow = (long)(gm % g.M0) * g.sCm0 + (long)(gm / g.M0) * g.sCm1;
+#pragma unroll
+            for (int i = 0; i < 4 * NB; ++i) {
+                const int nl = lk + 4 * i;
+                const int gn = n0 + nl;
+                if (gn >= nend) continue;
+                double *dst = C + crow + (long)gn * g.sCn;
+                const double v = alpha * R[lrow * LDB + nl];
+                *dst = g.accumulate ? *dst + v : v;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < MW; ++i) {
 #pragma unroll
