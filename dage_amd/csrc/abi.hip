@@ -719,18 +719,33 @@ int dage_bubbles_set(dage_bubbles b, int ibub, const double center[3], double ch
             else if (b->k < 0) rp = (i == 0) ? 0.0 : std::pow(ax.points[i], b->k);
             f[i + (size_t)npts * lm] = (b->k == 0) ? v : v * rp;
         }
+    // cell of r_k = k*qmax/kCellTab: 1 + the number of cell starts 1..ncell-1
+    // at or below r_k (the cell the bisection of grid.F90:1212-1246 finds);
+    // the kernel starts from the entry one bin below r and steps upward
+    const double qmax = ax.points[npts - 1];
+    std::vector<int> tab(kCellTab);
+    for (int k = 0, ic = 1; k < kCellTab; ++k) {
+        const double rk = k * qmax / kCellTab;
+        while (ic < ncell && ax.celld[ic] <= rk) ++ic;
+        tab[k] = ic;
+    }
     BubbleHost &h = b->host[ibub];
     DAGE_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
     if (h.d_buf) DAGE_HIP_CHECK(hipFree(h.d_buf));
+    if (h.d_tab) DAGE_HIP_CHECK(hipFree(h.d_tab));
     h.d_buf = nullptr;
+    h.d_tab = nullptr;
     h.set = false;
     DAGE_HIP_CHECK(hipMalloc((void **)&h.d_buf, sizeof(double) * buf.size()));
     DAGE_HIP_CHECK(hipMemcpy(h.d_buf, buf.data(), sizeof(double) * buf.size(),
                              hipMemcpyHostToDevice));
+    DAGE_HIP_CHECK(hipMalloc((void **)&h.d_tab, sizeof(int) * tab.size()));
+    DAGE_HIP_CHECK(hipMemcpy(h.d_tab, tab.data(), sizeof(int) * tab.size(),
+                             hipMemcpyHostToDevice));
     for (int a = 0; a < 3; ++a) h.center[a] = center[a];
     h.charge = charge;
     h.ncell = ncell;
-    h.qmax = ax.points[npts - 1];
+    h.qmax = qmax;
     h.set = true;
     b->dirty = true;
     return DAGE_OK;
@@ -753,8 +768,10 @@ int dage_bubbles_destroy(dage_bubbles b) {
     dage_ctx ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &h : b->host)
+    for (auto &h : b->host) {
         if (h.d_buf) (void)hipFree(h.d_buf);
+        if (h.d_tab) (void)hipFree(h.d_tab);
+    }
     if (b->d_desc) (void)hipFree(b->d_desc);
     delete b;
     ctx_release(ctx);

@@ -129,7 +129,11 @@ struct BubbleDev {
     const double *steps;   // ncell cell steps
     const double *f;       // npts x (lmax+1)^2, r^k folded in, point fastest
     int ncell, npts;
+    // cell-lookup table: ctab[k] = cell (1-based) holding r = k*qmax/kCellTab
+    const int *ctab;
+    double tscale;         // kCellTab / qmax
 };
+constexpr int kCellTab = 1024;
 
 struct BubbleHost {
     double center[3] = {0.0, 0.0, 0.0};
@@ -137,6 +141,7 @@ struct BubbleHost {
     int ncell = 0;
     double qmax = 0.0;
     double *d_buf = nullptr;  // celld | steps | f
+    int *d_tab = nullptr;     // kCellTab cell-lookup entries
     bool set = false;
 };
 }  // namespace dage

@@ -5,9 +5,8 @@
 //
 // One fused launch for all bubbles and all (l,m): one thread per cube point,
 // a wavefront per 4 x 4 x 4 patch of points.  Per bubble the thread finds
-// the radial cell by the reference's bisection (grid.F90:1212-1246),
-// evaluates the nlip cell
-// polynomials once by Horner (lip.F90:551-566), and reuses them for every
+// the radial cell (grid.F90:1212-1246, from a lookup table), evaluates the
+// nlip cell polynomials once by Horner (lip.F90:551-566), reuses them for every
 // (l,m); the Racah solid harmonics come from the recursion of
 // harmonics.F90:586-659, fully unrolled for the compile-time lmax.
 #include <cmath>
@@ -17,6 +16,12 @@
 
 namespace dage {
 namespace {
+
+// Global-address-space views of the bubble tables: the pointers come out of
+// device memory (the descriptors), where the compiler may otherwise treat
+// them as generic and emit flat loads, or vector loads for uniform addresses.
+typedef const double __attribute__((address_space(1))) *gdptr;
+typedef const int __attribute__((address_space(1))) *giptr;
 
 struct InjectArgs {
     const double *gx, *gy, *gz;  // cube grid points
@@ -56,24 +61,20 @@ __device__ inline void solid_harmonics(double x, double y, double z, double *S) 
     }
 }
 
-// Grid1D_get_icell: 1-based cell of r, 0 outside [0, qmax].
-__device__ inline int radial_icell(const double *celld, int ncell, double qmax, double r) {
-    if (!(r >= 0.0 && r <= qmax)) return 0;  // also rejects NaN
-    if (celld[ncell - 1] <= r) return ncell;
-    int icell = ncell / 2, maxi = ncell - 1, mini = 1;
-    for (int it = 0; it < 64; ++it) {  // bisection over <= 2^63 cells
-        if (icell < 1 || icell > ncell - 1) return 0;
-        if (r >= celld[icell]) {
-            if (r < celld[icell + 1]) return icell + 1;
-            mini = icell + 2;
-            icell = (mini + maxi) / 2;
-        } else {
-            if (r >= celld[icell - 1]) return icell;
-            maxi = icell - 1;
-            icell = (mini + maxi) / 2;
-        }
-    }
-    return 0;
+// Grid1D_get_icell (grid.F90:1212-1246): the 1-based cell c with
+// celld[c-1] <= r < celld[c] (the last cell from its start up to qmax), 0
+// outside [0, qmax].  The reference bisects; here the lookup-table entry one
+// bin below r (a margin for the rounding of r*tscale) is at or below r's
+// cell, and upward steps stop at the first cell start above r: the same
+// cell, in one or two dependent loads instead of about log2(ncell).
+__device__ inline int radial_icell_tab(const BubbleDev &B, double r) {
+    if (!(r >= 0.0 && r <= B.qmax)) return 0;  // also rejects NaN
+    const gdptr celld = (gdptr)B.celld;
+    if (celld[B.ncell - 1] <= r) return B.ncell;
+    const int k = min(max((int)(r * B.tscale) - 1, 0), kCellTab - 1);
+    int ic = ((giptr)B.ctab)[k];
+    while (celld[ic] <= r) ++ic;
+    return ic;
 }
 
 // NLIP > 0: compile-time cell size (unrolled Horner and dot products);
@@ -108,11 +109,11 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
         const BubbleDev B = A.bubs[b];
         const double dx = px - B.cx, dy = py - B.cy, dz = pz - B.cz;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const int ic = inside ? radial_icell(B.celld, B.ncell, B.qmax, r) : 0;
+        const int ic = inside ? radial_icell_tab(B, r) : 0;
         unsigned long long todo = __ballot(ic != 0);
         if (todo == 0ull) continue;
         const int icl = ic > 0 ? ic - 1 : 0;
-        const double crd = (r - B.celld[icl]) / B.steps[icl] + A.first;
+        const double crd = (r - ((gdptr)B.celld)[icl]) / ((gdptr)B.steps)[icl] + A.first;
         double polys[NMAX];
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
             // (inside the branch the compiler may otherwise substitute the
             // per-lane ic for c)
             const int cs = __builtin_amdgcn_readfirstlane(c);
-            const double *f = B.f + (long)(cs - 1) * (nlip - 1);
+            const gdptr f = (gdptr)(B.f + (long)(cs - 1) * (nlip - 1));
             double res = 0.0;
             double ipow = 1.0;
 #pragma unroll
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
 #pragma unroll
                     for (int m = -l; m <= l; ++m) {
                         const int lm = l * l + l + m;
-                        const double *fl = f + (long)lm * B.npts;
+                        const gdptr fl = f + (long)lm * B.npts;
                         double fv = 0.0;
 #pragma unroll
                         for (int j = 0; j < NMAX; ++j) {
@@ -197,6 +198,8 @@ int bubbles_sync_device(dage_bubbles bub) {
         d.celld = h.d_buf;
         d.steps = h.d_buf + h.ncell;
         d.f = h.d_buf + 2 * h.ncell;
+        d.ctab = h.d_tab;
+        d.tscale = kCellTab / h.qmax;
     }
     DAGE_HIP_CHECK(hipMemcpy(bub->d_desc, desc.data(), sizeof(BubbleDev) * bub->nbub,
                              hipMemcpyHostToDevice));

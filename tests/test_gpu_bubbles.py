@@ -65,3 +65,34 @@ def test_inject_errors(ctx):
     assert "not been set" in str(e.value)
     with pytest.raises(ValueError):
         b.set_bubble(0, [0, 0, 0], 1.0, np.full(3, 0.1), np.zeros((5, 9)))
+
+
+def test_inject_points_on_radial_cell_starts(ctx):
+    """Cube points at exactly the radial cell starts: a bubble at a grid point
+    whose radial cells are as wide as the cube spacing (0.25) puts the points
+    along each axis through the centre on cell boundaries, where the cell
+    lookup must pick the cell that starts there (grid.F90:1212-1246), and the
+    far corners beyond the radial grid get nothing."""
+    rng = np.random.default_rng(91)
+    lmax = 3
+    ax = O.Axis.ncell_fixed(-3.0, 3.0, 7, 4)  # 25 points, h = 0.25
+    steps = np.full(10, 0.25 / 6)             # 10 cells of 0.25: r up to 2.5
+    r = O.Axis(0.0, steps, 7).points
+    c = rng.uniform(-1.0, 1.0, (lmax + 1) ** 2)
+    bf = np.zeros((len(r), (lmax + 1) ** 2), order="F")
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            bf[:, l * l + l + m] = c[l * l + l + m] * (1.0 + r) ** l * np.exp(-r)
+    center = np.array([[0.0, 0.5, -0.25]])
+    bub = O.BubbleSet(center, [steps], [bf], lmax, 0, 7)
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    cube = dage_amd.DeviceCube(ctx, (ax.n,) * 3)
+    cube.set_to_zero()
+    b = lib_bubbles(ctx, bub)
+    b.inject(g, cube)
+    ref = O.bubbles_eval_grid(bub, ax.points, ax.points, ax.points)
+    got = cube.download()
+    err = rel_maxdiff(got, ref)
+    print(f"points on cell starts: {err:.3e}")
+    assert err < 1e-12
+    assert np.any(ref == 0.0) and np.array_equal(got == 0.0, ref == 0.0)
