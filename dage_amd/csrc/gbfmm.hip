@@ -51,6 +51,10 @@ struct dage_gbfmm_s {
     // per axis and box coordinate: box and near-field point ranges
     std::vector<int> box_lo[3], box_n[3], box_cut[3], nf_lo[3], nf_n[3];
     dage_operator nf_op = nullptr;    // whole-cube operator with the banded near-field matrices
+    // the far field up to the local expansions runs on its own stream,
+    // beside the near field (both only read the density)
+    hipStream_t s_far = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<int> boxvec;          // 3 per maxlevel box, by box index
     // far field
     double *d_q = nullptr, *d_L = nullptr, *d_part = nullptr;
@@ -558,6 +562,9 @@ void free_gbfmm(dage_gbfmm f) {
         if (p) (void)hipFree(p);
     if (f->d_m2l_G) (void)hipFree(f->d_m2l_G);
     if (f->d_m2l_out) (void)hipFree(f->d_m2l_out);
+    if (f->ev_fork) (void)hipEventDestroy(f->ev_fork);
+    if (f->ev_join) (void)hipEventDestroy(f->ev_join);
+    if (f->s_far) (void)hipStreamDestroy(f->s_far);
     if (f->ctx) ctx_release(f->ctx);
     delete f;
 }
@@ -878,15 +885,74 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     return DAGE_OK;
 }
 
+// Runs the context's launches on another stream while in scope.
+struct StreamSwap {
+    dage_ctx ctx;
+    hipStream_t saved;
+    StreamSwap(dage_ctx c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+    ~StreamSwap() { ctx->stream = saved; }
+};
+
+// Far field of the finest boxes up to their local-expansion coefficients
+// d_L: multipoles, M2M, M2L, L2L (reads only rho).
+int gbfmm_far_coefficients(dage_gbfmm f, const double *rho);
+
 int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
+    dage_ctx ctx = f->ctx;
+    hipStream_t s = ctx->stream;
+    const int nb = f->nb, N = f->N;
+    const long ldx = f->npt[0], ldy = f->npt[1];
+    hipEvent_t ev;
+    const int nbox_max = nb * nb * nb;
+    // near field of every box: one whole-cube apply with the banded matrices
+    // (context stream); the far-field coefficients beside it on s_far
+    DAGE_HIP_CHECK(hipEventRecord(f->ev_fork, s));
+    DAGE_HIP_CHECK(hipStreamWaitEvent(f->s_far, f->ev_fork, 0));
+    if (precision == DAGE_PREC_FP64) DAGE_TRY(apply_f64(f->nf_op, rho, pot));
+    else DAGE_TRY(apply_mixed(f->nf_op, rho, pot, precision == DAGE_PREC_BF16X6 ? 6 : 3));
+    {
+        StreamSwap swap(ctx, f->s_far);
+        DAGE_TRY(gbfmm_far_coefficients(f, rho));
+    }
+    DAGE_HIP_CHECK(hipEventRecord(f->ev_join, f->s_far));
+    DAGE_HIP_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));
+    // local expansions on the finest boxes' points (after the near field)
+    LeArgs la;
+    la.out = pot; la.ldx = ldx; la.ldy = ldy;
+    la.boxvec = f->d_boxvec; la.geo = f->d_geo; la.nb = nb;
+    la.points = f->d_points; la.points_ld = f->points_ld;
+    la.centers = f->d_centers; la.coef = f->d_coef;
+    la.L = f->d_L;  // level maxlevel has offset 0
+    la.lmax = f->lmax; la.N = N;
+    if (f->uniform) {
+        // potential at the box's points = S^T L, added over the points each box owns
+        prof_begin(ctx, "local_expansion_gemm", &ev);
+        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_local_expansion", (int)f->P, nbox_max, N, f->d_S,
+                               f->P, f->d_L, N, f->d_Pbox, 1, nullptr));
+        const long tot = f->P * nbox_max;
+        const int gs = (int)std::min<long>((tot + 255) / 256, 65536);
+        hipLaunchKernelGGL(box_scatter_kernel, dim3(gs), dim3(256), 0, s, f->d_Pbox, ldx, ldy,
+                           f->d_boxvec, f->d_geo, nb, f->pb[0], f->pb[1], f->pb[2], nbox_max,
+                           pot);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, "local_expansion_gemm", ev);
+        return DAGE_OK;
+    }
+    int maxz = 0;
+    for (int b = 0; b < nb; ++b) maxz = std::max(maxz, f->box_cut[2][b]);
+    prof_begin(ctx, "local_expansion_kernel", &ev);
+    hipLaunchKernelGGL(local_expansion_kernel, dim3(nbox_max, maxz), dim3(256), 0, s, la);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, "local_expansion_kernel", ev);
+    return DAGE_OK;
+}
+
+int gbfmm_far_coefficients(dage_gbfmm f, const double *rho) {
     dage_ctx ctx = f->ctx;
     hipStream_t s = ctx->stream;
     const int nb = f->nb, N = f->N, L = f->maxlevel;
     const long ldx = f->npt[0], ldy = f->npt[1];
     hipEvent_t ev;
-    // near field of every box: one whole-cube apply with the banded matrices
-    if (precision == DAGE_PREC_FP64) DAGE_TRY(apply_f64(f->nf_op, rho, pot));
-    else DAGE_TRY(apply_mixed(f->nf_op, rho, pot, precision == DAGE_PREC_BF16X6 ? 6 : 3));
     // multipoles of the finest boxes
     const int nbox_max = nb * nb * nb;
     DAGE_TRY(fill_zero(ctx, f->d_L, (size_t)N * f->nbox_total));
@@ -958,34 +1024,6 @@ int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
         DAGE_HIP_CHECK(hipGetLastError());
         prof_end(ctx, "m2l_gemm", ev);
     }
-    // local expansions on the finest boxes' points
-    LeArgs la;
-    la.out = pot; la.ldx = ldx; la.ldy = ldy;
-    la.boxvec = f->d_boxvec; la.geo = f->d_geo; la.nb = nb;
-    la.points = f->d_points; la.points_ld = f->points_ld;
-    la.centers = f->d_centers; la.coef = f->d_coef;
-    la.L = f->d_L;  // level maxlevel has offset 0
-    la.lmax = f->lmax; la.N = N;
-    if (f->uniform) {
-        // potential at the box's points = S^T L, added over the points each box owns
-        prof_begin(ctx, "local_expansion_gemm", &ev);
-        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_local_expansion", (int)f->P, nbox_max, N, f->d_S,
-                               f->P, f->d_L, N, f->d_Pbox, 1, nullptr));
-        const long tot = f->P * nbox_max;
-        const int gs = (int)std::min<long>((tot + 255) / 256, 65536);
-        hipLaunchKernelGGL(box_scatter_kernel, dim3(gs), dim3(256), 0, s, f->d_Pbox, ldx, ldy,
-                           f->d_boxvec, f->d_geo, nb, f->pb[0], f->pb[1], f->pb[2], nbox_max,
-                           pot);
-        DAGE_HIP_CHECK(hipGetLastError());
-        prof_end(ctx, "local_expansion_gemm", ev);
-        return DAGE_OK;
-    }
-    int maxz = 0;
-    for (int b = 0; b < nb; ++b) maxz = std::max(maxz, f->box_cut[2][b]);
-    prof_begin(ctx, "local_expansion_kernel", &ev);
-    hipLaunchKernelGGL(local_expansion_kernel, dim3(nbox_max, maxz), dim3(256), 0, s, la);
-    DAGE_HIP_CHECK(hipGetLastError());
-    prof_end(ctx, "local_expansion_kernel", ev);
     return DAGE_OK;
 }
 
@@ -1026,7 +1064,14 @@ int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax, co
     f->N = (lmax + 1) * (lmax + 1);
     f->nb = nb;
     f->nlip = grid->axis[0].nlip;
-    const int rc = build_gbfmm(f, grid, tp, tw, ntp);
+    int rc = DAGE_OK;
+    if (hipStreamCreateWithFlags(&f->s_far, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming) != hipSuccess) {
+        set_error("GBFMM stream or event creation failed");
+        rc = DAGE_ERR_HIP;
+    }
+    if (rc == DAGE_OK) rc = build_gbfmm(f, grid, tp, tw, ntp);
     if (rc != DAGE_OK) {
         free_gbfmm(f);
         return rc;
