@@ -94,6 +94,40 @@ def test_gbfmm_device_path_and_repeat(ctx):
     assert np.array_equal(fmm.apply(rho), first)
 
 
+def test_gbfmm_on_a_caller_stream(ctx):
+    """The far field runs on the GBFMM object's own stream, forked from and
+    joined back into the context stream: on a stream the caller set (created
+    here through the HIP runtime), results are bit-identical to the default
+    stream's, and the context stream alone orders them for the download."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    ax = O.Axis.ncell_fixed(-4.0, 4.0, 5, 8)
+    rng = np.random.default_rng(19)
+    q_, a, c = random_gaussians(rng, 2, box=2.0)
+    rho = O.gauss_density(q_, a, c, ax.points, ax.points, ax.points)
+    g = grid_of(ctx, [ax] * 3)
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=6, gauss=dage_amd.GaussQuad(8, 8))
+    d_in = dage_amd.DeviceCube(ctx, rho.shape, rho)
+    d_out = dage_amd.DeviceCube(ctx, rho.shape)
+    fmm.apply_device(d_in, d_out)
+    ref = d_out.download()
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        ctx.set_stream(stream.value)
+        d_out.set_to_zero()
+        for _ in range(3):
+            fmm.apply_device(d_in, d_out)
+        ctx.synchronize()
+        got = d_out.download()
+    finally:
+        ctx.set_stream(None)
+        assert hip.hipStreamSynchronize(stream) == 0
+        assert hip.hipStreamDestroy(stream) == 0
+    assert np.array_equal(got, ref)
+    fmm.close()
+
+
 def test_gbfmm_against_analytic(ctx):
     """121^3 points over [-6,6]^3 (nlip 7, 20 cells, 4x4x4 boxes), lmax 12,
     near field with the reference default GaussQuad(25, 25)."""
