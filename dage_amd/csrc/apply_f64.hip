@@ -62,11 +62,12 @@ constexpr int kMaxPanels = 2048;  // K panels a block can list (K <= 32768)
 // layout on gfx950 (measured with a one-hot probe): lane l holds A(l%16, l/16)
 // and B(l/16, l%16); result register r of lane l is D(4*r + l/16, l%16).
 //
-// Operand panels of kBK k-values are double-buffered in LDS (one barrier per
-// panel); the next panel's global loads are issued before the current panel's
-// MFMAs, from clamped addresses so no load sits behind a branch.  Values
-// outside the matrix are loaded as zeros, so padded k-steps add nothing.
-// PASS only distinguishes the three apply passes in profiles.
+// Operand panels of kBK k-values are staged in LDS; the next panel's global
+// loads are issued into registers before the current panel's MFMAs, from
+// clamped addresses so no load sits behind a branch.  Values outside the
+// matrix are loaded as zeros, so padded k-steps add nothing.  PASS 0-2 are
+// the three apply passes (one LDS buffer), PASS 3 the column-major GEMMs
+// (two buffers).
 template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG, int PASS>
 __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     constexpr int BM = 64 * MW;
@@ -75,8 +76,14 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     constexpr int B_PER = NB;                   // BN * kBK / kThreads
     constexpr int LDA = BM + 1;  // +1 double breaks power-of-two strides
     constexpr int LDB = BN + 1;
-    __shared__ double As[2][kBK * LDA];
-    __shared__ double Bs[2][kBK * LDB];
+    // The mode passes stage one panel at a time (next panel prefetched into
+    // registers, two barriers per panel): 21 KB of LDS lets five blocks share
+    // a CU, and independent blocks hide more latency than a second buffer
+    // does (x and z passes 5 % faster at 241^3 than double buffering at four
+    // blocks per CU).  The column-major GEMMs keep two buffers.
+    constexpr int NBUF = (PASS <= 2) ? 1 : 2;
+    __shared__ double As[NBUF][kBK * LDA];
+    __shared__ double Bs[NBUF][kBK * LDB];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -228,7 +235,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         __syncthreads();
     }
     for (int p = 0; p < npanels; ++p) {
-        const int buf = p & 1;
+        const int buf = NBUF == 2 ? (p & 1) : 0;
         const bool more = p + 1 < npanels;
         if (more) load_panel(panel_k(p + 1));
         const double *as = As[buf];
@@ -238,7 +245,12 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks)
             if (ks < ksteps) kstep(as, bs, ks);
-        if (more) store_panel(buf ^ 1);
+        if constexpr (NBUF == 1) {
+            __syncthreads();  // every wavefront is done with the panel
+            if (more) store_panel(0);
+        } else {
+            if (more) store_panel(buf ^ 1);
+        }
         __syncthreads();
     }
 
@@ -246,29 +258,38 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     if constexpr (PASS == 1 && MW == 1 && NB <= 4) {
         // y pass (its output rows are the contiguous axis and it writes the
         // largest array, U): each wavefront's 16 x 16*NB tile goes through
-        // LDS (the operand buffers, free after the loop's last barrier:
-        // waves 0-1 in As, 2-3 in Bs, 16 rows of LDB each) so that stores
-        // run along rows, 16 consecutive rows (128 B) per column instead of
-        // 4 (32 B).  The x pass's output is contiguous along columns already.
-        static_assert(32 * LDB <= 2 * kBK * LDA && 32 * LDB <= 2 * kBK * LDB,
-                      "two wavefront tiles fit each operand buffer");
-        double *R = (wave < 2 ? &As[0][0] : &Bs[0][0]) + (wave & 1) * 16 * LDB;
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) R[(4 * r + lk) * LDB + j * 16 + lrow] = acc[0][j][r];
-        __syncthreads();
+        // LDS (the operand buffers, free after the loop's last barrier; one
+        // tile fits each, so wavefronts 0-1 go first, then 2-3) so that
+        // stores run along rows, 16 consecutive rows (128 B) per column
+        // instead of 4 (32 B).  The x pass's output is contiguous along
+        // columns already.
+        static_assert(16 * LDB <= kBK * LDA && 16 * LDB <= kBK * LDB,
+                      "a wavefront tile fits each operand buffer");
+        double *R = (wave & 1) ? &Bs[0][0] : &As[0][0];
         const int gm = m0 + wrow + lrow;
-        if (gm < g.M) {
-            const long crow = (long)(gm % g.M0) * g.sCm0 + (long)(gm / g.M0) * g.sCm1;
+        const long crow =
+            gm < g.M ? (long)(gm % g.M0) * g.sCm0 + (long)(gm / g.M0) * g.sCm1 : 0;
+        for (int round = 0; round < 2; ++round) {
+            const bool mine = (wave >> 1) == round;
+            if (round > 0) __syncthreads();  // the first round's reads are done
+            if (mine) {
 #pragma unroll
-            for (int i = 0; i < 4 * NB; ++i) {
-                const int nl = lk + 4 * i;
-                const int gn = n0 + nl;
-                if (gn >= nend) continue;
-                double *dst = C + crow + (long)gn * g.sCn;
-                const double v = alpha * R[lrow * LDB + nl];
-                *dst = g.accumulate ? *dst + v : v;
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        R[(4 * r + lk) * LDB + j * 16 + lrow] = acc[0][j][r];
+            }
+            __syncthreads();
+            if (mine && gm < g.M) {
+#pragma unroll
+                for (int i = 0; i < 4 * NB; ++i) {
+                    const int nl = lk + 4 * i;
+                    const int gn = n0 + nl;
+                    if (gn >= nend) continue;
+                    double *dst = C + crow + (long)gn * g.sCn;
+                    const double v = alpha * R[lrow * LDB + nl];
+                    *dst = g.accumulate ? *dst + v : v;
+                }
             }
         }
         return;
