@@ -53,15 +53,29 @@ __device__ inline uint16_t f32_to_bf16(float f) {
     return (uint16_t)(r >> 16);
 }
 
+// The VALU layers' weight table: w[(o*cin + c)*27 + t] (bf16) as FP32 at
+// wf[(c*27 + t)*cout + o], so one (c, t) tap's cout weights are contiguous.
+__global__ void __launch_bounds__(256) weights_f32_kernel(const uint16_t *w, int cout, int cin,
+                                                          float *wf) {
+    const int n = cout * cin * 27;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int o = i / (cin * 27), ct = i - o * (cin * 27);
+        wf[ct * cout + o] = bf16_to_f32(w[i]);
+    }
+}
+
+// The weights (FP32 copies of the bf16 values) are read at wavefront-uniform
+// indices through scalar loads, so every multiply-add takes its weight from
+// a scalar register; no LDS, no per-lane weight loads.
 template <int COUT>
-__global__ void __launch_bounds__(256) conv_fwd_kernel(const float *in, int cin,
-                                                       const uint16_t *w, const float *bias,
-                                                       int relu, float *out, int nx, int ny,
+__global__ void __launch_bounds__(256) conv_fwd_kernel(const float *__restrict__ in, int cin,
+                                                       const float *__restrict__ wf,
+                                                       const float *__restrict__ bias, int relu,
+                                                       float *__restrict__ out, int nx, int ny,
                                                        int nz) {
-    __shared__ float ws[COUT * kMaxC * 27];
-    const int nw = COUT * cin * 27;
-    for (int i = threadIdx.x; i < nw; i += 256) ws[i] = bf16_to_f32(w[i]);
-    __syncthreads();
+    typedef const float __attribute__((address_space(1))) *gfptr;
+    const gfptr ws = (gfptr)wf;
+    const gfptr bs = (gfptr)bias;
     const long vol = (long)nx * ny * nz;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < vol;
          p += (long)gridDim.x * blockDim.x) {
@@ -71,7 +85,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const float *in, int cin,
         const int z = (int)(r / ny);
         float acc[COUT];
 #pragma unroll
-        for (int o = 0; o < COUT; ++o) acc[o] = bias[o];
+        for (int o = 0; o < COUT; ++o) acc[o] = bs[o];
         for (int c = 0; c < cin; ++c) {
             const float *ic = in + (long)c * vol;
 #pragma unroll
@@ -81,7 +95,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const float *in, int cin,
                 const bool ok = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
                 const float v = ok ? ic[xx + (long)nx * (yy + (long)ny * zz)] : 0.0f;
 #pragma unroll
-                for (int o = 0; o < COUT; ++o) acc[o] += ws[(o * cin + c) * 27 + t] * v;
+                for (int o = 0; o < COUT; ++o) acc[o] += ws[(c * 27 + t) * COUT + o] * v;
             }
         }
 #pragma unroll
@@ -496,12 +510,19 @@ int dage_conv3d_forward(dage_ctx ctx, int nx, int ny, int nz, int cin, int cout,
     const long vol = (long)nx * ny * nz;
     if (cin == kMaxC && cout == kMaxC)
         return launch_conv16(ctx, nx, ny, nz, in, w, bias, relu, out);
+    void *scr = nullptr;  // FP32 weight table
+    const int nw = cout * cin * 27;
+    DAGE_TRY(ctx_scratch(ctx, sizeof(float) * nw, &scr));
+    float *wf = static_cast<float *>(scr);
+    hipLaunchKernelGGL(weights_f32_kernel, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, w,
+                       cout, cin, wf);
+    DAGE_HIP_CHECK(hipGetLastError());
     if (cout == 1)
         hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(grid_of(vol)), dim3(256), 0, ctx->stream, in,
-                           cin, w, bias, relu, out, nx, ny, nz);
+                           cin, wf, bias, relu, out, nx, ny, nz);
     else
         hipLaunchKernelGGL(conv_fwd_kernel<kMaxC>, dim3(grid_of(vol)), dim3(256), 0, ctx->stream,
-                           in, cin, w, bias, relu, out, nx, ny, nz);
+                           in, cin, wf, bias, relu, out, nx, ny, nz);
     DAGE_HIP_CHECK(hipGetLastError());
     return DAGE_OK;
 }
