@@ -894,7 +894,8 @@ struct StreamSwap {
 };
 
 // Far field of the finest boxes up to their local-expansion coefficients
-// d_L: multipoles, M2M, M2L, L2L (reads only rho).
+// d_L: multipoles, M2M, M2L, L2L (reads only rho), and for equal boxes the
+// expansions evaluated at every box's points (d_Pbox).
 int gbfmm_far_coefficients(dage_gbfmm f, const double *rho);
 
 int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
@@ -925,10 +926,9 @@ int gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision) {
     la.L = f->d_L;  // level maxlevel has offset 0
     la.lmax = f->lmax; la.N = N;
     if (f->uniform) {
-        // potential at the box's points = S^T L, added over the points each box owns
+        // potential at the box's points = S^T L (computed on s_far), added
+        // over the points each box owns
         prof_begin(ctx, "local_expansion_gemm", &ev);
-        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_local_expansion", (int)f->P, nbox_max, N, f->d_S,
-                               f->P, f->d_L, N, f->d_Pbox, 1, nullptr));
         const long tot = f->P * nbox_max;
         const int gs = (int)std::min<long>((tot + 255) / 256, 65536);
         hipLaunchKernelGGL(box_scatter_kernel, dim3(gs), dim3(256), 0, s, f->d_Pbox, ldx, ldy,
@@ -1023,6 +1023,14 @@ int gbfmm_far_coefficients(dage_gbfmm f, const double *rho) {
                            f->d_list[l], f->d_m2l_pos[l], N, Ll);
         DAGE_HIP_CHECK(hipGetLastError());
         prof_end(ctx, "m2l_gemm", ev);
+    }
+    if (f->uniform) {
+        // the local expansions at every box's points, P_box = S^T L (needs
+        // only L; the scatter into the potential waits for the near field)
+        prof_begin(ctx, "local_expansion_gemm", &ev);
+        DAGE_TRY(gemm_colmajor(ctx, "gemm_f64_local_expansion", (int)f->P, nbox_max, N, f->d_S,
+                               f->P, f->d_L, N, f->d_Pbox, 1, nullptr));
+        prof_end(ctx, "local_expansion_gemm", ev);
     }
     return DAGE_OK;
 }
