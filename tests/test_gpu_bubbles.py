@@ -96,3 +96,23 @@ def test_inject_points_on_radial_cell_starts(ctx):
     print(f"points on cell starts: {err:.3e}")
     assert err < 1e-12
     assert np.any(ref == 0.0) and np.array_equal(got == 0.0, ref == 0.0)
+
+
+def test_inject_headline_size_planes(ctx):
+    """The headline workload's inject (4 atoms, lmax 4, 199-point radial
+    grids, 241^3 over [-12, 12]^3) checked against the oracle on 4 z-planes
+    spread through the cube (the oracle on all 241 planes takes minutes)."""
+    rng = np.random.default_rng(2410)
+    bub, _ = slater_bubbles(rng, 4, lmax=4)
+    ax = O.Axis.ncell_fixed(-12.0, 12.0, 7, 40)
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    cube = dage_amd.DeviceCube(ctx, (ax.n,) * 3)
+    cube.set_to_zero()
+    b = lib_bubbles(ctx, bub)
+    b.inject(g, cube)
+    got = cube.download()
+    planes = [0, 97, 120, 240]
+    ref = O.bubbles_eval_grid(bub, ax.points, ax.points, ax.points[planes])
+    err = rel_maxdiff(got[:, :, planes], ref)
+    print(f"241^3 inject on planes {planes}: {err:.3e}")
+    assert err < 1e-12
