@@ -542,6 +542,10 @@ class HelmholtzBatch:
         call("dage_batch_apply_eager", self.handle)
 
     def tune(self, reps: int = 2) -> np.ndarray:
+        """Time every GEMM column-block count (1-8) and the operator's own
+        tiling (0) per pass and grid group, keep the fastest and re-capture
+        the graph; returns the choices, one row (x, y, z) per group.
+        Overwrites the outputs."""
         tiles = (ctypes.c_int * (3 * self.ngroups))()
         call("dage_batch_tune", self.handle, int(reps), tiles)
         return np.array(tiles[:], dtype=int).reshape(self.ngroups, 3)
