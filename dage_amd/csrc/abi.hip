@@ -201,6 +201,11 @@ int dage_ctx_destroy(dage_ctx ctx) {
 int dage_ctx_set_stream(dage_ctx ctx, void *stream) {
     DAGE_TRY(use_device(ctx));
     DAGE_TRY(prof_resolve(ctx));
+    // work on the outgoing stream may still use the scratch buffer, and that
+    // stream may be destroyed by its owner once switched away: wait for it
+    // here and forget it, so ctx_scratch never touches a stale handle
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->scratch_stream = nullptr;
     ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
     return DAGE_OK;
 }

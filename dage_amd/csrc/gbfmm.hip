@@ -1093,6 +1093,11 @@ int dage_gbfmm_apply_device(dage_gbfmm f, const double *rho, double *pot, int pr
     DAGE_REQUIRE(precision == DAGE_PREC_FP64 || precision == DAGE_PREC_BF16X6 ||
                      precision == DAGE_PREC_BF16X3,
                  DAGE_ERR_ARG, "unknown precision mode %d", precision);
+    // the far field reads rho on a second stream while the near field writes
+    // pot on the context stream, so the two buffers must not overlap
+    const size_t n = (size_t)f->npt[0] * f->npt[1] * f->npt[2];
+    DAGE_REQUIRE(rho + n <= pot || pot + n <= rho, DAGE_ERR_ARG,
+                 "rho and pot overlap (the far field reads rho while pot is written)");
     DAGE_TRY(use_device(f->ctx));
     return gbfmm_apply(f, rho, pot, precision);
 }

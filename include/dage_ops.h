@@ -76,7 +76,9 @@ const char *dage_last_error(void);
  * (src/bubbles/streamcontainer.cpp:377) for one device. */
 int dage_ctx_create(int device, dage_ctx *ctx);
 int dage_ctx_destroy(dage_ctx ctx);
-/* Use a caller-provided hipStream_t (NULL restores the context's own). */
+/* Use a caller-provided hipStream_t (NULL restores the context's own).
+ * Synchronises the outgoing stream first, so the caller may destroy it
+ * afterwards. */
 int dage_ctx_set_stream(dage_ctx ctx, void *hip_stream);
 int dage_ctx_get_stream(dage_ctx ctx, void **hip_stream);
 /* Replaces cuda_sync_all_ (src/bubbles/cublas_iface.cpp) for this stream. */
@@ -241,7 +243,8 @@ int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax,
  * pot = near field + local expansions, on the grid's points (device buffers,
  * asynchronous on the context stream).  precision selects the near-field
  * arithmetic (DAGE_PREC_*); multipoles, translations and local expansions
- * are always FP64. */
+ * are always FP64.  rho and pot must not overlap (DAGE_ERR_ARG): the far
+ * field reads rho on a second stream while pot is written. */
 int dage_gbfmm_apply_device(dage_gbfmm gbfmm, const double *rho_dev,
                             double *pot_dev, int precision);
 /* Host-buffer variant (uploads, applies, downloads, synchronises). */

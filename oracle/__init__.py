@@ -73,6 +73,7 @@ def _load():
                                               _dp, _dp]),
         "dage_oracle_eval_local_expansion": (None, [i, _dp, _dp, i, _dp, i, _dp, i, _dp, _dp, i,
                                                     i]),
+        "dage_oracle_set_threads": (i, [i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -80,6 +81,12 @@ def _load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of transform_cube and bubbles_eval_grid (their results
+    do not depend on it); returns the count in use."""
+    return int(_load().dage_oracle_set_threads(int(n)))
 
 
 def _p(a: np.ndarray):
@@ -380,6 +387,34 @@ def fmm_translation_full(lmax: int, dR) -> np.ndarray:
     return W
 
 
+def box_multipoles(lmax: int, rho_box, axes, centre) -> np.ndarray:
+    """Multipoles of one box (integrator.cu:1122-1196): q_lm = sum_p wx wy wz
+    rho_p S_lm(r_p - centre) over the box sub-grid `axes` (its own LIP
+    weights); rho_box is the density on those points."""
+    rho_box = np.asfortranarray(rho_box, dtype=np.float64)
+    n = [a.n for a in axes]
+    if list(rho_box.shape) != n:
+        raise ValueError("density shape does not match the box grid")
+    q = np.zeros((lmax + 1) ** 2)
+    _load().dage_oracle_box_multipoles(
+        lmax, rho_box.ctypes.data_as(_dp), n[0], n[1],
+        _p(_f64(axes[0].points)), _p(_f64(axes[0].ints())), n[0],
+        _p(_f64(axes[1].points)), _p(_f64(axes[1].ints())), n[1],
+        _p(_f64(axes[2].points)), _p(_f64(axes[2].ints())), n[2], _p(_f64(centre)), _p(q))
+    return q
+
+
+def eval_local_expansion(lmax: int, L, x, y, z, centre) -> np.ndarray:
+    """Local-expansion potential sum_lm L_lm S_lm(r - centre) on the points
+    x (x) y (x) z (gbfmm_coulomb3d.cu:81-280)."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.zeros((len(x), len(y), len(z)), order="F")
+    _load().dage_oracle_eval_local_expansion(lmax, _p(_f64(L)), _p(x), len(x), _p(y), len(y),
+                                             _p(z), len(z), _p(_f64(centre)),
+                                             out.ctypes.data_as(_dp), len(x), len(y))
+    return out
+
+
 def sub_axis(ax: Axis, c0: int, c1: int) -> Axis:
     """Grid1D_get_subgrid (grid.F90:1474-1506) for cells [c0, c1) (0-based)."""
     return Axis(float(ax.celld[c0]), ax.steps[c0:c1].copy(), ax.nlip, ax.grid_type)
@@ -474,7 +509,11 @@ def gbfmm_potential(axes, rho, maxlevel, lmax, tp, tw, start_level=2):
             W = fmm_translation_packed(lmax, centre(_box_vector(child, l + 1), l + 1)
                                        - centre(_box_vector(parent, l), l))
             q[:, offs[l] + parent] += fmm_translate(lmax, W, q[:, offs[l + 1] + child])
-    # far field per level and downward pass
+    # far field per level and downward pass; an interaction matrix depends
+    # only on the integer box offset at a level, so it is built once per
+    # offset (the reference builds one per pair: equal up to the rounding of
+    # the centre difference)
+    tcache = {}
     for l in range(start_level, maxlevel + 1):
         n = 1 << l
         for t in range(8 ** l):
@@ -497,7 +536,10 @@ def gbfmm_potential(axes, rho, maxlevel, lmax, tp, tw, start_level=2):
                                     2 * pn[2] + ((o >> 2) & 1)]
                             if max(abs(cvec[a] - v[a]) for a in range(3)) <= 1:
                                 continue
-                            T = fmm_interaction(lmax, centre(cvec, l) - centre(v, l))
+                            key = (l, cvec[0] - v[0], cvec[1] - v[1], cvec[2] - v[2])
+                            if key not in tcache:
+                                tcache[key] = fmm_interaction(lmax, centre(cvec, l) - centre(v, l))
+                            T = tcache[key]
                             L[:, offs[l] + t] += T @ q[:, offs[l] + _box_index(cvec, l)]
     # local expansions on each box's own points
     for idx in range(nb ** 3):

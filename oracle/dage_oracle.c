@@ -551,62 +551,79 @@ int dage_oracle_transform_cube(const int di[3], const int dout[3], int ntp,
                                int add_coda) {
     const size_t nxi = di[0], nyi = di[1], nzi = di[2];
     const size_t nxo = dout[0], nyo = dout[1], nzo = dout[2];
-    double *tmp1 = (double *)malloc(sizeof(double) * nxo * nyi);
+    if (add_coda && coda != 0.0 && (nxi != nxo || nyi != nyo || nzi != nzo))
+        return -2; /* cube_project onto a different grid: not restated */
     double *tmpu = (double *)malloc(sizeof(double) * nxo * nyo * nzi);
-    double *m = (double *)malloc(sizeof(double) * nxo * nzo);
-    if (!tmp1 || !tmpu || !m) { free(tmp1); free(tmpu); free(m); return -1; }
+    if (!tmpu) return -1;
+    int fail = 0;
     memset(out, 0, sizeof(double) * nxo * nyo * nzo);
     for (int ip = t0; ip < t1 && ip < ntp; ++ip) {
         const double *Fx = fx + nxo * nxi * ip;
         const double *Fy = fy + nyi * nyo * ip;
         const double *Fz = fz + nzi * nzo * ip;
-        for (size_t iz = 0; iz < nzi; ++iz) {
-            const double *slice = in + nxi * nyi * iz;
-            memset(tmp1, 0, sizeof(double) * nxo * nyi);
-            for (size_t yi = 0; yi < nyi; ++yi)
-                for (size_t xi = 0; xi < nxi; ++xi) {
-                    double s = slice[xi + nxi * yi];
-                    const double *col = Fx + nxo * xi;
-                    double *dst = tmp1 + nxo * yi;
-                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += col[xo] * s;
-                }
-            double *tu = tmpu + nxo * nyo * iz;
-            memset(tu, 0, sizeof(double) * nxo * nyo);
-            for (size_t yo = 0; yo < nyo; ++yo)
-                for (size_t yi = 0; yi < nyi; ++yi) {
-                    double f = Fy[yi + nyi * yo];
-                    const double *src = tmp1 + nxo * yi;
-                    double *dst = tu + nxo * yo;
-                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
-                }
-        }
-        for (size_t iy = 0; iy < nyo; ++iy) {
-            memset(m, 0, sizeof(double) * nxo * nzo);
-            for (size_t zo = 0; zo < nzo; ++zo)
-                for (size_t zi = 0; zi < nzi; ++zi) {
-                    double f = Fz[zi + nzi * zo];
-                    const double *src = tmpu + nxo * (iy + nyo * zi);
-                    double *dst = m + nxo * zo;
-                    for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
-                }
-            for (size_t zo = 0; zo < nzo; ++zo) {
-                double *o = out + nxo * (iy + nyo * zo);
-                const double *src = m + nxo * zo;
-                for (size_t xo = 0; xo < nxo; ++xo) o[xo] = o[xo] + w[ip] * src[xo];
+        /* The z-slices (and below the output y-lines) are independent: with
+         * OpenMP each thread takes whole slices with its own temporaries, and
+         * every element is summed in the same order as the serial loop. */
+#pragma omp parallel reduction(| : fail)
+        {
+            double *tmp1 = (double *)malloc(sizeof(double) * nxo * nyi);
+            if (!tmp1) fail = 1;
+#pragma omp for schedule(static)
+            for (size_t iz = 0; iz < nzi; ++iz) {
+                if (!tmp1) continue;
+                const double *slice = in + nxi * nyi * iz;
+                memset(tmp1, 0, sizeof(double) * nxo * nyi);
+                for (size_t yi = 0; yi < nyi; ++yi)
+                    for (size_t xi = 0; xi < nxi; ++xi) {
+                        double s = slice[xi + nxi * yi];
+                        const double *col = Fx + nxo * xi;
+                        double *dst = tmp1 + nxo * yi;
+                        for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += col[xo] * s;
+                    }
+                double *tu = tmpu + nxo * nyo * iz;
+                memset(tu, 0, sizeof(double) * nxo * nyo);
+                for (size_t yo = 0; yo < nyo; ++yo)
+                    for (size_t yi = 0; yi < nyi; ++yi) {
+                        double f = Fy[yi + nyi * yo];
+                        const double *src = tmp1 + nxo * yi;
+                        double *dst = tu + nxo * yo;
+                        for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
+                    }
             }
+            free(tmp1);
         }
+        if (fail) break;
+#pragma omp parallel reduction(| : fail)
+        {
+            double *m = (double *)malloc(sizeof(double) * nxo * nzo);
+            if (!m) fail = 1;
+#pragma omp for schedule(static)
+            for (size_t iy = 0; iy < nyo; ++iy) {
+                if (!m) continue;
+                memset(m, 0, sizeof(double) * nxo * nzo);
+                for (size_t zo = 0; zo < nzo; ++zo)
+                    for (size_t zi = 0; zi < nzi; ++zi) {
+                        double f = Fz[zi + nzi * zo];
+                        const double *src = tmpu + nxo * (iy + nyo * zi);
+                        double *dst = m + nxo * zo;
+                        for (size_t xo = 0; xo < nxo; ++xo) dst[xo] += src[xo] * f;
+                    }
+                for (size_t zo = 0; zo < nzo; ++zo) {
+                    double *o = out + nxo * (iy + nyo * zo);
+                    const double *src = m + nxo * zo;
+                    for (size_t xo = 0; xo < nxo; ++xo) o[xo] = o[xo] + w[ip] * src[xo];
+                }
+            }
+            free(m);
+        }
+        if (fail) break;
     }
+    free(tmpu);
+    if (fail) return -1;
     if (add_coda && coda != 0.0) {
-        if (nxi != nxo || nyi != nyo || nzi != nzo) {
-            free(tmp1); free(tmpu); free(m);
-            return -2; /* cube_project onto a different grid: not restated */
-        }
         size_t n = nxo * nyo * nzo;
         for (size_t i = 0; i < n; ++i) out[i] = out[i] + coda * in[i];
     }
-    free(tmp1);
-    free(tmpu);
-    free(m);
     return 0;
 }
 
@@ -785,6 +802,9 @@ int dage_oracle_bubbles_eval_grid(int nbub, const double *centers, int lmax,
             }
         const double qmax = pts[npts - 1];
         const double cx = centers[3 * b], cy = centers[3 * b + 1], cz = centers[3 * b + 2];
+        /* points are independent: with OpenMP the z-planes are shared out,
+         * each point evaluated exactly as in the serial loop */
+#pragma omp parallel for schedule(dynamic, 1) private(S, polys)
         for (int kk = 0; kk < nz; ++kk)
             for (int jj = 0; jj < ny; ++jj)
                 for (int ii = 0; ii < nx; ++ii) {
@@ -1001,4 +1021,19 @@ void dage_oracle_eval_local_expansion(int lmax, const double *L, const double *x
                 for (int n = 0; n < N; ++n) v += S[n] * L[n];
                 out[i + (size_t)ld_x * (j + (size_t)ld_y * k)] += v;
             }
+}
+
+/* OpenMP threads of the loops above (no-op without OpenMP); returns the
+ * count in use. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int dage_oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

@@ -170,6 +170,11 @@ void dage_oracle_eval_local_expansion(int lmax, const double *L, const double *x
                                       const double *y, int ny, const double *z, int nz,
                                       const double centre[3], double *out, int ld_x, int ld_y);
 
+/* OpenMP threads of transform_cube and bubbles_eval_grid (their parallel
+ * loops give the serial loops' results bit for bit); returns the count in
+ * use (1 without OpenMP). */
+int dage_oracle_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -294,3 +294,50 @@ def test_apply_box_banded_matrices_pitch_tiles(ctx, monkeypatch):
     print(f"box-banded matrices vs oracle: {err:.3e}")
     assert err < 1e-12
     assert np.array_equal(got, grid16)
+
+
+def test_direct_z_pass_accumulates_over_t_chunks(ctx, monkeypatch):
+    """One right-hand side, no coda and a z pass without K split: the z pass
+    accumulates every t-chunk straight into the output (no partials, no
+    reduction).  A 1 MB workspace forces one t-point per chunk at 37^3 (40
+    accumulating chunks, K = 37, so no split); the result matches the oracle
+    and the split-K path (full workspace), with dense and with box-banded
+    (pitch-tiled) matrices."""
+    rng = np.random.default_rng(91)
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 7, 6)  # 37 points
+    n = ax.n
+    q = quad()
+    T = q.get_npoints()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    cube = np.asfortranarray(rng.standard_normal((n,) * 3))
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, q.tp)
+    box = np.minimum(np.arange(n) // 12, 2)
+    near = np.abs(box[:, None] - box[None, :]) <= 1
+    banded = []
+    for a in range(3):
+        F = rng.standard_normal((n, n, T))
+        F[~near, :] = 0.0
+        banded.append(np.asfortranarray(F))
+    bfx = banded[0]
+    bfy = np.asfortranarray(banded[1].transpose(1, 0, 2))
+    bfz = np.asfortranarray(banded[2].transpose(1, 0, 2))
+    for label, mats in (("coulomb", (fx, fy, fz)), ("box-banded", (bfx, bfy, bfz))):
+        ref = O.transform_cube(*mats, O.coulomb_weights(q.tw), 0.0, cube, add_coda=False)
+        monkeypatch.delenv("DAGE_APPLY_WORKSPACE_MB", raising=False)
+        op = dage_amd.Coulomb3D(ctx, g, g, q, coda=False)
+        op.set_matrices(*mats)
+        split = op.transform_cube(cube)
+        full_ws = op.workspace_bytes()
+        op.close()
+        monkeypatch.setenv("DAGE_APPLY_WORKSPACE_MB", "1")
+        op = dage_amd.Coulomb3D(ctx, g, g, q, coda=False)
+        op.set_matrices(*mats)
+        direct = op.transform_cube(cube)
+        # V, U of one t-point and one unsplit z-pass partial (unused: the
+        # z pass writes the output directly)
+        assert op.workspace_bytes() == 3 * 8 * n ** 3 < full_ws
+        op.close()
+        e_ref, e_split = rel_maxdiff(direct, ref), rel_maxdiff(direct, split)
+        print(f"{label}: direct z pass vs oracle {e_ref:.2e}, vs split-K path {e_split:.2e}")
+        assert e_ref < 1e-12
+        assert e_split < 1e-14

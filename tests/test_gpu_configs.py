@@ -1,0 +1,199 @@
+"""The configurations at their real shapes on the GPU (rows A9, A12-A17,
+A27 and the batched Helmholtz apply), where the oracle is too slow for whole
+cubes:
+
+* config 3: 32 Helmholtz right-hand sides, k^2 in [0.1, 4], cubes of
+  65/97/129/161 points at h 0.15 (nlip 5), T = 40, 4 Gaussians each (the
+  seeded inputs of the benchmark).  The FP64 batch graph equals 32 separate
+  FP64 applies (1e-13); one 161^3 right-hand side matches the oracle's
+  Operator3D_transform_cube (function3d.F90:4230-4349) on four output
+  x-rows (1e-10); the split-BF16 (bf16x6) batch is within 1e-7 of FP64.
+* config 4: the 10-atom, 12-orbital Helmholtz SCF on a 129^3 cube: one
+  batched Helmholtz step (scf_cycle.F90:2927-3054) equals 12 single
+  applies (1e-13), and after an iteration the Ritz-rotated orbitals are
+  orthonormal (overlap = I to 1e-12).
+* config 5: 385^3 (57 M points).  The GBFMM with 8x8x8 boxes and lmax 12
+  (gbfmm_coulomb3d.F90:650-821) is linear in the density to rounding and
+  matches the direct no-coda apply to the far-field truncation (1e-4); a
+  64-orbital Helmholtz sweep (k^2 in [0.05, 6], T = 40) as one batch graph
+  equals single applies on three sampled orbitals (1e-13).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import dage_amd
+import dage_amd.scf as scf
+import oracle as O
+from _helpers import rel_maxdiff
+
+pytestmark = pytest.mark.gpu
+
+
+def _centred_axis(n, h=0.15, nlip=5):
+    ncell = (n - 1) // (nlip - 1)
+    half = ncell * (nlip - 1) * h / 2
+    return O.Axis(-half, np.full(ncell, h), nlip)
+
+
+def test_config3_batch_full_shapes(ctx):
+    rng = np.random.default_rng(0xDA6E0003)
+    nrhs = 32
+    sizes = rng.choice([65, 97, 129, 161], nrhs)
+    k2 = rng.uniform(0.1, 4.0, nrhs)
+    energies = -0.5 * k2
+    gq = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
+    axes = {int(n): _centred_axis(int(n)) for n in set(sizes.tolist())}
+    grids = {n: dage_amd.Grid3D(ctx, [a.qmin] * 3, [a.steps] * 3, 5) for n, a in axes.items()}
+    rhos = []
+    for s in sizes:
+        ax = axes[int(s)]
+        a = rng.uniform(0.5, 5.0, 4)
+        c = rng.uniform(-2.0, 2.0, (4, 3))
+        rhos.append(O.gauss_density(np.ones(4), a, c, ax.points, ax.points, ax.points))
+    print(f"config 3: sizes {sizes.tolist()}", flush=True)
+    gl = [grids[int(s)] for s in sizes]
+    batch = dage_amd.HelmholtzBatch(ctx, gl, energies, gq)
+    for i, r in enumerate(rhos):
+        batch.upload(i, r)
+    batch.apply()
+    fp64 = [batch.download(i) for i in range(nrhs)]
+    batch.close()
+    ops = {n: dage_amd.Helmholtz3D(ctx, g, g, gq) for n, g in grids.items()}
+    worst = 0.0
+    for i, s in enumerate(sizes):
+        op = ops[int(s)]
+        op.set_energy(energies[i])
+        worst = max(worst, rel_maxdiff(fp64[i], op.transform_cube(rhos[i])))
+    print(f"config 3: FP64 batch vs 32 single applies {worst:.2e}", flush=True)
+    assert worst < 1e-13
+    # the first 161^3 right-hand side against the oracle on four x-rows
+    i161 = int(np.flatnonzero(sizes == 161)[0])
+    ax = axes[161]
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, gq.tp)
+    rows = [0, 40, 80, 123]
+    ref = O.transform_cube(np.asfortranarray(fx[rows]), fy, fz,
+                           O.helmholtz_weights(gq.tp, gq.tw, energies[i161]), 0.0, rhos[i161],
+                           add_coda=False)
+    ref += O.coda(500.0) * rhos[i161][rows]
+    err = rel_maxdiff(fp64[i161][rows], ref)
+    print(f"config 3: 161^3 right-hand side vs oracle on 4 x-rows {err:.2e}", flush=True)
+    assert err < 1e-10
+    mixed = dage_amd.HelmholtzBatch(ctx, gl, energies, gq, precision=dage_amd.DAGE_PREC_BF16X6)
+    for i, r in enumerate(rhos):
+        mixed.upload(i, r)
+    mixed.apply()
+    emix = max(rel_maxdiff(mixed.download(i), fp64[i]) for i in range(nrhs))
+    mixed.close()
+    print(f"config 3: bf16x6 batch vs FP64 {emix:.2e}", flush=True)
+    assert emix < 1e-7
+    for obj in list(ops.values()) + list(grids.values()):
+        obj.close()
+
+
+def test_config4_scf_step_full_shape(ctx):
+    grid = dage_amd.Grid3D(ctx, [-9.6] * 3, [np.full(32, 0.15)] * 3, 5)
+    mol = scf.random_molecule(np.random.default_rng(0), 10)
+    S = scf.HelmholtzSCF(ctx, grid, mol, nocc=12, beta=4.0)
+    S.initial_guess(exponents=mol.Z.copy())
+    S.build_potential()
+    S.helmholtz_step()
+    delta = [d.download() for d in S.delta]
+    psi = S.orbitals()
+    V = S.V.download()
+    op = dage_amd.Helmholtz3D(ctx, grid, grid, S.gauss)
+    worst = 0.0
+    for i in range(12):
+        e = float(S.eps[i])
+        rhs = V * psi[i]
+        if e > 0.0:  # the level shift of scf_cycle.F90:2972-2978
+            rhs = rhs - (e + 0.1) * psi[i]
+            e = -0.1
+        op.set_energy(e)
+        ref = -op.transform_cube(np.asfortranarray(rhs)) / (2.0 * math.pi) - psi[i]
+        worst = max(worst, rel_maxdiff(delta[i], ref))
+    op.close()
+    print(f"config 4: batched Helmholtz step vs 12 single applies {worst:.2e}", flush=True)
+    assert worst < 1e-13
+    r = S.run(max_iter=2)
+    assert r.iterations == 2
+    ov = S._gram(S.psi, S.psi)
+    err = np.max(np.abs(ov - np.eye(12)))
+    print(f"config 4: overlap of the rotated orbitals - I: {err:.2e}", flush=True)
+    assert err < 1e-12
+    S.close()
+    grid.close()
+
+
+def test_config5_gbfmm_and_sweep_full_shape(ctx):
+    rng = np.random.default_rng(0xDA6E0005)
+    ax = O.Axis.ncell_fixed(-19.2, 19.2, 7, 64)
+    n = ax.n
+    shape = (n, n, n)
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    q1 = np.ones(8)
+    a1 = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 8))
+    c1 = rng.uniform(-6.0, 6.0, (8, 3))
+    a2 = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 8))
+    c2 = rng.uniform(-6.0, 6.0, (8, 3))
+    d1 = dage_amd.DeviceCube(ctx, shape, O.gauss_density(q1, a1, c1, ax.points, ax.points,
+                                                         ax.points))
+    d2 = dage_amd.DeviceCube(ctx, shape, O.gauss_density(q1, a2, c2, ax.points, ax.points,
+                                                         ax.points))
+    dmix = dage_amd.DeviceCube(ctx, shape)
+    dpot = dage_amd.DeviceCube(ctx, shape)
+    npts = n ** 3
+
+    def combo(a, b, out):  # out = a*d1 + b*d2 on the device
+        out.copy_from(d2)
+        dage_amd._lib.call("dage_vec_axpby", ctx.handle, npts, float(a), ctypes.c_void_p(d1.ptr),
+                           float(b), ctypes.c_void_p(out.ptr))
+
+    gq = dage_amd.GaussQuad(25, 25)
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=3, lmax=12, gauss=gq)
+    fmm.apply_device(d1, dpot)
+    p1 = dpot.download()
+    fmm.apply_device(d2, dpot)
+    p2 = dpot.download()
+    combo(2.0, -0.5, dmix)
+    fmm.apply_device(dmix, dpot)
+    lin = rel_maxdiff(dpot.download(), 2.0 * p1 - 0.5 * p2)
+    fmm.close()
+    op = dage_amd.Coulomb3D(ctx, grid, grid, gq, coda=False)
+    op.transform_cuda_cube(d1, dpot)
+    far = rel_maxdiff(p1, dpot.download())
+    op.close()
+    del p2
+    print(f"config 5: GBFMM 8x8x8 linearity {lin:.2e}, vs direct apply {far:.2e}", flush=True)
+    assert lin < 1e-12
+    assert far < 1e-4
+    # 64-orbital Helmholtz sweep as one batch graph
+    nrhs = 64
+    k2 = rng.uniform(0.05, 6.0, nrhs)
+    energies = -0.5 * k2
+    hq = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
+    batch = dage_amd.HelmholtzBatch(ctx, [grid] * nrhs, energies, hq)
+    coef = rng.uniform(-1.0, 1.0, nrhs)
+    for i in range(nrhs):  # in_i = d1 + coef_i d2, built on the device
+        dst = ctypes.c_void_p(batch.input_ptr(i))
+        dage_amd._lib.call("dage_memcpy", ctx.handle, dst, ctypes.c_void_p(d2.ptr),
+                           ctypes.c_size_t(8 * npts), 2)
+        dage_amd._lib.call("dage_vec_axpby", ctx.handle, npts, 1.0, ctypes.c_void_p(d1.ptr),
+                           float(coef[i]), dst)
+    batch.apply()
+    hop = dage_amd.Helmholtz3D(ctx, grid, grid, hq)
+    worst = 0.0
+    for i in (0, 31, 63):
+        combo(1.0, coef[i], dmix)
+        hop.set_energy(energies[i])
+        hop.transform_cuda_cube(dmix, dpot)
+        worst = max(worst, rel_maxdiff(batch.download(i), dpot.download()))
+    hop.close()
+    batch.close()
+    print(f"config 5: 64-orbital batch vs single applies (orbitals 0, 31, 63) {worst:.2e}",
+          flush=True)
+    assert worst < 1e-13
+    for obj in (d1, d2, dmix, dpot, grid):
+        obj.close()

@@ -158,6 +158,13 @@ def test_gbfmm_rejects_bad_arguments(ctx):
         dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=1, lmax=4)
     with pytest.raises(dage_amd.DageError):
         dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=16)
+    # rho and pot may not overlap: refused before any kernel is launched
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=4)
+    d = dage_amd.DeviceCube(ctx, (ax.n, ax.n, ax.n))
+    with pytest.raises(dage_amd.DageError, match="overlap"):
+        fmm.apply_device(d, d)
+    d.close()
+    fmm.close()
 
 
 def test_gbfmm_mixed_near_field(ctx):

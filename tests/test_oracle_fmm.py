@@ -112,3 +112,85 @@ def test_library_matrices_match_oracle(lmax):
         assert np.max(np.abs(W - Wo)) <= 1e-13 * np.max(np.abs(Wo))
     with pytest.raises(dage_amd.DageError):
         dage_amd.interaction_matrix(lmax, [0.0, 0.0, 0.0])
+
+
+# ------------------------------------------------------------------------
+# The oracle's GBFMM pieces, pinned on the CPU without the HIP library.
+
+def test_box_multipoles_of_gaussians_are_solid_harmonics_at_centres():
+    """A spherically symmetric density about R integrates any harmonic
+    polynomial P to q P(R) (mean-value property), so the box multipoles
+    (integrator.cu:1122-1196) of a sum of normalised Gaussians are
+    sum_g q_g S_lm(R_g - c) exactly, up to the grid quadrature (here < 1e-7
+    on a 145^3 box grid with 6-point-wide cells of 0.25 bohr)."""
+    ax = O.Axis.ncell_fixed(-3.0, 3.0, 7, 24)
+    R = np.array([[0.2, -0.3, 0.15], [-0.25, 0.1, 0.3], [0.1, 0.35, -0.45]])
+    q = np.array([1.0, -0.7, 0.4])
+    rho = O.gauss_density(q, np.array([4.0, 6.0, 5.0]), R, ax.points, ax.points, ax.points)
+    c = np.array([0.1, 0.05, -0.2])
+    lmax = 12
+    got = O.box_multipoles(lmax, rho, [ax] * 3, c)
+    ref = sum(qg * O.solid_harmonics(lmax, *(Rg - c)) for qg, Rg in zip(q, R))
+    err = np.max(np.abs(got - ref))
+    print(f"box multipoles vs sum q S(R - c): {err:.2e}")
+    assert err < 1e-7
+    # a plausible slip (a dropped weight axis) is far outside that bar
+    bad = O.box_multipoles(lmax, rho, [ax, ax, O.Axis(ax.qmin, ax.steps * 1.01, 7)], c)
+    assert np.max(np.abs(bad - ref)) > 1e-4
+
+
+def test_local_expansion_of_a_point_charge_is_its_potential():
+    """The local expansion of a unit charge (its multipoles S(R - c_s) about
+    a far box, moved by the pinned interaction matrix) evaluated by the
+    oracle at points of the target box gives 1/|r - R| to the lmax-12
+    truncation, and equals the direct sum_lm L_lm S_lm(r - c_t)."""
+    lmax = 12
+    ct = np.array([0.3, -0.1, 0.2])
+    cs = np.array([4.0, 3.0, -3.5])
+    Rp = cs + np.array([0.3, -0.2, 0.4])
+    N = (lmax + 1) ** 2
+    e0 = np.zeros(N)
+    e0[0] = 1.0
+    qs = O.fmm_translate(lmax, O.fmm_translation_packed(lmax, Rp - cs), e0)
+    L = O.fmm_interaction(lmax, cs - ct) @ qs
+    x = ct[0] + np.linspace(-0.5, 0.5, 7)
+    y = ct[1] + np.linspace(-0.5, 0.5, 6)
+    z = ct[2] + np.linspace(-0.5, 0.5, 5)
+    V = O.eval_local_expansion(lmax, L, x, y, z, ct)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    exact = 1.0 / np.sqrt((X - Rp[0]) ** 2 + (Y - Rp[1]) ** 2 + (Z - Rp[2]) ** 2)
+    assert np.max(np.abs(V - exact)) < 1e-10 * np.max(exact)
+    direct = np.array([[[L @ O.solid_harmonics(lmax, xi - ct[0], yi - ct[1], zi - ct[2])
+                         for zi in z] for yi in y] for xi in x])
+    assert np.max(np.abs(V - direct)) < 1e-14 * np.max(np.abs(direct))
+
+
+def test_gbfmm_oracle_matches_direct_apply_and_analytic():
+    """The oracle's whole GBFMM (near field on sub-grids, multipoles, M2L,
+    local expansions; gbfmm_coulomb3d.F90:650-821) against the oracle's
+    direct whole-grid apply without coda (pinned by the reference's
+    gauss_test goldens) on a 49^3 grid with 4x4x4 boxes: they differ only by
+    the far-field multipole approximation, which shrinks with lmax (1e-3 at
+    lmax 4, 1.1e-5 at lmax 12, where the multipole quadrature on the coarse
+    grid sets the floor); both sit at the same grid error from the analytic
+    potential."""
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 7, 8)
+    rng = np.random.default_rng(3)
+    al = rng.uniform(1.0, 4.0, 5)
+    R = rng.uniform(-3.5, 3.5, (5, 3))
+    q = rng.uniform(0.5, 1.5, 5)
+    rho = O.gauss_density(q, al, R, ax.points, ax.points, ax.points)
+    tp, tw = O.gauss_quad(25, 25)
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, tp)
+    direct = O.transform_cube(fx, fy, fz, O.coulomb_weights(tw), 0.0, rho, add_coda=False)
+    exact = O.gauss_potential(q, al, R, ax.points, ax.points, ax.points)
+    errs = {}
+    for lmax in (4, 12):
+        pot, _, _ = O.gbfmm_potential([ax] * 3, rho, 2, lmax, tp, tw)
+        errs[lmax] = np.max(np.abs(pot - direct)) / np.max(np.abs(direct))
+    print(f"oracle GBFMM vs direct apply: lmax 4 {errs[4]:.2e}, lmax 12 {errs[12]:.2e}")
+    assert errs[12] < 3e-5
+    assert errs[4] > 30 * errs[12]
+    e_an = np.max(np.abs(pot - exact)) / np.max(exact)
+    e_dir = np.max(np.abs(direct - exact)) / np.max(exact)
+    assert e_an < 2e-3 and abs(e_an - e_dir) < 5e-5
