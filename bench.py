@@ -45,8 +45,8 @@ FLOP_PER_POINT_BUBBLE = 2 * 49 + 2 * 25 * 7 + 75 + 2 * 25
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-config1", dest="config1", action="store_false",
                    help="skip the config-1 apply measurement")
@@ -54,8 +54,10 @@ def parse():
                    help="skip the config-3 batched Helmholtz measurement")
     p.add_argument("--no-config4", dest="config4", action="store_false",
                    help="skip the config-4 SCF (plain vs learned) measurement")
-    p.add_argument("--config5", action="store_true",
-                   help="also measure config-5 shapes (385^3 GBFMM, 64-orbital sweep; ~1 min)")
+    p.add_argument("--no-config2", dest="config2", action="store_false",
+                   help="skip the config-2 GBFMM (241^3) measurement")
+    p.add_argument("--config2-cpu", action="store_true",
+                   help="also time the oracle's sample of a config-2 step")
     p.add_argument("--scf-max-iter", type=int, default=1500,
                    help="iteration cap of the timed plain and learned SCF runs")
     p.add_argument("--scf-train-iter", type=int, default=600,
@@ -76,6 +78,19 @@ def measured_traffic(workload: str, kernel: str):
     try:
         with open(path) as f:
             return json.load(f)["workloads"][workload]["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def measured_executed_fraction(workload: str, kernel: str):
+    """Fraction of the dense-equivalent FLOPs that `kernel` executes (the
+    rest are skipped all-zero panels), from the committed MFMA counter
+    summary (profiles/hbm_traffic.json: SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 per
+    dispatch over the dense count), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["workloads"][workload]["kernels"][kernel]["executed_fraction"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -163,15 +178,16 @@ def bench_config2(ctx, args):
         bub.inject(grid, d_tot)
         fmm.apply_device(d_tot, d_pot, precision)
 
-    for _ in range(args.warmup):
+    steps = max(args.steps, 10)  # a 17-ms step: time at least 10
+    for _ in range(max(args.warmup, 2)):
         step()
     ctx.synchronize()
     ctx.profile(True)
     wall0 = time.perf_counter()
     ctx.time_begin()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
-    ms = ctx.time_end() / args.steps
+    ms = ctx.time_end() / steps
     wall = time.perf_counter() - wall0
     names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
@@ -181,8 +197,8 @@ def bench_config2(ctx, args):
     for k in names:
         kms, kn = ctx.profile_query(k)
         if kn:
-            kernels[k] = {"ms_per_launch": round(kms / kn, 4), "launches_per_step": kn // args.steps,
-                          "ms_per_step": round(kms / args.steps, 4)}
+            kernels[k] = {"ms_per_launch": round(kms / kn, 4), "launches_per_step": kn // steps,
+                          "ms_per_step": round(kms / steps, 4)}
     ctx.profile(False)
     T = gq.get_npoints()
     # dominant kernel: largest time per step; GEMM passes do 2 T n^4 dense
@@ -207,9 +223,9 @@ def bench_config2(ctx, args):
     # split-BF16 near field in the same step
     step(dage_amd.DAGE_PREC_BF16X6)
     ctx.time_begin()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step(dage_amd.DAGE_PREC_BF16X6)
-    ms_mixed = ctx.time_end() / args.steps
+    ms_mixed = ctx.time_end() / steps
     pot_mixed = d_pot.download()
     step()
     pot = d_pot.download()
@@ -246,7 +262,7 @@ def bench_config2(ctx, args):
     # 61^3 each, all 50 t-points; the four boxes whose neighbourhoods start
     # at cell 0 or 10 along x and y) and its inject on 32 z-planes, scaled
     # by work to the whole step
-    if not args.no_cpu_baseline:
+    if args.config2_cpu and not args.no_cpu_baseline:
         s = 10
         sub_in = O.sub_axis(ax, 0, 3 * s)
         sub_out = O.sub_axis(ax, s, 2 * s)
@@ -546,19 +562,13 @@ def bench_config4(ctx, args):
 
 
 # ----------------------------------------------------------------- config 5
-def bench_config5(ctx, args):
-    """Config-5 shapes (opt-in, --config5): a 385^3 cube over [-19.2,19.2]^3
-    (nlip 7, 64 cells, h 0.1; 57.07 M points, 456.5 MB per cube).  One GBFMM
-    Coulomb apply with 8x8x8 boxes (lmax 12, near field T = 50), FP64 and
-    bf16x6, checked against the analytic potential of 8 Gaussians on 8
-    z-planes; one Helmholtz sweep over 64 orbitals (k^2 uniform in
-    [0.05, 6], T = 40) as one batch graph in FP64 and bf16x6 against 64
-    plain FP64 applies.  Not the full config-5 SCF (hours on one GPU)."""
+def config5_setup(ctx):
+    """The 385^3 combined workload's inputs, resident on the GPU."""
     import numpy as np
 
     import dage_amd
+    import dage_amd.scf as scf
     import oracle as O
-    from dage_amd._lib import call
     rng = np.random.default_rng(0xDA6E0005)
     ax = O.Axis.ncell_fixed(-19.2, 19.2, 7, 64)
     n = ax.n
@@ -567,71 +577,307 @@ def bench_config5(ctx, args):
     alpha = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 8))
     centers = rng.uniform(-6.0, 6.0, (8, 3))
     rho = O.gauss_density(q, alpha, centers, ax.points, ax.points, ax.points)
+    alpha2 = np.exp(rng.uniform(np.log(0.3), np.log(30.0), 8))
+    centers2 = rng.uniform(-6.0, 6.0, (8, 3))
     d_rho = dage_amd.DeviceCube(ctx, rho.shape, rho)
     del rho
-    d_pot = dage_amd.DeviceCube(ctx, (n, n, n))
-    out = {"workload": "config-5 shapes: 385^3 over [-19.2,19.2]^3 (nlip 7, h 0.1); GBFMM 8x8x8 "
-                       "boxes lmax 12 T=50; 64-orbital Helmholtz sweep k^2 U[0.05,6] T=40",
-           "n": n}
-    gq = dage_amd.GaussQuad(25, 25)
-    c0 = time.perf_counter()
-    fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=3, lmax=12, gauss=gq)
-    out["gbfmm_setup_s"] = round(time.perf_counter() - c0, 2)
-    pots = {}
-    for label, prec in (("fp64", dage_amd.DAGE_PREC_FP64), ("bf16x6", dage_amd.DAGE_PREC_BF16X6)):
-        fmm.apply_device(d_rho, d_pot, prec)
-        ctx.time_begin()
-        for _ in range(2):
-            fmm.apply_device(d_rho, d_pot, prec)
-        ms = ctx.time_end() / 2
-        pots[label] = d_pot.download()
-        out[f"gbfmm_{label}_ms"] = round(ms, 2)
-        out[f"gbfmm_{label}_Gpts_s"] = round(n ** 3 / (ms * 1e-3) / 1e9, 4)
-    zs = ax.points[n // 2 - 4:n // 2 + 4]
-    exact = O.gauss_potential(q, alpha, centers, ax.points, ax.points, zs)
-    out["gbfmm_vs_analytic_8_planes"] = rel(pots["fp64"][:, :, n // 2 - 4:n // 2 + 4], exact)
-    out["gbfmm_bf16x6_vs_fp64"] = rel(pots["bf16x6"], pots["fp64"])
-    del pots, exact
-    fmm.close()
-    progress("config5: GBFMM done")
-    # 64-orbital Helmholtz sweep
+    d_rho2 = dage_amd.DeviceCube(
+        ctx, (n, n, n), O.gauss_density(q, alpha2, centers2, ax.points, ax.points, ax.points))
+    # 24 atoms from the molecule generator (Z in {1,6,7,8}, >= 1.8 bohr apart
+    # in a 6-bohr sphere), bubbles lmax 4 on 199-point radial grids
+    mol = scf.random_molecule(rng, 24)
+    lmax = 4
+    bub = dage_amd.Bubbles(ctx, 24, lmax, 0, 7)
+    steps, bfs = [], []
+    for a in range(24):
+        st = dage_amd.radial_grid_steps(mol.Z[a], 33, 7, 20.0)
+        r = O.Axis(0.0, st, 7).points
+        c = rng.uniform(-1.0, 1.0, (lmax + 1) ** 2)
+        bf = np.zeros((len(r), (lmax + 1) ** 2), order="F")
+        for l in range(lmax + 1):
+            for m in range(-l, l + 1):
+                bf[:, l * l + l + m] = c[l * l + l + m] * r ** l * np.exp(-mol.Z[a] * r)
+        bub.set_bubble(a, mol.R[a], mol.Z[a], st, bf)
+        steps.append(st)
+        bfs.append(bf)
+    bset = O.BubbleSet(mol.R, steps, bfs, lmax, 0, 7)
     nrhs = 64
     k2 = rng.uniform(0.05, 6.0, nrhs)
-    energies = -0.5 * k2
+    coef = rng.uniform(-1.0, 1.0, nrhs)
+    return dict(ax=ax, n=n, grid=grid, gauss=(q, alpha, centers), d_rho=d_rho, d_rho2=d_rho2,
+                mol=mol, bub=bub, bset=bset, energies=-0.5 * k2, coef=coef, nrhs=nrhs)
+
+
+def fill_orbitals(ctx, S, batch):
+    """Orbital i's right-hand side = rho + coef_i rho2, built on the device."""
+    from dage_amd._lib import call
+    npts = S["n"] ** 3
+    for i in range(S["nrhs"]):
+        dst = ctypes.c_void_p(batch.input_ptr(i))
+        call("dage_memcpy", ctx.handle, dst, ctypes.c_void_p(S["d_rho2"].ptr),
+             ctypes.c_size_t(8 * npts), 2)
+        call("dage_vec_axpby", ctx.handle, npts, 1.0, ctypes.c_void_p(S["d_rho"].ptr),
+             float(S["coef"][i]), dst)
+
+
+def bench_config5(ctx, args):
+    """Headline: one step of the 385^3 combined workload = the resident
+    density (8 Gaussians) plus 24 atoms' bubbles injected, its Coulomb
+    potential by GBFMM (8x8x8 boxes, lmax 12, near field T = 50), and one
+    Helmholtz sweep over 64 orbitals (k^2 uniform in [0.05, 6], T = 40) as
+    one batch, all FP64 on the GPU; `value` = output points of the step
+    (65 x 385^3) per second."""
+    import numpy as np
+
+    import dage_amd
+    import oracle as O
+    S = config5_setup(ctx)
+    n, grid, nrhs = S["n"], S["grid"], S["nrhs"]
+    gq = dage_amd.GaussQuad(25, 25)
     hq = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
-    T = hq.get_npoints()
-    flops = 6.0 * T * float(n) ** 4 * nrhs
-    op = dage_amd.Helmholtz3D(ctx, grid, grid, hq)
-    op.set_energy(energies[0])
-    op.transform_cuda_cube(d_rho, d_pot)
+    c0 = time.perf_counter()
+    fmm = dage_amd.GBFMMCoulomb3D(ctx, grid, maxlevel=3, lmax=12, gauss=gq)
+    batch = dage_amd.HelmholtzBatch(ctx, [grid] * nrhs, S["energies"], hq)
+    setup_s = time.perf_counter() - c0
+    fill_orbitals(ctx, S, batch)
+    d_tot = dage_amd.DeviceCube(ctx, (n, n, n))
+    d_pot = dage_amd.DeviceCube(ctx, (n, n, n))
+    progress(f"config5: setup {setup_s:.1f} s")
+
+    def step(eager=False):
+        d_tot.copy_from(S["d_rho"])
+        S["bub"].inject(grid, d_tot)
+        fmm.apply_device(d_tot, d_pot)
+        if eager:
+            batch.apply_eager()
+        else:
+            batch.apply()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    # timed region: the library's eager launches with per-kernel HIP events
+    # on the context stream (the kernels' own durations for the roofline)
+    ctx.profile(True)
+    wall0 = time.perf_counter()
+    ctx.time_begin()
+    for _ in range(args.steps):
+        step(eager=True)
+    ms = ctx.time_end() / args.steps
+    wall = time.perf_counter() - wall0
+    names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+             "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
+             "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
+             "local_expansion_gemm", "cube_copy_kernel")
+    kernels = {}
+    for k in names:
+        kms, kn = ctx.profile_query(k)
+        if kn:
+            kernels[k] = {"ms_per_launch": round(kms / kn, 4), "launches_per_step": kn // args.steps,
+                          "ms_per_step": round(kms / args.steps, 4)}
+    ctx.profile(False)
+    # the same step with the batch as one graph launch
+    ctx.time_begin()
+    for _ in range(args.steps):
+        step()
+    ms_graph = ctx.time_end() / args.steps
+    progress(f"config5: {ms:.0f} ms per step (graph {ms_graph:.0f} ms)")
+    T, Tn = hq.get_npoints(), gq.get_npoints()
+    # dominant kernel of the step; a mode pass does 2 T n^4 dense FLOPs per
+    # right-hand side (near field: per apply), counted dense although the
+    # all-zero K panels of the banded matrices are skipped
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dom]
+    if dom.startswith("gemm_f64"):
+        per_step = 2.0 * float(n) ** 4 * (T * nrhs + Tn)  # sweep + near field, per pass
+        per_launch = per_step / dk["launches_per_step"]
+        achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
+                    "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                    "traffic": measured_traffic("config5", dom),
+                    "flops": "dense-equivalent (2 T n^4 per pass and right-hand side)",
+                    "avg_launch_ms": dk["ms_per_launch"],
+                    "algorithmic_flops_per_launch": per_launch}
+        exe = measured_executed_fraction("config5", dom)
+        if exe is not None:
+            roofline["executed_fraction"] = exe
+            roofline["frac_executed"] = round(exe * achieved / FP64_MFMA_PEAK_TFLOPS, 4)
+    else:
+        per_launch = 16.0 * n ** 3
+        achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic("config5", dom),
+                    "avg_launch_ms": dk["ms_per_launch"], "algorithmic_bytes_per_launch": per_launch}
+    out = {"ms_per_step": ms, "ms_per_step_graph": round(ms_graph, 2), "wall_s": wall, "n": n,
+           "T_sweep": T, "T_near": Tn, "points_per_step": (nrhs + 1) * n ** 3,
+           "kernels": kernels, "roofline": roofline, "setup_s": round(setup_s, 2)}
+    # parity on the GPU results of the step
+    inj = S["bset"]
+    d_tot.set_to_zero()
+    S["bub"].inject(grid, d_tot)
+    z0 = n // 2
+    got = d_tot.download()[:, :, z0:z0 + 2]
+    ax = S["ax"]
+    out["inject_vs_oracle_2_planes"] = rel(got, O.bubbles_eval_grid(inj, ax.points, ax.points,
+                                                                    ax.points[z0:z0 + 2]))
+    fmm.apply_device(S["d_rho"], d_pot)
+    q, alpha, centers = S["gauss"]
+    zs = ax.points[z0 - 4:z0 + 4]
+    pot_fp64 = d_pot.download()
+    out["gbfmm_vs_analytic_8_planes"] = rel(pot_fp64[:, :, z0 - 4:z0 + 4],
+                                            O.gauss_potential(q, alpha, centers, ax.points,
+                                                              ax.points, zs))
+    hop = dage_amd.Helmholtz3D(ctx, grid, grid, hq)
+    d_mix = dage_amd.DeviceCube(ctx, (n, n, n))
+    i_s = nrhs - 1
+    d_mix.copy_from(S["d_rho2"])
+    from dage_amd._lib import call
+    call("dage_vec_axpby", ctx.handle, n ** 3, 1.0, ctypes.c_void_p(S["d_rho"].ptr),
+         float(S["coef"][i_s]), ctypes.c_void_p(d_mix.ptr))
+    hop.set_energy(S["energies"][i_s])
+    hop.transform_cuda_cube(d_mix, d_pot)
+    single = d_pot.download()
+    sweep_fp64 = batch.download(i_s)
+    out["sweep_vs_single_apply_orbital_63"] = rel(sweep_fp64, single)
+    # plain per-orbital loop (the reference's order: set_energy + apply per
+    # orbital) for the batching gain
     ctx.time_begin()
     for i in range(nrhs):
-        op.set_energy(energies[i])
-        op.transform_cuda_cube(d_rho, d_pot)
+        hop.set_energy(S["energies"][i])
+        hop.transform_cuda_cube(S["d_rho"], d_pot)
     plain_ms = ctx.time_end()
-    last_plain = d_pot.download()
-    op.close()
-    out["sweep_plain_fp64_ms"] = round(plain_ms, 1)
-    sweep = {}
-    for label, prec in (("fp64", dage_amd.DAGE_PREC_FP64), ("bf16x6", dage_amd.DAGE_PREC_BF16X6)):
-        b = dage_amd.HelmholtzBatch(ctx, [grid] * nrhs, energies, hq, precision=prec)
-        for i in range(nrhs):
-            call("dage_memcpy", ctx.handle, ctypes.c_void_p(b.input_ptr(i)),
-                 ctypes.c_void_p(d_rho.ptr), ctypes.c_size_t(8 * n ** 3), 2)
-        b.apply()
-        ctx.time_begin()
-        b.apply()
-        ms = ctx.time_end()
-        sweep[label] = {"ms_per_sweep": round(ms, 1),
-                        "dense_fp64_equiv_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
-                        "speedup_vs_plain_fp64": round(plain_ms / ms, 3),
-                        "last_orbital_vs_plain": rel(b.download(nrhs - 1), last_plain)}
-        b.close()
-        progress(f"config5: {label} sweep {ms:.0f} ms")
-    out["sweep_batched"] = sweep
-    for obj in (d_rho, d_pot, grid):
+    batch.apply()
+    ctx.time_begin()
+    batch.apply()
+    sweep_ms = ctx.time_end()
+    hop.close()
+    batch.close()
+    progress("config5: FP64 step and parity done")
+    # split-BF16 (bf16x6) near field and sweep, error against FP64
+    fmm.apply_device(S["d_rho"], d_pot, dage_amd.DAGE_PREC_BF16X6)
+    ctx.time_begin()
+    fmm.apply_device(S["d_rho"], d_pot, dage_amd.DAGE_PREC_BF16X6)
+    gbfmm_mixed_ms = ctx.time_end()
+    gbfmm_mixed_err = rel(d_pot.download(), pot_fp64)
+    ctx.time_begin()
+    fmm.apply_device(S["d_rho"], d_pot)
+    gbfmm_ms = ctx.time_end()
+    del pot_fp64
+    mb = dage_amd.HelmholtzBatch(ctx, [grid] * nrhs, S["energies"], hq,
+                                 precision=dage_amd.DAGE_PREC_BF16X6)
+    fill_orbitals(ctx, S, mb)
+    mb.apply()
+    ctx.time_begin()
+    mb.apply()
+    sweep_mixed_ms = ctx.time_end()
+    sweep_mixed_err = rel(mb.download(i_s), sweep_fp64)
+    mb.close()
+    flops_sweep = 6.0 * T * float(n) ** 4 * nrhs
+    out["gbfmm"] = {"fp64_ms": round(gbfmm_ms, 2), "bf16x6_ms": round(gbfmm_mixed_ms, 2),
+                    "bf16x6_vs_fp64": gbfmm_mixed_err}
+    out["sweep_64"] = {
+        "plain_fp64_per_orbital_ms": round(plain_ms, 1),
+        "batched_fp64_ms": round(sweep_ms, 1),
+        "batched_bf16x6_ms": round(sweep_mixed_ms, 1),
+        "batched_fp64_dense_equiv_tflops": round(flops_sweep / (sweep_ms * 1e-3) / 1e12, 2),
+        "speedup_fp64_batch_vs_plain": round(plain_ms / sweep_ms, 3),
+        "speedup_bf16x6_batch_vs_plain_fp64": round(plain_ms / sweep_mixed_ms, 3),
+        "bf16x6_vs_fp64_orbital_63": sweep_mixed_err}
+    progress("config5: split-BF16 legs done")
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_config5(S, gq, hq)
+    fmm.close()
+    for obj in (d_tot, d_pot, d_mix, S["d_rho"], S["d_rho2"], S["bub"], grid):
         obj.close()
     return out
+
+
+def cpu_threads():
+    """Host threads for the all-core CPU baseline: OMP_NUM_THREADS when set
+    (the GPU host's CPU share), else the cores this process may use."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_config5(S, gq, hq):
+    """The oracle (C restatement of Operator3D_transform_cube, the GBFMM near
+    field and Bubbles_eval_3dgrid) on bounded samples of one config-5 step,
+    at 1 thread and at all threads (OpenMP over independent slices, same
+    results), scaled by FLOPs / points to the whole step."""
+    import numpy as np
+
+    import oracle as O
+    ax, n = S["ax"], S["n"]
+    T, Tn = hq.get_npoints(), gq.get_npoints()
+    # (a) one orbital's Helmholtz apply on 8 input z-planes and 10 of 40
+    # t-points: 6 n^3 * 8 * 10 FLOPs of the 6 n^4 * 40 per orbital
+    zs = slice(n // 2 - 4, n // 2 + 4)
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, hq.tp)
+    rho = O.gauss_density(*S["gauss"], ax.points, ax.points, ax.points[zs])
+    fz_s = np.asfortranarray(fz[zs])
+    w = O.helmholtz_weights(hq.tp, hq.tw, S["energies"][0])
+    # (b) the near field of one interior box (145^3 -> 49^3, all 50 t-points)
+    s = 8
+    sub_in = O.sub_axis(ax, 2 * s, 5 * s)
+    sub_out = O.sub_axis(ax, 3 * s, 4 * s)
+    nmats = O.coulomb3d_matrices([sub_in] * 3, [sub_out] * 3, gq.tp)
+    m = sub_in.n
+    o0 = 2 * s * 6
+    nf = np.asfortranarray(O.gauss_density(*S["gauss"], ax.points[o0:o0 + m], ax.points[o0:o0 + m],
+                                           ax.points[o0:o0 + m]))
+    wn = O.coulomb_weights(gq.tw)
+
+    def box_flops(nx, ny, nz, no=49):
+        return 2 * Tn * (no * nx * ny * nz + no * no * ny * nz + no ** 3 * nz)
+    nfs = [97] + [145] * 6 + [97]
+    near_total = sum(box_flops(nfs[x], nfs[y], nfs[z])
+                     for x in range(8) for y in range(8) for z in range(8))
+    res = {}
+    for threads in (1, cpu_threads()):
+        O.set_threads(threads)
+        c0 = time.perf_counter()
+        O.transform_cube(fx, fy, fz_s, w, 0.0, rho, t0=0, t1=10, add_coda=False)
+        t_apply = time.perf_counter() - c0
+        c0 = time.perf_counter()
+        O.transform_cube(*nmats, wn, 0.0, nf, add_coda=False)
+        t_box = time.perf_counter() - c0
+        c0 = time.perf_counter()
+        O.bubbles_eval_grid(S["bset"], ax.points, ax.points, ax.points[zs][:2])
+        t_inj = time.perf_counter() - c0
+        est = (t_apply * (n / 8) * (T / 10) * S["nrhs"]
+               + t_box * near_total / box_flops(145, 145, 145) + t_inj * n / 2)
+        res[threads] = (est, t_apply + t_box + t_inj, t_apply, t_box, t_inj)
+    O.set_threads(cpu_threads())
+    pts = (S["nrhs"] + 1) * n ** 3
+    nt = cpu_threads()
+    est1, sample1 = res[1][0], res[1][1]
+    estn, samplen = res[nt][0], res[nt][1]
+    return {
+        "value": round(pts / estn / 1e9, 8), "unit": "Gpts/s", "cores": nt, "kind": "port",
+        "value_1_core": round(pts / est1 / 1e9, 8),
+        "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+        "seconds_per_step": {"1": round(est1, 1), str(nt): round(estn, 1)},
+        "sample": f"oracle C restatement on the GPU host: one orbital's Helmholtz apply on 8 of "
+                  f"{n} input z-planes and 10 of {T} t-points (scaled x{n // 8 * T // 10} and x64 "
+                  f"orbitals), the near field of 1 of 512 boxes (145^3 -> 49^3, T={Tn}, scaled by "
+                  f"FLOPs), the 24-atom inject on 2 of {n} z-planes; multipoles, translations and "
+                  f"local expansions (about 1 % of the GPU step) not included; sample "
+                  f"{sample1:.1f} s at 1 thread, {samplen:.1f} s at {nt} threads"}
 
 
 def main():
@@ -645,39 +891,47 @@ def main():
         return
     import dage_amd
     ctx = dage_amd.Context(0)
-    c2 = bench_config2(ctx, args)
-    progress(f"config2: {c2['ms_per_step']:.2f} ms per step")
-    n = c2["n"]
+    c5 = bench_config5(ctx, args)
+    progress(f"config5: {c5['ms_per_step']:.1f} ms per step")
+    n = c5["n"]
     line = {
         "metric": METRIC,
-        "value": round(n ** 3 / (c2["ms_per_step"] * 1e-3) / 1e9, 4),
+        "value": round(c5["points_per_step"] / (c5["ms_per_step"] * 1e-3) / 1e9, 4),
         "unit": "Gpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(c2["ms_per_step"], 4),
+        "ms_per_step": round(c5["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "config 2: 241^3 cube over [-12,12]^3 (nlip 7, 40 cells); 16 Gaussians "
-                        "+ 4 atoms' bubbles (lmax 4, 199-point radial grids) injected, then the "
-                        "GBFMM Coulomb potential (4x4x4 boxes, lmax 12, near field T=50), FP64",
-            "n": n, "T": c2["T"], "maxlevel": 2, "lmax": 12, "atoms": 4,
+            "workload": "config 5: 385^3 cube over [-19.2,19.2]^3 (nlip 7, 64 cells, 57.07 M "
+                        "points); per step: 8 Gaussians + 24 atoms' bubbles (lmax 4, 199-point "
+                        "radial grids) injected, their Coulomb potential by GBFMM (8x8x8 boxes, "
+                        "lmax 12, near field T=50), and one Helmholtz sweep over 64 orbitals "
+                        "(k^2 U[0.05,6], T=40, one batch); FP64; value = 65 x 385^3 output "
+                        "points per second",
+            "n": n, "orbitals": 64, "atoms": 24, "maxlevel": 3, "lmax": 12,
+            "T_sweep": c5["T_sweep"], "T_near": c5["T_near"],
         },
-        "roofline": c2["roofline"],
-        "cpu_baseline": c2.get("cpu_baseline"),
-        "kernels": c2["kernels"],
-        "bf16x6_nearfield": c2["bf16x6_nearfield"],
-        "gbfmm_vs_analytic": c2["gbfmm_vs_analytic"],
-        "gbfmm_vs_direct_apply": c2["gbfmm_vs_direct_apply"],
-        "direct_apply_ms": c2["direct_apply_ms"],
-        "inject_vs_oracle_8_planes": c2["inject_vs_oracle_8_planes"],
-        "inject_valu_fp64_tflops": c2["inject_valu_fp64_tflops"],
-        "wall_s_timed": round(c2["wall_s"], 4),
+        "roofline": c5["roofline"],
+        "cpu_baseline": c5.get("cpu_baseline"),
+        "kernels": c5["kernels"],
+        "ms_per_step_graph": c5["ms_per_step_graph"],
+        "inject_vs_oracle_2_planes": c5["inject_vs_oracle_2_planes"],
+        "gbfmm_vs_analytic_8_planes": c5["gbfmm_vs_analytic_8_planes"],
+        "sweep_vs_single_apply_orbital_63": c5["sweep_vs_single_apply_orbital_63"],
+        "gbfmm": c5["gbfmm"],
+        "sweep_64": c5["sweep_64"],
+        "setup_s": c5["setup_s"],
+        "wall_s_timed": round(c5["wall_s"], 3),
     }
+    if args.config2:
+        line["config2"] = bench_config2(ctx, args)
+        progress(f"config2: {line['config2']['ms_per_step']:.2f} ms per step")
     if args.config1:
         line["config1"] = bench_config1(ctx, args)
         progress("config1 done")
@@ -686,8 +940,6 @@ def main():
         progress("config3 done")
     if args.config4:
         line["config4"] = bench_config4(ctx, args)
-    if args.config5:
-        line["config5"] = bench_config5(ctx, args)
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -76,6 +76,24 @@ _SIGNATURES = {
     "dage_bubbles_set": [_c_p, _int, _dp, _dbl, _int, _dp, _dp],
     "dage_bubbles_inject_device": [_c_p, _c_p, _int, _dbl, _c_p],
     "dage_bubbles_destroy": [_c_p],
+    "dage_bubbles_create_like": [_c_p, _int, _int, ctypes.POINTER(_c_p)],
+    "dage_bubbles_info": [_c_p, _ip, _ip, _ip, _ip],
+    "dage_bubbles_npoints": [_c_p, _int, _ip],
+    "dage_bubbles_get": [_c_p, _int, _dp, _dp, _dp, _dp],
+    "dage_bubbles_axpby": [_dbl, _c_p, _dbl, _c_p],
+    "dage_bubbles_coulomb_transform": [_c_p, _c_p],
+    "dage_bubbles_helmholtz_transform": [_c_p, _dbl, _c_p],
+    "dage_bubbles_copy_lmax": [_c_p, _int, ctypes.POINTER(_c_p)],
+    "dage_function3d_inject_extra_bubbles": [_c_p, _c_p, _c_p, _int, ctypes.POINTER(_c_p)],
+    "dage_operator_operate_on": [_c_p, _c_p, _c_p, _c_p, _c_p, _int],
+    "dage_bubbles_multiply": [_c_p, _c_p, _c_p],
+    "dage_function3d_multiply": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "dage_fun_write": [ctypes.c_char_p, ctypes.c_char_p, _int, _ip, _dp, _int, _int, _int, _dp,
+                       _dp, _ip, ctypes.POINTER(_dp)],
+    "dage_fun_read_info": [ctypes.c_char_p, ctypes.c_char_p, _ip, _ip, _ip, _ip, _ip],
+    "dage_fun_read": [ctypes.c_char_p, _dp, _dp, _dp, _ip, ctypes.POINTER(_dp)],
+    "dage_g3d_write": [ctypes.c_char_p, _int, _int, _ip, _dp, ctypes.POINTER(_dp)],
+    "dage_g3d_read": [ctypes.c_char_p, _ip, _ip, _ip, _dp, ctypes.POINTER(_dp)],
     "dage_radial_grid_steps": [_dbl, _int, _int, _dbl, _dp],
     "dage_gbfmm_create": [_c_p, _c_p, _int, _int, _dp, _dp, _int, ctypes.POINTER(_c_p)],
     "dage_gbfmm_apply_device": [_c_p, _c_p, _c_p, _int],

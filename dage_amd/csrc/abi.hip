@@ -561,6 +561,8 @@ int dage_operator_set_energy(dage_operator op, double energy) {
         if (energy < -0.005) wi = wi * std::exp(2.0 * energy / (4.0 * (op->tp[i] * op->tp[i])));
         op->w[i] = c * wi;
     }
+    op->helmholtz = true;
+    op->energy = energy;
     return upload_weights(op);
 }
 
@@ -708,51 +710,26 @@ int dage_bubbles_set(dage_bubbles b, int ibub, const double center[3], double ch
     for (int c = 0; c < ncell; ++c)
         DAGE_REQUIRE(steps[c] > 0.0 && std::isfinite(steps[c]), DAGE_ERR_ARG,
                      "radial step %d is not positive", c);
+    DAGE_TRY(bubble_alloc_host(b, ibub, center, charge, ax));
     const int npts = ax.n();
-    const int nlm = (b->lmax + 1) * (b->lmax + 1);
+    const size_t nv = (size_t)npts * b->nlm();
     // r^k folded into the radial values on the grid (Bubbles_rpow,
-    // bubbles.F90:3363-3378: r(1)^k = 0 for k < 0)
-    std::vector<double> buf(2 * (size_t)ncell + (size_t)npts * nlm);
-    std::memcpy(buf.data(), ax.celld.data(), sizeof(double) * ncell);
-    std::memcpy(buf.data() + ncell, steps, sizeof(double) * ncell);
-    double *f = buf.data() + 2 * ncell;
-    for (int lm = 0; lm < nlm; ++lm)
+    // bubbles.F90:3363-3378: r(1)^k = 0 for k < 0); the raw values are kept
+    // beside them when k != 0 (the radial transforms and products use them)
+    std::vector<double> fk(nv);
+    for (int lm = 0; lm < b->nlm(); ++lm)
         for (int i = 0; i < npts; ++i) {
             const double v = bf[i + (size_t)npts * lm];
             double rp = 1.0;
             if (b->k > 0) rp = std::pow(ax.points[i], b->k);
             else if (b->k < 0) rp = (i == 0) ? 0.0 : std::pow(ax.points[i], b->k);
-            f[i + (size_t)npts * lm] = (b->k == 0) ? v : v * rp;
+            fk[i + (size_t)npts * lm] = (b->k == 0) ? v : v * rp;
         }
-    // cell of r_k = k*qmax/kCellTab: 1 + the number of cell starts 1..ncell-1
-    // at or below r_k (the cell the bisection of grid.F90:1212-1246 finds);
-    // the kernel starts from the entry one bin below r and steps upward
-    const double qmax = ax.points[npts - 1];
-    std::vector<int> tab(kCellTab);
-    for (int k = 0, ic = 1; k < kCellTab; ++k) {
-        const double rk = k * qmax / kCellTab;
-        while (ic < ncell && ax.celld[ic] <= rk) ++ic;
-        tab[k] = ic;
-    }
     BubbleHost &h = b->host[ibub];
-    DAGE_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
-    if (h.d_buf) DAGE_HIP_CHECK(hipFree(h.d_buf));
-    if (h.d_tab) DAGE_HIP_CHECK(hipFree(h.d_tab));
-    h.d_buf = nullptr;
-    h.d_tab = nullptr;
-    h.set = false;
-    DAGE_HIP_CHECK(hipMalloc((void **)&h.d_buf, sizeof(double) * buf.size()));
-    DAGE_HIP_CHECK(hipMemcpy(h.d_buf, buf.data(), sizeof(double) * buf.size(),
-                             hipMemcpyHostToDevice));
-    DAGE_HIP_CHECK(hipMalloc((void **)&h.d_tab, sizeof(int) * tab.size()));
-    DAGE_HIP_CHECK(hipMemcpy(h.d_tab, tab.data(), sizeof(int) * tab.size(),
-                             hipMemcpyHostToDevice));
-    for (int a = 0; a < 3; ++a) h.center[a] = center[a];
-    h.charge = charge;
-    h.ncell = ncell;
-    h.qmax = qmax;
+    DAGE_HIP_CHECK(hipMemcpy(h.fk(), fk.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+    if (b->k != 0)
+        DAGE_HIP_CHECK(hipMemcpy(b->f(ibub), bf, sizeof(double) * nv, hipMemcpyHostToDevice));
     h.set = true;
-    b->dirty = true;
     return DAGE_OK;
 }
 

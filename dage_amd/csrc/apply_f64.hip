@@ -420,28 +420,47 @@ struct Plan {
     size_t v_elems, u_elems, p_elems;
 };
 
-Plan make_plan(const ApplyMats &m, int B = 1) {
+// cap > 0: the whole workspace must fit cap bytes (the largest t-chunk, then
+// the largest K split, that does); tchunk = 0 when nothing fits.
+Plan make_plan(const ApplyMats &m, int B = 1, size_t cap = 0) {
     const long nyi = m.din[1], nzi = m.din[2];
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
     size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
     const size_t total_cap = 64ull << 30;  // and for all B right-hand sides together
     const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
-    Plan p;
-    p.tchunk = (int)std::max<size_t>(
-        1, std::min<size_t>({(size_t)m.ntp, budget / per_t, total_cap / (per_t * (size_t)B)}));
     const long M3 = nxo * nyo;
     const int mtiles = (int)((M3 + 64 * kMW - 1) / (64 * kMW));
     const int nb = pick_nb((int)nzo);
     const int ntiles = (int)((nzo + 16 * nb - 1) / (16 * nb));
-    const long K3 = nzi * p.tchunk;
-    int splits = std::max(1, (1024 + mtiles * ntiles * B - 1) / (mtiles * ntiles * B));
-    splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 64));
-    p.splits = splits;
-    p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk * B;
-    p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk * B;
-    p.p_elems = (size_t)M3 * nzo * splits * B;
-    return p;
+    auto fill = [&](int tchunk) {
+        Plan p;
+        p.tchunk = tchunk;
+        const long K3 = nzi * p.tchunk;
+        int splits = std::max(1, (1024 + mtiles * ntiles * B - 1) / (mtiles * ntiles * B));
+        splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 64));
+        p.splits = splits;
+        p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk * B;
+        p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk * B;
+        p.p_elems = (size_t)M3 * nzo * splits * B;
+        return p;
+    };
+    auto bytes = [](const Plan &p) { return 8 * (p.v_elems + p.u_elems + p.p_elems); };
+    if (cap > 0) {
+        for (int tc = m.ntp; tc >= 1; --tc) {
+            Plan p = fill(tc);
+            while (bytes(p) > cap && p.splits > 1) {
+                --p.splits;
+                p.p_elems = (size_t)M3 * nzo * p.splits * B;
+            }
+            if (bytes(p) <= cap) return p;
+        }
+        Plan none = fill(1);
+        none.tchunk = 0;
+        return none;
+    }
+    return fill((int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)m.ntp, budget / per_t, total_cap / (per_t * (size_t)B)})));
 }
 
 // out[b*len + i] = sum_s part[(b*splits + s)*len + i] + coda*in[b*len + i]
@@ -688,7 +707,10 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const long nxi = m.din[0], nyi = m.din[1], nzi = m.din[2];
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
     const int T = m.ntp, B = bs.B;
-    const Plan plan = make_plan(m, B);
+    const Plan plan = make_plan(m, B, bs.ws_cap);
+    DAGE_REQUIRE(plan.tchunk > 0, DAGE_ERR_ARG,
+                 "the apply needs more than %zu bytes of workspace (V, U and partials of one "
+                 "t-point)", bs.ws_cap);
     const size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
     if (need > ws->bytes) {
         DAGE_REQUIRE(!bs.no_alloc, DAGE_ERR_STATE, "workspace of %zu bytes not reserved", need);

@@ -51,6 +51,7 @@ int gauss_lobatto_grid(int n, double left, double right, double *tp);
 void lip_first_last(int nlip, double *first, double *last);
 int lip_coeffs(int nlip, int grid_type, double *c);  // c[j + nlip*k]
 int lip_integrals(int nlip, int grid_type, double *v);
+int lip_inout_integrals(int nlip, double *ii, double *io);  // (nlip, nlip-1) each
 double two_over_sqrtpi();
 double pi();
 
@@ -119,7 +120,7 @@ struct dage_grid3d_s {
 };
 
 namespace dage {
-constexpr int kMaxBubbleLmax = 6;
+constexpr int kMaxBubbleLmax = 12;
 
 // Device view of one bubble (read by the inject kernel).
 struct BubbleDev {
@@ -138,11 +139,16 @@ constexpr int kCellTab = 1024;
 struct BubbleHost {
     double center[3] = {0.0, 0.0, 0.0};
     double charge = 0.0;
-    int ncell = 0;
+    int ncell = 0, npts = 0;
     double qmax = 0.0;
-    double *d_buf = nullptr;  // celld | steps | f
+    // celld | steps | r^k f | f (the last block only when k != 0; with k = 0
+    // the raw values are the r^k-folded ones)
+    double *d_buf = nullptr;
     int *d_tab = nullptr;     // kCellTab cell-lookup entries
     bool set = false;
+    double *celld() const { return d_buf; }
+    double *steps() const { return d_buf + ncell; }
+    double *fk() const { return d_buf + 2 * ncell; }
 };
 }  // namespace dage
 
@@ -152,6 +158,15 @@ struct dage_bubbles_s {
     std::vector<dage::BubbleHost> host;
     dage::BubbleDev *d_desc = nullptr;
     bool dirty = true;
+    int nlm() const { return (lmax + 1) * (lmax + 1); }
+    // raw radial values f (npts x nlm, point fastest) of bubble i
+    double *f(int i) const {
+        const auto &h = host[i];
+        return h.fk() + (k != 0 ? (size_t)h.npts * nlm() : 0);
+    }
+    size_t buf_doubles(int ncell, int npts) const {
+        return 2 * (size_t)ncell + (size_t)npts * nlm() * (k != 0 ? 2 : 1);
+    }
 };
 
 // Layout of the split-BF16 matrix planes (apply_mixed.hip): per mode, column
@@ -167,6 +182,10 @@ struct dage_operator_s {
     dage_ctx ctx = nullptr;
     dage_grid3d gin = nullptr, gout = nullptr;  // set during creation only
     int ntp = 0;
+    // Helmholtz3D once dage_operator_set_energy has been called (its bubble
+    // transform then convolves with exp(-kappa r)/r, kappa = sqrt(-2 energy))
+    bool helmholtz = false;
+    double energy = 0.0;
     std::vector<double> tp, tw, w;  // host copies
     double coda = 0.0;
     int flags = 0;
@@ -245,6 +264,10 @@ struct BatchSpec {
     double coda = 0.0;
     int nb[3] = {0, 0, 0};
     bool no_alloc = false;
+    // > 0: the workspace (V, U and the z-pass partials) must fit this many
+    // bytes; t-chunk and K split shrink to fit (DAGE_ERR_ARG when even one
+    // t-point without split does not)
+    size_t ws_cap = 0;
 };
 int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *out,
                     const BatchSpec &bs, Workspace *ws);
@@ -294,6 +317,16 @@ int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, c
 int build_masks(dage_operator op);
 // Bubbles (inject.hip, abi.hip).
 int bubbles_sync_device(dage_bubbles bub);
+// Bubble ibub's buffers on the radial grid ax (cell-lookup table built from
+// it), radial values zeroed.
+int bubble_alloc_host(dage_bubbles b, int ibub, const double center[3], double charge,
+                      const Axis &ax);
+// Bubble ibub on the radial grid of bubble sbub of src (device copies),
+// radial values zeroed.
+int bubble_alloc_like(dage_bubbles b, int ibub, dage_bubbles src, int sbub);
+// r^k f -> the inject block after the raw values changed on the device
+// (k != 0 only; asynchronous on the context stream).
+int bubbles_fold_rk(dage_bubbles b);
 int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multiplier,
                    double *cube);
 // Zero n doubles with a fill kernel on the context stream (graph-safe, unlike

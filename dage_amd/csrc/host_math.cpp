@@ -131,6 +131,36 @@ int lip_integrals(int nlip, int grid_type, double *v) {
     return 0;
 }
 
+// lip.F90:414-485: integrals of each basis polynomial from node x_j to the
+// cell end (ii, LIPBasis_inward_integrals) and from the cell start to node
+// x_(j+1) (io, LIPBasis_outward_integrals), both (nlip, nlip-1)
+// column-major, nodes at first + j (equidistant cells).
+int lip_inout_integrals(int nlip, double *ii, double *io) {
+    double c[kMaxNlip * kMaxNlip];
+    if (lip_coeffs(nlip, 1, c)) return -1;
+    double first, last;
+    lip_first_last(nlip, &first, &last);
+    auto antider = [&](int j, double x) {  // Horner over the antiderivative
+        double v = 0.0;
+        for (int k = 0; k < nlip; ++k) v = (v + c[j + nlip * k] / (nlip - k)) * x;
+        return v;
+    };
+    for (int r = 0; r < nlip; ++r) {
+        const double at_last = antider(r, last), at_first = antider(r, first);
+        double x = first;
+        for (int j = 0; j < nlip - 1; ++j) {
+            ii[r + nlip * j] = at_last - antider(r, x);
+            x = x + 1;
+        }
+        x = first;
+        for (int j = 0; j < nlip - 1; ++j) {
+            x = x + 1.0;
+            io[r + nlip * j] = antider(r, x) - at_first;
+        }
+    }
+    return 0;
+}
+
 // grid.F90:446-537: points of each cell measured from the cell start.
 int make_axis(double qmin, int ncell, int nlip, const double *steps,
               int grid_type, Axis *ax) {

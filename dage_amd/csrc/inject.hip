@@ -171,6 +171,123 @@ __global__ void __launch_bounds__(256) bubbles_inject_kernel(InjectArgs A) {
     }
 }
 
+// Bubbles with 6 < lmax <= 12 (products of two bubble sets, l > lmax parts
+// injected after a transform): the (lmax+1)^2 solid harmonics would not fit
+// in registers, so each l's 2l+1 values are generated from the previous two
+// l's by the same recursion (harmonics.F90:586-659) right before they are
+// consumed, per radial cell present in the wavefront.
+template <int LMAX, int NLIP>
+__global__ void __launch_bounds__(256) bubbles_inject_hi_kernel(InjectArgs A) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = blockIdx.x * 16 + wave * 4 + (lane & 3);
+    const int iy = blockIdx.y * 4 + ((lane >> 2) & 3);
+    const int iz = blockIdx.z * 4 + (lane >> 4);
+    const bool inside = ix < A.nx && iy < A.ny && iz < A.nz;
+    constexpr int NMAX = NLIP > 0 ? NLIP : kMaxNlip;
+    constexpr int W = 2 * LMAX + 1;
+    const int nlip = NLIP > 0 ? NLIP : A.nlip;
+    const double px = A.gx[inside ? ix : 0], py = A.gy[inside ? iy : 0],
+                 pz = A.gz[inside ? iz : 0];
+    constexpr double kEps = 2.220446049250313e-16;  // epsilon(0.d0)
+    double total = 0.0;
+    for (int b = 0; b < A.nbub; ++b) {
+        const BubbleDev B = A.bubs[b];
+        const double dx = px - B.cx, dy = py - B.cy, dz = pz - B.cz;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const int ic = inside ? radial_icell_tab(B, r) : 0;
+        unsigned long long todo = __ballot(ic != 0);
+        if (todo == 0ull) continue;
+        const int icl = ic > 0 ? ic - 1 : 0;
+        const double crd = (r - ((gdptr)B.celld)[icl]) / ((gdptr)B.steps)[icl] + A.first;
+        double polys[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            if (j >= nlip) break;
+            double v = A.lip[j];
+#pragma unroll
+            for (int q = 1; q < NMAX; ++q) {
+                if (q >= nlip) break;
+                v = v * crd + A.lip[j + nlip * q];
+            }
+            polys[j] = v;
+        }
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double inv = (r > kEps) ? 1.0 / r : 1.0;
+        while (todo != 0ull) {
+            const int leader = __builtin_ctzll(todo);
+            const int c = __builtin_amdgcn_readlane(ic, leader);
+            const bool mine = ic == c;
+            todo &= ~__ballot(mine);
+            if (!mine) continue;
+            const int cs = __builtin_amdgcn_readfirstlane(c);
+            const gdptr f = (gdptr)(B.f + (long)(cs - 1) * (nlip - 1));
+            // rolling solid harmonics: pp = S_(l-2), p = S_(l-1), q = S_l,
+            // entry m + l of each
+            double pp[W], p[W], q[W];
+            double res = 0.0, ipow = 1.0;
+#pragma unroll
+            for (int l = 0; l <= LMAX; ++l) {
+                if (l == 0) {
+                    q[0] = 1.0;
+                } else if (l == 1) {
+                    q[0] = dy;
+                    q[1] = dz;
+                    q[2] = dx;
+                } else {
+                    const double fl = sqrt((2 * l - 1.0) / (2.0 * l));
+                    q[0] = fl * (dy * p[2 * (l - 1)] + dx * p[0]);
+#pragma unroll
+                    for (int m = -(l - 1); m <= l - 1; ++m) {
+                        double v = (2 * l - 1.0) / sqrt(1.0 * (l + m) * (l - m)) *
+                                   (dz * p[m + l - 1]);
+                        if (l - (m < 0 ? -m : m) > 1)
+                            v = v - sqrt((l + m - 1.0) * (l - m - 1.0) / ((l + m) * (l - m))) *
+                                        (r2 * pp[m + l - 2]);
+                        q[m + l] = v;
+                    }
+                    q[2 * l] = fl * (dx * p[2 * (l - 1)] - dy * p[0]);
+                }
+                if (l >= A.lmin) {
+                    double sl = 0.0;
+#pragma unroll
+                    for (int m = -l; m <= l; ++m) {
+                        const gdptr fl_ = f + (long)(l * l + l + m) * B.npts;
+                        double fv = 0.0;
+#pragma unroll
+                        for (int j = 0; j < NMAX; ++j) {
+                            if (j >= nlip) break;
+                            fv += fl_[j] * polys[j];
+                        }
+                        sl += fv * q[m + l];
+                    }
+                    res += sl * ipow;
+                }
+                ipow *= inv;
+#pragma unroll
+                for (int m = 0; m < W; ++m) {
+                    pp[m] = p[m];
+                    p[m] = q[m];
+                }
+            }
+            total += res;
+        }
+    }
+    if (inside) {
+        const long id = ix + (long)A.nx * (iy + (long)A.ny * iz);
+        A.cube[id] = A.cube[id] + A.multiplier * total;
+    }
+}
+
+template <int LMAX>
+void launch_inject_hi(const InjectArgs &A, dim3 g, dim3 block, hipStream_t s) {
+    if (A.nlip == 7)
+        hipLaunchKernelGGL((bubbles_inject_hi_kernel<LMAX, 7>), g, block, 0, s, A);
+    else if (A.nlip == 5)
+        hipLaunchKernelGGL((bubbles_inject_hi_kernel<LMAX, 5>), g, block, 0, s, A);
+    else
+        hipLaunchKernelGGL((bubbles_inject_hi_kernel<LMAX, 0>), g, block, 0, s, A);
+}
+
 template <int LMAX>
 void launch_inject(const InjectArgs &A, dim3 g, dim3 block, hipStream_t s) {
     if (A.nlip == 7)
@@ -239,6 +356,12 @@ int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multipli
     case 4: launch_inject<4>(A, g, block, ctx->stream); break;
     case 5: launch_inject<5>(A, g, block, ctx->stream); break;
     case 6: launch_inject<6>(A, g, block, ctx->stream); break;
+    case 7: launch_inject_hi<7>(A, g, block, ctx->stream); break;
+    case 8: launch_inject_hi<8>(A, g, block, ctx->stream); break;
+    case 9: launch_inject_hi<9>(A, g, block, ctx->stream); break;
+    case 10: launch_inject_hi<10>(A, g, block, ctx->stream); break;
+    case 11: launch_inject_hi<11>(A, g, block, ctx->stream); break;
+    case 12: launch_inject_hi<12>(A, g, block, ctx->stream); break;
     default:
         set_error("bubbles lmax %d outside 0..%d", bub->lmax, kMaxBubbleLmax);
         return DAGE_ERR_UNSUPPORTED;

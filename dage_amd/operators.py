@@ -174,6 +174,8 @@ class Grid3D:
         self.nlip = nlip
         self.grid_type = grid_type
         self._steps = steps
+        self.steps = steps
+        self.qmin = qmin
 
     @classmethod
     def equidistant(cls, ctx, ranges, nlip=7, stepmax=0.2, grid_type=1):
@@ -396,6 +398,57 @@ class Bubbles:
             raise ValueError(f"cube shape {cube.shape} != grid shape {grid.get_shape()}")
         call("dage_bubbles_inject_device", self.handle, grid.handle, lmin, float(multiplier),
              ctypes.c_void_p(cube.ptr))
+
+    @classmethod
+    def like(cls, src: "Bubbles", lmax: int | None = None, k: int = 0) -> "Bubbles":
+        """Bubbles(src, lmax=..., k=...) without content: same centres,
+        charges and radial grids, zero radial values."""
+        lmax = src.lmax if lmax is None else lmax
+        h = ctypes.c_void_p()
+        call("dage_bubbles_create_like", src.handle, lmax, k, ctypes.byref(h))
+        obj = cls.__new__(cls)
+        obj.handle, obj.ctx = h, src.ctx
+        obj.nbub, obj.lmax, obj.k, obj.nlip = src.nbub, lmax, k, src.nlip
+        return obj
+
+    def npoints(self, ibub: int) -> int:
+        n = ctypes.c_int()
+        call("dage_bubbles_npoints", self.handle, ibub, ctypes.byref(n))
+        return n.value
+
+    def get(self, ibub: int):
+        """(centre, charge, cell steps, radial values f (npts, (lmax+1)^2))."""
+        npts = self.npoints(ibub)
+        ncell = (npts - 1) // (self.nlip - 1)
+        c, q = np.zeros(3), ctypes.c_double()
+        st = np.zeros(ncell)
+        bf = np.zeros((npts, (self.lmax + 1) ** 2), order="F")
+        call("dage_bubbles_get", self.handle, ibub, _ptr(c), ctypes.byref(q), _ptr(st),
+             bf.ctypes.data_as(_dp))
+        return c, q.value, st, bf
+
+    def axpby(self, a: float, x: "Bubbles", b: float):
+        """self = a x + b self (l above x's lmax: b self)."""
+        call("dage_bubbles_axpby", float(a), x.handle, float(b), self.handle)
+
+    def multiply(self, other: "Bubbles", out: "Bubbles | None" = None) -> "Bubbles":
+        """Bubble-by-bubble product (lmax and k add up)."""
+        out = out if out is not None else Bubbles.like(self, lmax=self.lmax + other.lmax,
+                                                       k=self.k + other.k)
+        call("dage_bubbles_multiply", self.handle, other.handle, out.handle)
+        return out
+
+    def coulomb_transform(self, out: "Bubbles | None" = None) -> "Bubbles":
+        """Coulomb3D_evaluate_bubbles_potential (coulomb.F90:222-336)."""
+        out = out if out is not None else Bubbles.like(self, k=0)
+        call("dage_bubbles_coulomb_transform", self.handle, out.handle)
+        return out
+
+    def helmholtz_transform(self, energy: float, out: "Bubbles | None" = None) -> "Bubbles":
+        """Helmholtz3D_transform_bubbles_sub (helmholtz3d.F90:519-657)."""
+        out = out if out is not None else Bubbles.like(self, k=0)
+        call("dage_bubbles_helmholtz_transform", self.handle, float(energy), out.handle)
+        return out
 
 
 class GBFMMCoulomb3D:

@@ -194,9 +194,9 @@ int dage_operator_destroy(dage_operator op);
  * Atom-centred radial functions times real solid harmonics
  * (src/bubbles/bubbles.F90:90-230).  Replaces bubbles_init_cuda,
  * bubble_init_cuda, bubble_upload_all_cuda and bubbles_inject_cuda
- * (src/bubbles/bubbles_cuda.cu:3715-3776).  lmax and k are shared by all
- * bubbles (r^k f_lm(r) is the radial part); radial grids start at r = 0 with
- * equidistant-cell LIP basis of nlip points. */
+ * (src/bubbles/bubbles_cuda.cu:3715-3776).  lmax (0..12) and k are shared
+ * by all bubbles (r^k f_lm(r) is the radial part); radial grids start at
+ * r = 0 with equidistant-cell LIP basis of nlip points. */
 int dage_bubbles_create(dage_ctx ctx, int nbub, int lmax, int k, int nlip,
                         dage_bubbles *bubbles);
 /* Bubble ibub (0-based): centre, nuclear charge (informational), radial grid
@@ -212,10 +212,101 @@ int dage_bubbles_set(dage_bubbles bubbles, int ibub, const double center[3],
 int dage_bubbles_inject_device(dage_bubbles bubbles, dage_grid3d grid, int lmin,
                                double multiplier, double *cube_dev);
 int dage_bubbles_destroy(dage_bubbles bubbles);
+/* Bubbles(src, copy_content=.false., lmax=lmax, k=k) (Bubbles_init_copy,
+ * src/bubbles/bubbles.F90:802-830):
+ * the same centres, charges and radial grids as src, radial values zero. */
+int dage_bubbles_create_like(dage_bubbles src, int lmax, int k, dage_bubbles *out);
+int dage_bubbles_info(dage_bubbles bubbles, int *nbub, int *lmax, int *k, int *nlip);
+/* Radial points of bubble ibub (ncell*(nlip-1)+1). */
+int dage_bubbles_npoints(dage_bubbles bubbles, int ibub, int *npts);
+/* Bubble ibub's centre[3], charge, cell steps (ncell) and radial values f
+ * (npts x (lmax+1)^2, the values given to dage_bubbles_set, without r^k);
+ * any output may be NULL; synchronises. */
+int dage_bubbles_get(dage_bubbles bubbles, int ibub, double *center, double *charge,
+                     double *steps, double *bf);
+/* y = a*x + beta*y on bubbles with the same grids and k (Bubbles_add and
+ * Bubbles_times_REAL64, src/bubbles/bubbles.F90:2499-2535, 3311-3330); l
+ * above x's lmax: y = beta*y.  Asynchronous. */
+int dage_bubbles_axpby(double a, dage_bubbles x, double beta, dage_bubbles y);
+/* Coulomb3D_evaluate_bubbles_potential (src/bubbles/coulomb.F90:222-336):
+ * out = radial Poisson potential of in (out: same grids and lmax, k = 0),
+ * V_lm(r) = 4pi/(2l+1) [r^-(l+1) int_0^r s^(l+2+k) f_lm + r^l int_r s^(1-l+k) f_lm].
+ * Asynchronous on the context stream. */
+int dage_bubbles_coulomb_transform(dage_bubbles in, dage_bubbles out);
+/* Helmholtz3D_transform_bubbles_sub (src/bubbles/helmholtz3d.F90:519-657):
+ * out = radial convolution of in with exp(-kappa r)/r, kappa = sqrt(-2E),
+ * E < 0 (out: same grids and lmax, k = 0; 7-point radial cells).
+ * Asynchronous on the context stream. */
+int dage_bubbles_helmholtz_transform(dage_bubbles in, double energy, dage_bubbles out);
+/* Bubbles(src, copy_content=.true., lmax=lmax) (Bubbles_init_copy,
+ * bubbles.F90:802-830): a new set with lmax (values above src's lmax zero,
+ * l above lmax dropped); dispose_extra_bubbles
+ * (src/bubbles/function3d.F90:2225-2240) keeps it as the function's
+ * bubbles. */
+int dage_bubbles_copy_lmax(dage_bubbles src, int lmax, dage_bubbles *out);
+/* Function3D_inject_extra_bubbles (src/bubbles/function3d.F90:2198-2222):
+ * cube_dev += the l > lmax part of `in` on grid, and *kept = the l <=
+ * min(lmax, in's lmax) part of `in` (a new bubbles handle). */
+int dage_function3d_inject_extra_bubbles(dage_grid3d grid, double *cube_dev,
+                                         dage_bubbles in, int lmax, dage_bubbles *kept);
+/* Operator3D_operate_on_Function3D (src/bubbles/function3d.F90:3727-3792)
+ * and Helmholtz3D_operate (src/bubbles/helmholtz3d.F90:446-481) on a function
+ * of a device cube (operator's input grid) and bubbles: the bubble transform
+ * (Coulomb: radial Poisson; after dage_operator_set_energy: Helmholtz with
+ * the energy set, which must be negative) into bub_out (same grids and lmax,
+ * k = 0), then the FP64 cube transform into cube_out.  only_cube != 0 (or
+ * bub_in NULL) skips the bubbles.  A Helmholtz operator is reset to energy 0
+ * afterwards, as the reference does (helmholtz3d.F90:479).  Asynchronous on
+ * the context stream. */
+int dage_operator_operate_on(dage_operator op, const double *cube_in, dage_bubbles bub_in,
+                             double *cube_out, dage_bubbles bub_out, int only_cube);
+/* BubblesMultiplier_multiply, simple product (src/bubbles/bubbles.F90:
+ * 3553-3760): out = a * b centre by centre, g_LM = sum c(l1m1,l2m2->LM) f1 f2
+ * with the product coefficients of the Racah harmonics (YProduct,
+ * src/bubbles/harmonic.F90:884-1000).  out: same grids, lmax = la + lb,
+ * k = ka + kb.  The Taylor-series terms the reference adds for cusp-type
+ * factors are not included.  Asynchronous. */
+int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out);
+/* Function3D_multiply_sub (src/bubbles/function3d.F90:1097-1308, the product
+ * without Taylor-series terms): bub_out = b1 * b2 (when both given) and
+ * cube_out = (cube2 + B2)(cube1 + B1) - B_out, B = bubbles injected on grid.
+ * Device cubes on grid; cube_out may be cube2 but not cube1.  Asynchronous. */
+int dage_function3d_multiply(dage_grid3d grid, const double *cube1, dage_bubbles b1,
+                             const double *cube2, dage_bubbles b2, double *cube_out,
+                             dage_bubbles bub_out);
 /* Grid1D_init_radial (src/bubbles/grid.F90:569-603) cell steps for ncell
  * cells: x_i = c/((r_max+c)/(i*dx)-1), c = 8 z^-1.5, dx = r_max/ncell. */
 int dage_radial_grid_steps(double z, int ncell, int nlip, double r_max,
                            double *steps);
+
+/* ------------------------------------------------------------ binary files
+ * DAGE's Function3D binary format (.fun: Function3D_dump / Function3D_load,
+ * src/bubbles/function3d.F90:812-935; bubbles part Bubbles_write /
+ * Bubbles_read_and_init, src/bubbles/bubbles.F90:712-756, 1251-1276) and
+ * grid format (.g3d: Grid3D_dump / Grid3D_init_read,
+ * src/bubbles/grid.F90:1072-1145).  Host buffers; no GPU work.
+ * dage_fun_write: shape NULL stores no cube; nbub < 0 stores no bubbles,
+ * otherwise charges[nbub], centers[3*nbub], npts[nbub] and bf[i] (npts[i] x
+ * (lmax+1)^2, without r^k) per bubble.  The label is stored in 20 blank-padded
+ * characters. */
+int dage_fun_write(const char *path, const char *label, int type, const int shape[3],
+                   const double *cube, int nbub, int lmax, int k, const double *charges,
+                   const double *centers, const int *npts, const double *const *bf);
+/* Header of a .fun file: label (21 bytes incl. terminator), type, cube shape
+ * (-1s without a cube), bubble count (-1 without bubbles), lmax, k.  Any
+ * output may be NULL. */
+int dage_fun_read_info(const char *path, char *label, int *type, int shape[3], int *nbub,
+                       int *lmax, int *k);
+/* Contents of a .fun file (outputs may be NULL); the radial grids are not in
+ * the file, so npts (per bubble) comes from the caller's mould. */
+int dage_fun_read(const char *path, double *cube, double *charges, double *centers,
+                  const int *npts, double *const *bf);
+int dage_g3d_write(const char *path, int nlip, int grid_type, const int ncell[3],
+                   const double qmin[3], const double *const steps[3]);
+/* Reads a .g3d file: nlip, grid_type, ncell[3], qmin[3], and the cell steps
+ * into steps[a] (ncell[a] values; steps or steps[a] NULL skips them). */
+int dage_g3d_read(const char *path, int *nlip, int *grid_type, int ncell[3], double qmin[3],
+                  double *const steps[3]);
 
 /* ---------------------------------------------------------------- cube sums
  * Integrator3D reduce3d (src/bubbles/integrator.cu:320-373) /

@@ -74,6 +74,14 @@ def _load():
         "dage_oracle_eval_local_expansion": (None, [i, _dp, _dp, i, _dp, i, _dp, i, _dp, _dp, i,
                                                     i]),
         "dage_oracle_set_threads": (i, [i]),
+        "dage_oracle_lip_inout_integrals": (i, [i, i, _dp, _dp]),
+        "dage_oracle_integrate_inout": (i, [i, i, _dp, _dp, _dp, _dp]),
+        "dage_oracle_bessel_i": (None, [i, d, _dp]),
+        "dage_oracle_bessel_k": (None, [i, d, _dp]),
+        "dage_oracle_bubble_coulomb": (i, [i, i, i, i, _dp, _dp, _dp]),
+        "dage_oracle_bubble_helmholtz": (i, [i, i, i, i, _dp, d, _dp, _dp]),
+        "dage_oracle_yproduct": (i, [i, i, _dp]),
+        "dage_oracle_bubble_product": (i, [i, i, i, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -343,6 +351,82 @@ def bubbles_eval_grid(bub: BubbleSet, x, y, z, lmin: int = 0, cube=None):
                                                _p(z), len(z), out.ctypes.data_as(_dp))
     if rc:
         raise RuntimeError("bubbles_eval_grid failed")
+    return out
+
+
+# ------------------------------------------------- bubble radial transforms
+def lip_inout_integrals(nlip: int, grid_type: int = 1):
+    """LIPBasis inward and outward integrals (lip.F90:364-485), nlip x (nlip-1)."""
+    a, b = np.zeros(nlip * (nlip - 1)), np.zeros(nlip * (nlip - 1))
+    if _load().dage_oracle_lip_inout_integrals(nlip, grid_type, _p(a), _p(b)):
+        raise ValueError("unsupported nlip")
+    return a.reshape((nlip, nlip - 1), order="F"), b.reshape((nlip, nlip - 1), order="F")
+
+
+def integrate_inout(steps, nlip: int, f):
+    """InOutIntegrator1D (evaluators.F90:1263-1320): (inwards, outwards)
+    cumulative integrals of f on the radial grid of `steps` from r = 0."""
+    steps, f = _f64(steps), _f64(f)
+    inw, outw = np.zeros(len(f)), np.zeros(len(f))
+    if _load().dage_oracle_integrate_inout(len(steps), nlip, _p(steps), _p(f), _p(inw),
+                                           _p(outw)):
+        raise ValueError("integrate_inout failed")
+    return inw, outw
+
+
+def bessel_i(lmax: int, x: float) -> np.ndarray:
+    out = np.zeros(lmax + 1)
+    _load().dage_oracle_bessel_i(lmax, float(x), _p(out))
+    return out
+
+
+def bessel_k(lmax: int, x: float) -> np.ndarray:
+    out = np.zeros(lmax + 1)
+    _load().dage_oracle_bessel_k(lmax, float(x), _p(out))
+    return out
+
+
+def bubble_coulomb(lmax: int, k: int, steps, nlip: int, bf) -> np.ndarray:
+    """Coulomb3D_evaluate_bubbles_potential of one bubble (coulomb.F90:222-336)."""
+    steps = _f64(steps)
+    bf = np.asfortranarray(bf, dtype=np.float64)
+    out = np.zeros(bf.shape, order="F")
+    if _load().dage_oracle_bubble_coulomb(lmax, k, len(steps), nlip, _p(steps),
+                                          bf.ctypes.data_as(_dp), out.ctypes.data_as(_dp)):
+        raise ValueError("bubble_coulomb failed")
+    return out
+
+
+def bubble_helmholtz(lmax: int, k: int, steps, nlip: int, energy: float, bf) -> np.ndarray:
+    """Helmholtz3D_transform_bubbles_sub of one bubble (helmholtz3d.F90:519-657)."""
+    steps = _f64(steps)
+    bf = np.asfortranarray(bf, dtype=np.float64)
+    out = np.zeros(bf.shape, order="F")
+    if _load().dage_oracle_bubble_helmholtz(lmax, k, len(steps), nlip, _p(steps), float(energy),
+                                            bf.ctypes.data_as(_dp), out.ctypes.data_as(_dp)):
+        raise ValueError("bubble_helmholtz failed")
+    return out
+
+
+def yproduct(l1max: int, l2max: int) -> np.ndarray:
+    """Y_lm1 Y_lm2 = sum_o c[lm1, lm2, o] Y_o (Racah real harmonics)."""
+    L = l1max + l2max
+    n1, n2, nL = (l1max + 1) ** 2, (l2max + 1) ** 2, (L + 1) ** 2
+    c = np.zeros(n1 * n2 * nL)
+    if _load().dage_oracle_yproduct(l1max, l2max, _p(c)):
+        raise ValueError("yproduct failed")
+    return c.reshape((n1, n2, nL), order="F")
+
+
+def bubble_product(l1max: int, bf1, l2max: int, bf2) -> np.ndarray:
+    """Product of two bubbles' radial functions on the same radial grid."""
+    bf1 = np.asfortranarray(bf1, dtype=np.float64)
+    bf2 = np.asfortranarray(bf2, dtype=np.float64)
+    npts = bf1.shape[0]
+    out = np.zeros((npts, (l1max + l2max + 1) ** 2), order="F")
+    if _load().dage_oracle_bubble_product(l1max, l2max, npts, bf1.ctypes.data_as(_dp),
+                                          bf2.ctypes.data_as(_dp), out.ctypes.data_as(_dp)):
+        raise ValueError("bubble_product failed")
     return out
 
 

@@ -1037,3 +1037,308 @@ int dage_oracle_set_threads(int n) {
     return 1;
 #endif
 }
+
+/* ===================================================== bubble radial transforms
+ * Radial parts of the Coulomb and Helmholtz operators acting on bubbles
+ * (coulomb.F90:222-336, helmholtz3d.F90:519-657) with the LIP cumulative
+ * integrals they use (lip.F90:364-485, evaluators.F90:1263-1320) and the
+ * modified spherical Bessel functions, evaluated in quadruple precision as the
+ * reference does (REAL128, helmholtz3d.F90:879-1000, 1133-1210). */
+#include <quadmath.h>
+
+/* x**n for integer n as gfortran evaluates it (binary powering; 1/x^|n|). */
+static double o_powi(double x, int n) { return __builtin_powi(x, n); }
+
+/* LIPBasis_integral_coeffs (lip.F90:364-383): antiderivative coefficients,
+ * column j (0-based) the coefficient of x^(nlip-j), column nlip zero. */
+static void lip_integral_coeffs(int nlip, const double *c, double *ic) {
+    for (int r = 0; r < nlip; ++r) ic[r + nlip * nlip] = 0.0;
+    for (int i = 0; i < nlip; ++i)  /* int_coeffs(:,n-i) = coeffs(:,n-i)/(i+1) */
+        for (int r = 0; r < nlip; ++r)
+            ic[r + nlip * (nlip - 1 - i)] = c[r + nlip * (nlip - 1 - i)] / (i + 1);
+}
+
+int dage_oracle_lip_inout_integrals(int nlip, int grid_type, double *ints_in,
+                                    double *ints_out) {
+    double c[64], ic[72], first, last, tmp1[8], tmp2[8];
+    if (nlip < 2 || nlip > 8 || dage_oracle_lip_coeffs(nlip, grid_type, c)) return -1;
+    dage_oracle_lip_first_last(nlip, grid_type, &first, &last);
+    lip_integral_coeffs(nlip, c, ic);
+    /* inward: int_{x_j}^{last} chi_i, x_j = first + j (lip.F90:421-446) */
+    for (int r = 0; r < nlip; ++r) tmp2[r] = 0.0;
+    for (int i = 0; i < nlip; ++i)
+        for (int r = 0; r < nlip; ++r) tmp2[r] = (tmp2[r] + ic[r + nlip * i]) * last;
+    double x = first;
+    for (int j = 0; j < nlip - 1; ++j) {
+        for (int r = 0; r < nlip; ++r) tmp1[r] = 0.0;
+        for (int i = 0; i < nlip; ++i)
+            for (int r = 0; r < nlip; ++r) tmp1[r] = (tmp1[r] + ic[r + nlip * i]) * x;
+        for (int r = 0; r < nlip; ++r) ints_in[r + nlip * j] = tmp2[r] - tmp1[r];
+        x = x + 1;
+    }
+    /* outward: int_{first}^{x_j} chi_i, x_j = first + j + 1 (lip.F90:457-485) */
+    for (int r = 0; r < nlip; ++r) tmp1[r] = 0.0;
+    for (int i = 0; i < nlip; ++i)
+        for (int r = 0; r < nlip; ++r) tmp1[r] = (tmp1[r] + ic[r + nlip * i]) * first;
+    x = first;
+    for (int j = 0; j < nlip - 1; ++j) {
+        x = x + 1.0;
+        for (int r = 0; r < nlip; ++r) tmp2[r] = 0.0;
+        for (int i = 0; i < nlip; ++i)
+            for (int r = 0; r < nlip; ++r) tmp2[r] = (tmp2[r] + ic[r + nlip * i]) * x;
+        for (int r = 0; r < nlip; ++r) ints_out[r + nlip * j] = tmp2[r] - tmp1[r];
+    }
+    return 0;
+}
+
+int dage_oracle_integrate_inout(int ncell, int nlip, const double *steps, const double *f,
+                                double *inw, double *outw) {
+    double ii[64], io[64];
+    if (dage_oracle_lip_inout_integrals(nlip, 1, ii, io)) return -1;
+    const int n = nlip - 1, sz = ncell * n + 1;
+    /* inwards (evaluators.F90:1283-1302): from the last cell down */
+    if (inw) {
+        inw[sz - 1] = 0.0;
+        int start = sz - 1;
+        for (int c = ncell - 1; c >= 0; --c) {
+            start -= n;
+            const double carry = inw[start + n];
+            for (int j = 0; j < n; ++j) {
+                double s = 0.0;
+                for (int i = 0; i < nlip; ++i) s += f[start + i] * ii[i + nlip * j];
+                inw[start + j] = s * steps[c] + carry;
+            }
+        }
+    }
+    /* outwards (evaluators.F90:1304-1320): from the first cell up */
+    if (outw) {
+        outw[0] = 0.0;
+        int start = 0;
+        for (int c = 0; c < ncell; ++c) {
+            const double carry = outw[start];
+            for (int j = 0; j < n; ++j) {
+                double s = 0.0;
+                for (int i = 0; i < nlip; ++i) s += f[start + i] * io[i + nlip * j];
+                outw[start + 1 + j] = s * steps[c] + carry;
+            }
+            start += n;
+        }
+    }
+    return 0;
+}
+
+/* Bubbles_rpow (bubbles.F90:3363-3378) */
+static void o_rpow(const double *r, int npts, int n, double *out) {
+    for (int i = 0; i < npts; ++i) out[i] = o_powi(r[i], n);
+    if (n < 0) out[0] = 0.0;
+}
+
+static __float128 q_powi(__float128 x, int n) {
+    /* binary powering, as gfortran's REAL(16)**INTEGER */
+    __float128 y = 1.0Q;
+    int m = n < 0 ? -n : n;
+    while (m) {
+        if (m & 1) y *= x;
+        m >>= 1;
+        if (m) x *= x;
+    }
+    return n < 0 ? 1.0Q / y : y;
+}
+
+/* FirstModSphBesselCollection_evaluate_small (helmholtz3d.F90:935-980) */
+static __float128 q_bessel_i_series(__float128 z, int n, int maxterms) {
+    __float128 prefactor = q_powi(z / 1.0Q, n);
+    __float128 divider = 1.0Q;
+    for (int i = 1; i <= 2 * n + 1; i += 2) divider = divider * i;
+    prefactor = prefactor / divider;
+    __float128 addition = 1.0Q, reslt = 1.0Q, fact = 1.0Q, a = 0.5Q * (z * z), b = 1.0Q;
+    divider = 1.0Q;
+    const __float128 tol = (__float128)1e-15;
+    int i = 1;
+    while (addition > tol * reslt && i < maxterms) {
+        b = b * a;
+        fact = fact * i;
+        divider = divider * (2 * n + 2 * i + 1);
+        addition = b / (fact * divider);
+        reslt = reslt + addition;
+        ++i;
+    }
+    return prefactor * reslt;
+}
+
+void dage_oracle_bessel_i(int lmax, double xd, double *out) {
+    const __float128 x = xd;
+    const __float128 eps = (__float128)2.220446049250313e-16;
+    __float128 r[64];
+    if (fabsq(x) > 0.0Q) {
+        r[lmax] = q_bessel_i_series(x, lmax, 300);
+        if (lmax >= 1) r[lmax - 1] = q_bessel_i_series(x, lmax - 1, 300);
+        for (int l = lmax - 2; l >= 0; --l)
+            r[l] = (2 * l + 3) * r[l + 1] / x * 1.0Q + r[l + 2] * 1.0Q;
+    } else {
+        for (int l = 0; l <= lmax; ++l) r[l] = 0.0Q;
+        if (fabsq(x) < eps) r[0] = 1.0Q;
+    }
+    for (int l = 0; l <= lmax; ++l) out[l] = (double)r[l];
+}
+
+void dage_oracle_bessel_k(int lmax, double xd, double *out) {
+    const __float128 x = xd;
+    const __float128 pi = (__float128)o_pi();
+    __float128 r[64];
+    if (fabsq(x) > 0.0Q) {
+        r[0] = expq(-x) / x * pi / 2;
+        if (lmax >= 1) r[1] = expq(-x) * (x + 1) * 1.0Q / (x * x) * pi / 2;
+        for (int l = 2; l <= lmax; ++l)
+            r[l] = (2 * l - 1) * r[l - 1] * 1.0Q / x + r[l - 2] * 1.0Q;
+    } else {
+        for (int l = 0; l <= lmax; ++l) r[l] = 0.0Q;
+    }
+    for (int l = 0; l <= lmax; ++l) out[l] = (double)r[l];
+}
+
+int dage_oracle_bubble_coulomb(int lmax, int k, int ncell, int nlip, const double *steps,
+                               const double *bf, double *out) {
+    const int npts = ncell * (nlip - 1) + 1, nlm = (lmax + 1) * (lmax + 1);
+    double *r = malloc(sizeof(double) * npts), *celld = malloc(sizeof(double) * ncell);
+    double *w = malloc(sizeof(double) * 8 * npts);
+    if (!r || !celld || !w || lmax > 62) { free(r); free(celld); free(w); return -1; }
+    dage_oracle_grid1d(0.0, ncell, nlip, steps, 1, r, celld);
+    double *p1 = w, *p2 = w + npts, *p3 = w + 2 * npts, *p4 = w + 3 * npts;
+    double *g = w + 4 * npts, *o = w + 5 * npts, *in = w + 6 * npts;
+    const double fourpi = 4.0 * o_pi();
+    int rc = 0;
+    for (int l = 0; l <= lmax && !rc; ++l) {
+        o_rpow(r, npts, -(l + 1), p1);
+        o_rpow(r, npts, l + 2 + k, p2);
+        o_rpow(r, npts, l, p3);
+        o_rpow(r, npts, -(l + 1) + 2 + k, p4);
+        for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            const double *f = bf + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) g[i] = f[i] * p2[i];
+            rc |= dage_oracle_integrate_inout(ncell, nlip, steps, g, NULL, o);
+            for (int i = 0; i < npts; ++i) g[i] = f[i] * p4[i];
+            rc |= dage_oracle_integrate_inout(ncell, nlip, steps, g, in, NULL);
+            const double pre = fourpi / (2 * l + 1);
+            double *v = out + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) v[i] = pre * (p1[i] * o[i] + p3[i] * in[i]);
+        }
+    }
+    (void)nlm;
+    free(r); free(celld); free(w);
+    return rc;
+}
+
+int dage_oracle_bubble_helmholtz(int lmax, int k, int ncell, int nlip, const double *steps,
+                                 double energy, const double *bf, double *out) {
+    const int npts = ncell * (nlip - 1) + 1, nlm = (lmax + 1) * (lmax + 1);
+    if (lmax > 62) return -1;
+    double *r = malloc(sizeof(double) * npts), *celld = malloc(sizeof(double) * ncell);
+    double *bi = malloc(sizeof(double) * (size_t)npts * (lmax + 1));
+    double *bk = malloc(sizeof(double) * (size_t)npts * (lmax + 1));
+    double *w = malloc(sizeof(double) * 5 * npts);
+    if (!r || !celld || !bi || !bk || !w) {
+        free(r); free(celld); free(bi); free(bk); free(w);
+        return -1;
+    }
+    dage_oracle_grid1d(0.0, ncell, nlip, steps, 1, r, celld);
+    const double kappa = sqrt(-2.0 * energy);
+    double tmp[64];
+    for (int i = 0; i < npts; ++i) {
+        dage_oracle_bessel_i(lmax, kappa * r[i], tmp);
+        for (int l = 0; l <= lmax; ++l) bi[i + (size_t)npts * l] = tmp[l];
+        dage_oracle_bessel_k(lmax, kappa * r[i], tmp);
+        for (int l = 0; l <= lmax; ++l) bk[i + (size_t)npts * l] = tmp[l];
+    }
+    double *rp = w, *t1 = w + npts, *t2 = w + 2 * npts, *o = w + 3 * npts, *in = w + 4 * npts;
+    o_rpow(r, npts, 2 + k, rp);
+    int rc = 0;
+    for (int l = 0; l <= lmax && !rc; ++l) {
+        const double *I = bi + (size_t)npts * l, *K = bk + (size_t)npts * l;
+        for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            const double *f = bf + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) {
+                t1[i] = rp[i] * I[i] * f[i];
+                t2[i] = rp[i] * K[i] * f[i];
+            }
+            /* extrapolate_first_nlip7(3, equidistant) for l = 0
+             * (evaluators.F90:1588-1640) */
+            if (l == 0) t2[0] = 3.0 * t2[1] - 3.0 * t2[2] + 1.0 * t2[3];
+            else t2[0] = 0.0;
+            rc |= dage_oracle_integrate_inout(ncell, nlip, steps, t1, NULL, o);
+            rc |= dage_oracle_integrate_inout(ncell, nlip, steps, t2, in, NULL);
+            double *v = out + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) v[i] = K[i] * o[i] + I[i] * in[i];
+        }
+    }
+    for (size_t e = 0; e < (size_t)npts * nlm; ++e) out[e] = out[e] * 8.0 * kappa;
+    free(r); free(celld); free(bi); free(bk); free(w);
+    return rc;
+}
+
+/* ================================================================ products
+ * YProduct (harmonic.F90:884-1000) restated as the projection
+ * c(lm1, lm2 -> LM) = (2L+1)/(4 pi) int Y1 Y2 Y_LM dOmega with Y = S/r^l
+ * (Racah), by Gauss-Legendre (cos theta) x uniform (phi) quadrature exact
+ * for these degrees; coef is dense, (n1, n2, nL) column-major with
+ * n = (lmax+1)^2 and L = l1max + l2max. */
+int dage_oracle_yproduct(int l1max, int l2max, double *coef) {
+    const int L = l1max + l2max;
+    const int n1 = (l1max + 1) * (l1max + 1), n2 = (l2max + 1) * (l2max + 1);
+    const int nL = (L + 1) * (L + 1), nt = L + 1, np = 2 * L + 2;
+    if (L > 30) return -1;
+    double *xt = malloc(sizeof(double) * nt), *wt = malloc(sizeof(double) * nt);
+    double *S = malloc(sizeof(double) * nL);
+    double *Y = malloc(sizeof(double) * (size_t)nt * np * nL);
+    if (!xt || !wt || !S || !Y) { free(xt); free(wt); free(S); free(Y); return -1; }
+    dage_oracle_gauleg(nt, xt, wt);
+    const double pi = o_pi();
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < np; ++b) {
+            const double ct = xt[a], st = sqrt(1.0 - ct * ct), ph = 2.0 * pi * b / np;
+            dage_oracle_solid_harmonics(L, st * cos(ph), st * sin(ph), ct,
+                                        Y + ((size_t)a * np + b) * nL);
+        }
+    for (int o = 0; o < nL; ++o) {
+        const int lo = (int)sqrt((double)o);
+        for (int j = 0; j < n2; ++j)
+            for (int i = 0; i < n1; ++i) {
+                double s = 0.0;
+                for (int a = 0; a < nt; ++a)
+                    for (int b = 0; b < np; ++b) {
+                        const double *y = Y + ((size_t)a * np + b) * nL;
+                        s += wt[a] * (2.0 * pi / np) * y[i] * y[j] * y[o];
+                    }
+                double c = s * (2 * lo + 1) / (4.0 * pi);
+                if (fabs(c) < 1e-10) c = 0.0;  /* quadrature noise of zero entries */
+                coef[i + (size_t)n1 * (j + (size_t)n2 * o)] = c;
+            }
+    }
+    free(xt); free(wt); free(S); free(Y);
+    return 0;
+}
+
+/* Bubble product on one bubble's radial points (bubbles.F90:3703-3756):
+ * out(:, LM) = sum over (l1 m1), (l2 m2) in that order of c f1 f2. */
+int dage_oracle_bubble_product(int l1max, int l2max, int npts, const double *f1,
+                               const double *f2, double *out) {
+    const int L = l1max + l2max;
+    const int n1 = (l1max + 1) * (l1max + 1), n2 = (l2max + 1) * (l2max + 1);
+    const int nL = (L + 1) * (L + 1);
+    double *coef = malloc(sizeof(double) * (size_t)n1 * n2 * nL);
+    if (!coef || dage_oracle_yproduct(l1max, l2max, coef)) { free(coef); return -1; }
+    memset(out, 0, sizeof(double) * (size_t)npts * nL);
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int o = 0; o < nL; ++o) {
+                const double c = coef[i + (size_t)n1 * (j + (size_t)n2 * o)];
+                if (c == 0.0) continue;
+                for (int p = 0; p < npts; ++p)
+                    out[p + (size_t)npts * o] =
+                        out[p + (size_t)npts * o] + c * f1[p + (size_t)npts * i] * f2[p + (size_t)npts * j];
+            }
+    free(coef);
+    return 0;
+}

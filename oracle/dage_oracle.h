@@ -170,6 +170,38 @@ void dage_oracle_eval_local_expansion(int lmax, const double *L, const double *x
                                       const double *y, int ny, const double *z, int nz,
                                       const double centre[3], double *out, int ld_x, int ld_y);
 
+/* ------------------------------------------------ bubble radial transforms */
+/* LIPBasis_inward_integrals / _outward_integrals (lip.F90:364-485): nlip x
+ * (nlip-1) column-major each. */
+int dage_oracle_lip_inout_integrals(int nlip, int grid_type, double *ints_in,
+                                    double *ints_out);
+/* InOutIntegrator1D_inwards / _outwards (evaluators.F90:1263-1320) on a
+ * radial grid from r = 0 (either output may be NULL). */
+int dage_oracle_integrate_inout(int ncell, int nlip, const double *steps, const double *f,
+                                double *inw, double *outw);
+/* FirstModSphBesselCollection_evaluate / SecondModSphBesselCollection_evaluate
+ * (helmholtz3d.F90:879-1000, 1133-1210) at one point, l = 0..lmax, computed in
+ * __float128 as the reference's REAL128. */
+void dage_oracle_bessel_i(int lmax, double x, double *out);
+void dage_oracle_bessel_k(int lmax, double x, double *out);
+/* Coulomb3D_evaluate_bubbles_potential (coulomb.F90:222-336) for one bubble:
+ * bf, out: npts x (lmax+1)^2 column-major; k of the input bubbles. */
+int dage_oracle_bubble_coulomb(int lmax, int k, int ncell, int nlip, const double *steps,
+                               const double *bf, double *out);
+/* Helmholtz3D_transform_bubbles_sub (helmholtz3d.F90:519-657) for one bubble
+ * (equidistant cells), kappa = sqrt(-2 energy). */
+int dage_oracle_bubble_helmholtz(int lmax, int k, int ncell, int nlip, const double *steps,
+                                 double energy, const double *bf, double *out);
+
+/* --------------------------------------------------------------- products */
+/* YProduct (harmonic.F90:884-1000): dense product coefficients of Racah real
+ * spherical harmonics, coef[i + n1*(j + n2*o)], L = l1max + l2max. */
+int dage_oracle_yproduct(int l1max, int l2max, double *coef);
+/* BubblesMultiplier_multiply without Taylor terms (bubbles.F90:3703-3756),
+ * one bubble: out (npts x (l1max+l2max+1)^2). */
+int dage_oracle_bubble_product(int l1max, int l2max, int npts, const double *f1,
+                               const double *f2, double *out);
+
 /* OpenMP threads of transform_cube and bubbles_eval_grid (their parallel
  * loops give the serial loops' results bit for bit); returns the count in
  * use (1 without OpenMP). */
