@@ -87,6 +87,7 @@ _SIGNATURES = {
     "dage_function3d_inject_extra_bubbles": [_c_p, _c_p, _c_p, _int, ctypes.POINTER(_c_p)],
     "dage_operator_operate_on": [_c_p, _c_p, _c_p, _c_p, _c_p, _int],
     "dage_bubbles_multiply": [_c_p, _c_p, _c_p],
+    "dage_bubbles_integrate": [_c_p, _dp],
     "dage_function3d_multiply": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "dage_fun_write": [ctypes.c_char_p, ctypes.c_char_p, _int, _ip, _dp, _int, _int, _int, _dp,
                        _dp, _ip, ctypes.POINTER(_dp)],

@@ -214,16 +214,29 @@ __global__ void __launch_bounds__(64) bubbles_radial_kernel(RadialArgs A) {
     }
 }
 
-// dst(lm) = a * src(lm) + beta * dst(lm) for lm < nlm_common, dst(lm) =
-// beta * dst(lm) above (src has no such l)
+// dst(lm) = a * src(lm) r^dk + beta * dst(lm) for lm < nlm_common, dst(lm)
+// = beta * dst(lm) above (src has no such l); dk = k_src - k_dst >= 0 puts
+// src into dst's r^k representation (Bubbles_add, bubbles.F90:2499-2535)
 __global__ void __launch_bounds__(256) bubble_axpby_kernel(const double *src, int nlm_src,
                                                            double a, double *dst, int nlm_dst,
-                                                           double beta, int npts) {
+                                                           double beta, int npts, int dk,
+                                                           const double *celld,
+                                                           const double *steps, int n) {
     const long total = (long)npts * nlm_dst;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
          e += (long)gridDim.x * blockDim.x) {
         const int lm = (int)(e / npts);
-        const double s = lm < nlm_src ? a * src[e] : 0.0;
+        double s = 0.0;
+        if (lm < nlm_src) {
+            double x = src[e];
+            if (dk != 0) {
+                const int i = (int)(e % npts);
+                const int c = min(i / n, (npts - 1) / n - 1);  // cell, last point in the last
+                const double r = celld[c] + (i - c * n) * steps[c];
+                x = x * powi(r, dk);
+            }
+            s = a * x;
+        }
         dst[e] = beta == 0.0 ? s : s + beta * dst[e];
     }
 }
@@ -458,13 +471,16 @@ int dage_bubbles_get(dage_bubbles b, int ibub, double *center, double *charge, d
 int dage_bubbles_axpby(double a, dage_bubbles x, double beta, dage_bubbles y) {
     DAGE_REQUIRE(x && y, DAGE_ERR_ARG, "null argument");
     DAGE_TRY(require_same_grids(x, y));
-    DAGE_REQUIRE(x->k == y->k, DAGE_ERR_ARG, "bubbles differ in k (%d, %d)", x->k, y->k);
+    DAGE_REQUIRE(x->k >= y->k, DAGE_ERR_ARG,
+                 "x has k %d below y's %d (the sum would need y's k lowered)", x->k, y->k);
     DAGE_TRY(use_ctx(y->ctx));
     for (int i = 0; i < y->nbub; ++i) {
-        const long total = (long)y->host[i].npts * y->nlm();
+        const BubbleHost &h = y->host[i];
+        const long total = (long)h.npts * y->nlm();
         hipLaunchKernelGGL(bubble_axpby_kernel, dim3(grid_blocks(total)), dim3(256), 0,
                            y->ctx->stream, x->f(i), std::min(x->nlm(), y->nlm()), a, y->f(i),
-                           y->nlm(), beta, y->host[i].npts);
+                           y->nlm(), beta, h.npts, x->k - y->k, h.celld(), h.steps(),
+                           y->nlip - 1);
         DAGE_HIP_CHECK(hipGetLastError());
     }
     return bubbles_fold_rk(y);
@@ -791,6 +807,46 @@ int dage_function3d_multiply(dage_grid3d grid, const double *cube1, dage_bubbles
     DAGE_HIP_CHECK(hipGetLastError());
     // minus the product bubbles
     if (bub_out && b1 && b2) DAGE_TRY(dage_bubbles_inject_device(bub_out, grid, 0, -1.0, cube_out));
+    return DAGE_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int dage_bubbles_integrate(dage_bubbles b, double *result) {
+    DAGE_REQUIRE(b && result, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(use_ctx(b->ctx));
+    DAGE_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    double ints[kMaxNlip];
+    DAGE_REQUIRE(lip_integrals(b->nlip, 1, ints) == 0, DAGE_ERR_ARG, "unsupported nlip");
+    const double fourpi = 4.0 * 3.141592653589793;
+    const int n = b->nlip - 1;
+    double res = 0.0;
+    for (int i = 0; i < b->nbub; ++i) {
+        const BubbleHost &h = b->host[i];
+        DAGE_REQUIRE(h.set, DAGE_ERR_STATE, "bubble %d has not been set", i);
+        std::vector<double> st(h.ncell), f(h.npts), cd(h.ncell);
+        DAGE_HIP_CHECK(hipMemcpy(st.data(), h.steps(), 8 * h.ncell, hipMemcpyDeviceToHost));
+        DAGE_HIP_CHECK(hipMemcpy(cd.data(), h.celld(), 8 * h.ncell, hipMemcpyDeviceToHost));
+        DAGE_HIP_CHECK(hipMemcpy(f.data(), b->f(i), 8 * h.npts, hipMemcpyDeviceToHost));
+        // Bubbles_integrate_pick (bubbles.F90:3146-3180): only the s part
+        // survives the angular integral; 4 pi int r^(2+k) f_00 dr with the
+        // radial LIP weights (Integrator1D_eval, evaluators.F90:1130-1145)
+        double sum = 0.0;
+        for (int c = 0; c < h.ncell; ++c) {
+            double s = 0.0;
+            for (int j = 0; j <= n; ++j) {
+                const int p = c * n + j;
+                const double r = cd[c] + j * st[c];
+                const double rp = (2 + b->k < 0 && p == 0) ? 0.0 : powi(r, 2 + b->k);
+                s += f[p] * rp * ints[j];
+            }
+            sum += s * st[c];
+        }
+        res += fourpi * sum;
+    }
+    *result = res;
     return DAGE_OK;
 }
 

@@ -24,6 +24,9 @@ module dage_ops_m
     public :: dage_coulomb3d_create, dage_operator_set_weights, dage_operator_set_energy
     public :: dage_operator_apply, dage_operator_destroy, dage_last_error_f
     public :: dage_bubbles_create, dage_bubbles_set, dage_bubbles_destroy
+    public :: dage_bubbles_create_like, dage_bubbles_get, dage_bubbles_coulomb_transform
+    public :: dage_bubbles_helmholtz_transform, dage_bubbles_multiply, dage_bubbles_integrate
+    public :: dage_operator_operate_on, dage_fun_write, dage_fun_read_info
     public :: dage_gbfmm_create, dage_gbfmm_apply, dage_gbfmm_box_count
     public :: dage_gbfmm_get_multipoles, dage_gbfmm_destroy
     public :: dage_helmholtz_batch_create, dage_batch_upload, dage_batch_download
@@ -151,6 +154,78 @@ module dage_ops_m
         integer(c_int) function dage_bubbles_destroy(bubbles) bind(C, name="dage_bubbles_destroy")
             import :: c_int, c_ptr
             type(c_ptr), value :: bubbles
+        end function
+
+        ! Bubbles(src, lmax=, k=) without content (bubbles.F90:802-830)
+        integer(c_int) function dage_bubbles_create_like(src, lmax, k, bubbles) &
+                bind(C, name="dage_bubbles_create_like")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: src
+            integer(c_int), value :: lmax, k
+            type(c_ptr)           :: bubbles
+        end function
+
+        integer(c_int) function dage_bubbles_get(bubbles, ibub, center, charge, steps, bf) &
+                bind(C, name="dage_bubbles_get")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: bubbles
+            integer(c_int), value :: ibub
+            real(c_double)        :: center(3), charge, steps(*), bf(*)
+        end function
+
+        ! Coulomb3D_evaluate_bubbles_potential (coulomb.F90:222-336)
+        integer(c_int) function dage_bubbles_coulomb_transform(bin, bout) &
+                bind(C, name="dage_bubbles_coulomb_transform")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: bin, bout
+        end function
+
+        ! Helmholtz3D_transform_bubbles_sub (helmholtz3d.F90:519-657)
+        integer(c_int) function dage_bubbles_helmholtz_transform(bin, energy, bout) &
+                bind(C, name="dage_bubbles_helmholtz_transform")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: bin, bout
+            real(c_double), value :: energy
+        end function
+
+        ! BubblesMultiplier_multiply, simple product (bubbles.F90:3553-3760)
+        integer(c_int) function dage_bubbles_multiply(a, b, bout) bind(C, name="dage_bubbles_multiply")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: a, b, bout
+        end function
+
+        integer(c_int) function dage_bubbles_integrate(bubbles, res) &
+                bind(C, name="dage_bubbles_integrate")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: bubbles
+            real(c_double)     :: res
+        end function
+
+        ! Operator3D_operate_on_Function3D (function3d.F90:3727-3792) on
+        ! device cubes (cube_in, cube_out are device addresses)
+        integer(c_int) function dage_operator_operate_on(op, cube_in, bin, cube_out, bout, &
+                                                         only_cube) bind(C, name="dage_operator_operate_on")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: op, cube_in, bin, cube_out, bout
+            integer(c_int), value :: only_cube
+        end function
+
+        ! Function3D_dump / Function3D_load (function3d.F90:812-935), host arrays
+        integer(c_int) function dage_fun_write(path, label, ftype, shape, cube, nbub, lmax, k, &
+                                               charges, centers, npts, bf) bind(C, name="dage_fun_write")
+            import :: c_int, c_ptr, c_double, c_char
+            character(kind=c_char) :: path(*), label(*)
+            integer(c_int), value  :: ftype, nbub, lmax, k
+            integer(c_int)         :: shape(3), npts(*)
+            real(c_double)         :: cube(*), charges(*), centers(*)
+            type(c_ptr)            :: bf(*)
+        end function
+
+        integer(c_int) function dage_fun_read_info(path, label, ftype, shape, nbub, lmax, k) &
+                bind(C, name="dage_fun_read_info")
+            import :: c_int, c_char
+            character(kind=c_char) :: path(*), label(*)
+            integer(c_int)         :: ftype, shape(3), nbub, lmax, k
         end function
 
         ! GBFMMCoulomb3D (gbfmm_coulomb3d.F90:306-1330)

@@ -47,6 +47,49 @@ class Function3D:
                 obj.close()
         self.cube = self.bubbles = None
 
+    # ------------------------------------------------------------- arithmetic
+    def copy(self) -> "Function3D":
+        """Function3D(self, copy_content=.true.)"""
+        f = Function3D(self.grid, type=self.type, label=self.label)
+        f.cube.copy_from(self.cube)
+        if self.bubbles is not None:
+            f.bubbles = Bubbles.like(self.bubbles, k=self.bubbles.k)
+            f.bubbles.axpby(1.0, self.bubbles, 0.0)
+        return f
+
+    def axpby(self, a: float, x: "Function3D", b: float):
+        """self = a x + b self, cube and bubbles (add_in_place,
+        subtract_in_place, product_in_place_REAL64 of function3d.F90)."""
+        n = int(np.prod(self.grid.get_shape()))
+        call("dage_vec_axpby", self.ctx.handle, n, float(a), ctypes.c_void_p(x.cube.ptr),
+             float(b), ctypes.c_void_p(self.cube.ptr))
+        if x.bubbles is not None and self.bubbles is not None:
+            self.bubbles.axpby(a, x.bubbles, b)
+        elif self.bubbles is not None:
+            self.bubbles.axpby(0.0, self.bubbles, b)
+        elif x.bubbles is not None:
+            self.bubbles = Bubbles.like(x.bubbles, k=x.bubbles.k)
+            self.bubbles.axpby(a, x.bubbles, 0.0)
+
+    def scale(self, a: float):
+        self.axpby(0.0, self, a)
+
+    def integrate(self) -> float:
+        """Function3D_integrate (function3d.F90:1793-1846): the cube's LIP
+        integral plus the bubbles' radial integrals."""
+        val = self.grid.integrate(self.cube)
+        if self.bubbles is not None:
+            val += self.bubbles.integrate()
+        return val
+
+    def dot(self, f2: "Function3D") -> float:
+        """Function3D_dot_product (function3d.F90:1497-1530): integral of the
+        product (bubbles multiplied centre by centre)."""
+        p = self.multiply(f2)
+        val = p.integrate()
+        p.close()
+        return val
+
     # ---------------------------------------------------------------- bubbles
     def inject_bubbles(self) -> DeviceCube:
         """A new cube holding the bubbles evaluated on the grid (the cube part

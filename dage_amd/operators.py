@@ -431,6 +431,12 @@ class Bubbles:
         """self = a x + b self (l above x's lmax: b self)."""
         call("dage_bubbles_axpby", float(a), x.handle, float(b), self.handle)
 
+    def integrate(self) -> float:
+        """sum over bubbles of 4 pi int r^(2+k) f_00 dr (Bubbles_integrate_all)."""
+        r = ctypes.c_double()
+        call("dage_bubbles_integrate", self.handle, ctypes.byref(r))
+        return r.value
+
     def multiply(self, other: "Bubbles", out: "Bubbles | None" = None) -> "Bubbles":
         """Bubble-by-bubble product (lmax and k add up)."""
         out = out if out is not None else Bubbles.like(self, lmax=self.lmax + other.lmax,

@@ -224,9 +224,10 @@ int dage_bubbles_npoints(dage_bubbles bubbles, int ibub, int *npts);
  * any output may be NULL; synchronises. */
 int dage_bubbles_get(dage_bubbles bubbles, int ibub, double *center, double *charge,
                      double *steps, double *bf);
-/* y = a*x + beta*y on bubbles with the same grids and k (Bubbles_add and
- * Bubbles_times_REAL64, src/bubbles/bubbles.F90:2499-2535, 3311-3330); l
- * above x's lmax: y = beta*y.  Asynchronous. */
+/* y = a*x + beta*y on bubbles with the same grids (Bubbles_add and
+ * Bubbles_times_REAL64, src/bubbles/bubbles.F90:2499-2535, 3311-3330): x's
+ * k may exceed y's (x enters as r^(kx-ky) f_x in y's representation), not
+ * fall below it; l above x's lmax: y = beta*y.  Asynchronous. */
 int dage_bubbles_axpby(double a, dage_bubbles x, double beta, dage_bubbles y);
 /* Coulomb3D_evaluate_bubbles_potential (src/bubbles/coulomb.F90:222-336):
  * out = radial Poisson potential of in (out: same grids and lmax, k = 0),
@@ -260,6 +261,10 @@ int dage_function3d_inject_extra_bubbles(dage_grid3d grid, double *cube_dev,
  * the context stream. */
 int dage_operator_operate_on(dage_operator op, const double *cube_in, dage_bubbles bub_in,
                              double *cube_out, dage_bubbles bub_out, int only_cube);
+/* Bubbles_integrate_all (src/bubbles/bubbles.F90:3146-3220): sum over
+ * bubbles of 4 pi int r^(2+k) f_00(r) dr with the radial LIP weights (only
+ * the s part survives the angular integral); synchronises. */
+int dage_bubbles_integrate(dage_bubbles bubbles, double *result);
 /* BubblesMultiplier_multiply, simple product (src/bubbles/bubbles.F90:
  * 3553-3760): out = a * b centre by centre, g_LM = sum c(l1m1,l2m2->LM) f1 f2
  * with the product coefficients of the Racah harmonics (YProduct,

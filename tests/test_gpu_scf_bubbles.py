@@ -1,0 +1,174 @@
+"""The Helmholtz orbital update on orbitals that carry bubbles (row A27),
+HelmholtzSCFCycle_update_orbitals_worker (scf_cycle.F90:2927-3054), on the
+GPU against the same steps composed from the oracle's pinned pieces
+(product, cube transform with coda, radial Bessel convolution, inject,
+integrals), with a nuclear-potential bubble (-Z/r, k = -1) in V so that the
+bubbles of V psi, of the Helmholtz result and of the updated orbital are all
+exercised; one eigenvalue is positive (the level-shift path).  Bar 1e-10 of
+max|psi|.  At the config-4 shape (10 atoms, 12 orbitals with lmax-4 bubbles
+on 199-point radial grids, 129^3) the update keeps every orbital normalised
+(1e-12) and leaves the Helmholtz operator at energy 0.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dage_amd
+import dage_amd.function3d as F3
+import dage_amd.scf as scf
+import oracle as O
+from _helpers import rel_maxdiff
+
+pytestmark = pytest.mark.gpu
+
+
+def radial_weights(st, nlip=7):
+    ints = O.lip_integrals(nlip)
+    r = O.Axis(0.0, st, nlip).points
+    return r, ints
+
+
+def oracle_bubble_integral(bset):
+    """sum_b 4 pi sum_cells (sum_j f00 r^(2+k) ints_j) h_c (bubbles.F90:3146-3220)."""
+    total = 0.0
+    for st, bf in zip(bset.steps, bset.bf):
+        r, ints = radial_weights(st, bset.nlip)
+        n = bset.nlip - 1
+        g = bf[:, 0] * (r ** (2 + bset.k) if 2 + bset.k >= 0 else np.where(r > 0, r, 1) ** (2 + bset.k))
+        s = 0.0
+        for c, h in enumerate(st):
+            s += np.sum(g[c * n:c * n + n + 1] * ints) * h
+        total += 4 * math.pi * s
+    return total
+
+
+def build(ctx, ax, centers, Z, lmax, rng, norb, ncell=33):
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    P = ax.points
+    steps = [O.radial_steps(z, ncell, 7, 20.0) for z in Z]
+    # V: a smooth cube part and -Z/r nuclear bubbles (k = -1, f_00 = -Z)
+    X, Y, Zc = np.meshgrid(P, P, P, indexing="ij")
+    vcube = -1.0 / (1.0 + X ** 2 + Y ** 2 + Zc ** 2)
+    vb = dage_amd.Bubbles(ctx, len(Z), 0, -1, 7)
+    vbf = []
+    for i, z in enumerate(Z):
+        r = O.Axis(0.0, steps[i], 7).points
+        bf = np.full((len(r), 1), -float(z), order="F")
+        vb.set_bubble(i, centers[i], z, steps[i], bf)
+        vbf.append(bf)
+    V = F3.Function3D(g, dage_amd.DeviceCube(ctx, vcube.shape, np.asfortranarray(vcube)), vb)
+    orbs, obfs, ocubes = [], [], []
+    for o in range(norb):
+        a = rng.uniform(0.8, 2.0)
+        c0 = centers[o % len(centers)]
+        cube = np.asfortranarray(np.exp(-a * ((X - c0[0]) ** 2 + (Y - c0[1]) ** 2 + (Zc - c0[2]) ** 2)))
+        b = dage_amd.Bubbles(ctx, len(Z), lmax, 0, 7)
+        bfs = []
+        for i, z in enumerate(Z):
+            r = O.Axis(0.0, steps[i], 7).points
+            coef = rng.uniform(-0.5, 0.5, (lmax + 1) ** 2)
+            bf = np.asfortranarray(np.outer(np.exp(-z * r), np.ones((lmax + 1) ** 2)) * coef
+                                   * np.array([r ** int(np.sqrt(k)) for k in range((lmax + 1) ** 2)]).T)
+            b.set_bubble(i, centers[i], z, steps[i], bf)
+            bfs.append(bf)
+        orbs.append(F3.Function3D(g, dage_amd.DeviceCube(ctx, cube.shape, cube), b,
+                                  type=F3.F3D_TYPE_CUSP))
+        obfs.append(bfs)
+        ocubes.append(cube)
+    return g, V, vcube, vbf, steps, orbs, ocubes, obfs
+
+
+def test_update_with_bubbles_matches_oracle(ctx):
+    rng = np.random.default_rng(12)
+    ax = O.Axis.ncell_fixed(-4.0, 4.0, 7, 6)  # 37^3
+    centers = np.array([[0.8, 0.1, -0.2], [-0.7, -0.2, 0.3]])
+    Z = [2.0, 1.0]
+    lmax = 2
+    g, V, vcube, vbf, steps, orbs, ocubes, obfs = build(ctx, ax, centers, Z, lmax, rng, 2)
+    eigs = [-0.9, 0.2]
+    quad = dage_amd.GaussQuad(10, 10, tlin=2.0, tlog=500.0)
+    op = dage_amd.Helmholtz3D(ctx, g, g, quad)
+    pots = [V.multiply(psi) for psi in orbs]
+    scf.helmholtz_update_functions(op, orbs, pots, eigs, scaling_factor=0.7)
+    # the same with the oracle's pieces
+    P = ax.points
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, quad.tp)
+    vset = O.BubbleSet(centers, steps, vbf, 0, -1, 7)
+    BV = O.bubbles_eval_grid(vset, P, P, P)
+    for o in range(2):
+        pset = O.BubbleSet(centers, steps, obfs[o], lmax, 0, 7)
+        Bp = O.bubbles_eval_grid(pset, P, P, P)
+        # V psi: bubbles (k = -1) and cube
+        prod_bf = [O.bubble_product(0, vbf[i], lmax, obfs[o][i]) for i in range(2)]
+        prod_set = O.BubbleSet(centers, steps, prod_bf, lmax, -1, 7)
+        in_cube = (ocubes[o] + Bp) * (vcube + BV) - O.bubbles_eval_grid(prod_set, P, P, P)
+        in_bf = [b.copy() for b in prod_bf]
+        e = eigs[o]
+        if e > 0:
+            in_cube = in_cube - (e + 0.1) * ocubes[o]
+            # the orbital bubbles (k = 0) enter the k = -1 bubbles as r f
+            # (Bubbles_add, bubbles.F90:2499-2535)
+            in_bf = [in_bf[i] - (e + 0.1) * (obfs[o][i] * O.Axis(0.0, steps[i], 7).points[:, None])
+                     for i in range(2)]
+            e = -0.1
+        Gc = O.transform_cube(fx, fy, fz, O.helmholtz_weights(quad.tp, quad.tw, e), O.coda(500.0),
+                              in_cube) * (-1 / (2 * math.pi))
+        Gb = [O.bubble_helmholtz(lmax, -1, steps[i], 7, e, in_bf[i]) * (-1 / (2 * math.pi))
+              for i in range(2)]
+        new_c = ocubes[o] + 0.7 * (Gc - ocubes[o])
+        new_b = [obfs[o][i] + 0.7 * (Gb[i] - obfs[o][i]) for i in range(2)]
+        nset = O.BubbleSet(centers, steps, new_b, lmax, 0, 7)
+        Bn = O.bubbles_eval_grid(nset, P, P, P)
+        sq_b = [O.bubble_product(lmax, new_b[i], lmax, new_b[i]) for i in range(2)]
+        sq_set = O.BubbleSet(centers, steps, sq_b, 2 * lmax, 0, 7)
+        sq_cube = (new_c + Bn) * (new_c + Bn) - O.bubbles_eval_grid(sq_set, P, P, P)
+        w = ax.ints()
+        norm = O.cube_integral(w, w, w, sq_cube) + oracle_bubble_integral(sq_set)
+        new_c, new_b = new_c / math.sqrt(norm), [b / math.sqrt(norm) for b in new_b]
+        got_c = orbs[o].cube.download()
+        ec = rel_maxdiff(got_c, new_c)
+        eb = max(rel_maxdiff(orbs[o].bubbles.get(i)[3], new_b[i]) for i in range(2))
+        print(f"orbital {o}: cube {ec:.2e}, bubbles {eb:.2e}")
+        assert ec < 1e-10 and eb < 1e-10
+    for obj in pots + orbs + [V, op, g]:
+        obj.close()
+
+
+def test_update_with_bubbles_config4_shape(ctx):
+    rng = np.random.default_rng(4)
+    mol = scf.random_molecule(np.random.default_rng(0), 10)
+    ax = O.Axis(-9.6, np.full(32, 0.15), 5)  # 129^3 (nlip 5)
+    grid = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 5)
+    P = ax.points
+    X, Y, Zc = np.meshgrid(P, P, P, indexing="ij")
+    steps = [O.radial_steps(z, 33, 7, 20.0) for z in mol.Z]
+    vb = dage_amd.Bubbles(ctx, 10, 0, -1, 7)
+    for i, z in enumerate(mol.Z):
+        r = O.Axis(0.0, steps[i], 7).points
+        vb.set_bubble(i, mol.R[i], z, steps[i], np.full((len(r), 1), -float(z), order="F"))
+    V = F3.Function3D(grid, dage_amd.DeviceCube(ctx, X.shape), vb)
+    V.cube.set_to_zero()
+    orbs = []
+    for o in range(12):
+        c0 = mol.R[o % 10]
+        cube = np.asfortranarray(np.exp(-((X - c0[0]) ** 2 + (Y - c0[1]) ** 2 + (Zc - c0[2]) ** 2)))
+        b = dage_amd.Bubbles(ctx, 10, 4, 0, 7)
+        for i, z in enumerate(mol.Z):
+            r = O.Axis(0.0, steps[i], 7).points
+            b.set_bubble(i, mol.R[i], z, steps[i],
+                         np.asfortranarray(rng.uniform(-0.1, 0.1, (len(r), 25))
+                                           * np.exp(-z * r)[:, None]))
+        orbs.append(F3.Function3D(grid, dage_amd.DeviceCube(ctx, cube.shape, cube), b))
+    pots = [V.multiply(psi) for psi in orbs]
+    quad = dage_amd.GaussQuad(20, 20, tlin=2.0, tlog=500.0)
+    op = dage_amd.Helmholtz3D(ctx, grid, grid, quad)
+    eigs = np.linspace(-2.0, 0.3, 12)
+    scf.helmholtz_update_functions(op, orbs, pots, eigs)
+    for o in range(12):
+        n = orbs[o].dot(orbs[o])
+        assert abs(n - 1.0) < 1e-12, (o, n)
+        assert orbs[o].bubbles.lmax == 4
+    assert op.energy == 0.0
+    for obj in pots + orbs + [V, op, grid]:
+        obj.close()
