@@ -189,7 +189,8 @@ def bench_config2(ctx, args):
         step()
     ms = ctx.time_end() / steps
     wall = time.perf_counter() - wall0
-    names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+    names = ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
+             "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm")
@@ -201,12 +202,14 @@ def bench_config2(ctx, args):
                           "ms_per_step": round(kms / steps, 4)}
     ctx.profile(False)
     T = gq.get_npoints()
-    # dominant kernel: largest time per step; GEMM passes do 2 T n^4 dense
-    # FLOPs per apply (the banded near-field matrices are applied densely)
+    # dominant kernel: largest time per step; a GEMM pass does 2 T n^4
+    # dense-equivalent FLOPs per apply, the fused x/y pass twice that (the
+    # zero K panels of the banded near-field matrices are skipped but counted)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
-    if dom.startswith("gemm_f64"):
-        per_launch = 2.0 * T * float(n) ** 4 / dk["launches_per_step"]
+    if dom.startswith("gemm_f64") or dom == "xy_fused_kernel":
+        passes = 2.0 if dom == "xy_fused_kernel" else 1.0
+        per_launch = passes * 2.0 * T * float(n) ** 4 / dk["launches_per_step"]
         achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -326,7 +329,8 @@ def bench_config1(ctx, args):
         op.transform_cuda_cube(d_in, d_out)
     ms = ctx.time_end() / steps
     kernels = {}
-    for name in ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel"):
+    for name in ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
+                 "splitk_reduce_kernel"):
         tot, k = ctx.profile_query(name)
         if k:
             kernels[name] = {"ms_per_launch": round(tot / k, 4), "launches_per_apply": k // steps}
@@ -665,7 +669,8 @@ def bench_config5(ctx, args):
         step(eager=True)
     ms = ctx.time_end() / args.steps
     wall = time.perf_counter() - wall0
-    names = ("gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "splitk_reduce_kernel",
+    names = ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
+             "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm", "cube_copy_kernel")
@@ -684,19 +689,22 @@ def bench_config5(ctx, args):
     progress(f"config5: {ms:.0f} ms per step (graph {ms_graph:.0f} ms)")
     T, Tn = hq.get_npoints(), gq.get_npoints()
     # dominant kernel of the step; a mode pass does 2 T n^4 dense FLOPs per
-    # right-hand side (near field: per apply), counted dense although the
-    # all-zero K panels of the banded matrices are skipped
+    # right-hand side (near field: per apply), the fused x/y pass twice that,
+    # counted dense although the all-zero K panels (products with zero matrix
+    # entries outside the bands) of the banded matrices are skipped
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
-    if dom.startswith("gemm_f64"):
-        per_step = 2.0 * float(n) ** 4 * (T * nrhs + Tn)  # sweep + near field, per pass
+    if dom.startswith("gemm_f64") or dom == "xy_fused_kernel":
+        passes = 2.0 if dom == "xy_fused_kernel" else 1.0
+        per_step = passes * 2.0 * float(n) ** 4 * (T * nrhs + Tn)  # sweep + near field
         per_launch = per_step / dk["launches_per_step"]
         achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
                     "traffic": measured_traffic("config5", dom),
-                    "flops": "dense-equivalent (2 T n^4 per pass and right-hand side)",
+                    "flops": "dense-equivalent (2 T n^4 per pass and right-hand side; the "
+                             "fused x/y kernel does two passes)",
                     "avg_launch_ms": dk["ms_per_launch"],
                     "algorithmic_flops_per_launch": per_launch}
         exe = measured_executed_fraction("config5", dom)

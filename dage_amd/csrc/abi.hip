@@ -139,6 +139,10 @@ void free_operator_buffers(dage_operator op) {
     op->d_mask = nullptr;
     if (op->d_tmask) (void)hipFree(op->d_tmask);
     op->d_tmask = nullptr;
+    if (op->d_xband) (void)hipFree(op->d_xband);
+    op->d_xband = nullptr;
+    if (op->d_yband) (void)hipFree(op->d_yband);
+    op->d_yband = nullptr;
     op->planes_valid = false;
     op->ws_bytes = 0;
 }
@@ -188,6 +192,12 @@ int dage_ctx_create(int device, dage_ctx *out) {
         return DAGE_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    int rc = prepare_fused_kernels();
+    if (rc != DAGE_OK) {
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return DAGE_OK;
 }
@@ -278,6 +288,14 @@ int dage_cube_create(dage_ctx ctx, const int shape[3], dage_cube *out) {
         set_error("hipMalloc of a %d x %d x %d cube failed: %s", shape[0], shape[1], shape[2],
                   hipGetErrorString(e));
         return DAGE_ERR_ALLOC;
+    }
+    // a new cube holds zeros (cube += bubbles and the other accumulating
+    // calls may be its first writers)
+    const int rc = fill_zero(ctx, c->d, c->count());
+    if (rc != DAGE_OK) {
+        (void)hipFree(c->d);
+        delete c;
+        return rc;
     }
     ctx_retain(ctx);
     *out = c;

@@ -53,7 +53,44 @@ struct Gemm {
     // >0: column tile j is [j*pitch, (j+1)*pitch), the last one running to N
     // (each at most 16*NB wide), and bit j of a mask row stands for tile j
     int pitch;
+    // Tile grid (row tiles, column tiles, batch indices), set at launch; the
+    // launch itself is one-dimensional (see block_coords).
+    int gx, gy, gz;
+    int remap;  // 0: plain row-tile-fastest order (for comparison runs)
 };
+
+// Block coordinates of a one-dimensional launch.  Hardware hands block L to
+// XCD L % 8, so L is first mapped to a logical tile index that gives each XCD
+// one contiguous range of the tile order; the order walks column tiles
+// fastest, so the blocks that share an A row tile run together on one XCD and
+// read it through that XCD's L2 once instead of once per column tile.
+// BATCH_INNER (the x pass, whose A is the input cube for every t) walks the
+// batch index z_lo before the row tile for the same reason.
+template <bool BATCH_INNER>
+__device__ inline void block_coords(const Gemm &g, int *bx, int *by, int *bz) {
+    unsigned L = blockIdx.x;
+    if (g.remap) {
+        const unsigned total = gridDim.x;
+        const unsigned q = total >> 3, r = total & 7u, x = L & 7u;
+        L = x * q + min(x, r) + (L >> 3);
+        *by = (int)(L % (unsigned)g.gy);
+        L /= (unsigned)g.gy;
+        if (BATCH_INNER && g.Z0 > 0) {
+            const unsigned zl = L % (unsigned)g.Z0;
+            L /= (unsigned)g.Z0;
+            *bx = (int)(L % (unsigned)g.gx);
+            *bz = (int)(zl + (unsigned)g.Z0 * (L / (unsigned)g.gx));
+        } else {
+            *bx = (int)(L % (unsigned)g.gx);
+            *bz = (int)(L / (unsigned)g.gx);
+        }
+    } else {
+        *bx = (int)(L % (unsigned)g.gx);
+        L /= (unsigned)g.gx;
+        *by = (int)(L % (unsigned)g.gy);
+        *bz = (int)(L / (unsigned)g.gy);
+    }
+}
 
 constexpr int kMaxPanels = 2048;  // K panels a block can list (K <= 32768)
 
@@ -88,17 +125,19 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int m0 = blockIdx.x * BM;
-    const int n0 = g.pitch > 0 ? blockIdx.y * g.pitch : blockIdx.y * BN;
-    const int nend = g.pitch > 0 && blockIdx.y + 1 < gridDim.y ? n0 + g.pitch : g.N;
+    int bx, by, bz;
+    block_coords<PASS == 0>(g, &bx, &by, &bz);
+    const int m0 = bx * BM;
+    const int n0 = g.pitch > 0 ? by * g.pitch : by * BN;
+    const int nend = g.pitch > 0 && by + 1 < g.gy ? n0 + g.pitch : g.N;
 
     const double *A = g.A;
     const double *B = g.B;
     double *C = g.C;
     double alpha = 1.0;
     int kbeg = 0, kend = g.K;
-    const int zlo = g.Z0 > 0 ? (int)blockIdx.z % g.Z0 : (int)blockIdx.z;
-    const int zhi = g.Z0 > 0 ? (int)blockIdx.z / g.Z0 : 0;
+    const int zlo = g.Z0 > 0 ? bz % g.Z0 : bz;
+    const int zhi = g.Z0 > 0 ? bz / g.Z0 : 0;
     A += (long)zhi * g.bA1;
     B += (long)zhi * g.bB1;
     C += (long)zlo * g.bC + (long)zhi * g.bC1;
@@ -205,7 +244,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const bool listed = mrow != nullptr && npanels <= kMaxPanels;
     if (listed) {
         if (wave == 0) {
-            const uint64_t cols = g.pitch > 0 ? 1ull << blockIdx.y
+            const uint64_t cols = g.pitch > 0 ? 1ull << by
                                               : ((1ull << NB) - 1ull) << (n0 / 16);
             int cnt = 0;
             for (int base = 0; base < npanels; base += 64) {
@@ -313,6 +352,176 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     }
 }
 
+// Fused x and y passes: for one (right-hand side b, t, input z-plane zi) and
+// 64 output x points (16 per wavefront),
+//   U(xo, yo) = w_t sum_yi [sum_xi Fx_t(xo,xi) in(xi,yi,zi)] Fy_t(yi,yo),
+// without the x-pass intermediate V reaching memory.  The input y axis is
+// walked in chunks of 16 points.  Per chunk a wavefront first forms the
+// 16 x 16 tile V^T(yi, xo) = in^T(yi, xi) Fx_t^T(xi, xo) on the MFMA (A = the
+// input chunk from LDS, B = Fx_t from global memory); result register r of
+// lane l then holds V(xo = l%16, yi = 4r + l/16), which is exactly the
+// A-operand element of k-step r of the y product, so the tile feeds
+//   U(xo, yo) += V(xo, yi-chunk) Fy_t(yi-chunk, yo)
+// straight from registers (B = the Fy_t chunk from LDS), one accumulator per
+// 16-point output y tile.  The k-grouping, operand roles and k order are those
+// of the separate x and y passes, and only products with a zero matrix entry
+// outside the bands are skipped, so U is bit-identical to theirs.
+//
+// The input chunk (16 columns of nxi points, contiguous in memory) and the
+// Fy_t chunk (16 rows) are staged in LDS, the next chunk's loads issued into
+// registers before the current chunk's MFMAs.  NT >= the output y tiles and
+// NT >= nxi / 16 (it also sizes the loader's registers).
+struct XYF {
+    const double *in;   // in_b(xi, yi, zi) at in + b*nin + xi + nxi*(yi + nyi*zi)
+    const double *fx;   // Fx_t(xo, xi) at fx + xo + nxo*(xi + nxi*t)
+    const double *fy;   // Fy_t(yi, yo) at fy + yi + nyi*(yo + nyo*t)
+    double *U;          // U + xo + nxo*(yo + nyo*zi) + Ut*(tl + tc*b)
+    const double *w;    // alpha = w[t0 + tl + wS*b]
+    long wS;
+    int nxi, nyi, nxo, nyo, nzi;
+    int t0, tc;
+    long nin, Ut;
+    const int2 *xband, *yband;
+    int nxt, nyc;
+    int gx;      // 64-point output x blocks
+    int ldy;     // LDS row count of the Fy chunk (16 * output y tiles)
+    int remap;
+};
+
+template <int NT>
+__global__ void __launch_bounds__(kThreads) xy_fused_kernel(XYF g) {
+    extern __shared__ double smem[];
+    double *Is = smem;                   // input chunk: (xi, yl) at yl*nxi + xi
+    double *Ys = smem + 16 * g.nxi;      // Fy chunk: (yl, yo) at yo*17 + yl
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int l16 = lane & 15, lk = lane >> 4;
+
+    // block -> (b, tl, zi, xb), xb fastest; XCD-aware as in block_coords
+    unsigned L = blockIdx.x;
+    if (g.remap) {
+        const unsigned total = gridDim.x;
+        const unsigned q = total >> 3, r = total & 7u, x = L & 7u;
+        L = x * q + min(x, r) + (L >> 3);
+    }
+    const int xb = (int)(L % (unsigned)g.gx);
+    L /= (unsigned)g.gx;
+    const int zi = (int)(L % (unsigned)g.nzi);
+    L /= (unsigned)g.nzi;
+    const int tl = (int)(L % (unsigned)g.tc);
+    const int b = (int)(L / (unsigned)g.tc);
+    const int t = g.t0 + tl;
+
+    const double *in = g.in + (long)b * g.nin + (long)zi * g.nxi * g.nyi;
+    const double *fx = g.fx + (long)t * g.nxo * g.nxi;
+    const double *fy = g.fy + (long)t * g.nyi * g.nyo;
+    const int xt = xb * 4 + wave;  // this wavefront's 16-point output x tile
+    const bool active = xt < g.nxt;
+    const int xo = xt * 16 + l16;
+    const bool xo_ok = xo < g.nxo;
+    const int2 xr = active ? g.xband[(long)t * g.nxt + xt] : make_int2(0, 0);
+    const double *fxl = fx + (xo_ok ? xo : 0);
+
+    // loaders: input chunk element e = tid + 256*s (contiguous in memory),
+    // Fy chunk element e: yl = e % 16, yo = e / 16
+    const int ichunk = 16 * g.nxi;
+    double ri[NT], ry[NT];
+    auto load_chunk = [&](int c) {
+        const int yc = 16 * c;
+        const int ilim = min(16, g.nyi - yc) * g.nxi;  // valid elements
+        const double *src = in + (long)yc * g.nxi;
+#pragma unroll
+        for (int s = 0; s < NT; ++s) {
+            const int e = tid + kThreads * s;
+            const bool ok = e < ilim;
+            const double v = src[ok ? e : 0];
+            ri[s] = ok ? v : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < NT; ++s) {
+            const int e = tid + kThreads * s;
+            const int yl = e & 15, yo = e >> 4;
+            const bool ok = yo < g.nyo && yc + yl < g.nyi;
+            const double v = fy[ok ? (long)(yc + yl) + (long)g.nyi * yo : 0];
+            ry[s] = ok ? v : 0.0;
+        }
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int s = 0; s < NT; ++s) {
+            const int e = tid + kThreads * s;
+            if (e < ichunk) Is[e] = ri[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NT; ++s) {
+            const int e = tid + kThreads * s;
+            const int yl = e & 15, yo = e >> 4;
+            if (yo < g.ldy) Ys[yo * 17 + yl] = ry[s];
+        }
+    };
+
+    d4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const int nchunks = (g.nyi + 15) / 16;
+    load_chunk(0);
+    for (int c = 0; c < nchunks; ++c) {
+        store_chunk();
+        __syncthreads();
+        if (c + 1 < nchunks) load_chunk(c + 1);
+        const int2 yr = g.yband[(long)t * g.nyc + c];
+        if (active && yr.x < yr.y && xr.x < xr.y) {
+            // x part: V^T(yi, xo) over the k-steps of this tile's band
+            d4 v = d4{0.0, 0.0, 0.0, 0.0};
+            const double *ia = Is + l16 * g.nxi;
+            for (int s0 = xr.x; s0 < xr.y; s0 += 8) {
+                double fa[8], fb[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int xi = 4 * (s0 + u) + lk;
+                    const bool ok = s0 + u < xr.y && xi < g.nxi;
+                    const double bv = fxl[ok && xo_ok ? (long)g.nxo * xi : 0];
+                    fb[u] = ok && xo_ok ? bv : 0.0;
+                    fa[u] = ok ? ia[ok ? xi : 0] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (s0 + u < xr.y) v = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u], fb[u], v, 0, 0, 0);
+            }
+            // y part: k-steps with some input y point inside the grid
+            const int nks = min(4, (g.nyi - 16 * c + 3) / 4);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                if (s < nks) {
+#pragma unroll
+                    for (int j = 0; j < NT; ++j)
+                        if (j >= yr.x && j < yr.y) {
+                            const double bv = Ys[(16 * j + l16) * 17 + 4 * s + lk];
+                            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[s], bv, acc[j], 0, 0, 0);
+                        }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    // Register r of lane l holds U(xo = xt*16 + 4r + l/16, yo = 16j + l%16).
+    const double alpha = g.w[g.t0 + tl + g.wS * b];
+    double *U = g.U + (long)zi * g.nxo * g.nyo + g.Ut * ((long)tl + (long)g.tc * b);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int yo = 16 * j + l16;
+        if (yo >= g.nyo) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = xt * 16 + 4 * r + lk;
+            if (x < g.nxo) U[x + (long)g.nxo * yo] = alpha * acc[j][r];
+        }
+    }
+}
+
 // Fz (nzi, nzo, T) -> Gz((zi + nzi*t), zo)
 __global__ void __launch_bounds__(256) relayout_fz_kernel(const double *fz, double *gz, int nzi,
                                                           int nzo, int T) {
@@ -375,14 +584,32 @@ void launch_t(const Gemm &g, dim3 grid, hipStream_t s) {
     hipLaunchKernelGGL((gemm_f64_kernel<kMW, NB, AM, BK, PASS>), grid, dim3(kThreads), 0, s, g);
 }
 
+// XCD-aware block order (block_coords); DAGE_APPLY_REMAP=0 restores the
+// plain order for comparison runs (read at every launch).
+int remap_default() {
+    const char *e = std::getenv("DAGE_APPLY_REMAP");
+    return (e && e[0] == '0') ? 0 : 1;
+}
+
 template <bool AM, bool BK, int PASS>
-int launch_nb(const Gemm &g, int nb, int zdim, hipStream_t s) {
+int launch_nb(const Gemm &g0, int nb, int zdim, hipStream_t s) {
     const int BM = 64 * kMW;
-    const int ntiles = g.pitch > 0 ? pitch_tiles(g.N, g.pitch) : (g.N + 16 * nb - 1) / (16 * nb);
-    DAGE_REQUIRE(g.pitch <= 0 || (ntiles <= 64 && g.pitch <= 16 * nb &&
-                                  g.N - (ntiles - 1) * g.pitch <= 16 * nb),
-                 DAGE_ERR_STATE, "column pitch %d does not fit %d-column tiles", g.pitch, 16 * nb);
-    const dim3 grid((g.M + BM - 1) / BM, ntiles, zdim);
+    const int ntiles =
+        g0.pitch > 0 ? pitch_tiles(g0.N, g0.pitch) : (g0.N + 16 * nb - 1) / (16 * nb);
+    DAGE_REQUIRE(g0.pitch <= 0 || (ntiles <= 64 && g0.pitch <= 16 * nb &&
+                                   g0.N - (ntiles - 1) * g0.pitch <= 16 * nb),
+                 DAGE_ERR_STATE, "column pitch %d does not fit %d-column tiles", g0.pitch, 16 * nb);
+    Gemm g = g0;
+    g.gx = (g.M + BM - 1) / BM;
+    g.gy = ntiles;
+    g.gz = zdim;
+    g.remap = remap_default();
+    const long blocks = (long)g.gx * g.gy * g.gz;
+    DAGE_REQUIRE(g.Z0 <= 0 || zdim % g.Z0 == 0, DAGE_ERR_STATE,
+                 "batch grid %d is not a multiple of %d", zdim, g.Z0);
+    DAGE_REQUIRE(blocks > 0 && blocks < (1L << 31), DAGE_ERR_ARG,
+                 "GEMM of %ld tiles does not fit one launch", blocks);
+    const dim3 grid((unsigned)blocks);
     switch (nb) {
     case 1: launch_t<1, AM, BK, PASS>(g, grid, s); break;
     case 2: launch_t<2, AM, BK, PASS>(g, grid, s); break;
@@ -413,7 +640,69 @@ int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim, int nb_
     return rc;
 }
 
-// Workspace layout: V | U | P (split-K partials)
+// Fused x/y pass (xy_fused_kernel): opt-in with DAGE_APPLY_FUSED=1, when the
+// bands are built and both the output y axis and the input x axis fit the
+// largest register tile.  Measured on MI355X it is 1.5x (97^3) to 4.4x
+// (385^3) slower than the separate x and y passes: per chunk each wavefront's
+// x product is one dependent MFMA chain fed by global loads of Fx_t, at one
+// wavefront per SIMD once the y accumulators pass 128 registers.  Returns
+// the kernel's NT (0: separate passes).
+constexpr int kMaxNT = 28;
+
+int fused_nt(const ApplyMats &m) {
+    if (!m.xband || !m.yband) return 0;
+    const char *e = std::getenv("DAGE_APPLY_FUSED");
+    if (!e || e[0] != '1') return 0;
+    const int need = std::max((m.dout[1] + 15) / 16, (m.din[0] + 15) / 16);
+    if (need > kMaxNT) return 0;
+    return (need + 3) / 4 * 4;
+}
+
+// Largest dynamic LDS of the fused pass: 16 input columns of up to 16*kMaxNT
+// points and 16*kMaxNT Fy rows of 17 doubles.
+constexpr size_t kFusedMaxLds = 8 * (16 * 16 * kMaxNT + 17 * 16 * kMaxNT);
+
+template <int NT>
+int set_xy_attr() {
+    DAGE_HIP_CHECK(hipFuncSetAttribute((const void *)xy_fused_kernel<NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kFusedMaxLds));
+    return DAGE_OK;
+}
+
+template <int NT>
+int launch_xy_t(const XYF &g, unsigned blocks, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(xy_fused_kernel<NT>, dim3(blocks), dim3(kThreads), lds, s, g);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+
+int launch_xy(dage_ctx ctx, int nt, const XYF &g, int B) {
+    const long blocks = (long)g.gx * g.nzi * g.tc * B;
+    DAGE_REQUIRE(blocks > 0 && blocks < (1L << 31), DAGE_ERR_ARG,
+                 "fused pass of %ld blocks does not fit one launch", blocks);
+    DAGE_REQUIRE(16 * nt >= g.ldy && 16 * nt >= g.nxi, DAGE_ERR_STATE,
+                 "fused pass tile %d too small", nt);
+    const size_t lds = 8 * (16 * (size_t)g.nxi + 17 * (size_t)g.ldy);
+    hipEvent_t ev;
+    prof_begin(ctx, "xy_fused_kernel", &ev);
+    int rc;
+    const unsigned nb = (unsigned)blocks;
+    switch (nt) {
+    case 4: rc = launch_xy_t<4>(g, nb, lds, ctx->stream); break;
+    case 8: rc = launch_xy_t<8>(g, nb, lds, ctx->stream); break;
+    case 12: rc = launch_xy_t<12>(g, nb, lds, ctx->stream); break;
+    case 16: rc = launch_xy_t<16>(g, nb, lds, ctx->stream); break;
+    case 20: rc = launch_xy_t<20>(g, nb, lds, ctx->stream); break;
+    case 24: rc = launch_xy_t<24>(g, nb, lds, ctx->stream); break;
+    case 28: rc = launch_xy_t<28>(g, nb, lds, ctx->stream); break;
+    default: DAGE_REQUIRE(false, DAGE_ERR_STATE, "no fused pass for tile %d", nt);
+    }
+    prof_end(ctx, "xy_fused_kernel", ev);
+    return rc;
+}
+
+// Workspace layout: V | U | P (split-K partials); no V with the fused pass
 struct Plan {
     int tchunk;
     int splits;
@@ -423,12 +712,13 @@ struct Plan {
 // cap > 0: the whole workspace must fit cap bytes (the largest t-chunk, then
 // the largest K split, that does); tchunk = 0 when nothing fits.
 Plan make_plan(const ApplyMats &m, int B = 1, size_t cap = 0) {
+    const bool fused = fused_nt(m) > 0;
     const long nyi = m.din[1], nzi = m.din[2];
     const long nxo = m.dout[0], nyo = m.dout[1], nzo = m.dout[2];
     size_t budget = 4ull << 30;  // bytes for V + U per right-hand side
     if (const char *e = std::getenv("DAGE_APPLY_WORKSPACE_MB")) budget = (size_t)atol(e) << 20;
     const size_t total_cap = 64ull << 30;  // and for all B right-hand sides together
-    const size_t per_t = 8ull * (size_t)(nxo * nyi * nzi + nxo * nyo * nzi);
+    const size_t per_t = 8ull * (size_t)((fused ? 0 : nxo * nyi * nzi) + nxo * nyo * nzi);
     const long M3 = nxo * nyo;
     const int mtiles = (int)((M3 + 64 * kMW - 1) / (64 * kMW));
     const int nb = pick_nb((int)nzo);
@@ -440,14 +730,17 @@ Plan make_plan(const ApplyMats &m, int B = 1, size_t cap = 0) {
         int splits = std::max(1, (1024 + mtiles * ntiles * B - 1) / (mtiles * ntiles * B));
         splits = (int)std::min<long>(splits, std::max<long>(1, K3 / 64));
         p.splits = splits;
-        p.v_elems = (size_t)nxo * nyi * nzi * p.tchunk * B;
+        p.v_elems = fused ? 0 : (size_t)nxo * nyi * nzi * p.tchunk * B;
         p.u_elems = (size_t)nxo * nyo * nzi * p.tchunk * B;
         p.p_elems = (size_t)M3 * nzo * splits * B;
         return p;
     };
     auto bytes = [](const Plan &p) { return 8 * (p.v_elems + p.u_elems + p.p_elems); };
     if (cap > 0) {
-        for (int tc = m.ntp; tc >= 1; --tc) {
+        int tmax = m.ntp;
+        if (const char *e = std::getenv("DAGE_APPLY_TCHUNK"))
+            if (atoi(e) > 0) tmax = std::min(tmax, atoi(e));
+        for (int tc = tmax; tc >= 1; --tc) {
             Plan p = fill(tc);
             while (bytes(p) > cap && p.splits > 1) {
                 --p.splits;
@@ -459,6 +752,10 @@ Plan make_plan(const ApplyMats &m, int B = 1, size_t cap = 0) {
         none.tchunk = 0;
         return none;
     }
+    // DAGE_APPLY_TCHUNK: a fixed t-chunk (comparison runs; the z pass then adds
+    // the same partial sums in the same order with or without the fused pass)
+    if (const char *e = std::getenv("DAGE_APPLY_TCHUNK"))
+        if (atoi(e) > 0) return fill(std::min(m.ntp, atoi(e)));
     return fill((int)std::max<size_t>(
         1, std::min<size_t>({(size_t)m.ntp, budget / per_t, total_cap / (per_t * (size_t)B)})));
 }
@@ -648,6 +945,50 @@ int build_masks(dage_operator op) {
         }
     }
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_tmask, ht.data(), 8 * ht.size(), hipMemcpyHostToDevice, s));
+    // Bands of the fused x/y pass from the row spans: output x tile j (16
+    // points) takes input points xi whose Fx_t row reaches it, input y chunk
+    // c (16 points) reaches the output y tiles its Fy_t rows span.  Zero
+    // entries inside a band are computed (they add exact zeros).
+    const int nxt = (int)((nxo + 15) / 16), nyc = (int)((nyi + 15) / 16);
+    std::vector<int2> xb((size_t)T * nxt), yb((size_t)T * nyc);
+    for (long t = 0; t < T; ++t) {
+        for (int j = 0; j < nxt; ++j) {
+            const int a = 16 * j, b = (int)std::min<long>(a + 16, nxo);
+            long lo = -1, hi = -1;
+            for (long xi = 0; xi < nxi; ++xi) {
+                const long r = t * nxi + xi;
+                if (hs[2 * r] < b && hs[2 * r + 1] >= a) {
+                    if (lo < 0) lo = xi;
+                    hi = xi;
+                }
+            }
+            xb[t * nxt + j] = lo < 0 ? make_int2(0, 0) : make_int2((int)(lo / 4), (int)(hi / 4 + 1));
+        }
+        for (int c = 0; c < nyc; ++c) {
+            int lo = 1 << 30, hi = -1;
+            for (long yi = 16L * c; yi < std::min<long>(16L * c + 16, nyi); ++yi) {
+                const long r = rx + t * nyi + yi;
+                if (hs[2 * r + 1] >= 0) {
+                    lo = std::min(lo, hs[2 * r] / 16);
+                    hi = std::max(hi, hs[2 * r + 1] / 16);
+                }
+            }
+            yb[t * nyc + c] = hi < 0 ? make_int2(0, 0) : make_int2(lo, hi + 1);
+        }
+    }
+    if (op->nxt != nxt || op->nyc != nyc || !op->d_xband) {
+        if (op->d_xband) DAGE_HIP_CHECK(hipFree(op->d_xband));
+        if (op->d_yband) DAGE_HIP_CHECK(hipFree(op->d_yband));
+        op->d_xband = op->d_yband = nullptr;
+        DAGE_HIP_CHECK(hipMalloc((void **)&op->d_xband, sizeof(int2) * xb.size()));
+        DAGE_HIP_CHECK(hipMalloc((void **)&op->d_yband, sizeof(int2) * yb.size()));
+        op->nxt = nxt;
+        op->nyc = nyc;
+    }
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_xband, xb.data(), sizeof(int2) * xb.size(),
+                                  hipMemcpyHostToDevice, s));
+    DAGE_HIP_CHECK(hipMemcpyAsync(op->d_yband, yb.data(), sizeof(int2) * yb.size(),
+                                  hipMemcpyHostToDevice, s));
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
     // tuning knob "x,y,z": fixed column-block counts, masks on wherever defined
     if (const char *e = std::getenv("DAGE_APPLY_NB")) {
@@ -674,6 +1015,10 @@ ApplyMats operator_mats(dage_operator op) {
         m.dout[i] = op->dout[i];
     }
     m.ntp = op->ntp;
+    m.xband = op->d_xband;
+    m.yband = op->d_yband;
+    m.nxt = op->nxt;
+    m.nyc = op->nyc;
     if (op->d_mask) {
         const long rx = (long)op->ntp * op->din[0], ry = (long)op->ntp * op->din[1];
         const uint64_t *base[3] = {op->d_mask, op->d_mask + rx, op->d_mask + rx + ry};
@@ -688,6 +1033,17 @@ ApplyMats operator_mats(dage_operator op) {
         }
     }
     return m;
+}
+
+int prepare_fused_kernels() {
+    DAGE_TRY(set_xy_attr<4>());
+    DAGE_TRY(set_xy_attr<8>());
+    DAGE_TRY(set_xy_attr<12>());
+    DAGE_TRY(set_xy_attr<16>());
+    DAGE_TRY(set_xy_attr<20>());
+    DAGE_TRY(set_xy_attr<24>());
+    DAGE_TRY(set_xy_attr<28>());
+    return DAGE_OK;
 }
 
 int apply_f64_workspace(dage_operator op, size_t *bytes) {
@@ -709,8 +1065,8 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const int T = m.ntp, B = bs.B;
     const Plan plan = make_plan(m, B, bs.ws_cap);
     DAGE_REQUIRE(plan.tchunk > 0, DAGE_ERR_ARG,
-                 "the apply needs more than %zu bytes of workspace (V, U and partials of one "
-                 "t-point)", bs.ws_cap);
+                 "the apply needs more than the %zu bytes of workspace it may use (the "
+                 "intermediates of one t-point do not fit)", bs.ws_cap);
     const size_t need = 8 * (plan.v_elems + plan.u_elems + plan.p_elems);
     if (need > ws->bytes) {
         DAGE_REQUIRE(!bs.no_alloc, DAGE_ERR_STATE, "workspace of %zu bytes not reserved", need);
@@ -750,8 +1106,29 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     // which later t-chunks still read.
     const bool direct = B == 1 && plan.splits == 1 && bs.coda == 0.0 &&
                         (out + nout <= in || in + nin <= out);
+    const int nt = fused_nt(m);
     for (int t0 = 0; t0 < T; t0 += tcmax) {
         const int tc = std::min(tcmax, T - t0);
+        if (nt > 0) {
+            // passes 1 and 2 fused: U_b((xo,yo),(zi,t)) straight from in_b
+            XYF f{};
+            f.in = in;
+            f.fx = m.fx;
+            f.fy = m.fy;
+            f.U = U;
+            f.w = W;
+            f.wS = wS;
+            f.nxi = (int)nxi; f.nyi = (int)nyi; f.nxo = (int)nxo; f.nyo = (int)nyo;
+            f.nzi = (int)nzi;
+            f.t0 = t0; f.tc = tc;
+            f.nin = nin; f.Ut = Ut;
+            f.xband = m.xband; f.yband = m.yband;
+            f.nxt = m.nxt; f.nyc = m.nyc;
+            f.gx = (int)((nxo + 63) / 64);
+            f.ldy = 16 * (int)((nyo + 15) / 16);
+            f.remap = remap_default();
+            DAGE_TRY(launch_xy(ctx, nt, f, B));
+        } else {
         // pass 1, batch (t, b): V_{b,t}((yi,zi), xo) = in_b((yi,zi), xi) . Fx_t^T(xi, xo)
         Gemm g1{};
         g1.A = in;
@@ -783,6 +1160,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         int nb2;
         set_tiles(g2, 1, (long)t0 * nyi, nyi, &nb2);
         DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc * B, nb2));
+        }
 
         // pass 3, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
         const int K3 = (int)(nzi * tc);

@@ -205,6 +205,12 @@ struct dage_operator_s {
     // then holds per row one bit per such tile, same row order as d_mask
     uint64_t *d_tmask = nullptr;
     int pitch[3] = {0, 0, 0}, nbp[3] = {0, 0, 0};
+    // Nonzero bands for the fused x/y pass: d_xband[t*nxt + j] = [first,
+    // last+1) k-step (4 input x points) of Fx_t rows reaching output x tile j
+    // (16 points); d_yband[t*nyc + c] = [first, last+1) 16-point output y tile
+    // reached by Fy_t rows of input y chunk c (16 points)
+    int2 *d_xband = nullptr, *d_yband = nullptr;
+    int nxt = 0, nyc = 0;
     void *d_planes = nullptr;   // split-BF16 matrix planes (built on first mixed apply)
     bool planes_valid = false;  // cleared when the matrices change
     MixedLayout mixed;
@@ -246,6 +252,9 @@ struct ApplyMats {
     // pitch tiling (pitch[p] > 0): tile masks, same row order, bit j = tile j
     const uint64_t *tmask[3] = {nullptr, nullptr, nullptr};
     int pitch[3] = {0, 0, 0}, nbp[3] = {0, 0, 0};
+    // nonzero bands of Fx and Fy (dage_operator_s::d_xband/d_yband; null: none)
+    const int2 *xband = nullptr, *yband = nullptr;
+    int nxt = 0, nyc = 0;
 };
 // Device scratch owned by the caller, grown on demand.
 struct Workspace {
@@ -273,6 +282,9 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
                     const BatchSpec &bs, Workspace *ws);
 int apply_f64_batch_workspace(const ApplyMats &m, int B, size_t *bytes);
 int apply_f64_workspace(dage_operator op, size_t *bytes);
+// Allows the fused x/y pass its dynamic LDS on the current device (called
+// once per context, outside any stream capture).
+int prepare_fused_kernels();
 // Mixed-precision apply (apply_mixed.hip): npair = 6 (bf16x6) or 3 (bf16x3).
 int apply_mixed(dage_operator op, const double *in_dev, double *out_dev, int npair);
 // Batch of right-hand sides on one operator (same conventions as apply_f64_batch).

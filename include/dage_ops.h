@@ -103,7 +103,8 @@ int dage_ctx_time_end(dage_ctx ctx, double *ms);
  * (src/bubbles/cube.cu:1182-1269; Fortran side src/bubbles/cuda.F90:70-200):
  * cudacube_init_cuda, cudacube_upload_cuda, cudacube_download_cuda,
  * cudacube_set_to_zero_cuda, cudacube_destroy_cuda.  The memory is one
- * contiguous allocation (no pitch, no multi-device split). */
+ * contiguous allocation (no pitch, no multi-device split); a new cube holds
+ * zeros (filled on the context stream). */
 int dage_cube_create(dage_ctx ctx, const int shape[3], dage_cube *cube);
 int dage_cube_destroy(dage_cube cube);
 /* Synchronises the context stream before returning (the host buffer may be

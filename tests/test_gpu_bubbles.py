@@ -55,7 +55,7 @@ def test_inject_multiplier_and_uneven_grid(ctx):
 
 def test_inject_errors(ctx):
     with pytest.raises(dage_amd.DageError):
-        dage_amd.Bubbles(ctx, 2, 9)
+        dage_amd.Bubbles(ctx, 2, 13)  # above the largest lmax (12)
     ax = O.Axis.ncell_fixed(-2.0, 2.0, 7, 2)
     g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
     cube = dage_amd.DeviceCube(ctx, (ax.n,) * 3)
