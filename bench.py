@@ -64,6 +64,9 @@ def parse():
                    help="plain iterations on the training molecule (targets: its last orbitals)")
     p.add_argument("--scf-record-iter", type=int, default=200,
                    help="training pairs come from every 4th of the first this-many iterations")
+    p.add_argument("--headline-only", action="store_true",
+                   help="run only the headline's warmup and timed steps (for profiler runs: "
+                        "no parity, split-BF16, CPU-baseline or other-config legs)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="time a plain device copy of a 257^3 cube instead (the known-size "
                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -687,6 +690,12 @@ def bench_config5(ctx, args):
         step()
     ms_graph = ctx.time_end() / args.steps
     progress(f"config5: {ms:.0f} ms per step (graph {ms_graph:.0f} ms)")
+    if args.headline_only:
+        print(json.dumps({"mode": "headline_only", "ms_per_step": round(ms, 2),
+                          "kernels": kernels}), flush=True)
+        fmm.close()
+        batch.close()
+        return None
     T, Tn = hq.get_npoints(), gq.get_npoints()
     # dominant kernel of the step; a mode pass does 2 T n^4 dense FLOPs per
     # right-hand side (near field: per apply), the fused x/y pass twice that,
@@ -900,6 +909,9 @@ def main():
     import dage_amd
     ctx = dage_amd.Context(0)
     c5 = bench_config5(ctx, args)
+    if c5 is None:  # --headline-only
+        ctx.close()
+        return
     progress(f"config5: {c5['ms_per_step']:.1f} ms per step")
     n = c5["n"]
     line = {
