@@ -67,6 +67,8 @@ _SIGNATURES = {
     "dage_operator_get_matrices": [_c_p, _dp, _dp, _dp],
     "dage_operator_apply": [_c_p, _dp, _dp, _int],
     "dage_operator_apply_device": [_c_p, _c_p, _c_p, _int],
+    "dage_operator_apply_device_noalloc": [_c_p, _c_p, _c_p],
+    "dage_operator_noalloc_limit": [ctypes.POINTER(ctypes.c_size_t)],
     "dage_operator_workspace_bytes": [_c_p, ctypes.POINTER(ctypes.c_size_t)],
     "dage_operator_get_tiling": [_c_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int)],

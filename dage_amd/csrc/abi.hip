@@ -118,6 +118,35 @@ int prof_resolve(dage_ctx ctx) {
 
 using namespace dage;
 
+// ------------------------------------------------ allocation-free apply
+// dage_operator_apply_device_noalloc keeps its intermediates in this fixed
+// array: one copy per GPU, held for the life of the process.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in);
+// it uses the device address hipGetSymbolAddress returns, looked up once per
+// device at context creation with that device current.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// This device's address of `scratch` (the device must be current).
+int scratch_address(int device, void **p) {
+    DAGE_REQUIRE(device >= 0 && device < kMaxDevices, DAGE_ERR_UNSUPPORTED,
+                 "device %d beyond the %d the library tracks", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        DAGE_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = ptr;
+    }
+    *p = g_scratch_dev[device];
+    return DAGE_OK;
+}
+}  // namespace
+
 namespace {
 
 int use_device(dage_ctx ctx) {
@@ -193,6 +222,7 @@ int dage_ctx_create(int device, dage_ctx *out) {
     }
     ctx->stream = ctx->own_stream;
     int rc = prepare_fused_kernels();
+    if (rc == DAGE_OK) rc = scratch_address(device, &ctx->noalloc_scratch);
     if (rc != DAGE_OK) {
         (void)hipStreamDestroy(ctx->own_stream);
         delete ctx;
@@ -629,6 +659,22 @@ int dage_operator_apply_device(dage_operator op, const double *in, double *out, 
     if (precision == DAGE_PREC_BF16X6) return apply_mixed(op, in, out, 6);
     if (precision == DAGE_PREC_BF16X3) return apply_mixed(op, in, out, 3);
     return apply_f64(op, in, out);
+}
+
+int dage_operator_apply_device_noalloc(dage_operator op, const double *in, double *out) {
+    DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE((const void *)in != (const void *)out, DAGE_ERR_ARG,
+                 "in-place apply is not supported");
+    DAGE_TRY(use_device(op->ctx));
+    DAGE_REQUIRE(op->ctx->noalloc_scratch != nullptr, DAGE_ERR_STATE,
+                 "no fixed scratch on device %d", op->ctx->device);
+    return apply_f64_fixed(op, in, out, op->ctx->noalloc_scratch, kNoAllocScratchBytes);
+}
+
+int dage_operator_noalloc_limit(size_t *bytes) {
+    DAGE_REQUIRE(bytes != nullptr, DAGE_ERR_ARG, "null argument");
+    *bytes = kNoAllocScratchBytes;
+    return DAGE_OK;
 }
 
 int dage_operator_apply(dage_operator op, const double *in, double *out, int precision) {

@@ -1259,4 +1259,15 @@ int apply_f64(dage_operator op, const double *in, double *out) {
     return rc;
 }
 
+int apply_f64_fixed(dage_operator op, const double *in, double *out, void *ws_ptr,
+                    size_t ws_bytes) {
+    BatchSpec bs;
+    const bool coda = (op->coda != 0.0) && !(op->flags & DAGE_OP_NO_CODA);
+    bs.coda = coda ? op->coda : 0.0;
+    bs.no_alloc = true;
+    bs.ws_cap = ws_bytes;
+    Workspace ws{static_cast<double *>(ws_ptr), ws_bytes};
+    return apply_f64_batch(op->ctx, operator_mats(op), in, out, bs, &ws);
+}
+
 }  // namespace dage

@@ -98,6 +98,9 @@ struct dage_ctx_s {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     hipStream_t scratch_stream = nullptr;
+    // this device's copy of the allocation-free apply's fixed scratch array
+    // (abi.hip), looked up at context creation
+    void *noalloc_scratch = nullptr;
 };
 
 struct dage_cube_s {
@@ -237,6 +240,11 @@ int prof_resolve(dage_ctx ctx);
 int generate_matrices(dage_operator op);
 // FP64 apply (apply_f64.hip).
 int apply_f64(dage_operator op, const double *in_dev, double *out_dev);
+// The same apply with its intermediates in a caller-given device buffer of
+// ws_bytes: the t-chunk and K split shrink to fit, nothing is allocated, and
+// DAGE_ERR_ARG is returned (nothing launched) when one t-point does not fit.
+int apply_f64_fixed(dage_operator op, const double *in_dev, double *out_dev, void *ws,
+                    size_t ws_bytes);
 // One operator's matrices in the library's device layouts (Fx (nxo,nxi,T),
 // Fy (nyi,nyo,T), Gz ((zi + nzi*t), zo)) and weights.
 // mask (optional): per B row of each pass, bit c set when the row has a

@@ -310,6 +310,23 @@ class Operator3D:
         call("dage_operator_apply_device", self.handle, ctypes.c_void_p(cubein.ptr),
              ctypes.c_void_p(cubeout.ptr), precision)
 
+    def transform_cuda_cube_noalloc(self, cubein: DeviceCube, cubeout: DeviceCube):
+        """FP64 transform_cuda_cube that allocates nothing: the intermediates
+        live in the library's fixed per-GPU scratch (noalloc_limit() bytes,
+        shared by every such call on the GPU, so overlapping calls race)."""
+        nin, nout = self.get_dims()
+        if cubein.shape != tuple(nin) or cubeout.shape != tuple(nout):
+            raise ValueError(f"device cubes {cubein.shape}->{cubeout.shape} do not match "
+                             f"the operator {tuple(nin)}->{tuple(nout)}")
+        call("dage_operator_apply_device_noalloc", self.handle, ctypes.c_void_p(cubein.ptr),
+             ctypes.c_void_p(cubeout.ptr))
+
+    @staticmethod
+    def noalloc_limit() -> int:
+        n = ctypes.c_size_t()
+        call("dage_operator_noalloc_limit", ctypes.byref(n))
+        return n.value
+
     def workspace_bytes(self) -> int:
         n = ctypes.c_size_t()
         call("dage_operator_workspace_bytes", self.handle, ctypes.byref(n))

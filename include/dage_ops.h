@@ -181,6 +181,20 @@ int dage_operator_apply(dage_operator op, const double *cubein,
  * buffers; asynchronous on the context stream. */
 int dage_operator_apply_device(dage_operator op, const double *cubein_dev,
                                double *cubeout_dev, int precision);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 apply of dage_operator_apply_device, but with
+ * its intermediates in a fixed per-GPU scratch array of
+ * dage_operator_noalloc_limit() bytes (128 MiB) that the library holds for
+ * the life of the process.  The t-chunk shrinks to fit; when the
+ * intermediates of one t-point (8*nxo*nyo*nzi bytes with the fused x/y pass,
+ * plus the z-pass partials) do not fit, DAGE_ERR_ARG is returned and nothing
+ * is launched.  Calls no allocator.  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams, other contexts,
+ * other host threads) race; calls on one stream are ordered. */
+int dage_operator_apply_device_noalloc(dage_operator op, const double *cubein_dev,
+                                       double *cubeout_dev);
+int dage_operator_noalloc_limit(size_t *bytes);
 /* Bytes of device workspace the operator holds for its intermediates. */
 int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
 /* No reference counterpart (diagnostic): the output-column tiling the FP64
