@@ -672,8 +672,11 @@ def bench_config5(ctx, args):
         step(eager=True)
     ms = ctx.time_end() / args.steps
     wall = time.perf_counter() - wall0
-    names = ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
-             "splitk_reduce_kernel",
+    # batch_*: the sweep's passes (the batch's own event names); gemm_f64_*:
+    # the GBFMM near field's
+    names = ("batch_xy_fused_kernel", "batch_f64_xmode", "batch_f64_ymode", "batch_f64_zmode",
+             "batch_splitk_reduce", "xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode",
+             "gemm_f64_zmode", "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm", "cube_copy_kernel")
@@ -698,14 +701,17 @@ def bench_config5(ctx, args):
         return None
     T, Tn = hq.get_npoints(), gq.get_npoints()
     # dominant kernel of the step; a mode pass does 2 T n^4 dense FLOPs per
-    # right-hand side (near field: per apply), the fused x/y pass twice that,
-    # counted dense although the all-zero K panels (products with zero matrix
-    # entries outside the bands) of the banded matrices are skipped
+    # right-hand side (sweep: T = 40 for each of 64 orbitals; GBFMM near
+    # field: T = 50 once), the fused x/y pass twice that, counted dense
+    # although the all-zero K panels (products with zero matrix entries
+    # outside the bands) of the banded matrices are skipped
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
-    if dom.startswith("gemm_f64") or dom == "xy_fused_kernel":
-        passes = 2.0 if dom == "xy_fused_kernel" else 1.0
-        per_step = passes * 2.0 * float(n) ** 4 * (T * nrhs + Tn)  # sweep + near field
+    mfma_kernels = ("batch_f64_", "gemm_f64_", "batch_xy_fused", "xy_fused")
+    if dom.startswith(mfma_kernels):
+        passes = 2.0 if "xy_fused" in dom else 1.0
+        tsum = T * nrhs if dom.startswith("batch_") else Tn
+        per_step = passes * 2.0 * float(n) ** 4 * tsum
         per_launch = per_step / dk["launches_per_step"]
         achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
@@ -715,6 +721,7 @@ def bench_config5(ctx, args):
                     "flops": "dense-equivalent (2 T n^4 per pass and right-hand side; the "
                              "fused x/y kernel does two passes)",
                     "avg_launch_ms": dk["ms_per_launch"],
+                    "launches_per_step": dk["launches_per_step"],
                     "algorithmic_flops_per_launch": per_launch}
         exe = measured_executed_fraction("config5", dom)
         if exe is not None:

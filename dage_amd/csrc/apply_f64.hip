@@ -684,7 +684,7 @@ int launch_xy_t(const XYF &g, unsigned blocks, size_t lds, hipStream_t s) {
     return DAGE_OK;
 }
 
-int launch_xy(dage_ctx ctx, int nt, const XYF &g, int B) {
+int launch_xy(dage_ctx ctx, int nt, const XYF &g, int B, const char *name) {
     const long blocks = (long)g.gx * g.nzi * g.tc * B;
     DAGE_REQUIRE(blocks > 0 && blocks < (1L << 31), DAGE_ERR_ARG,
                  "fused pass of %ld blocks does not fit one launch", blocks);
@@ -692,7 +692,7 @@ int launch_xy(dage_ctx ctx, int nt, const XYF &g, int B) {
                  "fused pass tile %d too small", nt);
     const size_t lds = 8 * (16 * (size_t)g.nxi + 17 * (size_t)g.ldy);
     hipEvent_t ev;
-    prof_begin(ctx, "xy_fused_kernel", &ev);
+    prof_begin(ctx, name, &ev);
     int rc;
     const unsigned nb = (unsigned)blocks;
     switch (nt) {
@@ -705,7 +705,7 @@ int launch_xy(dage_ctx ctx, int nt, const XYF &g, int B) {
     case 28: rc = launch_xy_t<28>(g, nb, lds, ctx->stream); break;
     default: DAGE_REQUIRE(false, DAGE_ERR_STATE, "no fused pass for tile %d", nt);
     }
-    prof_end(ctx, "xy_fused_kernel", ev);
+    prof_end(ctx, name, ev);
     return rc;
 }
 
@@ -1114,6 +1114,14 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const bool direct = B == 1 && plan.splits == 1 && bs.coda == 0.0 &&
                         (out + nout <= in || in + nin <= out);
     const int nt = fused_nt(m);
+    // event names (profiling): batched right-hand sides with their own
+    // weights (the batch graph) apart from single applies
+    const bool bat = bs.w != nullptr;
+    const char *nm_xy = bat ? "batch_xy_fused_kernel" : "xy_fused_kernel";
+    const char *nm_x = bat ? "batch_f64_xmode" : "gemm_f64_xmode";
+    const char *nm_y = bat ? "batch_f64_ymode" : "gemm_f64_ymode";
+    const char *nm_z = bat ? "batch_f64_zmode" : "gemm_f64_zmode";
+    const char *nm_r = bat ? "batch_splitk_reduce" : "splitk_reduce_kernel";
     for (int t0 = 0; t0 < T; t0 += tcmax) {
         const int tc = std::min(tcmax, T - t0);
         if (nt > 0) {
@@ -1134,7 +1142,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
             f.gx = (int)((nxo + 63) / 64);
             f.ldy = 16 * (int)((nyo + 15) / 16);
             f.remap = remap_default(8.0 * nxi * nyi * nzi);
-            DAGE_TRY(launch_xy(ctx, nt, f, B));
+            DAGE_TRY(launch_xy(ctx, nt, f, B, nm_xy));
         } else {
         // pass 1, batch (t, b): V_{b,t}((yi,zi), xo) = in_b((yi,zi), xi) . Fx_t^T(xi, xo)
         Gemm g1{};
@@ -1149,7 +1157,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
         int nb1;
         set_tiles(g1, 0, (long)t0 * nxi, nxi, &nb1);
-        DAGE_TRY(launch_gemm<0>(ctx, "gemm_f64_xmode", g1, tc * B, nb1));
+        DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
 
         // pass 2, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
@@ -1166,7 +1174,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g2.aS1 = wS;
         int nb2;
         set_tiles(g2, 1, (long)t0 * nyi, nyi, &nb2);
-        DAGE_TRY(launch_gemm<1>(ctx, "gemm_f64_ymode", g2, tc * B, nb2));
+        DAGE_TRY(launch_gemm<1>(ctx, nm_y, g2, tc * B, nb2));
         }
 
         // pass 3, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
@@ -1191,18 +1199,18 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
             for (int b = 0; b < B; ++b)
                 DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + splits) * M3 * nzo,
                                    (size_t)(plan.splits - splits) * M3 * nzo));
-        DAGE_TRY(launch_gemm<2>(ctx, "gemm_f64_zmode", g3, splits * B, nb3));
+        DAGE_TRY(launch_gemm<2>(ctx, nm_z, g3, splits * B, nb3));
     }
     if (direct) return DAGE_OK;
     const long len = M3 * nzo;
     const long total = len * B;
     const int grid = (int)std::min<long>((total + 255) / 256, 65536);
     hipEvent_t ev;
-    prof_begin(ctx, "splitk_reduce_kernel", &ev);
+    prof_begin(ctx, nm_r, &ev);
     hipLaunchKernelGGL(splitk_reduce_batch_kernel, dim3(grid), dim3(256), 0, ctx->stream, P,
                        plan.splits, len, B, bs.coda != 0.0 ? in : nullptr, bs.coda, out);
     DAGE_HIP_CHECK(hipGetLastError());
-    prof_end(ctx, "splitk_reduce_kernel", ev);
+    prof_end(ctx, nm_r, ev);
     return DAGE_OK;
 }
 
