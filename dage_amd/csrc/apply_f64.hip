@@ -57,6 +57,7 @@ struct Gemm {
     // launch itself is one-dimensional (see block_coords).
     int gx, gy, gz;
     int remap;  // 0: plain row-tile-fastest order (for comparison runs)
+    int batch_inner;  // remap order walks z_lo before the row tile (x pass)
 };
 
 // Block coordinates of a one-dimensional launch.  Hardware hands block L to
@@ -64,9 +65,8 @@ struct Gemm {
 // one contiguous range of the tile order; the order walks column tiles
 // fastest, so the blocks that share an A row tile run together on one XCD and
 // read it through that XCD's L2 once instead of once per column tile.
-// BATCH_INNER (the x pass, whose A is the input cube for every t) walks the
+// batch_inner (the x pass, whose A is the input cube for every t) walks the
 // batch index z_lo before the row tile for the same reason.
-template <bool BATCH_INNER>
 __device__ inline void block_coords(const Gemm &g, int *bx, int *by, int *bz) {
     unsigned L = blockIdx.x;
     if (g.remap) {
@@ -75,7 +75,7 @@ __device__ inline void block_coords(const Gemm &g, int *bx, int *by, int *bz) {
         L = x * q + min(x, r) + (L >> 3);
         *by = (int)(L % (unsigned)g.gy);
         L /= (unsigned)g.gy;
-        if (BATCH_INNER && g.Z0 > 0) {
+        if (g.batch_inner && g.Z0 > 0) {
             const unsigned zl = L % (unsigned)g.Z0;
             L /= (unsigned)g.Z0;
             *bx = (int)(L % (unsigned)g.gx);
@@ -99,18 +99,18 @@ constexpr int kMaxPanels = 2048;  // K panels a block can list (K <= 32768)
 // layout on gfx950 (measured with a one-hot probe): lane l holds A(l%16, l/16)
 // and B(l/16, l%16); result register r of lane l is D(4*r + l/16, l%16).
 //
-// Operand panels of kBK k-values are staged in LDS; the next panel's global
+// Operand panels of KB k-values are staged in LDS; the next panel's global
 // loads are issued into registers before the current panel's MFMAs, from
 // clamped addresses so no load sits behind a branch.  Values outside the
 // matrix are loaded as zeros, so padded k-steps add nothing.  PASS 0-2 are
 // the three apply passes (one LDS buffer), PASS 3 the column-major GEMMs
 // (two buffers).
-template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG, int PASS>
+template <int MW, int NB, bool A_MCONTIG, bool B_KCONTIG, int PASS, int KB = kBK>
 __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     constexpr int BM = 64 * MW;
     constexpr int BN = 16 * NB;
-    constexpr int A_PER = BM * kBK / kThreads;  // elements per thread
-    constexpr int B_PER = NB;                   // BN * kBK / kThreads
+    constexpr int A_PER = BM * KB / kThreads;  // elements per thread
+    constexpr int B_PER = BN * KB / kThreads;
     constexpr int LDA = BM + 1;  // +1 double breaks power-of-two strides
     constexpr int LDB = BN + 1;
     // The mode passes stage one panel at a time (next panel prefetched into
@@ -119,14 +119,14 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     // does (x and z passes 5 % faster at 241^3 than double buffering at four
     // blocks per CU).  The column-major GEMMs keep two buffers.
     constexpr int NBUF = (PASS <= 2) ? 1 : 2;
-    __shared__ double As[NBUF][kBK * LDA];
-    __shared__ double Bs[NBUF][kBK * LDB];
+    __shared__ double As[NBUF][KB * LDA];
+    __shared__ double Bs[NBUF][KB * LDB];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     int bx, by, bz;
-    block_coords<PASS == 0>(g, &bx, &by, &bz);
+    block_coords(g, &bx, &by, &bz);
     const int m0 = bx * BM;
     const int n0 = g.pitch > 0 ? by * g.pitch : by * BN;
     const int nend = g.pitch > 0 && by + 1 < g.gy ? n0 + g.pitch : g.N;
@@ -163,9 +163,9 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     //   B k-contiguous: k = tid % 16, n = tid / 16 + slot * 16
     //   B n-contiguous: n = tid % 16 + slot * 16, k = tid / 16
     constexpr int A_DK = A_MCONTIG ? kThreads / BM : 0;
-    constexpr int A_DM = A_MCONTIG ? 0 : kThreads / kBK;
-    const int a_m = A_MCONTIG ? tid % BM : tid / kBK;
-    const int a_k = A_MCONTIG ? tid / BM : tid % kBK;
+    constexpr int A_DM = A_MCONTIG ? 0 : kThreads / KB;
+    const int a_m = A_MCONTIG ? tid % BM : tid / KB;
+    const int a_k = A_MCONTIG ? tid / BM : tid % KB;
     long a_base;
     bool a_row_ok;  // m-contiguous: the single row of this thread is inside M
     {
@@ -178,11 +178,15 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const int a_sm0 = a_k * LDA + a_m;
     constexpr int a_sstep = A_MCONTIG ? A_DK * LDA : A_DM;
 
-    const int bk = B_KCONTIG ? tid % kBK : tid / 16;
-    const int bn = B_KCONTIG ? tid / kBK : tid % 16;
+    // B slot r sits (b_dk(r), b_dn(r)) from the thread's base (k, n):
+    //   k-contiguous: k = tid % KB, n = tid / KB + r * (256 / KB)
+    //   n-contiguous: n = tid % 16 + (r % NB) * 16, k = tid / 16 + (r / NB) * 16
+    const int bk = B_KCONTIG ? tid % KB : tid / 16;
+    const int bn = B_KCONTIG ? tid / KB : tid % 16;
     const long b_base = (long)bk * g.sBk + (long)(n0 + bn) * g.sBn;
-    const long b_step = 16 * g.sBn;
     const int b_sm0 = bk * LDB + bn;
+    auto b_dk = [](int r) { return B_KCONTIG ? 0 : (r / NB) * 16; };
+    auto b_dn = [](int r) { return B_KCONTIG ? r * (kThreads / KB) : (r % NB) * 16; };
 
     double ra[A_PER], rb[B_PER];
     auto load_panel = [&](int kp) {
@@ -198,8 +202,9 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         const long kboff = (long)kp * g.sBk;
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
-            const bool ok = bk < lim && n0 + bn + 16 * r < nend;
-            const double v = B[ok ? b_base + r * b_step + kboff : 0];
+            const bool ok = bk + b_dk(r) < lim && n0 + bn + b_dn(r) < nend;
+            const double v =
+                B[ok ? b_base + (long)b_dk(r) * g.sBk + (long)b_dn(r) * g.sBn + kboff : 0];
             rb[r] = ok ? v : 0.0;
         }
     };
@@ -207,7 +212,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
 #pragma unroll
         for (int r = 0; r < A_PER; ++r) As[buf][a_sm0 + r * a_sstep] = ra[r];
 #pragma unroll
-        for (int r = 0; r < B_PER; ++r) Bs[buf][b_sm0 + 16 * r] = rb[r];
+        for (int r = 0; r < B_PER; ++r) Bs[buf][b_sm0 + b_dk(r) * LDB + b_dn(r)] = rb[r];
     };
 
     d4 acc[MW][NB];
@@ -240,7 +245,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     // same as without masks.
     __shared__ unsigned short plist[kMaxPanels];
     __shared__ int plist_n;
-    int npanels = (K + kBK - 1) / kBK;
+    int npanels = (K + KB - 1) / KB;
     const bool listed = mrow != nullptr && npanels <= kMaxPanels;
     if (listed) {
         if (wave == 0) {
@@ -251,8 +256,8 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
                 const int p = base + lane;
                 bool need = false;
                 if (p < npanels) {
-                    const int k0 = p * kBK;
-                    const int kn = min(kBK, K - k0);
+                    const int k0 = p * KB;
+                    const int kn = min(KB, K - k0);
                     uint64_t any = 0;
                     for (int j = 0; j < kn; ++j) any |= mrow[k0 + j];
                     need = (any & cols) != 0;
@@ -266,7 +271,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         __syncthreads();
         npanels = plist_n;
     }
-    auto panel_k = [&](int p) { return (listed ? (int)plist[p] : p) * kBK; };
+    auto panel_k = [&](int p) { return (listed ? (int)plist[p] : p) * KB; };
 
     if (npanels > 0) {
         load_panel(panel_k(0));
@@ -280,9 +285,9 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         const double *as = As[buf];
         const double *bs = Bs[buf];
         // uniform bound: a panel at the end of K skips its all-padding k-steps
-        const int ksteps = min(kBK / 4, (K - panel_k(p) + 3) / 4);
+        const int ksteps = min(KB / 4, (K - panel_k(p) + 3) / 4);
 #pragma unroll
-        for (int ks = 0; ks < kBK / 4; ++ks)
+        for (int ks = 0; ks < KB / 4; ++ks)
             if (ks < ksteps) kstep(as, bs, ks);
         if constexpr (NBUF == 1) {
             __syncthreads();  // every wavefront is done with the panel
@@ -302,7 +307,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         // stores run along rows, 16 consecutive rows (128 B) per column
         // instead of 4 (32 B).  The x pass's output is contiguous along
         // columns already.
-        static_assert(16 * LDB <= kBK * LDA && 16 * LDB <= kBK * LDB,
+        static_assert(16 * LDB <= KB * LDA && 16 * LDB <= KB * LDB,
                       "a wavefront tile fits each operand buffer");
         double *R = (wave & 1) ? &Bs[0][0] : &As[0][0];
         const int gm = m0 + wrow + lrow;
@@ -611,6 +616,7 @@ int launch_nb(const Gemm &g0, int nb, int zdim, hipStream_t s) {
     g.gy = ntiles;
     g.gz = zdim;
     g.remap = remap_default(8.0 * g.M * (double)(g.Ksplit > 0 ? g.Ksplit : g.K));
+    g.batch_inner = PASS == 0;  // the x pass's A (the input cube) is the same for every t
     const long blocks = (long)g.gx * g.gy * g.gz;
     DAGE_REQUIRE(g.Z0 <= 0 || zdim % g.Z0 == 0, DAGE_ERR_STATE,
                  "batch grid %d is not a multiple of %d", zdim, g.Z0);
