@@ -77,6 +77,7 @@ _SIGNATURES = {
     "dage_bubbles_create": [_c_p, _int, _int, _int, _int, ctypes.POINTER(_c_p)],
     "dage_bubbles_set": [_c_p, _int, _dp, _dbl, _int, _dp, _dp],
     "dage_bubbles_inject_device": [_c_p, _c_p, _int, _dbl, _c_p],
+    "dage_bubbles_inject": [_c_p, _c_p, _int, _dbl, _dp],
     "dage_bubbles_destroy": [_c_p],
     "dage_bubbles_create_like": [_c_p, _int, _int, ctypes.POINTER(_c_p)],
     "dage_bubbles_info": [_c_p, _ip, _ip, _ip, _ip],

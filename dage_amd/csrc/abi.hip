@@ -809,6 +809,22 @@ int dage_bubbles_inject_device(dage_bubbles b, dage_grid3d grid, int lmin, doubl
     return bubbles_inject(b, grid, lmin, multiplier, cube);
 }
 
+int dage_bubbles_inject(dage_bubbles b, dage_grid3d grid, int lmin, double multiplier,
+                        double *cube) {
+    DAGE_REQUIRE(b && grid && cube, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(grid->ctx == b->ctx, DAGE_ERR_ARG, "grid belongs to another context");
+    DAGE_TRY(use_device(b->ctx));
+    const size_t n = (size_t)grid->shape(0) * grid->shape(1) * grid->shape(2);
+    double *d = nullptr;
+    DAGE_TRY(ctx_scratch(b->ctx, 8 * n, (void **)&d));
+    hipStream_t s = b->ctx->stream;
+    DAGE_HIP_CHECK(hipMemcpyAsync(d, cube, 8 * n, hipMemcpyHostToDevice, s));
+    DAGE_TRY(dage_bubbles_inject_device(b, grid, lmin, multiplier, d));
+    DAGE_HIP_CHECK(hipMemcpyAsync(cube, d, 8 * n, hipMemcpyDeviceToHost, s));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    return DAGE_OK;
+}
+
 int dage_bubbles_destroy(dage_bubbles b) {
     if (!b) return DAGE_OK;
     dage_ctx ctx = b->ctx;

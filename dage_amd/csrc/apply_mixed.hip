@@ -107,7 +107,7 @@ constexpr int kMaxSteps = 2048;  // K steps a block can list (K <= 65536)
 // MASKED: the block lists its nonzero K steps from g.mask (the unmasked
 // instantiation carries neither the list nor its LDS).
 template <int MW, int NB, int NPAIR, typename TA, typename TC, bool A_KCONTIG, int PASS,
-          bool MASKED>
+          bool MASKED, int PIPE>
 __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     constexpr int BM = 64 * MW;
     constexpr int BN = 16 * NB;
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     }
 
     TA ra[MW][8];
-    auto load_a = [&](int step) {
+    auto load_a_to = [&](TA (&dst)[MW][8], int step) {
         const int kb = kbeg + step * kKS + kg * 8;
 #pragma unroll
         for (int i = 0; i < MW; ++i)
@@ -192,9 +192,10 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
                 const int k = kb + j;
                 const bool ok = rok[i] && k < kend;
                 const TA v = A[ok ? arow[i] + (long)k * g.sAk : 0];
-                ra[i][j] = ok ? v : (TA)0;
+                dst[i][j] = ok ? v : (TA)0;
             }
     };
+    auto load_a = [&](int step) { load_a_to(ra, step); };
     // 16-byte chunks of the block's B planes for one K step
     auto copy_b = [&](int step, int buf) {
         constexpr int chunks = NQB * BN * (kKS / 8);
@@ -216,6 +217,111 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
+    auto split_a = [&](const TA (&src)[MW][8], bf8 (&a)[3][MW]) {
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                __bf16 h, md, l;
+                split3(src[i][j], h, md, l);
+                a[0][i][j] = h;
+                a[1][i][j] = md;
+                a[2][i][j] = l;
+            }
+    };
+    // The MFMAs of one K step on B buffer buf, the FP32 partials then added
+    // into the FP64 accumulators.
+    auto compute = [&](const bf8 (&a)[3][MW], int buf) {
+        f4 c[MW][NB];
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) c[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int qb = 0; qb < NQB; ++qb) {
+            bf8 b[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                b[j] = *reinterpret_cast<const bf8 *>(&Bs[buf][qb][j * 16 + lrow][kg * 8]);
+#pragma unroll
+            for (int qa = 2; qa >= 0; --qa) {  // small terms first
+                if (qa + qb > NQB - 1) continue;
+#pragma unroll
+                for (int i = 0; i < MW; ++i)
+#pragma unroll
+                    for (int j = 0; j < NB; ++j)
+                        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[qa][i], b[j], c[i][j],
+                                                                          0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)c[i][j][r];
+    };
+
+    if constexpr (PIPE == 2) {
+        // Two-deep pipeline: the A values of step s + 2 are loaded (into the
+        // register set of step s, free once split) while step s computes,
+        // and the B planes of step s + 2 into registers, stored to LDS one
+        // step later; every global load has one to two steps of MFMA work
+        // to arrive in.  Same products, same order as the one-deep loop.
+        constexpr int chunks = NQB * BN * (kKS / 8);
+        constexpr int CH = (chunks + kThreads - 1) / kThreads;
+        u4 rb[CH];
+        TA ra1[MW][8];
+        auto load_b_regs = [&](int step) {
+            const long kofs = (long)kbeg + (long)step * kKS;
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int c = tid + i * kThreads;
+                if (c < chunks) {
+                    const int q = c / (BN * 4);
+                    const int rem = c - q * (BN * 4);
+                    rb[i] = *reinterpret_cast<const u4 *>(Bp + q * g.sBq +
+                                                          (long)(n0 + (rem >> 2)) * g.Kp + kofs +
+                                                          (rem & 3) * 8);
+                }
+            }
+        };
+        auto store_b_regs = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int c = tid + i * kThreads;
+                if (c < chunks) {
+                    const int q = c / (BN * 4);
+                    const int rem = c - q * (BN * 4);
+                    *reinterpret_cast<u4 *>(&Bs[buf][q][rem >> 2][(rem & 3) * 8]) = rb[i];
+                }
+            }
+        };
+        auto body = [&](int s, TA (&rs)[MW][8], int buf) {
+            bf8 a[3][MW];
+            split_a(rs, a);
+            if (s + 1 < nsteps) store_b_regs(buf ^ 1);
+            if (s + 2 < nsteps) {
+                load_a_to(rs, step_of(s + 2));
+                load_b_regs(step_of(s + 2));
+            }
+            compute(a, buf);
+            __syncthreads();
+        };
+        if (nsteps > 0) {
+            copy_b(step_of(0), 0);
+            load_a_to(ra, step_of(0));
+        }
+        if (nsteps > 1) {
+            load_b_regs(step_of(1));
+            load_a_to(ra1, step_of(1));
+        }
+        __syncthreads();
+        for (int s = 0; s < nsteps; s += 2) {
+            body(s, ra, 0);
+            if (s + 1 < nsteps) body(s + 1, ra1, 1);
+        }
+    } else {
     if (nsteps > 0) {
         copy_b(step_of(0), 0);
         load_a(step_of(0));
@@ -224,16 +330,7 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
     for (int s = 0; s < nsteps; ++s) {
         const int buf = s & 1;
         bf8 a[3][MW];
-#pragma unroll
-        for (int i = 0; i < MW; ++i)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                __bf16 h, md, l;
-                split3(ra[i][j], h, md, l);
-                a[0][i][j] = h;
-                a[1][i][j] = md;
-                a[2][i][j] = l;
-            }
+        split_a(ra, a);
         if (s + 1 < nsteps) {
             copy_b(step_of(s + 1), buf ^ 1);
             load_a(step_of(s + 1));
@@ -267,6 +364,7 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(SGemm g) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)c[i][j][r];
         __syncthreads();
+    }
     }
 
 #pragma unroll
@@ -316,14 +414,35 @@ int pad_all_nb(int n) {
 // tiles need 256 VGPRs + 64 AGPRs, one wavefront, and ran 1.47x slower at 97^3)
 constexpr int kMWs = 1;
 
+// Pipeline depth of the split kernels: 2 (A two K steps ahead, B planes one
+// step ahead in registers; 229 registers, two wavefronts per SIMD), or the
+// one-deep loop (163 registers, three wavefronts per SIMD) with
+// DAGE_MIXED_PIPE=1 (comparison runs; read at every launch).  Measured
+// (bf16x6, same bits): 97^3 0.578 -> 0.564 ms, 241^3 14.5 -> 11.0 ms (z pass
+// 4.97 -> 2.73 ms), 8-right-hand-side Helmholtz batch at 385^3 536 -> 470 ms.
+int mixed_pipe() {
+    const char *e = std::getenv("DAGE_MIXED_PIPE");
+    return (e && e[0] == '1') ? 1 : 2;
+}
+
 template <int NB, int NPAIR, typename TA, typename TC, bool AK, int PASS>
 void launch_s(const SGemm &g, dim3 grid, hipStream_t s) {
-    if (g.mask)
-        hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, true>), grid,
-                           dim3(kThreads), 0, s, g);
-    else
-        hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, false>), grid,
-                           dim3(kThreads), 0, s, g);
+    const bool two = mixed_pipe() == 2;
+    if (g.mask) {
+        if (two)
+            hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, true, 2>),
+                               grid, dim3(kThreads), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, true, 1>),
+                               grid, dim3(kThreads), 0, s, g);
+    } else {
+        if (two)
+            hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, false, 2>),
+                               grid, dim3(kThreads), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm_split_kernel<kMWs, NB, NPAIR, TA, TC, AK, PASS, false, 1>),
+                               grid, dim3(kThreads), 0, s, g);
+    }
 }
 
 template <int NPAIR, typename TA, typename TC, bool AK, int PASS>

@@ -151,6 +151,16 @@ module dage_ops_m
             real(c_double)        :: steps(*), bf(*)
         end function
 
+        ! Bubbles_eval_3dgrid on a host cube: cube = cube + multiplier * bubbles (l >= lmin)
+        integer(c_int) function dage_bubbles_inject(bubbles, grid, lmin, multiplier, cube) &
+                bind(C, name="dage_bubbles_inject")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: bubbles, grid
+            integer(c_int), value :: lmin
+            real(c_double), value :: multiplier
+            real(c_double)        :: cube(*)
+        end function
+
         integer(c_int) function dage_bubbles_destroy(bubbles) bind(C, name="dage_bubbles_destroy")
             import :: c_int, c_ptr
             type(c_ptr), value :: bubbles

@@ -416,6 +416,17 @@ class Bubbles:
         call("dage_bubbles_inject_device", self.handle, grid.handle, lmin, float(multiplier),
              ctypes.c_void_p(cube.ptr))
 
+    def eval_3dgrid(self, grid: Grid3D, lmin: int = 0, cube: np.ndarray | None = None,
+                    multiplier: float = 1.0) -> np.ndarray:
+        """Bubbles_eval_3dgrid on host memory: cube (zeros when not given) +
+        multiplier * bubbles (l >= lmin) on the points of grid."""
+        out = np.zeros(grid.get_shape(), order="F") if cube is None else \
+            np.array(cube, dtype=np.float64, order="F", copy=True)
+        if out.shape != grid.get_shape():
+            raise ValueError(f"cube shape {out.shape} != grid shape {grid.get_shape()}")
+        call("dage_bubbles_inject", self.handle, grid.handle, lmin, float(multiplier), _ptr(out))
+        return out
+
     @classmethod
     def like(cls, src: "Bubbles", lmax: int | None = None, k: int = 0) -> "Bubbles":
         """Bubbles(src, lmax=..., k=...) without content: same centres,

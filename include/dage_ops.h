@@ -226,6 +226,12 @@ int dage_bubbles_set(dage_bubbles bubbles, int ibub, const double center[3],
  * grid (asynchronous on the context stream). */
 int dage_bubbles_inject_device(dage_bubbles bubbles, dage_grid3d grid, int lmin,
                                double multiplier, double *cube_dev);
+/* Bubbles_eval_3dgrid (src/bubbles/bubbles.F90:2148-2211) on a host cube:
+ * cube += multiplier * bubbles (l >= lmin) on the points of grid.  The cube
+ * is uploaded, injected on the GPU and downloaded; returns after the
+ * context stream has finished. */
+int dage_bubbles_inject(dage_bubbles bubbles, dage_grid3d grid, int lmin, double multiplier,
+                        double *cube);
 int dage_bubbles_destroy(dage_bubbles bubbles);
 /* Bubbles(src, copy_content=.false., lmax=lmax, k=k) (Bubbles_init_copy,
  * src/bubbles/bubbles.F90:802-830):

@@ -53,6 +53,23 @@ def test_inject_multiplier_and_uneven_grid(ctx):
     assert rel_maxdiff(cube.download(), -0.5 * ref) < 1e-12
 
 
+def test_inject_host_buffer(ctx):
+    """dage_bubbles_inject (Bubbles_eval_3dgrid on a host cube) equals the
+    device inject bit for bit and the oracle to 1e-12."""
+    rng = np.random.default_rng(78)
+    bub, _ = slater_bubbles(rng, 3, lmax=4)
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 7, 7)
+    g = dage_amd.Grid3D(ctx, [ax.qmin] * 3, [ax.steps] * 3, 7)
+    base = np.asfortranarray(rng.standard_normal((ax.n,) * 3))
+    b = lib_bubbles(ctx, bub)
+    host = b.eval_3dgrid(g, lmin=1, cube=base, multiplier=2.0)
+    dev = dage_amd.DeviceCube(ctx, base.shape, base)
+    b.inject(g, dev, lmin=1, multiplier=2.0)
+    assert np.array_equal(host, dev.download())
+    ref = O.bubbles_eval_grid(bub, ax.points, ax.points, ax.points, lmin=1)
+    assert rel_maxdiff(host - base, 2.0 * ref) < 1e-12
+
+
 def test_inject_errors(ctx):
     with pytest.raises(dage_amd.DageError):
         dage_amd.Bubbles(ctx, 2, 13)  # above the largest lmax (12)
