@@ -102,12 +102,16 @@ _SIGNATURES = {
     "dage_gbfmm_create": [_c_p, _c_p, _int, _int, _dp, _dp, _int, ctypes.POINTER(_c_p)],
     "dage_gbfmm_apply_device": [_c_p, _c_p, _c_p, _int],
     "dage_gbfmm_apply": [_c_p, _dp, _dp, _int],
+    "dage_gbfmm_set_energy": [_c_p, _dbl],
     "dage_gbfmm_box_count": [_c_p, _ip],
     "dage_gbfmm_get_multipoles": [_c_p, _dp],
     "dage_gbfmm_get_local_expansions": [_c_p, _dp],
     "dage_gbfmm_destroy": [_c_p],
     "dage_fmm_interaction_matrix": [_int, _dp, _dp],
     "dage_fmm_translation_matrix": [_int, _dp, _dp],
+    "dage_bessel_modified_spherical": [_int, _dbl, _dp, _dp],
+    "dage_fmm_yukawa_interaction_matrix": [_int, _dbl, _dp, _dbl, _dp],
+    "dage_fmm_yukawa_translation_matrix": [_int, _dbl, _dp, _dbl, _dp],
     "dage_helmholtz_batch_create": [_c_p, _int, ctypes.POINTER(_c_p), _dp, _dp, _dp, _int, _dbl,
                                     _int, _int, ctypes.POINTER(_c_p)],
     "dage_batch_input": [_c_p, _int, ctypes.POINTER(_c_p)],

@@ -317,6 +317,31 @@ void fmm_interaction(const FmmTables &tab, const double D[3], double *T);
 // W (N x N, column-major, unit lower triangular): q(new) = W(old - new) q(old).
 void fmm_translation(const FmmTables &tab, const double d[3], double *W);
 void solid_harmonics_host(int lmax, double x, double y, double z, double *S);
+
+// Yukawa (exp(-kappa r)/r) far-field math (yukawa_host.cpp).  Matrices are
+// column-major N x N, N = (lmax+1)^2, lm = l*l + l + m.
+constexpr int kMaxFmmBasisL = 16;
+void bessel_i_scaled(int lmax, double x, double *g);  // i_l(x)/x^l, 0 <= x <= 600
+void bessel_k(int lmax, double x, double *k);         // k_l(x), k_0 = (pi/2) e^-x / x
+void yukawa_regular(int lmax, double kappa, const double v[3], double *R);
+void yukawa_singular(int lmax, double kappa, const double v[3], double *K);
+// Product rule on the unit sphere: Q nodes, weights, C_lm at the nodes [lm + N q].
+struct YukawaSphere {
+    int lmax = 0, Q = 0;
+    std::vector<double> nodes, w, C;
+};
+YukawaSphere yukawa_sphere(int lmax);
+// P[lm + N q]: regular coefficients of a function sampled at rho W_q
+void yukawa_projector(const YukawaSphere &s, double kappa, double rho, double *P);
+// Kq[q + Q lm]: (2 kappa/pi)(2l+1) K_lm(rho W_q - D), the far field at the
+// target sphere of unit source multipoles at offset D (source - target)
+void yukawa_m2l_samples(const YukawaSphere &s, double kappa, const double D[3], double rho,
+                        double *Kq);
+// M2L matrix T = P Kq (target local coefficients from source multipoles)
+void yukawa_m2l(const YukawaSphere &s, double kappa, const double D[3], double rho, double *T);
+// A(lm, l'm'): R_lm(z + d) = sum A R_l'm'(z) for |z| <= rho (M2M: q_parent = A
+// q_child, L2L: L_child = A^T L_parent, d = child centre - parent centre)
+void yukawa_shift(const YukawaSphere &s, double kappa, const double d[3], double rho, double *A);
 // Cube integral (reduce.hip).
 int cube_integrate(dage_grid3d g, const double *f, const double *h,
                    double *result);

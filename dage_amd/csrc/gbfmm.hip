@@ -89,6 +89,15 @@ struct dage_gbfmm_s {
     double *d_Pbox = nullptr;         // [box][p] far-field potential at the box's points
     double *d_gpart = nullptr;        // split-K partials of the multipole GEMM
     int msplits = 1;
+    // Far-field kernel: 1/r, or exp(-kappa r)/r (kappa = sqrt(-2 E)) after
+    // dage_gbfmm_set_energy with E < -0.005 (uniform boxes only).  The
+    // inputs below are kept from construction to rebuild the far-field
+    // matrices and the box basis when the kernel changes.
+    bool yukawa = false;
+    double kappa = 0.0;
+    std::vector<double> box_rel, box_w;  // [axis][box_ld] box-relative coordinates, weights
+    int box_ld = 0;
+    std::vector<std::vector<int>> m2l_keys;  // per level and matrix: box-offset key
 };
 
 namespace dage {
@@ -380,16 +389,18 @@ __global__ void __launch_bounds__(256) sum_parts_kernel(const double *part, int 
 }
 
 // q(parent) = sum_children W_octant q(child)  (translate_multipoles,
-// gbfmm_coulomb3d.F90:1502-1558); W unit lower triangular.
+// gbfmm_coulomb3d.F90:1502-1558); W unit lower triangular for 1/r (only its
+// lower triangle is read), full for the Yukawa kernel.
 __global__ void __launch_bounds__(256) m2m_kernel(const double *W, const double *qchild,
-                                                  double *qparent, int N) {
+                                                  double *qparent, int N, int full) {
     const int P = blockIdx.x, i = threadIdx.x;
     if (i >= N) return;
     double acc = 0.0;
+    const int send = full ? N : i + 1;
     for (int o = 0; o < 8; ++o) {
         const double *Wo = W + (long)o * N * N;
         const double *qc = qchild + (long)(8 * P + o) * N;
-        for (int s = 0; s <= i; ++s) acc += Wo[i + (long)N * s] * qc[s];
+        for (int s = 0; s < send; ++s) acc += Wo[i + (long)N * s] * qc[s];
     }
     qparent[(long)P * N + i] = acc;
 }
@@ -421,13 +432,13 @@ __global__ void __launch_bounds__(256) m2l_scatter_kernel(const double *out, con
 
 // L(child) += W_octant^T L(parent)  (translate_potential, gbfmm_coulomb3d.F90:1562-1622).
 __global__ void __launch_bounds__(256) l2l_kernel(const double *W, const double *Lparent,
-                                                  double *Lchild, int N) {
+                                                  double *Lchild, int N, int full) {
     const int c = blockIdx.x, i = threadIdx.x;
     if (i >= N) return;
     const double *Wo = W + (long)(c & 7) * N * N;
     const double *Lp = Lparent + (long)(c >> 3) * N;
     double acc = 0.0;
-    for (int s = i; s < N; ++s) acc += Wo[s + (long)N * i] * Lp[s];
+    for (int s = full ? 0 : i; s < N; ++s) acc += Wo[s + (long)N * i] * Lp[s];
     Lchild[(long)c * N + i] += acc;
 }
 
@@ -567,6 +578,208 @@ void free_gbfmm(dage_gbfmm f) {
     if (f->s_far) (void)hipStreamDestroy(f->s_far);
     if (f->ctx) ctx_release(f->ctx);
     delete f;
+}
+
+// Size of a box at `level` along each axis.
+void box_size(const dage_gbfmm f, int level, double size[3]) {
+    for (int a = 0; a < 3; ++a) size[a] = (f->qmax[a] - f->qmin[a]) / (double)(1 << level);
+}
+
+// The shared basis of the uniform-box GEMMs: S[p + P lm] and Sw[lm + N p] =
+// S w at the box points p = i + pb0 (j + pb1 k), with S_lm(point - centre)
+// for 1/r (on the GPU) or the Yukawa regular basis i_l(kappa r) C_lm.
+int build_box_basis(dage_gbfmm f) {
+    const int N = f->N, lmax = f->lmax, ld = f->box_ld;
+    const long P = f->P;
+    if (!f->yukawa) {
+        double *d_rel = nullptr, *d_w = nullptr;
+        DAGE_TRY(upload((void **)&d_rel, f->box_rel.data(), 8 * f->box_rel.size()));
+        int rc = upload((void **)&d_w, f->box_w.data(), 8 * f->box_w.size());
+        if (rc == DAGE_OK) {
+            const int grid_sz = (int)std::min<long>((P + 255) / 256, 65536);
+            hipLaunchKernelGGL(box_harmonics_kernel, dim3(grid_sz), dim3(256), 0, f->ctx->stream,
+                               d_rel, d_w, f->pb[0], f->pb[1], f->pb[2], ld, lmax, f->d_coef,
+                               f->d_S, f->d_Sw);
+            if (hipGetLastError() != hipSuccess ||
+                hipStreamSynchronize(f->ctx->stream) != hipSuccess) {
+                set_error("box harmonics failed");
+                rc = DAGE_ERR_HIP;
+            }
+        }
+        if (d_rel) (void)hipFree(d_rel);
+        if (d_w) (void)hipFree(d_w);
+        return rc;
+    }
+    std::vector<double> S((size_t)P * N), Sw((size_t)P * N), R(N);
+    for (long p = 0; p < P; ++p) {
+        const int i = (int)(p % f->pb[0]), j = (int)((p / f->pb[0]) % f->pb[1]),
+                  k = (int)(p / ((long)f->pb[0] * f->pb[1]));
+        const double v[3] = {f->box_rel[i], f->box_rel[ld + j], f->box_rel[2 * ld + k]};
+        const double wp = f->box_w[i] * f->box_w[ld + j] * f->box_w[2 * ld + k];
+        yukawa_regular(lmax, f->kappa, v, R.data());
+        for (int lm = 0; lm < N; ++lm) {
+            S[(size_t)p + (size_t)P * lm] = R[lm];
+            Sw[(size_t)lm + (size_t)N * p] = R[lm] * wp;
+        }
+    }
+    DAGE_HIP_CHECK(hipMemcpy(f->d_S, S.data(), 8 * S.size(), hipMemcpyHostToDevice));
+    DAGE_HIP_CHECK(hipMemcpy(f->d_Sw, Sw.data(), 8 * Sw.size(), hipMemcpyHostToDevice));
+    return DAGE_OK;
+}
+
+// M2L matrices per level and box offset (L_target += T q_source) and the
+// M2M/L2L matrices per child level and octant, for the current kernel.
+// 1/r: the closed forms of fmm_host.cpp on the host.  Yukawa: projection on
+// the target sphere (yukawa_host.cpp), T = P K for all offsets of a level as
+// one batched GEMM on the GPU with the shared projector P, and A = R P^T for
+// the 8 octants likewise (R: the regular basis shifted by the octant vector).
+int build_far_matrices(dage_gbfmm f) {
+    const int L = f->maxlevel, N = f->N, lmax = f->lmax;
+    dage_ctx ctx = f->ctx;
+    hipStream_t s = ctx->stream;
+    const FmmTables *tab = f->yukawa ? nullptr : &fmm_tables(lmax);
+    YukawaSphere sph;
+    if (f->yukawa) sph = yukawa_sphere(lmax);
+    const int Q = sph.Q;
+    constexpr int kChunk = 32;  // M2L matrices sampled per upload
+    double *d_P = nullptr, *d_K = nullptr;
+    auto cleanup = [&]() {
+        if (d_P) (void)hipFree(d_P);
+        if (d_K) (void)hipFree(d_K);
+        d_P = d_K = nullptr;
+    };
+    if (f->yukawa) {
+        if (hipMalloc((void **)&d_P, 8 * (size_t)N * Q) != hipSuccess ||
+            hipMalloc((void **)&d_K, 8 * (size_t)Q * N * std::max(kChunk, 8)) != hipSuccess) {
+            cleanup();
+            set_error("Yukawa projection buffers: allocation failed");
+            return DAGE_ERR_HIP;
+        }
+    }
+    int rc = DAGE_OK;
+    for (int l = f->start_level; l <= L && rc == DAGE_OK; ++l) {
+        double size[3];
+        box_size(f, l, size);
+        const std::vector<int> &keys = f->m2l_keys[l];
+        const int nmat = f->m2l_nmat[l];
+        if (!f->d_T[l] && hipMalloc((void **)&f->d_T[l], 8 * (size_t)N * N * nmat) != hipSuccess) {
+            set_error("M2L matrices: allocation failed");
+            rc = DAGE_ERR_HIP;
+            break;
+        }
+        if (l > f->start_level && !f->d_W[l] &&
+            hipMalloc((void **)&f->d_W[l], 8 * (size_t)8 * N * N) != hipSuccess) {
+            set_error("translation matrices: allocation failed");
+            rc = DAGE_ERR_HIP;
+            break;
+        }
+        auto offset = [&](int key, double D[3]) {
+            const int ox = key % 17 - 8, oy = (key / 17) % 17 - 8, oz = key / 289 - 8;
+            D[0] = ox * size[0];
+            D[1] = oy * size[1];
+            D[2] = oz * size[2];
+        };
+        auto octant = [&](int o, double d[3]) {  // child centre - parent centre
+            d[0] = ((o & 1) - 0.5) * size[0];
+            d[1] = (((o >> 1) & 1) - 0.5) * size[1];
+            d[2] = (((o >> 2) & 1) - 0.5) * size[2];
+        };
+        if (!f->yukawa) {
+            std::vector<double> Tall((size_t)N * N * nmat, 0.0);
+            for (size_t id = 0; id < keys.size(); ++id) {
+                double D[3];
+                offset(keys[id], D);
+                fmm_interaction(*tab, D, &Tall[id * N * N]);
+            }
+            if (hipMemcpy(f->d_T[l], Tall.data(), 8 * Tall.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                set_error("M2L matrices: upload failed");
+                rc = DAGE_ERR_HIP;
+            }
+            if (rc == DAGE_OK && l > f->start_level) {
+                std::vector<double> Wall((size_t)8 * N * N);
+                for (int o = 0; o < 8; ++o) {
+                    double d[3];
+                    octant(o, d);
+                    fmm_translation(*tab, d, &Wall[(size_t)o * N * N]);
+                }
+                if (hipMemcpy(f->d_W[l], Wall.data(), 8 * Wall.size(), hipMemcpyHostToDevice) !=
+                    hipSuccess) {
+                    set_error("translation matrices: upload failed");
+                    rc = DAGE_ERR_HIP;
+                }
+            }
+            continue;
+        }
+        // Yukawa: target sphere of the level's box (its half diagonal)
+        const double rho = 0.5 * std::sqrt(size[0] * size[0] + size[1] * size[1] +
+                                           size[2] * size[2]);
+        if (!(f->kappa * rho <= 600.0)) {
+            set_error("kappa %g too large for boxes of half diagonal %g (kappa rho > 600)",
+                      f->kappa, rho);
+            rc = DAGE_ERR_ARG;
+            break;
+        }
+        std::vector<double> P((size_t)N * Q), K((size_t)Q * N * kChunk);
+        yukawa_projector(sph, f->kappa, rho, P.data());
+        if (hipMemcpy(d_P, P.data(), 8 * P.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("projector upload failed");
+            rc = DAGE_ERR_HIP;
+            break;
+        }
+        for (size_t id0 = 0; id0 < keys.size() && rc == DAGE_OK; id0 += kChunk) {
+            const int cnt = (int)std::min<size_t>(kChunk, keys.size() - id0);
+            for (int j = 0; j < cnt; ++j) {
+                double D[3];
+                offset(keys[id0 + j], D);
+                yukawa_m2l_samples(sph, f->kappa, D, rho, &K[(size_t)j * Q * N]);
+            }
+            if (hipMemcpy(d_K, K.data(), 8 * (size_t)Q * N * cnt, hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                set_error("M2L samples upload failed");
+                rc = DAGE_ERR_HIP;
+                break;
+            }
+            rc = gemm_colmajor_batched(ctx, "gemm_f64_yukawa_m2l", N, N, Q, d_P, N, 0, d_K, Q,
+                                       (long)Q * N, f->d_T[l] + id0 * N * N, (long)N * N, cnt);
+            if (rc == DAGE_OK && hipStreamSynchronize(s) != hipSuccess) {
+                set_error("M2L projection failed");
+                rc = DAGE_ERR_HIP;
+            }
+        }
+        if (rc != DAGE_OK || l == f->start_level) continue;
+        // A(lm, l'm') = sum_q R_lm(rho_c W_q + d) P_c(l'm', q) on the child sphere
+        const double rc_ = 0.5 * rho;
+        std::vector<double> Pc((size_t)N * Q), PcT((size_t)Q * N), Rq((size_t)8 * N * Q), R(N);
+        yukawa_projector(sph, f->kappa, rc_, Pc.data());
+        for (int q = 0; q < Q; ++q)
+            for (int c = 0; c < N; ++c) PcT[(size_t)q + (size_t)Q * c] = Pc[(size_t)c + (size_t)N * q];
+        for (int o = 0; o < 8; ++o) {
+            double d[3];
+            octant(o, d);
+            for (int q = 0; q < Q; ++q) {
+                const double v[3] = {rc_ * sph.nodes[3 * (size_t)q] + d[0],
+                                     rc_ * sph.nodes[3 * (size_t)q + 1] + d[1],
+                                     rc_ * sph.nodes[3 * (size_t)q + 2] + d[2]};
+                yukawa_regular(lmax, f->kappa, v, R.data());
+                std::copy(R.begin(), R.end(), Rq.begin() + ((size_t)o * Q + q) * N);
+            }
+        }
+        if (hipMemcpy(d_P, PcT.data(), 8 * PcT.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_K, Rq.data(), 8 * Rq.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("translation samples upload failed");
+            rc = DAGE_ERR_HIP;
+            break;
+        }
+        rc = gemm_colmajor_batched(ctx, "gemm_f64_yukawa_shift", N, N, Q, d_K, N, (long)N * Q,
+                                   d_P, Q, 0, f->d_W[l], (long)N * N, 8);
+        if (rc == DAGE_OK && hipStreamSynchronize(s) != hipSuccess) {
+            set_error("translation projection failed");
+            rc = DAGE_ERR_HIP;
+        }
+    }
+    cleanup();
+    return rc;
 }
 
 int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *tw, int ntp) {
@@ -760,9 +973,10 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
         f->uniform = uni;
         if (uni) {
             f->P = (long)f->pb[0] * f->pb[1] * f->pb[2];
-            double *d_rel = nullptr, *d_w = nullptr;
-            DAGE_TRY(upload((void **)&d_rel, rel.data(), 8 * rel.size()));
-            int rc = upload((void **)&d_w, w0.data(), 8 * w0.size());
+            f->box_rel = rel;
+            f->box_w = w0;
+            f->box_ld = maxn;
+            int rc = DAGE_OK;
             const size_t ps = (size_t)f->P;
             if (rc == DAGE_OK && (hipMalloc((void **)&f->d_S, 8 * ps * N) != hipSuccess ||
                                   hipMalloc((void **)&f->d_Sw, 8 * ps * N) != hipSuccess ||
@@ -779,25 +993,14 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
                 set_error("GBFMM partials allocation failed");
                 rc = DAGE_ERR_HIP;
             }
-            if (rc == DAGE_OK) {
-                const int grid_sz = (int)std::min<long>((f->P + 255) / 256, 65536);
-                hipLaunchKernelGGL(box_harmonics_kernel, dim3(grid_sz), dim3(256), 0, f->ctx->stream,
-                                   d_rel, d_w, f->pb[0], f->pb[1], f->pb[2], maxn, lmax,
-                                   f->d_coef, f->d_S, f->d_Sw);
-                if (hipGetLastError() != hipSuccess ||
-                    hipStreamSynchronize(f->ctx->stream) != hipSuccess) {
-                    set_error("box harmonics failed");
-                    rc = DAGE_ERR_HIP;
-                }
-            }
-            if (d_rel) (void)hipFree(d_rel);
-            if (d_w) (void)hipFree(d_w);
+            if (rc == DAGE_OK) rc = build_box_basis(f);
             DAGE_TRY(rc);
         }
     }
 
-    // ---- far-field matrices (init_interaction_matrices, get_local_farfield_indices)
-    const FmmTables &tab = fmm_tables(lmax);
+    // ---- interaction lists and far-field matrices (init_interaction_matrices,
+    // get_local_farfield_indices)
+    f->m2l_keys.assign(L + 1, {});
     f->d_T.assign(L + 1, nullptr);
     f->d_list.assign(L + 1, nullptr);
     f->list_len.assign(L + 1, 0);
@@ -807,11 +1010,8 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
     f->d_m2l_gsrc.assign(L + 1, nullptr);
     f->d_m2l_pos.assign(L + 1, nullptr);
     size_t m2l_cols = 1;
-    std::vector<double> M((size_t)N * N);
     for (int l = f->start_level; l <= L; ++l) {
         const int n = 1 << l, nbox = n * n * n;
-        double size[3];
-        for (int a = 0; a < 3; ++a) size[a] = (f->qmax[a] - f->qmin[a]) / (double)n;
         std::map<int, int> mat_id;
         std::vector<int> start(nbox + 1, 0), src, mat;
         for (int t = 0; t < nbox; ++t) {
@@ -842,14 +1042,8 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
                     }
             start[t + 1] = (int)src.size();
         }
-        std::vector<double> Tall((size_t)N * N * std::max<size_t>(1, mat_id.size()));
-        for (auto &kv : mat_id) {
-            const int key = kv.first;
-            const int ox = key % 17 - 8, oy = (key / 17) % 17 - 8, oz = key / 289 - 8;
-            const double D[3] = {ox * size[0], oy * size[1], oz * size[2]};
-            fmm_interaction(tab, D, &Tall[(size_t)kv.second * N * N]);
-        }
-        DAGE_TRY(upload((void **)&f->d_T[l], Tall.data(), 8 * Tall.size()));
+        f->m2l_keys[l].assign(mat_id.size(), 0);
+        for (auto &kv : mat_id) f->m2l_keys[l][kv.second] = kv.first;
         f->list_len[l] = (int)src.size();
         DAGE_TRY(upload((void **)&f->d_list[l], start.data(), 4 * start.size()));
         {   // group the pairs by matrix: slot s of matrix o = its s-th pair in list order
@@ -870,19 +1064,10 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
             DAGE_TRY(upload((void **)&f->d_m2l_pos[l], pos.data(), 4 * pos.size()));
             m2l_cols = std::max(m2l_cols, (size_t)nmat * smax);
         }
-        if (l > f->start_level) {  // child level l: W(child - parent) per octant
-            std::vector<double> Wall((size_t)8 * N * N);
-            for (int o = 0; o < 8; ++o) {
-                const double d[3] = {((o & 1) - 0.5) * size[0], (((o >> 1) & 1) - 0.5) * size[1],
-                                     (((o >> 2) & 1) - 0.5) * size[2]};
-                fmm_translation(tab, d, &Wall[(size_t)o * N * N]);
-            }
-            DAGE_TRY(upload((void **)&f->d_W[l], Wall.data(), 8 * Wall.size()));
-        }
     }
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_G, 8 * (size_t)N * m2l_cols));
     DAGE_HIP_CHECK(hipMalloc((void **)&f->d_m2l_out, 8 * (size_t)N * m2l_cols));
-    return DAGE_OK;
+    return build_far_matrices(f);
 }
 
 // Runs the context's launches on another stream while in scope.
@@ -991,7 +1176,7 @@ int gbfmm_far_coefficients(dage_gbfmm f, const double *rho) {
         prof_begin(ctx, "m2m_kernel", &ev);
         hipLaunchKernelGGL(m2m_kernel, dim3(1 << (3 * l)), dim3(256), 0, s, f->d_W[l + 1],
                            f->d_q + (size_t)N * f->level_offset[l + 1],
-                           f->d_q + (size_t)N * f->level_offset[l], N);
+                           f->d_q + (size_t)N * f->level_offset[l], N, f->yukawa ? 1 : 0);
         DAGE_HIP_CHECK(hipGetLastError());
         prof_end(ctx, "m2m_kernel", ev);
     }
@@ -1002,7 +1187,8 @@ int gbfmm_far_coefficients(dage_gbfmm f, const double *rho) {
         if (l > f->start_level) {
             prof_begin(ctx, "l2l_kernel", &ev);
             hipLaunchKernelGGL(l2l_kernel, dim3(nbox), dim3(256), 0, s, f->d_W[l],
-                               f->d_L + (size_t)N * f->level_offset[l - 1], Ll, N);
+                               f->d_L + (size_t)N * f->level_offset[l - 1], Ll, N,
+                               f->yukawa ? 1 : 0);
             DAGE_HIP_CHECK(hipGetLastError());
             prof_end(ctx, "l2l_kernel", ev);
         }
@@ -1133,6 +1319,34 @@ int dage_gbfmm_apply(dage_gbfmm f, const double *rho, double *pot, int precision
     return rc;
 }
 
+int dage_gbfmm_set_energy(dage_gbfmm f, double energy) {
+    DAGE_REQUIRE(f != nullptr, DAGE_ERR_ARG, "null GBFMM handle");
+    DAGE_REQUIRE(std::isfinite(energy), DAGE_ERR_ARG, "energy is not finite");
+    DAGE_TRY(use_device(f->ctx));
+    const bool yk = energy < -0.005;
+    DAGE_REQUIRE(!yk || f->uniform, DAGE_ERR_UNSUPPORTED,
+                 "the Helmholtz far field needs boxes of equal point counts and spacing");
+    DAGE_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+    DAGE_HIP_CHECK(hipStreamSynchronize(f->s_far));
+    // near field: the banded operator's weights (Helmholtz3D_set_energy)
+    DAGE_TRY(dage_operator_set_energy(f->nf_op, energy));
+    const bool was = f->yukawa;
+    const double kappa_was = f->kappa;
+    f->yukawa = yk;
+    f->kappa = yk ? std::sqrt(-2.0 * energy) : 0.0;
+    if (was == yk && kappa_was == f->kappa) return DAGE_OK;
+    int rc = build_far_matrices(f);
+    if (rc == DAGE_OK) rc = build_box_basis(f);
+    if (rc != DAGE_OK) {  // back to the 1/r far field, which always builds
+        f->yukawa = false;
+        f->kappa = 0.0;
+        (void)dage_operator_set_energy(f->nf_op, 0.0);
+        (void)build_far_matrices(f);
+        (void)build_box_basis(f);
+    }
+    return rc;
+}
+
 int dage_gbfmm_box_count(dage_gbfmm f, int *nbox_total) {
     DAGE_REQUIRE(f && nbox_total, DAGE_ERR_ARG, "null argument");
     *nbox_total = f->nbox_total;
@@ -1178,6 +1392,45 @@ int dage_fmm_translation_matrix(int lmax, const double dR[3], double *W) {
     DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
                  lmax, kMaxFmmLmax);
     fmm_translation(fmm_tables(lmax), dR, W);
+    return DAGE_OK;
+}
+
+int dage_bessel_modified_spherical(int lmax, double x, double *i_l, double *k_l) {
+    DAGE_REQUIRE(i_l && k_l, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmBasisL, DAGE_ERR_UNSUPPORTED,
+                 "lmax %d outside 0..%d", lmax, kMaxFmmBasisL);
+    DAGE_REQUIRE(x > 0.0 && x <= 600.0, DAGE_ERR_ARG, "x = %g outside (0, 600]", x);
+    double g[kMaxFmmBasisL + 1];
+    bessel_i_scaled(lmax, x, g);
+    double xl = 1.0;
+    for (int l = 0; l <= lmax; ++l) {
+        i_l[l] = g[l] * xl;
+        xl *= x;
+    }
+    bessel_k(lmax, x, k_l);
+    return DAGE_OK;
+}
+
+int dage_fmm_yukawa_interaction_matrix(int lmax, double kappa, const double dR[3], double rho,
+                                       double *T) {
+    DAGE_REQUIRE(dR && T, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
+                 lmax, kMaxFmmLmax);
+    const double d = std::sqrt(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    DAGE_REQUIRE(kappa > 0.0 && rho > 0.0 && d > 2.0 * rho && kappa * rho <= 600.0, DAGE_ERR_ARG,
+                 "needs kappa > 0, 0 < rho < |dR|/2 and kappa rho <= 600");
+    yukawa_m2l(yukawa_sphere(lmax), kappa, dR, rho, T);
+    return DAGE_OK;
+}
+
+int dage_fmm_yukawa_translation_matrix(int lmax, double kappa, const double d[3], double rho,
+                                       double *A) {
+    DAGE_REQUIRE(d && A, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(lmax >= 0 && lmax <= kMaxFmmLmax, DAGE_ERR_UNSUPPORTED, "lmax %d outside 0..%d",
+                 lmax, kMaxFmmLmax);
+    DAGE_REQUIRE(kappa > 0.0 && rho > 0.0 && kappa * rho <= 600.0, DAGE_ERR_ARG,
+                 "needs kappa > 0, rho > 0 and kappa rho <= 600");
+    yukawa_shift(yukawa_sphere(lmax), kappa, d, rho, A);
     return DAGE_OK;
 }
 

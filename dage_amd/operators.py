@@ -500,6 +500,7 @@ class GBFMMCoulomb3D:
         self.handle = h
         self.ctx, self.grid = ctx, grid
         self.maxlevel, self.lmax = maxlevel, lmax
+        self.energy = 0.0
         n = ctypes.c_int()
         call("dage_gbfmm_box_count", h, ctypes.byref(n))
         self.nbox_total = n.value
@@ -514,6 +515,13 @@ class GBFMMCoulomb3D:
             self.close()
         except Exception:
             pass
+
+    def set_energy(self, energy: float):
+        """GBFMMHelmholtz3D_set_energy: energy < -0.005 makes this the
+        Helmholtz operator (kernel exp(-kappa r)/r, kappa = sqrt(-2 energy)),
+        energy >= -0.005 the Coulomb one again."""
+        call("dage_gbfmm_set_energy", self.handle, float(energy))
+        self.energy = float(energy)
 
     def apply(self, rho: np.ndarray, precision: int = DAGE_PREC_FP64) -> np.ndarray:
         """Potential of the density cube rho (host arrays, Fortran order).

@@ -368,6 +368,19 @@ int dage_gbfmm_apply_device(dage_gbfmm gbfmm, const double *rho_dev,
 int dage_gbfmm_apply(dage_gbfmm gbfmm, const double *rho, double *pot,
                      int precision);
 /* Number of boxes over levels 2..maxlevel (the columns of the arrays below). */
+/* GBFMMHelmholtz3D_set_energy (src/gbfmm/gbfmm_helmholtz3d.F90:478-515) and
+ * the GBFMMHelmholtz3D far field (:730-964, gbfmm_helmholtz3d.cu:300-420):
+ * energy < -0.005 turns the operator into the Helmholtz one, near field with
+ * the t-weights times exp(2E/(4t^2)) and far field of the screened kernel
+ * exp(-kappa r)/r, kappa = sqrt(-2E) (multipoles and local expansions in
+ * i_l(kappa r) C_lm, translations from dage_fmm_yukawa_*_matrix); energy >=
+ * -0.005 restores the 1/r operator.  Needs boxes of equal point counts and
+ * spacing (DAGE_ERR_UNSUPPORTED otherwise).  Rebuilds the far-field
+ * matrices (host sampling and a GPU projection GEMM) and synchronises the
+ * context stream.  The potential of dage_gbfmm_apply is then
+ * G rho = int exp(-kappa |r - r'|)/|r - r'| rho(r') dr' (without the
+ * -1/(2 pi) of the Helmholtz update). */
+int dage_gbfmm_set_energy(dage_gbfmm gbfmm, double energy);
 int dage_gbfmm_box_count(dage_gbfmm gbfmm, int *nbox_total);
 /* After an apply: multipole moments and local expansions, (lmax+1)^2 x
  * nbox_total column-major, boxes in the reference's order (level maxlevel
@@ -383,6 +396,26 @@ int dage_fmm_interaction_matrix(int lmax, const double dR[3], double *T);
 /* TranslationMatrix%eval (multipole.F90:444-470) unpacked: W, (lmax+1)^2
  * square column-major, unit lower triangular, q(new) = W(old - new) q(old). */
 int dage_fmm_translation_matrix(int lmax, const double dR[3], double *W);
+/* Modified spherical Bessel functions i_l(x), k_l(x), l = 0..lmax (<= 16),
+ * 0 < x <= 600, k_0(x) = (pi/2) exp(-x)/x (the reference's first and second
+ * modified spherical Bessel functions, src/bubbles/modified_bessel.F90,
+ * bessels.cu:183-300).  Host computation. */
+int dage_bessel_modified_spherical(int lmax, double x, double *i_l, double *k_l);
+/* Screened-Coulomb (Yukawa, exp(-kappa r)/r) counterparts of the two
+ * matrices above, for the GBFMM-Helmholtz far field
+ * (HelmholtzMultipoleConverter / HelmholtzMultipoleTranslator,
+ * src/bubbles/helmholtz_multipole_tools.F90:338-426).  Multipoles
+ * q_lm = sum rho w i_l(kappa r) C_lm, local expansions
+ * phi = sum L_lm i_l(kappa r) C_lm (C_lm = S_lm / r^l).  Interaction: L_target
+ * += T q_source for dR = source centre - target centre, exact on the target
+ * sphere of radius rho (|dR| > 2 rho) up to the lmax truncation.
+ * Translation: A with q(parent) = A q(child) and L(child) = A^T L(parent)
+ * for d = child centre - parent centre, for points within rho of the child
+ * centre.  Built by projection on a sphere; host computation. */
+int dage_fmm_yukawa_interaction_matrix(int lmax, double kappa, const double dR[3], double rho,
+                                       double *T);
+int dage_fmm_yukawa_translation_matrix(int lmax, double kappa, const double d[3], double rho,
+                                       double *A);
 
 /* ---------------------------------------------------------- batched applies
  * Helmholtz applies of many orbitals at once: the per-orbital loop of

@@ -14,6 +14,7 @@ docstrings of tests/test_oracle_*.py.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -82,6 +83,8 @@ def _load():
         "dage_oracle_bubble_helmholtz": (i, [i, i, i, i, _dp, d, _dp, _dp]),
         "dage_oracle_yproduct": (i, [i, i, _dp]),
         "dage_oracle_bubble_product": (i, [i, i, i, _dp, _dp, _dp]),
+        "dage_oracle_gbfmm_far_direct": (None, [i, _ip, _ip, _ip, _dp, i, _dp, _dp, _dp, i, i,
+                                                _dp, d, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -642,6 +645,68 @@ def gbfmm_potential(axes, rho, maxlevel, lmax, tp, tw, start_level=2):
                                                  blk.ctypes.data_as(_dp), cut[0], cut[1])
         pot[lo[0]:lo[0] + cut[0], lo[1]:lo[1] + cut[1], lo[2]:lo[2] + cut[2]] += blk
     return pot, q, L
+
+
+def gbfmm_helmholtz_potential(axes, rho, maxlevel, energy, tp, tw):
+    """GBFMMHelmholtz3D_operate, cube part (gbfmm_helmholtz3d.F90:517-964):
+    the near field of every box (Helmholtz3D t-weights, near-field sub-grid
+    = box +- 1 box, no coda, output cut at shared box ends, as
+    gbfmm_potential) plus the far field the multipole expansions approximate,
+    summed directly: exp(-kappa r)/r (kappa = sqrt(-2 energy); 1/r for energy
+    >= -0.005) times rho and the source box's sub-grid weights over the
+    points of every box that is not a nearest neighbour."""
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    nb = 1 << maxlevel
+    w = helmholtz_weights(tp, tw, energy)
+    pot = np.zeros(rho.shape, order="F")
+    s = [a.ncell // nb for a in axes]
+    npl = [a.nlip - 1 for a in axes]
+    for a in range(3):
+        if axes[a].ncell % nb:
+            raise ValueError("cells do not split into boxes")
+    mats = {}
+    for idx in range(nb ** 3):
+        v = _box_vector(idx, maxlevel)
+        sub_in, sub_out, lo_in, lo_out, cut = [], [], [], [], []
+        for a in range(3):
+            b = v[a]
+            c0, c1 = max(0, b - 1) * s[a], min(nb, b + 2) * s[a]
+            sub_in.append(sub_axis(axes[a], c0, c1))
+            sub_out.append(sub_axis(axes[a], b * s[a], (b + 1) * s[a]))
+            lo_in.append(c0 * npl[a])
+            lo_out.append(b * s[a] * npl[a])
+            cut.append(s[a] * npl[a] + (1 if b == nb - 1 else 0))
+        ms = []
+        for a in range(3):
+            if (a, v[a]) not in mats:
+                mats[(a, v[a])] = exponentials(sub_in[a], sub_out[a], tp, a != 0)
+            ms.append(mats[(a, v[a])])
+        nf = rho[lo_in[0]:lo_in[0] + sub_in[0].n, lo_in[1]:lo_in[1] + sub_in[1].n,
+                 lo_in[2]:lo_in[2] + sub_in[2].n]
+        box = transform_cube(ms[0], ms[1], ms[2], w, 0.0, nf, add_coda=False)
+        pot[lo_out[0]:lo_out[0] + cut[0], lo_out[1]:lo_out[1] + cut[1],
+            lo_out[2]:lo_out[2] + cut[2]] = box[:cut[0], :cut[1], :cut[2]]
+    # far field, direct
+    lo = np.zeros(3 * nb, dtype=np.int32)
+    n = np.zeros(3 * nb, dtype=np.int32)
+    ct = np.zeros(3 * nb, dtype=np.int32)
+    wld = max(s[a] * npl[a] + 1 for a in range(3))
+    wts = np.zeros(3 * nb * wld)
+    for a in range(3):
+        for b in range(nb):
+            sub = sub_axis(axes[a], b * s[a], (b + 1) * s[a])
+            lo[a * nb + b] = b * s[a] * npl[a]
+            n[a * nb + b] = sub.n
+            ct[a * nb + b] = s[a] * npl[a] + (1 if b == nb - 1 else 0)
+            wts[(a * nb + b) * wld:(a * nb + b) * wld + sub.n] = sub.ints()
+    kappa = math.sqrt(-2.0 * energy) if energy < -0.005 else 0.0
+    ip = ctypes.POINTER(ctypes.c_int)
+    _load().dage_oracle_gbfmm_far_direct(
+        nb, lo.ctypes.data_as(ip), n.ctypes.data_as(ip), ct.ctypes.data_as(ip), _p(wts), wld,
+        _p(_f64(axes[0].points)), _p(_f64(axes[1].points)), _p(_f64(axes[2].points)),
+        rho.shape[0], rho.shape[1], rho.ctypes.data_as(_dp), float(kappa),
+        pot.ctypes.data_as(_dp))
+    return pot
 
 
 def yukawa_gauss_potential(q, alpha, centers, k, x, y, z):

@@ -1342,3 +1342,57 @@ int dage_oracle_bubble_product(int l1max, int l2max, int npts, const double *f1,
     free(coef);
     return 0;
 }
+
+/* ------------------------------------------------- GBFMM far field, direct
+ * What the GBFMM far field approximates (gbfmm_coulomb3d.F90:650-821 with
+ * 1/r, gbfmm_helmholtz3d.F90:730-964 with exp(-kappa r)/r): at every point x
+ * a box owns (its first cut points per axis), the sum over all boxes c of
+ * the finest level that are not nearest neighbours of x's box of
+ * rho(y) w_c(y) G(|x - y|) over the points y of c's whole sub-grid, with
+ * w_c the product of c's sub-axis integration weights.  kappa = 0 gives 1/r.
+ * Axis a, box b: first point lo[a*nb + b], sub-grid points n[a*nb + b],
+ * owned points cut[a*nb + b], weights w[(a*nb + b)*wld + i].  pot +=. */
+void dage_oracle_gbfmm_far_direct(int nb, const int *lo, const int *n, const int *cut,
+                                  const double *w, int wld, const double *x, const double *y,
+                                  const double *z, int ldx, int ldy, const double *rho,
+                                  double kappa, double *pot) {
+    const int nbox = nb * nb * nb;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < nbox; ++t) {
+        const int tb[3] = {t % nb, (t / nb) % nb, t / (nb * nb)};
+        for (int k = 0; k < cut[2 * nb + tb[2]]; ++k)
+            for (int j = 0; j < cut[nb + tb[1]]; ++j)
+                for (int i = 0; i < cut[tb[0]]; ++i) {
+                    const int gi = lo[tb[0]] + i, gj = lo[nb + tb[1]] + j,
+                              gk = lo[2 * nb + tb[2]] + k;
+                    const double px = x[gi], py = y[gj], pz = z[gk];
+                    double acc = 0.0;
+                    for (int c = 0; c < nbox; ++c) {
+                        const int cb[3] = {c % nb, (c / nb) % nb, c / (nb * nb)};
+                        if (abs(cb[0] - tb[0]) <= 1 && abs(cb[1] - tb[1]) <= 1 &&
+                            abs(cb[2] - tb[2]) <= 1)
+                            continue;
+                        const int x0 = lo[cb[0]], y0 = lo[nb + cb[1]], z0 = lo[2 * nb + cb[2]];
+                        const int nx = n[cb[0]], ny = n[nb + cb[1]], nz = n[2 * nb + cb[2]];
+                        const double *wx = w + (size_t)cb[0] * wld;
+                        const double *wy = w + (size_t)(nb + cb[1]) * wld;
+                        const double *wz = w + (size_t)(2 * nb + cb[2]) * wld;
+                        for (int c3 = 0; c3 < nz; ++c3) {
+                            const double dz = pz - z[z0 + c3];
+                            for (int c2 = 0; c2 < ny; ++c2) {
+                                const double dy = py - y[y0 + c2];
+                                const double wyz = wy[c2] * wz[c3];
+                                const double *rr = rho + (x0 + (size_t)ldx * ((y0 + c2) + (size_t)ldy * (z0 + c3)));
+                                for (int c1 = 0; c1 < nx; ++c1) {
+                                    const double dx = px - x[x0 + c1];
+                                    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                                    const double g = kappa > 0.0 ? exp(-kappa * r) / r : 1.0 / r;
+                                    acc += rr[c1] * wx[c1] * wyz * g;
+                                }
+                            }
+                        }
+                    }
+                    pot[gi + (size_t)ldx * (gj + (size_t)ldy * gk)] += acc;
+                }
+    }
+}

@@ -206,6 +206,12 @@ int dage_oracle_bubble_product(int l1max, int l2max, int npts, const double *f1,
  * loops give the serial loops' results bit for bit); returns the count in
  * use (1 without OpenMP). */
 int dage_oracle_set_threads(int n);
+/* GBFMM far field summed directly over the points of the non-adjacent
+ * boxes, kernel exp(-kappa r)/r (kappa = 0: 1/r); pot +=. */
+void dage_oracle_gbfmm_far_direct(int nb, const int *lo, const int *n, const int *cut,
+                                  const double *w, int wld, const double *x, const double *y,
+                                  const double *z, int ldx, int ldy, const double *rho,
+                                  double kappa, double *pot);
 
 #ifdef __cplusplus
 }

@@ -194,3 +194,44 @@ def test_gbfmm_oracle_matches_direct_apply_and_analytic():
     e_an = np.max(np.abs(pot - exact)) / np.max(exact)
     e_dir = np.max(np.abs(direct - exact)) / np.max(exact)
     assert e_an < 2e-3 and abs(e_an - e_dir) < 5e-5
+
+
+def test_gbfmm_helmholtz_oracle_pinned():
+    """The oracle's GBFMM-Helmholtz (near field with Helmholtz t-weights plus
+    the far field summed directly over the non-adjacent boxes' points,
+    gbfmm_helmholtz3d.F90:517-964) pinned on a 49^3 grid with 4x4x4 boxes:
+    * at energy 0 the direct far field is the 1/r one: it agrees with the
+      oracle's multipole GBFMM (lmax 12) to the multipole truncation;
+    * at energy -0.6 (kappa 1.095) it agrees with the whole-grid Helmholtz
+      apply (no coda; t-quadrature of exp(-kappa r)/r on the cell
+      interpolants) to the same level (6e-6), and both sit at the same
+      distance from the closed-form Yukawa potential of the Gaussians (2.9e-3
+      on this coarse grid)."""
+    ax = O.Axis.ncell_fixed(-6.0, 6.0, 7, 8)
+    rng = np.random.default_rng(3)
+    al = rng.uniform(1.0, 4.0, 5)
+    R = rng.uniform(-3.5, 3.5, (5, 3))
+    q = rng.uniform(0.5, 1.5, 5)
+    rho = O.gauss_density(q, al, R, ax.points, ax.points, ax.points)
+    tp, tw = O.gauss_quad(25, 25)
+    # 1/r: direct far field against the multipole one
+    direct0 = O.gbfmm_helmholtz_potential([ax] * 3, rho, 2, 0.0, tp, tw)
+    mp, _, _ = O.gbfmm_potential([ax] * 3, rho, 2, 12, tp, tw)
+    e0 = np.max(np.abs(direct0 - mp)) / np.max(np.abs(mp))
+    print(f"energy 0: direct far field vs multipoles (lmax 12): {e0:.2e}")
+    assert e0 < 3e-5
+    # Yukawa
+    E = -0.6
+    kappa = np.sqrt(-2 * E)
+    yk = O.gbfmm_helmholtz_potential([ax] * 3, rho, 2, E, tp, tw)
+    fx, fy, fz = O.coulomb3d_matrices([ax] * 3, [ax] * 3, tp)
+    whole = O.transform_cube(fx, fy, fz, O.helmholtz_weights(tp, tw, E), 0.0, rho,
+                             add_coda=False)
+    exact = O.yukawa_gauss_potential(q, al, R, kappa, ax.points, ax.points, ax.points)
+    e_whole = np.max(np.abs(yk - whole)) / np.max(np.abs(whole))
+    e_an = np.max(np.abs(yk - exact)) / np.max(np.abs(exact))
+    e_an_whole = np.max(np.abs(whole - exact)) / np.max(np.abs(exact))
+    print(f"energy {E}: vs whole-grid Helmholtz apply {e_whole:.2e}; vs closed form "
+          f"{e_an:.2e} (whole-grid apply {e_an_whole:.2e})")
+    assert e_whole < 3e-5
+    assert e_an < 5e-3 and abs(e_an - e_an_whole) < 5e-5
