@@ -27,6 +27,7 @@ launch from the committed rocprofv3 counter summary.
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -810,6 +811,33 @@ def bench_config5(ctx, args):
         "speedup_bf16x6_batch_vs_plain_fp64": round(plain_ms / sweep_mixed_ms, 3),
         "bf16x6_vs_fp64_orbital_63": sweep_mixed_err}
     progress("config5: split-BF16 legs done")
+    # GBFMM-Helmholtz on the same grid (kernel exp(-kappa r)/r, k^2 = 2):
+    # switching the operator (far-field matrices rebuilt), the apply, and
+    # the direct whole-grid Helmholtz apply with the same t-quadrature (no
+    # coda) beside it
+    eh = -1.0
+    c0 = time.perf_counter()
+    fmm.set_energy(eh)
+    switch_s = time.perf_counter() - c0
+    fmm.apply_device(S["d_rho"], d_pot)
+    ctx.time_begin()
+    fmm.apply_device(S["d_rho"], d_pot)
+    gh_ms = ctx.time_end()
+    gh = d_pot.download()
+    hd = dage_amd.Helmholtz3D(ctx, grid, grid, gq, coda=False)
+    hd.set_energy(eh)
+    hd.transform_cuda_cube(S["d_rho"], d_pot)
+    ctx.time_begin()
+    hd.transform_cuda_cube(S["d_rho"], d_pot)
+    hdir_ms = ctx.time_end()
+    out["gbfmm_helmholtz"] = {
+        "energy": eh, "kappa": round(math.sqrt(-2 * eh), 4), "set_energy_s": round(switch_s, 3),
+        "gbfmm_ms": round(gh_ms, 2), "direct_apply_ms": round(hdir_ms, 2),
+        "gbfmm_vs_direct": rel(gh, d_pot.download())}
+    del gh
+    hd.close()
+    fmm.set_energy(0.0)
+    progress("config5: GBFMM-Helmholtz leg done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_config5(S, gq, hq)
     fmm.close()
@@ -952,6 +980,7 @@ def main():
         "gbfmm_vs_analytic_8_planes": c5["gbfmm_vs_analytic_8_planes"],
         "sweep_vs_single_apply_orbital_63": c5["sweep_vs_single_apply_orbital_63"],
         "gbfmm": c5["gbfmm"],
+        "gbfmm_helmholtz": c5["gbfmm_helmholtz"],
         "sweep_64": c5["sweep_64"],
         "setup_s": c5["setup_s"],
         "wall_s_timed": round(c5["wall_s"], 3),

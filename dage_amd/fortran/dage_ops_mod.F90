@@ -28,7 +28,8 @@ module dage_ops_m
     public :: dage_bubbles_helmholtz_transform, dage_bubbles_multiply, dage_bubbles_integrate
     public :: dage_operator_operate_on, dage_fun_write, dage_fun_read_info
     public :: dage_gbfmm_create, dage_gbfmm_apply, dage_gbfmm_box_count
-    public :: dage_gbfmm_get_multipoles, dage_gbfmm_destroy
+    public :: dage_gbfmm_get_multipoles, dage_gbfmm_destroy, dage_gbfmm_set_energy
+    public :: dage_bubbles_inject
     public :: dage_helmholtz_batch_create, dage_batch_upload, dage_batch_download
     public :: dage_batch_set_energies, dage_batch_apply, dage_batch_tune, dage_batch_destroy
     public :: DAGE_PREC_BF16X6, DAGE_PREC_BF16X3
@@ -267,6 +268,14 @@ module dage_ops_m
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: fmm
             real(c_double)     :: q(*)
+        end function
+
+        ! GBFMMHelmholtz3D_set_energy: E < -0.005 gives the Helmholtz operator
+        integer(c_int) function dage_gbfmm_set_energy(fmm, energy) &
+                bind(C, name="dage_gbfmm_set_energy")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: fmm
+            real(c_double), value :: energy
         end function
 
         integer(c_int) function dage_gbfmm_destroy(fmm) bind(C, name="dage_gbfmm_destroy")
