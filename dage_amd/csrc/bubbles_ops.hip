@@ -46,19 +46,21 @@ __device__ inline double rpow(double r, int i, int n) {
     return (n < 0 && i == 0) ? 0.0 : powi(r, n);
 }
 
-// FirstModSphBesselCollection_evaluate_small (helmholtz3d.F90:935-980)
+// FirstModSphBesselCollection_evaluate_small (helmholtz3d.F90:935-980):
+// z^n/(2n+1)!! sum_i (z^2/2)^i / (i! (2n+3)(2n+5)...(2n+2i+1)), the same
+// terms to the same tolerance.  The reference accumulates the power and the
+// two factorials separately in REAL128; in FP64 those overflow beyond
+// z ~ 100, so here each term is formed from the previous one (the terms
+// themselves stay below e^z).
 __device__ double bessel_i_series(double z, int n) {
     double prefactor = powi(z, n);
     double divider = 1.0;
     for (int i = 1; i <= 2 * n + 1; i += 2) divider = divider * i;
     prefactor = prefactor / divider;
-    double addition = 1.0, reslt = 1.0, fact = 1.0, a = 0.5 * (z * z), b = 1.0;
-    divider = 1.0;
+    const double a = 0.5 * (z * z);
+    double addition = 1.0, reslt = 1.0;
     for (int i = 1; addition > 1e-15 * reslt && i < 300; ++i) {
-        b = b * a;
-        fact = fact * i;
-        divider = divider * (2 * n + 2 * i + 1);
-        addition = b / (fact * divider);
+        addition = addition * (a / (i * (double)(2 * n + 2 * i + 1)));
         reslt = reslt + addition;
     }
     return prefactor * reslt;
@@ -95,6 +97,26 @@ __device__ double bessel_k(int l, double x) {
     return k1;
 }
 
+// e^x k_l(x): the same recursion from (pi/2)/x and (pi/2)(x+1)/x^2
+__device__ double bessel_k_scaled(int l, double x) {
+    const double pi = 3.141592653589793;
+    double k0 = pi / 2 / x;
+    if (l == 0) return k0;
+    double k1 = (x + 1) / (x * x) * pi / 2;
+    for (int j = 2; j <= l; ++j) {
+        const double k2 = (2 * j - 1) * k1 / x + k0;
+        k0 = k1;
+        k1 = k2;
+    }
+    return k1;
+}
+
+// Above this kappa * (cell length) the LIP interpolant of k_l(kappa s) f(s)
+// or i_l(kappa s) f(s) over one cell carries errors of order exp(kappa L)
+// into the result (the reference's scheme); such bubbles use the
+// exponentially fitted one.
+constexpr double kFittedKappaL = 4.0;
+
 constexpr int kRadialChunk = 96;  // bubbles per launch (pointer tables by value)
 
 struct RadialArgs {
@@ -106,6 +128,10 @@ struct RadialArgs {
     double kappa;
     double ii[kMaxNlip * (kMaxNlip - 1)];  // inward integrals, (nlip, nlip-1)
     double io[kMaxNlip * (kMaxNlip - 1)];  // outward integrals
+    // exponentially fitted weights per bubble (nullptr: the reference's
+    // scheme): per cell the (nlip, nlip-1) inward then outward tables of
+    // lip_inout_integrals_exp at mu = kappa * step
+    const double *wfit[kRadialChunk];
 };
 
 __global__ void __launch_bounds__(64) bubbles_radial_kernel(RadialArgs A) {
@@ -122,6 +148,98 @@ __global__ void __launch_bounds__(64) bubbles_radial_kernel(RadialArgs A) {
         r[i] = B.celld[c] + j * B.steps[c];
     }
     __syncthreads();
+    if (A.helmholtz && A.wfit[b] != nullptr) {
+        // Exponentially fitted form.  With i_l(x) = e^x I(x), k_l(x) = e^-x K(x):
+        //   k_l(kr) int_0^r s^(2+k) i_l(ks) f ds = K(kr) int_0^r e^-k(r-s) u ds,
+        //   i_l(kr) int_r^R s^(2+k) k_l(ks) f ds = I(kr) int_r^R e^-k(s-r) v ds,
+        // u = s^(2+k) I(ks) f, v = s^(2+k) K(ks) f smooth and interpolated
+        // cell by cell, the exponentials integrated exactly (wfit), and the
+        // carries between cells scaled by e^(-k L_c): no term exceeds the
+        // result's own scale.
+        const double kap = A.kappa;
+        const double *W = A.wfit[b];
+        const int tab = A.nlip * n;
+        for (int i = lane; i < npts; i += 64) {
+            const double ri = r[i], x = kap * ri;
+            const double rp = rpow(ri, i, 2 + A.k);
+            const double I = bessel_i(l, A.lmax, x) * exp(-x);
+            const double K = i == 0 ? 0.0 : bessel_k_scaled(l, x);
+            p1[i] = K;          // K(kr_i)
+            p2[i] = rp * I;     // u / f
+            p3[i] = I;          // I(kr_i)
+            p4[i] = rp * K;     // v / f
+        }
+        __syncthreads();
+        for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            const double *f = A.fin[b] + (long)lm * npts;
+            double *out = A.fout[b] + (long)lm * npts;
+            for (int i = lane; i < npts; i += 64) g[i] = p2[i] * f[i];  // u
+            __syncthreads();
+            // outward partials: cum[s0 + 1 + j] = h int_0^(j+1) e^-mu(j+1-t) u
+            for (int c = lane; c < ncell; c += 64) {
+                const int s0 = c * n;
+                const double *wo = W + (long)c * 2 * tab + tab;
+                for (int j = 0; j < n; ++j) {
+                    double sum = 0.0;
+                    for (int i = 0; i <= n; ++i) sum += g[s0 + i] * wo[i + A.nlip * j];
+                    cum[s0 + 1 + j] = sum * B.steps[c];
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {  // D_c = int_0^a_c e^-k(a_c - s) u ds
+                double D = 0.0;
+                for (int c = 0; c < ncell; ++c) {
+                    carry_out[c] = D;
+                    D = exp(-kap * n * B.steps[c]) * D + cum[c * n + n];
+                }
+            }
+            __syncthreads();
+            for (int i = lane; i < npts; i += 64) {
+                if (i == 0) outw[i] = 0.0;
+                else {
+                    const int c = (i - 1) / n, j = i - c * n;  // j = 1..n in cell c
+                    outw[i] = exp(-kap * j * B.steps[c]) * carry_out[c] + cum[i];
+                }
+            }
+            __syncthreads();
+            // inward partials with v; the first value extrapolated for l = 0
+            for (int i = lane; i < npts; i += 64) g[i] = p4[i] * f[i];
+            __syncthreads();
+            if (lane == 0) g[0] = (l == 0) ? 3.0 * g[1] - 3.0 * g[2] + 1.0 * g[3] : 0.0;
+            __syncthreads();
+            for (int c = lane; c < ncell; c += 64) {
+                const int s0 = c * n;
+                const double *wi = W + (long)c * 2 * tab;
+                for (int j = 0; j < n; ++j) {
+                    double sum = 0.0;
+                    for (int i = 0; i <= n; ++i) sum += g[s0 + i] * wi[i + A.nlip * j];
+                    cum[s0 + j] = sum * B.steps[c];
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {  // C_c = int_(b_c)^R e^-k(s - b_c) v ds
+                double C = 0.0;
+                for (int c = ncell - 1; c >= 0; --c) {
+                    carry_in[c] = C;
+                    C = exp(-kap * n * B.steps[c]) * C + cum[c * n];
+                }
+            }
+            __syncthreads();
+            for (int i = lane; i < npts; i += 64) {
+                double inw;
+                if (i == npts - 1) inw = 0.0;
+                else {
+                    const int c = i / n, j = i - c * n;  // j = 0..n-1 in cell c
+                    inw = cum[i] + exp(-kap * (n - j) * B.steps[c]) * carry_in[c];
+                }
+                const double v = p1[i] * outw[i] + p3[i] * inw;
+                out[i] = v * 8.0 * kap;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     // per-point factors: p2/p4 multiply f before the outward/inward
     // integrals, p1/p3 the integrals afterwards
     for (int i = lane; i < npts; i += 64) {
@@ -375,6 +493,14 @@ int radial_transform(dage_bubbles in, dage_bubbles out, bool helmholtz, double e
                  "the Helmholtz bubble transform needs 7-point radial cells (got %d)", in->nlip);
     DAGE_REQUIRE(!helmholtz || (std::isfinite(energy) && energy < 0.0), DAGE_ERR_ARG,
                  "the Helmholtz bubble transform needs a negative energy (got %g)", energy);
+    // the i_l series (300 terms, as the reference) converges to 1e-15 up to
+    // kappa r ~ 400; i_l itself (~ e^(kappa r) / (2 kappa r)) fits FP64 to ~700
+    if (helmholtz)
+        for (int i = 0; i < in->nbub; ++i)
+            DAGE_REQUIRE(std::sqrt(-2.0 * energy) * in->host[i].qmax <= 400.0, DAGE_ERR_UNSUPPORTED,
+                         "Helmholtz bubble transform at energy %g: kappa * r_max = %g exceeds "
+                         "400 (bubble %d), where the i_l series stops converging", energy,
+                         std::sqrt(-2.0 * energy) * in->host[i].qmax, i);
     for (int i = 0; i < in->nbub; ++i)
         DAGE_REQUIRE(in->host[i].npts <= kMaxRadial && in->host[i].ncell * 4 <= kMaxRadial,
                      DAGE_ERR_UNSUPPORTED, "bubble %d has more than %d radial points", i,
@@ -391,6 +517,36 @@ int radial_transform(dage_bubbles in, dage_bubbles out, bool helmholtz, double e
     A.kappa = helmholtz ? std::sqrt(-2.0 * energy) : 0.0;
     DAGE_REQUIRE(lip_inout_integrals(in->nlip, A.ii, A.io) == 0, DAGE_ERR_ARG,
                  "unsupported radial nlip %d", in->nlip);
+    // bubbles with a cell of kappa L > kFittedKappaL take the exponentially
+    // fitted scheme: weight tables per cell at mu = kappa * step
+    const int n = in->nlip - 1, tab = in->nlip * n;
+    std::vector<long> woff(in->nbub, -1);
+    std::vector<double> wts;
+    if (helmholtz) {
+        for (int i = 0; i < in->nbub; ++i) {
+            const BubbleHost &h = in->host[i];
+            std::vector<double> st(h.ncell);
+            DAGE_HIP_CHECK(hipMemcpy(st.data(), h.steps(), 8 * h.ncell, hipMemcpyDeviceToHost));
+            double kl = 0.0;
+            for (double x : st) kl = std::max(kl, A.kappa * n * x);
+            if (kl <= kFittedKappaL) continue;
+            woff[i] = (long)wts.size();
+            wts.resize(wts.size() + (size_t)h.ncell * 2 * tab);
+            for (int c = 0; c < h.ncell; ++c) {
+                double *w = wts.data() + woff[i] + (size_t)c * 2 * tab;
+                DAGE_REQUIRE(lip_inout_integrals_exp(in->nlip, A.kappa * st[c], w, w + tab) == 0,
+                             DAGE_ERR_ARG, "unsupported radial nlip %d", in->nlip);
+            }
+        }
+    }
+    double *d_w = nullptr;
+    if (!wts.empty()) {
+        void *scr = nullptr;
+        DAGE_TRY(ctx_scratch(ctx, 8 * wts.size(), &scr));
+        d_w = (double *)scr;
+        DAGE_HIP_CHECK(hipMemcpyAsync(d_w, wts.data(), 8 * wts.size(), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    }
     hipEvent_t ev;
     prof_begin(ctx, "bubbles_radial_kernel", &ev);
     for (int b0 = 0; b0 < in->nbub; b0 += kRadialChunk) {
@@ -399,6 +555,7 @@ int radial_transform(dage_bubbles in, dage_bubbles out, bool helmholtz, double e
         for (int i = 0; i < nb; ++i) {
             A.fin[i] = in->f(b0 + i);
             A.fout[i] = out->f(b0 + i);
+            A.wfit[i] = woff[b0 + i] >= 0 ? d_w + woff[b0 + i] : nullptr;
         }
         hipLaunchKernelGGL(bubbles_radial_kernel, dim3(in->lmax + 1, nb), dim3(64), 0,
                            ctx->stream, A);

@@ -52,6 +52,7 @@ void lip_first_last(int nlip, double *first, double *last);
 int lip_coeffs(int nlip, int grid_type, double *c);  // c[j + nlip*k]
 int lip_integrals(int nlip, int grid_type, double *v);
 int lip_inout_integrals(int nlip, double *ii, double *io);  // (nlip, nlip-1) each
+int lip_inout_integrals_exp(int nlip, double mu, double *ii, double *io);
 double two_over_sqrtpi();
 double pi();
 

@@ -113,6 +113,78 @@ int lip_coeffs(int nlip, int grid_type, double *c) {
     return 0;
 }
 
+// int_0^T exp(-mu u) u^p du for mu >= 0 (the lower incomplete gamma
+// function gamma(p+1, mu T) / mu^(p+1)), by its all-positive series up to
+// mu T = 40 and by the finite upper-gamma sum beyond.
+static long double exp_moment(int p, long double mu, long double T) {
+    const long double x = mu * T;
+    long double Tp = 1.0L;
+    for (int i = 0; i <= p; ++i) Tp *= T;
+    if (x == 0.0L) return Tp / (p + 1);
+    if (x <= 40.0L) {
+        long double term = 1.0L / (p + 1), sum = term;
+        for (int m = 1; m < 400; ++m) {
+            term *= x / (p + 1 + m);
+            sum += term;
+            if (term < 1e-22L * sum) break;
+        }
+        return Tp * std::exp(-x) * sum;
+    }
+    long double fact = 1.0L, mp = 1.0L, part = 0.0L, xm = 1.0L;
+    for (int m = 0; m <= p; ++m) {
+        if (m > 0) {
+            xm *= x / m;
+            fact *= m;
+        }
+        part += xm;
+        mp *= mu;
+    }
+    return fact / mp * (1.0L - std::exp(-x) * part);
+}
+
+// Exponentially weighted inward / outward LIP integrals of one cell in cell
+// units (nodes t = 0..n, n = nlip - 1, basis L_r on those nodes):
+//   ii[r + nlip*j] = int_j^n exp(-mu (t - j)) L_r(t) dt,  j = 0..n-1
+//   io[r + nlip*j] = int_0^(j+1) exp(-mu (j+1 - t)) L_r(t) dt,  j = 0..n-1
+// (mu = 0: lip_inout_integrals).  Each basis polynomial is re-expanded about
+// the node and integrated term by term against exp_moment, in long double.
+int lip_inout_integrals_exp(int nlip, double mu, double *ii, double *io) {
+    double c[kMaxNlip * kMaxNlip];
+    if (lip_coeffs(nlip, 1, c) || mu < 0.0) return -1;
+    double first, last;
+    lip_first_last(nlip, &first, &last);
+    const int n = nlip - 1;
+    for (int r = 0; r < nlip; ++r) {
+        for (int j = 0; j <= n; ++j) {
+            // Taylor coefficients of L_r about node j (LIP coordinate first + j)
+            long double a[kMaxNlip];
+            for (int k = 0; k < nlip; ++k) a[k] = c[r + nlip * k];  // decreasing powers
+            const long double x0 = first + j;
+            long double d[kMaxNlip];
+            for (int q = 0; q < nlip; ++q) {  // synthetic division: d[q] = P^(q)(x0)/q!
+                long double v = 0.0L;
+                for (int k = 0; k < nlip - q; ++k) {
+                    v = v * x0 + a[k];
+                    a[k] = v;
+                }
+                d[q] = v;
+            }
+            if (j < n) {
+                long double s = 0.0L;
+                for (int q = 0; q < nlip; ++q) s += d[q] * exp_moment(q, mu, (long double)(n - j));
+                ii[r + nlip * j] = (double)s;
+            }
+            if (j > 0) {
+                long double s = 0.0L;
+                for (int q = 0; q < nlip; ++q)
+                    s += ((q % 2) ? -d[q] : d[q]) * exp_moment(q, mu, (long double)j);
+                io[r + nlip * (j - 1)] = (double)s;
+            }
+        }
+    }
+    return 0;
+}
+
 // lip.F90:367-415: antiderivative coefficients, evaluated at last and first.
 int lip_integrals(int nlip, int grid_type, double *v) {
     double c[kMaxNlip * kMaxNlip];

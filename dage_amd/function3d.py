@@ -40,6 +40,7 @@ class Function3D:
         self.cube = cube if cube is not None else DeviceCube(grid.ctx, grid.get_shape())
         self.bubbles = bubbles
         self.type, self.label = type, label
+        self.taylor_order = 2  # Function3D%taylor_order default (function3d.F90:109)
 
     def close(self):
         for obj in (self.cube, self.bubbles):
@@ -82,10 +83,11 @@ class Function3D:
             val += self.bubbles.integrate()
         return val
 
-    def dot(self, f2: "Function3D") -> float:
+    def dot(self, f2: "Function3D", taylor: bool | None = None) -> float:
         """Function3D_dot_product (function3d.F90:1497-1530): integral of the
-        product (bubbles multiplied centre by centre)."""
-        p = self.multiply(f2)
+        product (bubbles multiplied centre by centre; `taylor` as in
+        multiply)."""
+        p = self.multiply(f2, taylor=taylor)
         val = p.integrate()
         p.close()
         return val
@@ -100,14 +102,33 @@ class Function3D:
             self.bubbles.inject(self.grid, out)
         return out
 
-    def multiply(self, f2: "Function3D") -> "Function3D":
-        """Function3D_multiply_sub (function3d.F90:1097-1308) without the
-        Taylor-series terms: bubbles multiplied centre by centre, cube =
-        (cube2 + bubbles2)(cube1 + bubbles1) - product bubbles."""
+    def multiply(self, f2: "Function3D", taylor: bool | None = None) -> "Function3D":
+        """Function3D_multiply_sub (function3d.F90:1097-1308): bubbles
+        multiplied centre by centre, cube = (cube2 + bubbles2)(cube1 +
+        bubbles1) - product bubbles.  When both factors are cusp-type (CUSP or
+        NUCP, as the reference decides in Function3D_multiply_bubbles,
+        function3d.F90:1406-1470) or taylor=True, the product bubbles carry
+        the Taylor-series terms of both factors (order `taylor_order`):
+        (B1 + T1)(B2 + T2) - T1 T2, which keeps the cube free of the
+        singular products near the nuclei."""
         out = Function3D(self.grid, label=self.label)
         if F3D_TYPE_CUSP in (self.type, f2.type):
             out.type = F3D_TYPE_CUSP
         both = self.bubbles is not None and f2.bubbles is not None
+        cusp = (F3D_TYPE_CUSP, F3D_TYPE_NUCP)
+        if taylor is None:
+            taylor = self.type in cusp and f2.type in cusp
+        if both and taylor:
+            t = self.taylor_order
+            bo = Bubbles.like(self.bubbles,
+                              lmax=max(self.bubbles.lmax, t) + max(f2.bubbles.lmax, t),
+                              k=self.bubbles.k + f2.bubbles.k)
+            call("dage_function3d_multiply_taylor", self.grid.handle,
+                 ctypes.c_void_p(self.cube.ptr), self.bubbles.handle,
+                 ctypes.c_void_p(f2.cube.ptr), f2.bubbles.handle, int(t),
+                 ctypes.c_void_p(out.cube.ptr), bo.handle)
+            out.bubbles = bo
+            return out
         bo = Bubbles.like(self.bubbles, lmax=self.bubbles.lmax + f2.bubbles.lmax,
                           k=self.bubbles.k + f2.bubbles.k) if both else None
         call("dage_function3d_multiply", self.grid.handle, ctypes.c_void_p(self.cube.ptr),
@@ -116,6 +137,29 @@ class Function3D:
              ctypes.c_void_p(out.cube.ptr), bo.handle if bo is not None else None)
         out.bubbles = bo
         return out
+
+    def contaminants(self, order: int | None = None) -> np.ndarray:
+        """Function3D_get_cube_contaminants + Bubbles_get_contaminants: the
+        derivatives d^kappa (cube + other centres' bubbles) at every centre,
+        |kappa| <= order, rows in CartIter order, one column per centre."""
+        t = self.taylor_order if order is None else order
+        nd = (t + 1) * (t + 2) * (t + 3) // 6
+        out = np.zeros((nd, self.bubbles.nbub), order="F")
+        call("dage_function3d_contaminants", self.grid.handle, ctypes.c_void_p(self.cube.ptr),
+             self.bubbles.handle, int(t), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def taylor_series_bubbles(self, order: int | None = None) -> Bubbles:
+        """Function3D_get_taylor_series_bubbles (function3d.F90:1338-1404)."""
+        t = self.taylor_order if order is None else order
+        h = ctypes.c_void_p()
+        call("dage_function3d_taylor_bubbles", self.grid.handle, ctypes.c_void_p(self.cube.ptr),
+             self.bubbles.handle, int(t), ctypes.byref(h))
+        b = Bubbles.__new__(Bubbles)
+        b.handle, b.ctx = h, self.ctx
+        b.nbub, b.lmax, b.k, b.nlip = (self.bubbles.nbub, max(self.bubbles.lmax, t),
+                                       self.bubbles.k, self.bubbles.nlip)
+        return b
 
     def inject_extra_bubbles(self, lmax: int, injected: Bubbles | None = None):
         """Injects the l > lmax part of `injected` (default: own bubbles) into

@@ -405,3 +405,325 @@ def helmholtz_update_functions(op, orbitals, potentials, eigen_values, scaling_f
         G.close()
         psi.scale(1.0 / math.sqrt(psi.dot(psi)))
     return orbitals
+
+
+class BubbleHartreeFock:
+    """Closed-shell Hartree-Fock on orbitals that carry bubbles: the
+    reference's RHF Helmholtz cycle (HelmholtzSCFCycle, scf_cycle.F90:
+    2389-2395 Lowdin, 1434-1470 diagonalize, 2007-2070 Hamiltonian matrix,
+    2927-3054 update) with every orbital a Function3D: cube + lmax-`lmax`
+    bubbles on every atom's radial grid.
+
+    Operators, all through the C-ABI on the GPU:
+      V_nuc    bubbles -Z/r (k = -1) on every atom, zero cube
+      rho      2 sum_j psi_j psi_j (Function3D products, bubbles to 2 lmax)
+      J        Coulomb3D operate_on rho (radial Poisson + cube transform)
+      K psi_i  sum_j psi_j Coulomb3D[psi_i psi_j]   (exchange=True)
+      V psi_i  (V_nuc + J) psi_i - K psi_i          (bubbles to 3 lmax, k = -1)
+    One iteration: Hamiltonian matrix in span(psi), rotation to its
+    eigenvectors (eigenvalues eps), the Helmholtz update of every orbital
+    (helmholtz_update_functions' steps), Lowdin orthonormalisation, and the
+    potentials of the new orbitals.
+
+    The kinetic matrix needs no Laplacian: the update psi~ = -1/(2 pi)
+    G(e)[V psi - s psi] solves (T - e) psi~ = -(V psi - s psi), so
+    T psi~ = e psi~ - (V psi - s psi) is formed from functions already at
+    hand and carried through every linear step applied to the orbitals
+    (mixing, normalisation, Lowdin, rotation); <psi_i|T psi_j> is then the
+    kinetic matrix of the same Green's function the update uses, so its
+    fixed points are the Ritz pairs exactly.  The guess (single-exponential
+    s functions e^(-zeta r) on the atoms) has T psi in closed form:
+    -1/2 laplacian e^(-zeta r) = r^-1 (zeta - zeta^2 r / 2) e^(-zeta r).
+
+    Energy: E = sum_i (h_ii + eps_i) + E_nn with sum_i h_ii = tr T +
+    1/2 int V_nuc rho (occupation 2).
+
+    Products of the cusp-type functions carry the Taylor-series terms
+    (Function3D.multiply), which keep -Z/r times the cube and the other
+    centres' bubbles out of the cube.  taylor=None uses them when there is
+    more than one atom: with one atom every function lies in the bubbles and
+    the cube holds only rounding noise, whose interpolant derivatives at the
+    nucleus (noise / h^2) the Taylor series would feed back into the cube
+    (measured: the noise grows 1.4-2.8x per iteration at h = 0.2-0.4)."""
+
+    def __init__(self, ctx: Context, grid: Grid3D, molecule: Molecule, nocc: int,
+                 gauss: GaussQuad | None = None, lmax: int = 4, ncell: int = 33,
+                 r_max: float = 20.0, exchange: bool = True, interaction: bool = True,
+                 update_weight: float = 1.0, taylor: bool | None = None):
+        """interaction=False drops J and K (one-electron problem in V_nuc)."""
+        from .function3d import F3D_TYPE_CUSP, Function3D
+        from .operators import Bubbles, Helmholtz3D, radial_grid_steps
+        self.ctx, self.grid, self.mol, self.nocc = ctx, grid, molecule, nocc
+        self.gauss = gauss if gauss is not None else GaussQuad(20, 20, tlin=2.0, tlog=500.0)
+        self.lmax, self.exchange, self.interaction = lmax, exchange, interaction
+        self.w = float(update_weight)
+        self._F3, self._B, self._cusp = Function3D, Bubbles, F3D_TYPE_CUSP
+        self.steps = [radial_grid_steps(float(z), ncell, 7, r_max) for z in molecule.Z]
+        self.npts = [len(s) * 6 + 1 for s in self.steps]
+        self.nat = len(molecule.Z)
+        self.taylor = (self.nat > 1) if taylor is None else bool(taylor)
+        vb = Bubbles(ctx, self.nat, 0, -1, 7)
+        for a in range(self.nat):
+            vb.set_bubble(a, molecule.R[a], float(molecule.Z[a]), self.steps[a],
+                          np.full((self.npts[a], 1), -float(molecule.Z[a]), order="F"))
+        self.vnuc = Function3D(grid, DeviceCube(ctx, grid.get_shape()), vb, type=F3D_TYPE_CUSP,
+                               label="nuclear")
+        self.vnuc.cube.set_to_zero()
+        self.coul = Coulomb3D(ctx, grid, grid, self.gauss)
+        self.helm = Helmholtz3D(ctx, grid, grid, self.gauss)
+        self.psi, self.tpsi, self.vpsi = [], [], []
+        self.eps = np.zeros(nocc)
+        self.rho = None
+
+    # ------------------------------------------------------------ helpers
+    def _r(self, a):
+        """Radial points of atom a's bubble grid: 7 equidistant LIP nodes per
+        cell, steps[c] the node spacing of cell c."""
+        st = self.steps[a]
+        starts = np.concatenate([[0.0], np.cumsum(6.0 * st)[:-1]])
+        r = (starts[:, None] + st[:, None] * np.arange(6)[None, :]).ravel()
+        return np.concatenate([r, [6.0 * np.sum(st)]])
+
+    def _new(self, label="f"):
+        f = self._F3(self.grid, DeviceCube(self.ctx, self.grid.get_shape()), None,
+                     type=self._cusp, label=label)
+        f.cube.set_to_zero()
+        return f
+
+    def _lincomb(self, funcs, C):
+        """out_j = sum_i C[i, j] funcs_i (cubes in one launch, bubbles by axpby);
+        all inputs share lmax and k."""
+        n = int(np.prod(self.grid.get_shape()))
+        out = [self._new(funcs[0].label) for _ in range(C.shape[1])]
+        c = np.asfortranarray(C, dtype=np.float64)
+        call("dage_vec_lincomb", self.ctx.handle, n, len(funcs),
+             _ptr_array([f.cube for f in funcs]), c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+             len(out), _ptr_array([f.cube for f in out]))
+        src = funcs[0].bubbles
+        if src is not None:
+            for j, f in enumerate(out):
+                f.bubbles = self._B.like(src, k=src.k)
+                for i, g in enumerate(funcs):
+                    if C[i, j] != 0.0:
+                        f.bubbles.axpby(float(C[i, j]), g.bubbles, 1.0)
+        return out
+
+    def _replace(self, name, new):
+        for f in getattr(self, name):
+            f.close()
+        setattr(self, name, new)
+
+    def _mul(self, a, b):
+        return a.multiply(b, taylor=None if self.taylor else False)
+
+    def _dot(self, a, b):
+        return a.dot(b, taylor=None if self.taylor else False)
+
+    def _gram(self, f, g, symmetric=False):
+        M = np.zeros((len(f), len(g)))
+        for i in range(len(f)):
+            for j in range(len(g)):
+                if symmetric and j < i:
+                    M[i, j] = M[j, i]
+                else:
+                    M[i, j] = self._dot(f[i], g[j])
+        return M
+
+    # ------------------------------------------------------------- guess
+    def initial_guess(self, exponents=None):
+        """nocc s functions e^(-zeta r): zeta = Z on every atom (heaviest
+        first), then zeta = Z/2, Z/3, ... in further rounds; with T psi in
+        closed form, then orthonormalised and rotated like any iterate."""
+        order = np.argsort(-self.mol.Z, kind="stable")
+        funcs, tfuncs = [], []
+        rnd = 1
+        while len(funcs) < self.nocc:
+            for a in order:
+                if len(funcs) == self.nocc:
+                    break
+                zeta = float(self.mol.Z[a]) / rnd if exponents is None else exponents[a] / rnd
+                f, t = self._new("orbital"), self._new("T orbital")
+                f.bubbles = self._B(self.ctx, self.nat, self.lmax, 0, 7)
+                t.bubbles = self._B(self.ctx, self.nat, self.lmax, -1, 7)
+                nlm = (self.lmax + 1) ** 2
+                for b in range(self.nat):
+                    bf = np.zeros((self.npts[b], nlm), order="F")
+                    bt = np.zeros((self.npts[b], nlm), order="F")
+                    if b == a:
+                        r = self._r(b)
+                        e = np.exp(-zeta * r)
+                        bf[:, 0] = e
+                        bt[:, 0] = (zeta - 0.5 * zeta * zeta * r) * e
+                    args = (b, self.mol.R[b], float(self.mol.Z[b]), self.steps[b])
+                    f.bubbles.set_bubble(*args, bf)
+                    t.bubbles.set_bubble(*args, bt)
+                funcs.append(f)
+                tfuncs.append(t)
+            rnd += 1
+        self._replace("psi", funcs)
+        self._replace("tpsi", tfuncs)
+        self.eps = np.full(self.nocc, -0.5)
+        self.lowdin()
+        self.orbital_potentials()
+        return self.diagonalize()
+
+    # ------------------------------------------------------------ steps
+    def lowdin(self):
+        """HelmholtzSCFCycle_lowdin_orthogonalize_orbitals: psi <- psi S^-1/2."""
+        S = self._gram(self.psi, self.psi, symmetric=True)
+        s, U = np.linalg.eigh(0.5 * (S + S.T))
+        X = (U / np.sqrt(s)) @ U.T
+        self._replace("psi", self._lincomb(self.psi, X))
+        self._replace("tpsi", self._lincomb(self.tpsi, X))
+        return S
+
+    def orbital_potentials(self):
+        """V psi_i for the current (orthonormal) orbitals; also rho."""
+        F3 = self._F3
+        if self.rho is not None:
+            self.rho.close()
+        rho = self._new("density")
+        for p in self.psi:
+            sq = self._mul(p, p)
+            rho.axpby(2.0, sq, 1.0)
+            sq.close()
+        self.rho = rho
+        # (V_nuc + J): k = -1 bubbles of J's lmax
+        from .function3d import operate_on
+        if self.interaction:
+            J = operate_on(self.coul, rho)
+            vj = F3(self.grid, J.cube, self._B.like(J.bubbles, k=-1), type=self._cusp,
+                    label="V+J")
+            vj.bubbles.axpby(1.0, J.bubbles, 0.0)
+            vj.bubbles.axpby(1.0, self.vnuc.bubbles, 1.0)
+            J.bubbles.close()
+        else:
+            vj = self.vnuc.copy()
+        vpsi = [self._mul(vj, p) for p in self.psi]
+        vj.close()
+        if self.interaction and self.exchange:
+            n = len(self.psi)
+            for i in range(n):
+                for j in range(i, n):
+                    pair = self._mul(self.psi[i], self.psi[j])
+                    pot = operate_on(self.coul, pair)
+                    pair.close()
+                    kij = self._mul(self.psi[j], pot)
+                    vpsi[i].axpby(-1.0, kij, 1.0)
+                    kij.close()
+                    if j != i:
+                        kji = self._mul(self.psi[i], pot)
+                        vpsi[j].axpby(-1.0, kji, 1.0)
+                        kji.close()
+                    pot.close()
+        self._replace("vpsi", vpsi)
+
+    def hamiltonian(self):
+        """(H, tr T) in span(psi): H_ij = <psi_i | T psi_j + V psi_j>
+        symmetrised, and sum_i <psi_i | T psi_i>.  T psi_j + V psi_j is summed
+        into a k = -1 function whose bubbles above the orbitals' lmax are
+        injected into its cube, so that the products stay within 2 lmax."""
+        n = len(self.psi)
+        fpsi = []
+        for j in range(n):
+            f = self.vpsi[j].copy()
+            f.axpby(1.0, self.tpsi[j], 1.0)
+            f.inject_extra_bubbles(self.lmax)
+            fpsi.append(f)
+        H = self._gram(self.psi, fpsi)
+        for f in fpsi:
+            f.close()
+        trT = sum(self._dot(p, t) for p, t in zip(self.psi, self.tpsi))
+        return 0.5 * (H + H.T), float(trT)
+
+    def diagonalize(self):
+        """SCFCycle_diagonalize_worker on orthonormal orbitals: eigenvectors
+        of H, orbitals (and T psi, V psi) rotated to them; returns the
+        total energy."""
+        H, trT = self.hamiltonian()
+        e, C = np.linalg.eigh(H)
+        for j in range(C.shape[1]):
+            if C[np.argmax(np.abs(C[:, j])), j] < 0:
+                C[:, j] = -C[:, j]
+        self._replace("psi", self._lincomb(self.psi, C))
+        self._replace("tpsi", self._lincomb(self.tpsi, C))
+        self._replace("vpsi", self._lincomb(self.vpsi, C))
+        self.eps = e.copy()
+        en = self._dot(self.vnuc, self.rho)
+        self.energy = trT + 0.5 * en + float(np.sum(e)) + self.mol.nuclear_repulsion()
+        self.kinetic = 2.0 * trT
+        return self.energy
+
+    def update(self):
+        """HelmholtzSCFCycle_update_orbitals_worker, keeping T psi: for each
+        orbital (energy order) the level-shifted input, the Helmholtz
+        operate_on, x -1/(2 pi), bubbles above l = 13 dropped and above the
+        orbital's lmax injected, psi += w (psi~ - psi), normalisation."""
+        from .function3d import operate_on
+        order = np.argsort(self.eps, kind="stable")
+        for i in order:
+            psi, tpsi = self.psi[i], self.tpsi[i]
+            inp = self.vpsi[i].copy()
+            e = float(self.eps[i])
+            if e > 0.0:
+                inp.axpby(-(e + 0.1), psi, 1.0)
+                e = -0.1
+            self.helm.set_energy(e)
+            G = operate_on(self.helm, inp)
+            G.scale(-1.0 / (2.0 * math.pi))
+            G.dispose_extra_bubbles(13)
+            # T G = e G - inp, reduced to the orbitals' lmax like G
+            tg = G.copy()
+            tg_b = self._B.like(inp.bubbles, k=inp.bubbles.k)
+            tg_b.axpby(1.0, G.bubbles, 0.0)
+            tg.bubbles.close()
+            tg.bubbles = tg_b
+            tg.axpby(-1.0, inp, e)       # cube: e G - inp; bubbles: e G - inp
+            inp.close()
+            G.inject_extra_bubbles(self.lmax)
+            tg.inject_extra_bubbles(self.lmax)
+            # psi <- (1 - w) psi + w G and the same for T psi
+            G.axpby(1.0 - self.w, psi, self.w)
+            tg.axpby(1.0 - self.w, tpsi, self.w)
+            G.type = tg.type = self._cusp
+            norm = 1.0 / math.sqrt(self._dot(G, G))
+            G.scale(norm)
+            tg.scale(norm)
+            psi.close()
+            tpsi.close()
+            self.psi[i], self.tpsi[i] = G, tg
+
+    def iterate(self):
+        """One SCF iteration after the guess: update, Lowdin, potentials,
+        diagonalisation; returns the energy."""
+        self.update()
+        self.lowdin()
+        self.orbital_potentials()
+        return self.diagonalize()
+
+    def run(self, max_iter=50, e_tol=1e-6, eig_tol=1e-6, log=None) -> SCFResult:
+        res = SCFResult()
+        t0 = time.perf_counter()
+        res.energies.append(self.initial_guess())
+        res.eigen_values.append(self.eps.copy())
+        res.times.append(time.perf_counter() - t0)
+        for it in range(max_iter):
+            res.energies.append(self.iterate())
+            res.eigen_values.append(self.eps.copy())
+            res.times.append(time.perf_counter() - t0)
+            res.iterations = it + 1
+            if log is not None:
+                log(it + 1, res.energies[-1], self.eps)
+            if is_converged(res.energies, res.eigen_values, e_tol, eig_tol):
+                res.converged = True
+                break
+        res.wall_s = time.perf_counter() - t0
+        return res
+
+    def close(self):
+        for name in ("psi", "tpsi", "vpsi"):
+            self._replace(name, [])
+        for obj in (self.rho, self.vnuc, self.coul, self.helm):
+            if obj is not None:
+                obj.close()
+        self.rho = None

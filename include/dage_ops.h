@@ -291,15 +291,43 @@ int dage_bubbles_integrate(dage_bubbles bubbles, double *result);
  * with the product coefficients of the Racah harmonics (YProduct,
  * src/bubbles/harmonic.F90:884-1000).  out: same grids, lmax = la + lb,
  * k = ka + kb.  The Taylor-series terms the reference adds for cusp-type
- * factors are not included.  Asynchronous. */
+ * factors are dage_function3d_multiply_taylor's.  Asynchronous. */
 int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out);
 /* Function3D_multiply_sub (src/bubbles/function3d.F90:1097-1308, the product
- * without Taylor-series terms): bub_out = b1 * b2 (when both given) and
+ * without Taylor-series terms, as the reference forms it when a factor is not
+ * cusp-type): bub_out = b1 * b2 (when both given) and
  * cube_out = (cube2 + B2)(cube1 + B1) - B_out, B = bubbles injected on grid.
  * Device cubes on grid; cube_out may be cube2 but not cube1.  Asynchronous. */
 int dage_function3d_multiply(dage_grid3d grid, const double *cube1, dage_bubbles b1,
                              const double *cube2, dage_bubbles b2, double *cube_out,
                              dage_bubbles bub_out);
+/* Taylor-series "contaminants" of f = cube + bubbles at every centre of b
+ * (Function3D_get_cube_contaminants, src/bubbles/function3d.F90:1570-1622,
+ * plus Bubbles_get_contaminants, src/bubbles/bubbles.F90:2844-2903): the
+ * Cartesian derivatives d^kappa (cube + bubbles of the other centres) at the
+ * centre, |kappa| <= taylor_order (0..2), in CartIter order (x exponent
+ * outermost, z innermost); out is (taylor_order+1)(taylor_order+2)
+ * (taylor_order+3)/6 x nbub, column per centre.  cube may be NULL (bubbles
+ * only); a centre outside the grid gets no cube part.  Synchronous. */
+int dage_function3d_contaminants(dage_grid3d grid, const double *cube, dage_bubbles b,
+                                 int taylor_order, double *out);
+/* Function3D_get_taylor_series_bubbles (src/bubbles/function3d.F90:1338-1404,
+ * Bubbles_make_taylor, src/bubbles/bubbles.F90:1435-1533): new bubbles on
+ * b's centres and grids, k of b, lmax max(lmax_b, taylor_order), holding
+ * erfc(r - 2/r)/2 * sum_kappa d_kappa / |kappa|! (x - R)^kappa with the
+ * contaminants d above.  Synchronous; *out is the caller's to destroy. */
+int dage_function3d_taylor_bubbles(dage_grid3d grid, const double *cube, dage_bubbles b,
+                                   int taylor_order, dage_bubbles *out);
+/* Function3D_multiply_sub of two cusp-type functions, with the Taylor-series
+ * bubbles T1, T2 of both factors (Function3D_multiply_bubbles,
+ * src/bubbles/function3d.F90:1406-1470; BubblesMultiplier_multiply,
+ * src/bubbles/bubbles.F90:3567-3740): bub_out = (b1 + T1)(b2 + T2) - T1 T2,
+ * cube_out = (cube1 + B1)(cube2 + B2) - B_out on the grid.  bub_out: b1's
+ * grids, lmax max(l1, t) + max(l2, t), k = k1 + k2.  cube_out may be cube2
+ * but not cube1.  Synchronous (the contaminants are read on the host). */
+int dage_function3d_multiply_taylor(dage_grid3d grid, const double *cube1, dage_bubbles b1,
+                                    const double *cube2, dage_bubbles b2, int taylor_order,
+                                    double *cube_out, dage_bubbles bub_out);
 /* Grid1D_init_radial (src/bubbles/grid.F90:569-603) cell steps for ncell
  * cells: x_i = c/((r_max+c)/(i*dx)-1), c = 8 z^-1.5, dx = r_max/ncell. */
 int dage_radial_grid_steps(double z, int ncell, int nlip, double r_max,

@@ -81,6 +81,7 @@ def _load():
         "dage_oracle_bessel_k": (None, [i, d, _dp]),
         "dage_oracle_bubble_coulomb": (i, [i, i, i, i, _dp, _dp, _dp]),
         "dage_oracle_bubble_helmholtz": (i, [i, i, i, i, _dp, d, _dp, _dp]),
+        "dage_oracle_exp_inout": (None, [i, d, _dp, _dp]),
         "dage_oracle_yproduct": (i, [i, i, _dp]),
         "dage_oracle_bubble_product": (i, [i, i, i, _dp, _dp, _dp]),
         "dage_oracle_gbfmm_far_direct": (None, [i, _ip, _ip, _ip, _dp, i, _dp, _dp, _dp, i, i,
@@ -398,6 +399,15 @@ def bubble_coulomb(lmax: int, k: int, steps, nlip: int, bf) -> np.ndarray:
                                           bf.ctypes.data_as(_dp), out.ctypes.data_as(_dp)):
         raise ValueError("bubble_coulomb failed")
     return out
+
+
+def exp_inout(nlip: int, mu: float):
+    """(ii, io), each (nlip, nlip-1): the LIP in/out integrals of one cell
+    weighted by exp(-mu |t - t_j|) (quadrature in quadruple precision)."""
+    ii = np.zeros(nlip * (nlip - 1))
+    io = np.zeros(nlip * (nlip - 1))
+    _load().dage_oracle_exp_inout(nlip, float(mu), _p(ii), _p(io))
+    return ii.reshape((nlip, nlip - 1), order="F"), io.reshape((nlip, nlip - 1), order="F")
 
 
 def bubble_helmholtz(lmax: int, k: int, steps, nlip: int, energy: float, bf) -> np.ndarray:
@@ -783,3 +793,244 @@ def conv3d_backward(x, w, gout):
         gs = np.stack([_shift(gout[o], -dx, -dy, -dz) for o in range(gout.shape[0])])
         gx += np.einsum("oc,ozyx->czyx", w[:, :, t], gs)
     return gx, gw, gout.sum(axis=(1, 2, 3))
+
+
+# ------------------------------------------------------ Taylor-series bubbles
+def cart_indices(t: int):
+    """CartIter multi-indices |kappa| <= t, x exponent outermost
+    (src/bubbles/cartiter.F90:196-222)."""
+    return [(a, b, c) for a in range(t + 1) for b in range(t + 1 - a)
+            for c in range(t + 1 - a - b)]
+
+
+def _lip_basis_derivs(nlip: int, grid_type: int, tl: float) -> np.ndarray:
+    """(3, nlip): the LIP basis and its first two derivatives in the cell
+    coordinate (lip.F90:285-344 coefficients, decreasing powers)."""
+    C = lip_coeffs(nlip, grid_type)
+    out = np.zeros((3, nlip))
+    for j in range(nlip):
+        p = np.poly1d(C[j, :])
+        out[0, j], out[1, j], out[2, j] = p(tl), p.deriv(1)(tl), p.deriv(2)(tl)
+    return out
+
+
+def cube_contaminants(axes, cube, centres, t: int = 2) -> np.ndarray:
+    """Function3D_get_cube_contaminants (function3d.F90:1570-1622) with
+    Interpolator_eval (evaluators.F90:908-990): d^kappa of the cube's LIP
+    interpolant at each centre, (nder, ncentre); zero outside the grid."""
+    kap = cart_indices(t)
+    out = np.zeros((len(kap), len(centres)))
+    for c, R in enumerate(centres):
+        B, st = [], []
+        for a, ax in enumerate(axes):
+            ic = get_icell(ax, float(R[a]))
+            if ic == 0:
+                break
+            first, _ = lip_first_last(ax.nlip, ax.grid_type)
+            h = ax.steps[ic - 1]
+            L = _lip_basis_derivs(ax.nlip, ax.grid_type, (R[a] - ax.celld[ic - 1]) / h + first)
+            B.append(L / h ** np.arange(3)[:, None])
+            st.append((ic - 1) * (ax.nlip - 1))
+        else:
+            n = axes[0].nlip
+            blk = cube[st[0]:st[0] + n, st[1]:st[1] + n, st[2]:st[2] + n]
+            for d, kp in enumerate(kap):
+                out[d, c] = np.einsum("ijk,i,j,k->", blk, B[0][kp[0]], B[1][kp[1]], B[2][kp[2]])
+    return out
+
+
+def _solid_harmonics_jet(lmax: int, x):
+    """Racah solid harmonics with gradient and Hessian: lists of (v, g, H)."""
+    x = np.asarray(x, float)
+    r2 = float(x @ x)
+    I = np.eye(3)
+
+    def mul_var(p, a):
+        v, g, H = p
+        return (v * x[a], g * x[a] + v * I[a],
+                H * x[a] + np.outer(g, I[a]) + np.outer(I[a], g))
+
+    def mul_r2(p):
+        v, g, H = p
+        return (v * r2, g * r2 + 2 * v * x,
+                H * r2 + 2 * np.outer(g, x) + 2 * np.outer(x, g) + 2 * v * I)
+
+    def lin(a, p, b, q):
+        return (a * p[0] + b * q[0], a * p[1] + b * q[1], a * p[2] + b * q[2])
+
+    one = (1.0, np.zeros(3), np.zeros((3, 3)))
+    S = [one]
+    if lmax >= 1:
+        S += [mul_var(one, 1), mul_var(one, 2), mul_var(one, 0)]
+    for l in range(2, lmax + 1):
+        p, q = (l - 1) ** 2 + (l - 1), (l - 2) ** 2 + (l - 2)
+        f = math.sqrt((2 * l - 1) / (2 * l))
+        shell = [None] * (2 * l + 1)
+        shell[0] = lin(f, mul_var(S[p + l - 1], 1), f, mul_var(S[p - (l - 1)], 0))
+        for m in range(-(l - 1), l):
+            a = (2 * l - 1) / math.sqrt((l + m) * (l - m))
+            v = mul_var(S[p + m], 2)
+            if l - abs(m) > 1:
+                b = math.sqrt((l + m - 1) * (l - m - 1) / ((l + m) * (l - m)))
+                v = lin(a, v, -b, mul_r2(S[q + m]))
+            else:
+                v = lin(a, v, 0.0, v)
+            shell[m + l] = v
+        shell[2 * l] = lin(f, mul_var(S[p + l - 1], 0), -f, mul_var(S[p - (l - 1)], 1))
+        S += shell
+    return S
+
+
+def _jet_pick(jet, kp):
+    v, g, H = jet
+    idx = [a for a in range(3) for _ in range(kp[a])]
+    if not idx:
+        return v
+    return g[idx[0]] if len(idx) == 1 else H[idx[0], idx[1]]
+
+
+def bubble_contaminants(bub: BubbleSet, t: int = 2) -> np.ndarray:
+    """Bubbles_get_contaminants (bubbles.F90:2844-2903): at each centre B the
+    derivatives of sum_{A != B} P_lm(r) C_lm of the other bubbles, P the
+    radial LIP interpolant of r^k f_lm (Bubbles_get_radial_derivatives,
+    bubbles.F90:2272-2305, the interpolant the evaluation uses), taken
+    exactly as G(r) S_lm(x) with G = r^-l P; (nder, nbub)."""
+    kap = cart_indices(t)
+    nb = len(bub.steps)
+    out = np.zeros((len(kap), nb))
+    first, _ = lip_first_last(bub.nlip, 1)
+    for ib in range(nb):
+        for ia in range(nb):
+            if ia == ib:
+                continue
+            x = np.asarray(bub.centers[ib], float) - np.asarray(bub.centers[ia], float)
+            r = float(np.linalg.norm(x))
+            ax = bub.axis(ia)
+            ic = get_icell(ax, r)
+            if ic == 0 or r == 0.0:
+                continue
+            h = ax.steps[ic - 1]
+            L = _lip_basis_derivs(bub.nlip, 1, (r - ax.celld[ic - 1]) / h + first)
+            p0 = (ic - 1) * (bub.nlip - 1)
+            rn = ax.points[p0:p0 + bub.nlip]
+            rk = np.where(rn > 0, rn, 1.0) ** bub.k
+            if bub.k < 0:
+                rk = np.where(rn > 0, rk, 0.0)
+            vals = np.asarray(bub.bf[ia])[p0:p0 + bub.nlip, :] * rk[:, None]   # r^k f
+            P = (L @ vals) / h ** np.arange(3)[:, None]       # (3, nlm)
+            S = _solid_harmonics_jet(bub.lmax, x)
+            tot = (0.0, np.zeros(3), np.zeros((3, 3)))
+            u = x / r
+            for l in range(bub.lmax + 1):
+                a = -l
+                for m in range(-l, l + 1):
+                    lm = l * l + l + m
+                    G = r ** a * P[0, lm]
+                    G1 = a * r ** (a - 1) * P[0, lm] + r ** a * P[1, lm]
+                    G2 = (a * (a - 1) * r ** (a - 2) * P[0, lm] + 2 * a * r ** (a - 1) * P[1, lm]
+                          + r ** a * P[2, lm])
+                    gj = (G, G1 * u, G2 * np.outer(u, u) + G1 * (np.eye(3) - np.outer(u, u)) / r)
+                    s = S[lm]
+                    tot = (tot[0] + gj[0] * s[0], tot[1] + gj[1] * s[0] + gj[0] * s[1],
+                           tot[2] + gj[2] * s[0] + np.outer(gj[1], s[1]) + np.outer(s[1], gj[1])
+                           + gj[0] * s[2])
+            for d, kp in enumerate(kap):
+                out[d, ib] += _jet_pick(tot, kp)
+    return out
+
+
+def cart_to_sph(t: int) -> np.ndarray:
+    """c[kappa, lm]: u^kappa = sum_lm c C_lm(u) on the unit sphere (l <= t),
+    by projection with a product Gauss rule."""
+    kap = cart_indices(t)
+    N = (t + 1) ** 2
+    ct, wt = np.polynomial.legendre.leggauss(6)
+    c = np.zeros((len(kap), N))
+    for ci, wi in zip(ct, wt):
+        for j in range(12):
+            ph = 2 * math.pi * j / 12
+            st = math.sqrt(1 - ci * ci)
+            u = (st * math.cos(ph), st * math.sin(ph), ci)
+            S = solid_harmonics(t, *u)
+            w = wi * 2 * math.pi / 12
+            for d, kp in enumerate(kap):
+                mono = u[0] ** kp[0] * u[1] ** kp[1] * u[2] ** kp[2]
+                for l in range(t + 1):
+                    for m in range(-l, l + 1):
+                        lm = l * l + l + m
+                        c[d, lm] += (2 * l + 1) / (4 * math.pi) * w * mono * S[lm]
+    c[np.abs(c) < 1e-14] = 0.0
+    return c
+
+
+def taylor_weight(r):
+    """w(r) = erfc(r - 2/r) / 2, 1 at r = 0 (bubbles.F90:1537-1563)."""
+    from scipy.special import erfc
+    r = np.asarray(r, float)
+    rs = np.where(r > np.finfo(float).eps, r, 1.0)
+    return np.where(r > np.finfo(float).eps, 0.5 * erfc(rs - 2.0 / rs), 1.0)
+
+
+def make_taylor(bub: BubbleSet, contam, t: int = 2) -> BubbleSet:
+    """Bubbles_make_taylor (bubbles.F90:1435-1533): at every centre
+    f_lm(r) = sum_kappa d_kappa / |kappa|! w(r) r^(|kappa| - k) c2s(kappa, lm)."""
+    kap = cart_indices(t)
+    lt = max(bub.lmax, t)
+    c2s = cart_to_sph(t)
+    bfs = []
+    for i in range(len(bub.steps)):
+        r = bub.axis(i).points
+        f = np.zeros((len(r), (lt + 1) ** 2), order="F")
+        w = taylor_weight(r)
+        for d, kp in enumerate(kap):
+            n = sum(kp)
+            p = n - bub.k
+            rp = np.where(r > 0, r, 1.0) ** p
+            if p != 0:
+                rp = np.where(r > 0, rp, 0.0)
+            v = contam[d, i] / math.factorial(n) * w * rp
+            for lm in range((t + 1) ** 2):
+                if c2s[d, lm] != 0.0:
+                    f[:, lm] += c2s[d, lm] * v
+        bfs.append(f)
+    return BubbleSet(bub.centers, bub.steps, bfs, lt, bub.k, bub.nlip)
+
+
+def taylor_series_bubbles(axes, cube, bub: BubbleSet, t: int = 2) -> BubbleSet:
+    """Function3D_get_taylor_series_bubbles (function3d.F90:1338-1404)."""
+    contam = bubble_contaminants(bub, t)
+    if cube is not None:
+        contam = contam + cube_contaminants(axes, cube, bub.centers, t)
+    return make_taylor(bub, contam, t)
+
+
+def _pad_bf(bub: BubbleSet, lmax: int):
+    n = (lmax + 1) ** 2
+    out = []
+    for f in bub.bf:
+        g = np.zeros((f.shape[0], n), order="F")
+        g[:, :f.shape[1]] = f
+        out.append(g)
+    return out
+
+
+def function3d_multiply_taylor(axes, cube1, b1: BubbleSet, cube2, b2: BubbleSet, t: int = 2):
+    """Function3D_multiply_sub of two cusp-type functions with the Taylor
+    bubbles of both (function3d.F90:1406-1470, bubbles.F90:3567-3740):
+    bubbles (B1 + T1)(B2 + T2) - T1 T2; cube (c1 + B1)(c2 + B2) - bubbles."""
+    T1 = taylor_series_bubbles(axes, cube1, b1, t)
+    T2 = taylor_series_bubbles(axes, cube2, b2, t)
+    l1, l2 = max(b1.lmax, t), max(b2.lmax, t)
+    P1 = [a + b for a, b in zip(_pad_bf(b1, l1), T1.bf)]
+    P2 = [a + b for a, b in zip(_pad_bf(b2, l2), T2.bf)]
+    res = []
+    for i in range(len(b1.steps)):
+        pr = bubble_product(l1, P1[i], l2, P2[i])
+        tt = bubble_product(l1, T1.bf[i], l2, T2.bf[i])
+        res.append(pr - tt)
+    out_b = BubbleSet(b1.centers, b1.steps, res, l1 + l2, b1.k + b2.k, b1.nlip)
+    P = [ax.points for ax in axes]
+    B1 = bubbles_eval_grid(b1, *P)
+    B2 = bubbles_eval_grid(b2, *P)
+    cube = (cube1 + B1) * (cube2 + B2) - bubbles_eval_grid(out_b, *P)
+    return cube, out_b

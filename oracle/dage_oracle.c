@@ -1230,6 +1230,141 @@ int dage_oracle_bubble_coulomb(int lmax, int k, int ncell, int nlip, const doubl
     return rc;
 }
 
+/* Exponentially weighted inward / outward LIP integrals of one cell in cell
+ * units (nodes t = 0..n): ii[r + nlip*j] = int_j^n exp(-mu (t-j)) L_r(t) dt,
+ * io[r + nlip*j] = int_0^(j+1) exp(-mu (j+1-t)) L_r(t) dt, by composite
+ * 20-point Gauss-Legendre quadrature (4 panels per node interval) in
+ * quadruple precision: an evaluation independent of the closed-form moments
+ * the library uses. */
+void dage_oracle_exp_inout(int nlip, double mu, double *ii, double *io) {
+    double c[256], first, last, gx[20], gw[20];
+    dage_oracle_lip_coeffs(nlip, 1, c);
+    dage_oracle_lip_first_last(nlip, 1, &first, &last);
+    dage_oracle_gauleg(20, gx, gw);
+    const int n = nlip - 1, panels = 4;
+    for (int r = 0; r < nlip; ++r) {
+        for (int j = 0; j < n; ++j) {
+            __float128 sin_ = 0.0Q, sout = 0.0Q;
+            for (int q = 0; q < n; ++q)
+                for (int pnl = 0; pnl < panels; ++pnl) {
+                    const __float128 a = q + (__float128)pnl / panels;
+                    const __float128 hw = 0.5Q / panels;
+                    for (int g = 0; g < 20; ++g) {
+                        const __float128 t = a + hw * (1.0Q + gx[g]);
+                        __float128 L = 0.0Q;
+                        const __float128 X = t + first;
+                        for (int kk = 0; kk < nlip; ++kk) L = L * X + c[r + nlip * kk];
+                        const __float128 wl = hw * gw[g] * L;
+                        if (q >= j) sin_ += wl * expq(-(__float128)mu * (t - j));
+                        if (q <= j) sout += wl * expq(-(__float128)mu * (j + 1 - t));
+                    }
+                }
+            ii[r + nlip * j] = (double)sin_;
+            io[r + nlip * j] = (double)sout;
+        }
+    }
+}
+
+/* The exponentially fitted form of the radial Helmholtz convolution, used
+ * when a cell has kappa L > 4 (there the LIP interpolant of k_l(ks) f or
+ * i_l(ks) f over the cell carries errors of order exp(kappa L)): with
+ * i_l(x) = e^x I(x), k_l(x) = e^-x K(x),
+ *   V = 8k [K(kr) int_0^r e^-k(r-s) u ds + I(kr) int_r^R e^-k(s-r) v ds],
+ * u = s^(2+k) I(ks) f, v = s^(2+k) K(ks) f interpolated cell by cell, the
+ * exponentials integrated exactly (dage_oracle_exp_inout), carries between
+ * cells scaled by e^(-k L). */
+static int o_helmholtz_fitted(int lmax, int k, int ncell, int nlip, const double *steps,
+                              double kappa, const double *r, const double *bf, double *out) {
+    const int npts = ncell * (nlip - 1) + 1, n = nlip - 1, tab = nlip * n;
+    double *Is = malloc(sizeof(double) * (size_t)npts * (lmax + 1));
+    double *Ks = malloc(sizeof(double) * (size_t)npts * (lmax + 1));
+    double *W = malloc(sizeof(double) * (size_t)ncell * 2 * tab);
+    double *u = malloc(sizeof(double) * npts), *v = malloc(sizeof(double) * npts);
+    double *outw = malloc(sizeof(double) * npts), *inw = malloc(sizeof(double) * npts);
+    double *D = malloc(sizeof(double) * ncell), *C = malloc(sizeof(double) * ncell);
+    if (!Is || !Ks || !W || !u || !v || !outw || !inw || !D || !C) {
+        free(Is); free(Ks); free(W); free(u); free(v); free(outw); free(inw); free(D); free(C);
+        return -1;
+    }
+    for (int i = 0; i < npts; ++i) {
+        const __float128 x = (__float128)kappa * r[i];
+        double tmp[64];
+        dage_oracle_bessel_i(lmax, kappa * r[i], tmp);
+        for (int l = 0; l <= lmax; ++l) Is[i + (size_t)npts * l] = (double)(tmp[l] * expq(-x));
+        if (i == 0) {
+            for (int l = 0; l <= lmax; ++l) Ks[(size_t)npts * l] = 0.0;
+        } else {
+            const __float128 pi = (__float128)o_pi();
+            __float128 kq[64];
+            kq[0] = pi / 2 / x;
+            if (lmax >= 1) kq[1] = (x + 1) / (x * x) * pi / 2;
+            for (int l = 2; l <= lmax; ++l) kq[l] = (2 * l - 1) * kq[l - 1] / x + kq[l - 2];
+            for (int l = 0; l <= lmax; ++l) Ks[i + (size_t)npts * l] = (double)kq[l];
+        }
+    }
+    for (int c = 0; c < ncell; ++c)
+        dage_oracle_exp_inout(nlip, kappa * steps[c], W + (size_t)c * 2 * tab,
+                              W + (size_t)c * 2 * tab + tab);
+    double *rp = malloc(sizeof(double) * npts);
+    o_rpow(r, npts, 2 + k, rp);
+    for (int l = 0; l <= lmax; ++l) {
+        const double *I = Is + (size_t)npts * l, *K = Ks + (size_t)npts * l;
+        for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            const double *f = bf + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) {
+                u[i] = rp[i] * I[i] * f[i];
+                v[i] = rp[i] * K[i] * f[i];
+            }
+            v[0] = (l == 0) ? 3.0 * v[1] - 3.0 * v[2] + 1.0 * v[3] : 0.0;
+            /* outward: partials within each cell, then carries D_c */
+            outw[0] = 0.0;
+            for (int c = 0; c < ncell; ++c)
+                for (int j = 0; j < n; ++j) {
+                    double sum = 0.0;
+                    for (int i = 0; i <= n; ++i)
+                        sum += u[c * n + i] * W[(size_t)c * 2 * tab + tab + i + nlip * j];
+                    outw[c * n + 1 + j] = sum * steps[c];
+                }
+            double Dc = 0.0;
+            for (int c = 0; c < ncell; ++c) {
+                D[c] = Dc;
+                Dc = exp(-kappa * n * steps[c]) * Dc + outw[c * n + n];
+            }
+            for (int i = 1; i < npts; ++i) {
+                const int c = (i - 1) / n, j = i - c * n;
+                outw[i] = exp(-kappa * j * steps[c]) * D[c] + outw[i];
+            }
+            /* inward */
+            for (int c = 0; c < ncell; ++c)
+                for (int j = 0; j < n; ++j) {
+                    double sum = 0.0;
+                    for (int i = 0; i <= n; ++i)
+                        sum += v[c * n + i] * W[(size_t)c * 2 * tab + i + nlip * j];
+                    inw[c * n + j] = sum * steps[c];
+                }
+            inw[npts - 1] = 0.0;
+            double Cc = 0.0;
+            for (int c = ncell - 1; c >= 0; --c) {
+                C[c] = Cc;
+                Cc = exp(-kappa * n * steps[c]) * Cc + inw[c * n];
+            }
+            double *o = out + (size_t)npts * lm;
+            for (int i = 0; i < npts; ++i) {
+                double in_i = 0.0;
+                if (i < npts - 1) {
+                    const int c = i / n, j = i - c * n;
+                    in_i = inw[i] + exp(-kappa * (n - j) * steps[c]) * C[c];
+                }
+                o[i] = (K[i] * outw[i] + I[i] * in_i) * 8.0 * kappa;
+            }
+        }
+    }
+    free(Is); free(Ks); free(W); free(u); free(v); free(outw); free(inw); free(D); free(C);
+    free(rp);
+    return 0;
+}
+
 int dage_oracle_bubble_helmholtz(int lmax, int k, int ncell, int nlip, const double *steps,
                                  double energy, const double *bf, double *out) {
     const int npts = ncell * (nlip - 1) + 1, nlm = (lmax + 1) * (lmax + 1);
@@ -1244,6 +1379,14 @@ int dage_oracle_bubble_helmholtz(int lmax, int k, int ncell, int nlip, const dou
     }
     dage_oracle_grid1d(0.0, ncell, nlip, steps, 1, r, celld);
     const double kappa = sqrt(-2.0 * energy);
+    double kl = 0.0;
+    for (int c = 0; c < ncell; ++c)
+        if (kappa * (nlip - 1) * steps[c] > kl) kl = kappa * (nlip - 1) * steps[c];
+    if (kl > 4.0) {
+        const int rc = o_helmholtz_fitted(lmax, k, ncell, nlip, steps, kappa, r, bf, out);
+        free(r); free(celld); free(bi); free(bk); free(w);
+        return rc;
+    }
     double tmp[64];
     for (int i = 0; i < npts; ++i) {
         dage_oracle_bessel_i(lmax, kappa * r[i], tmp);

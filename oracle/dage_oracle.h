@@ -190,6 +190,7 @@ int dage_oracle_bubble_coulomb(int lmax, int k, int ncell, int nlip, const doubl
                                const double *bf, double *out);
 /* Helmholtz3D_transform_bubbles_sub (helmholtz3d.F90:519-657) for one bubble
  * (equidistant cells), kappa = sqrt(-2 energy). */
+void dage_oracle_exp_inout(int nlip, double mu, double *ii, double *io);
 int dage_oracle_bubble_helmholtz(int lmax, int k, int ncell, int nlip, const double *steps,
                                  double energy, const double *bf, double *out);
 

@@ -51,6 +51,50 @@ def test_coulomb_and_helmholtz_transforms_match_oracle(ctx, k):
         obj.close()
 
 
+@pytest.mark.parametrize("energy", [-31.0, -180.0])
+def test_helmholtz_transform_deep_core_energies(ctx, energy):
+    """Core-orbital energies (oxygen 1s ~ -20 Ha, and deeper): kappa r reaches
+    158 and 379 on a 20-bohr radial grid, where the i_l series must not
+    overflow (the reference sums it in REAL128) and the outer cells have
+    kappa L >> 1, which takes the exponentially fitted form; against the
+    oracle (REAL128 Bessel functions, quadrature weights), 1e-12 of max|V|,
+    and against the Yukawa closed form of a diffuse Gaussian, 1e-6.  Beyond
+    kappa r_max = 400 the series stops converging within its 300 terms:
+    refused."""
+    rng = np.random.default_rng(7)
+    bset, Z = slater_bubbles(rng, 2, lmax=12, k=-1)
+    b = lib_bubbles(ctx, bset, Z)
+    vh = b.helmholtz_transform(energy)
+    for i in range(2):
+        ref = O.bubble_helmholtz(12, -1, bset.steps[i], 7, energy, bset.bf[i])
+        got = vh.get(i)[3]
+        assert np.all(np.isfinite(got))
+        e = rel_maxdiff(got, ref)
+        print(f"E {energy}: bubble {i} Helmholtz transform vs oracle {e:.1e}")
+        assert e < 1e-12
+    vh.close()
+    # the fitted form against the closed form: the Yukawa potential of a
+    # diffuse normalised Gaussian on a Z = 8 grid (last cell 12.9 bohr)
+    st = O.radial_steps(8.0, 33, 7, 20.0)
+    r = O.Axis(0.0, st, 7).points
+    g = dage_amd.Bubbles(ctx, 1, 0, 0, 7)
+    f = (0.3 / np.pi) ** 1.5 * np.exp(-0.3 * r ** 2)
+    g.set_bubble(0, [0.0, 0.0, 0.0], 8.0, st, f[:, None])
+    V = g.helmholtz_transform(energy)
+    k = np.sqrt(-2 * energy)
+    ref = np.array([O.yukawa_gauss_potential([1.0], [0.3], [[0.0, 0.0, 0.0]], k, [x], [0.0],
+                                             [0.0])[0, 0, 0] for x in r])
+    e = np.max(np.abs(V.get(0)[3][:, 0] - ref)) / np.max(np.abs(ref))
+    print(f"E {energy}: diffuse Gaussian on the Z = 8 grid vs Yukawa closed form {e:.1e}")
+    assert e < 1e-6
+    V.close()
+    g.close()
+    qmax = max(O.Axis(0.0, st, 7).points[-1] for st in bset.steps)
+    with pytest.raises(dage_amd.DageError):
+        b.helmholtz_transform(-0.5 * (401.0 / qmax) ** 2)
+    b.close()
+
+
 def gaussian_bubbles(ctx, lmax, centers, alpha, coef, Z=1.0):
     bub = dage_amd.Bubbles(ctx, len(centers), lmax, 0, 7)
     sets = []

@@ -2,7 +2,8 @@
 HelmholtzSCFCycle_update_orbitals_worker (scf_cycle.F90:2927-3054), on the
 GPU against the same steps composed from the oracle's pinned pieces
 (product, cube transform with coda, radial Bessel convolution, inject,
-integrals), with a nuclear-potential bubble (-Z/r, k = -1) in V so that the
+integrals; the normalising dot product of the cusp-type orbitals with the
+Taylor-series terms), with a nuclear-potential bubble (-Z/r, k = -1) in V so that the
 bubbles of V psi, of the Helmholtz result and of the updated orbital are all
 exercised; one eigenvalue is positive (the level-shift path).  Bar 1e-10 of
 max|psi|.  At the config-4 shape (10 atoms, 12 orbitals with lmax-4 bubbles
@@ -119,10 +120,8 @@ def test_update_with_bubbles_matches_oracle(ctx):
         new_c = ocubes[o] + 0.7 * (Gc - ocubes[o])
         new_b = [obfs[o][i] + 0.7 * (Gb[i] - obfs[o][i]) for i in range(2)]
         nset = O.BubbleSet(centers, steps, new_b, lmax, 0, 7)
-        Bn = O.bubbles_eval_grid(nset, P, P, P)
-        sq_b = [O.bubble_product(lmax, new_b[i], lmax, new_b[i]) for i in range(2)]
-        sq_set = O.BubbleSet(centers, steps, sq_b, 2 * lmax, 0, 7)
-        sq_cube = (new_c + Bn) * (new_c + Bn) - O.bubbles_eval_grid(sq_set, P, P, P)
+        # psi . psi of cusp-type orbitals: the product with Taylor-series terms
+        sq_cube, sq_set = O.function3d_multiply_taylor([ax] * 3, new_c, nset, new_c, nset, 2)
         w = ax.ints()
         norm = O.cube_integral(w, w, w, sq_cube) + oracle_bubble_integral(sq_set)
         new_c, new_b = new_c / math.sqrt(norm), [b / math.sqrt(norm) for b in new_b]

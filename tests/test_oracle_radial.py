@@ -137,3 +137,64 @@ def test_helmholtz_radial_tends_to_coulomb():
     # and a finite energy really screens: |V| drops below the Coulomb tail
     H1 = O.bubble_helmholtz(lmax, 0, st, 7, -1.0, bf)
     assert abs(H1[-10, 0]) < 0.1 * abs(C[-10, 0])
+
+
+def test_exp_weighted_inout_integrals():
+    """The exponentially fitted scheme's cell weights: at mu = 0 the plain
+    LIP in/out integrals, and for mu > 0 exact for polynomials of degree <= 6
+    against exp(-mu |t - t_j|) (scipy quad)."""
+    from scipy.integrate import quad
+    ii0, io0 = O.exp_inout(7, 0.0)
+    ii, io = O.lip_inout_integrals(7)
+    assert np.max(np.abs(ii0 - ii)) < 1e-13 and np.max(np.abs(io0 - io)) < 1e-13
+    nodes = np.arange(7.0)
+    for mu in (0.3, 4.0, 17.0):
+        wi, wo = O.exp_inout(7, mu)
+        for q in range(7):
+            for j in range(6):
+                ref_in = quad(lambda t: math.exp(-mu * (t - j)) * t ** q, j, 6, epsabs=0,
+                              epsrel=1e-13, limit=200)[0]
+                ref_out = quad(lambda t: math.exp(-mu * (j + 1 - t)) * t ** q, 0, j + 1,
+                               epsabs=0, epsrel=1e-13, limit=200)[0]
+                got_in, got_out = np.dot(wi[:, j], nodes ** q), np.dot(wo[:, j], nodes ** q)
+                # relative to the sum's terms (nodes^6 reaches 6^6)
+                s_in = np.dot(np.abs(wi[:, j]), nodes ** q)
+                s_out = np.dot(np.abs(wo[:, j]), nodes ** q)
+                assert abs(got_in - ref_in) < 1e-13 * s_in, (mu, q, j, got_in, ref_in)
+                assert abs(got_out - ref_out) < 1e-13 * s_out, (mu, q, j, got_out, ref_out)
+
+
+@pytest.mark.parametrize("E", [-2.0, -20.0])
+def test_helmholtz_radial_deep_energies_fitted(E):
+    """kappa L >> 1 on the outer cells of a Z = 8 radial grid (33 cells to
+    20 bohr, the last 12.9 bohr long): the reference's scheme returns errors
+    of order exp(kappa L) (7.6 of max|V| at E = -2, 1e21 at E = -20 for a
+    diffuse Gaussian); the exponentially fitted form stays at the
+    interpolation error of the smooth parts.  l = 0: the Yukawa potential of
+    a normalised Gaussian (closed form); l = 2: the defining integrals with
+    scipy's Bessel functions and quad."""
+    from scipy.integrate import quad
+    k = math.sqrt(-2 * E)
+    st, r = radial(8.0, 33)
+    assert 6 * st.max() * k > 4.0
+    for alpha in (0.3, 3.0):
+        f = (alpha / math.pi) ** 1.5 * np.exp(-alpha * r ** 2)
+        V = O.bubble_helmholtz(0, 0, st, 7, E, f[:, None])[:, 0]
+        ref = np.array([O.yukawa_gauss_potential([1.0], [alpha], [[0.0, 0.0, 0.0]], k, [x], [0.0],
+                                                 [0.0])[0, 0, 0] for x in r])
+        err = np.max(np.abs(V - ref)) / np.max(np.abs(ref))
+        print(f"E {E} alpha {alpha}: l = 0 fitted vs closed form {err:.2e}")
+        assert err < 1e-6
+    # l = 2, f = r^2 exp(-r^2)
+    lmax, alpha = 2, 1.0
+    bf = np.zeros((len(r), 9), order="F")
+    bf[:, 6] = r ** 2 * np.exp(-alpha * r ** 2)
+    V = O.bubble_helmholtz(lmax, 0, st, 7, E, bf)[:, 6]
+    fl = lambda s: s ** 2 * math.exp(-alpha * s * s)
+    for i in range(5, len(r) - 1, 23):
+        x = r[i]
+        a = quad(lambda s: s * s * spherical_in(2, k * s) * fl(s), 0, x, epsrel=1e-12)[0]
+        b = quad(lambda s: s * s * spherical_kn(2, k * s) * fl(s), x, max(x, 12.0) + 8,
+                 epsrel=1e-12, limit=200)[0]
+        ref = 8 * k * (spherical_kn(2, k * x) * a + spherical_in(2, k * x) * b)
+        assert abs(V[i] - ref) < 1e-6 * np.max(np.abs(V)), (E, x, V[i], ref)
