@@ -49,6 +49,22 @@ int ctx_scratch(dage_ctx ctx, size_t bytes, void **p) {
     return DAGE_OK;
 }
 
+int ctx_aux(dage_ctx ctx, size_t bytes, void **p) {
+    if (bytes > ctx->aux_bytes) {
+        if (ctx->aux) {
+            DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->aux);
+        }
+        ctx->aux = nullptr;
+        ctx->aux_bytes = 0;
+        const size_t n = std::max(bytes, (size_t)1 << 20);
+        DAGE_HIP_CHECK(hipMalloc(&ctx->aux, n));
+        ctx->aux_bytes = n;
+    }
+    *p = ctx->aux;
+    return DAGE_OK;
+}
+
 void ctx_retain(dage_ctx ctx) {
     std::lock_guard<std::mutex> lock(g_ref_mutex);
     ++ctx->refs;
@@ -72,6 +88,7 @@ void ctx_release(dage_ctx ctx) {
     if (ctx->timer_start) (void)hipEventDestroy(ctx->timer_start);
     if (ctx->timer_stop) (void)hipEventDestroy(ctx->timer_stop);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->aux) (void)hipFree(ctx->aux);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

@@ -409,6 +409,8 @@ int bubble_alloc_host(dage_bubbles b, int ibub, const double center[3], double c
     DAGE_HIP_CHECK(hipMalloc((void **)&h.d_tab, sizeof(int) * tab.size()));
     DAGE_HIP_CHECK(hipMemcpy(h.d_tab, tab.data(), sizeof(int) * tab.size(),
                              hipMemcpyHostToDevice));
+    h.h_celld = ax.celld;
+    h.h_steps = ax.steps;
     for (int a = 0; a < 3; ++a) h.center[a] = center[a];
     h.charge = charge;
     h.ncell = ncell;
@@ -434,6 +436,8 @@ int bubble_alloc_like(dage_bubbles b, int ibub, dage_bubbles src, int sbub) {
                              hipMemcpyDeviceToDevice));
     DAGE_HIP_CHECK(hipMalloc((void **)&h.d_tab, sizeof(int) * kCellTab));
     DAGE_HIP_CHECK(hipMemcpy(h.d_tab, s.d_tab, sizeof(int) * kCellTab, hipMemcpyDeviceToDevice));
+    h.h_celld = s.h_celld;
+    h.h_steps = s.h_steps;
     for (int a = 0; a < 3; ++a) h.center[a] = s.center[a];
     h.charge = s.charge;
     h.ncell = s.ncell;
@@ -525,8 +529,7 @@ int radial_transform(dage_bubbles in, dage_bubbles out, bool helmholtz, double e
     if (helmholtz) {
         for (int i = 0; i < in->nbub; ++i) {
             const BubbleHost &h = in->host[i];
-            std::vector<double> st(h.ncell);
-            DAGE_HIP_CHECK(hipMemcpy(st.data(), h.steps(), 8 * h.ncell, hipMemcpyDeviceToHost));
+            const std::vector<double> &st = h.h_steps;
             double kl = 0.0;
             for (double x : st) kl = std::max(kl, A.kappa * n * x);
             if (kl <= kFittedKappaL) continue;
@@ -875,21 +878,10 @@ __global__ void __launch_bounds__(256) cube_mul_kernel(double *a, const double *
 
 }  // namespace
 
-}  // namespace dage
-
-extern "C" {
-
-int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out) {
-    DAGE_REQUIRE(a && b && out, DAGE_ERR_ARG, "null argument");
-    DAGE_TRY(require_same_grids(a, b));
-    DAGE_TRY(require_same_grids(a, out));
-    DAGE_REQUIRE(out->lmax == a->lmax + b->lmax && out->k == a->k + b->k, DAGE_ERR_ARG,
-                 "result bubbles need lmax %d and k %d (got %d, %d)", a->lmax + b->lmax,
-                 a->k + b->k, out->lmax, out->k);
-    DAGE_REQUIRE(out != a && out != b, DAGE_ERR_ARG, "the product cannot overwrite a factor");
-    DAGE_TRY(use_ctx(a->ctx));
-    dage_ctx ctx = a->ctx;
-    const YProd &P = yproduct(a->lmax, b->lmax);
+int bubble_product_acc(dage_ctx ctx, int nb, const double *const *f1, int l1max,
+                       const double *const *f2, int l2max, double *const *out, const int *npts) {
+    if (nb == 0) return DAGE_OK;
+    const YProd &P = yproduct(l1max, l2max);
     const size_t pairs = P.first.size(), terms = P.coef.size();
     const size_t bytes = sizeof(int) * (2 * pairs + terms) + sizeof(double) * terms + 64;
     void *scr = nullptr;
@@ -905,25 +897,23 @@ int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out) {
                                   hipMemcpyHostToDevice, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(d_out, P.out_lm.data(), sizeof(int) * terms,
                                   hipMemcpyHostToDevice, s));
-    // result = 0, then the products
-    DAGE_TRY(dage_bubbles_axpby(0.0, out, 0.0, out));
     hipEvent_t ev;
     prof_begin(ctx, "bubble_product_kernel", &ev);
-    for (int b0 = 0; b0 < a->nbub; b0 += kProdChunk) {
+    for (int b0 = 0; b0 < nb; b0 += kProdChunk) {
         ProdArgs A{};
-        A.nb = std::min(kProdChunk, a->nbub - b0);
-        A.n1 = a->nlm();
-        A.n2 = b->nlm();
+        A.nb = std::min(kProdChunk, nb - b0);
+        A.n1 = (l1max + 1) * (l1max + 1);
+        A.n2 = (l2max + 1) * (l2max + 1);
         A.first = d_first;
         A.count = d_count;
         A.out_lm = d_out;
         A.coef = d_coef;
         int maxp = 1;
         for (int i = 0; i < A.nb; ++i) {
-            A.f1[i] = a->f(b0 + i);
-            A.f2[i] = b->f(b0 + i);
-            A.out[i] = out->f(b0 + i);
-            A.npts[i] = a->host[b0 + i].npts;
+            A.f1[i] = f1[b0 + i];
+            A.f2[i] = f2[b0 + i];
+            A.out[i] = out[b0 + i];
+            A.npts[i] = npts[b0 + i];
             maxp = std::max(maxp, A.npts[i]);
         }
         hipLaunchKernelGGL(bubble_product_kernel, dim3((maxp + 255) / 256, A.nb), dim3(256), 0, s,
@@ -931,6 +921,36 @@ int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out) {
         DAGE_HIP_CHECK(hipGetLastError());
     }
     prof_end(ctx, "bubble_product_kernel", ev);
+    return DAGE_OK;
+}
+
+}  // namespace dage
+
+extern "C" {
+
+int dage_bubbles_multiply(dage_bubbles a, dage_bubbles b, dage_bubbles out) {
+    DAGE_REQUIRE(a && b && out, DAGE_ERR_ARG, "null argument");
+    DAGE_TRY(require_same_grids(a, b));
+    DAGE_TRY(require_same_grids(a, out));
+    DAGE_REQUIRE(out->lmax == a->lmax + b->lmax && out->k == a->k + b->k, DAGE_ERR_ARG,
+                 "result bubbles need lmax %d and k %d (got %d, %d)", a->lmax + b->lmax,
+                 a->k + b->k, out->lmax, out->k);
+    DAGE_REQUIRE(out != a && out != b, DAGE_ERR_ARG, "the product cannot overwrite a factor");
+    DAGE_TRY(use_ctx(a->ctx));
+    dage_ctx ctx = a->ctx;
+    // result = 0, then the products
+    DAGE_TRY(dage_bubbles_axpby(0.0, out, 0.0, out));
+    std::vector<const double *> f1(a->nbub), f2(a->nbub);
+    std::vector<double *> fo(a->nbub);
+    std::vector<int> np(a->nbub);
+    for (int i = 0; i < a->nbub; ++i) {
+        f1[i] = a->f(i);
+        f2[i] = b->f(i);
+        fo[i] = out->f(i);
+        np[i] = a->host[i].npts;
+    }
+    DAGE_TRY(bubble_product_acc(ctx, a->nbub, f1.data(), a->lmax, f2.data(), b->lmax, fo.data(),
+                                np.data()));
     return bubbles_fold_rk(out);
 }
 

@@ -98,6 +98,8 @@ struct dage_ctx_s {
     // grow-only device scratch for per-call temporaries (dage::ctx_scratch)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *aux = nullptr;      // dage::ctx_aux
+    size_t aux_bytes = 0;
     hipStream_t scratch_stream = nullptr;
     // this device's copy of the allocation-free apply's fixed scratch array
     // (abi.hip), looked up at context creation
@@ -149,6 +151,7 @@ struct BubbleHost {
     // the raw values are the r^k-folded ones)
     double *d_buf = nullptr;
     int *d_tab = nullptr;     // kCellTab cell-lookup entries
+    std::vector<double> h_celld, h_steps;  // host copies of the radial grid
     bool set = false;
     double *celld() const { return d_buf; }
     double *steps() const { return d_buf + ncell; }
@@ -373,6 +376,14 @@ int bubble_alloc_like(dage_bubbles b, int ibub, dage_bubbles src, int sbub);
 // r^k f -> the inject block after the raw values changed on the device
 // (k != 0 only; asynchronous on the context stream).
 int bubbles_fold_rk(dage_bubbles b);
+// out_i(:, LM) += sum c(lm1, lm2 -> LM) f1_i(:, lm1) f2_i(:, lm2) for nb
+// bubbles on npts_i radial points (raw values, l1max/l2max the factors'
+// lmax; out holds at least (l1max+l2max+1)^2 columns); asynchronous.
+int bubble_product_acc(dage_ctx ctx, int nb, const double *const *f1, int l1max,
+                       const double *const *f2, int l2max, double *const *out, const int *npts);
+// a second grow-only device buffer for temporaries that must outlive the
+// calls that use ctx_scratch within one library call
+int ctx_aux(dage_ctx ctx, size_t bytes, void **p);
 int bubbles_inject(dage_bubbles bub, dage_grid3d grid, int lmin, double multiplier,
                    double *cube);
 // Zero n doubles with a fill kernel on the context stream (graph-safe, unlike

@@ -19,7 +19,9 @@
 // bubbles changes.
 //
 // Product of two cusp-type functions (both carry Taylor bubbles T1, T2):
-//   result bubbles  = (B1 + T1)(B2 + T2) - T1 T2      (centre by centre)
+//   result bubbles  = (B1 + T1)(B2 + T2) - T1 T2      (centre by centre,
+//                     formed as B1 B2 + B1 T2 + T1 B2, the Taylor values in
+//                     the context's auxiliary buffer: no allocation)
 //   result cube     = (c1 + B1)(c2 + B2) - result bubbles (on the grid)
 // so near a nucleus A the cube holds B1_A (f2 - T2_A) + ..., in which the
 // Taylor series cancels the low orders of f2 that a singular B1_A (e.g. the
@@ -272,12 +274,10 @@ int download_bubble(dage_bubbles b, int i, RadialHost &R) {
     R.ncell = h.ncell;
     R.npts = h.npts;
     R.qmax = h.qmax;
-    R.celld.resize(h.ncell);
-    R.steps.resize(h.ncell);
+    R.celld = h.h_celld;
+    R.steps = h.h_steps;
     R.fk.resize((size_t)h.npts * b->nlm());
     hipStream_t s = b->ctx->stream;
-    DAGE_HIP_CHECK(hipMemcpyAsync(R.celld.data(), h.celld(), 8 * h.ncell, hipMemcpyDeviceToHost, s));
-    DAGE_HIP_CHECK(hipMemcpyAsync(R.steps.data(), h.steps(), 8 * h.ncell, hipMemcpyDeviceToHost, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(R.fk.data(), h.fk(), 8 * R.fk.size(), hipMemcpyDeviceToHost, s));
     return DAGE_OK;
 }
@@ -401,27 +401,29 @@ double rpow(double r, int p) {
     return std::pow(r, p);
 }
 
-// Bubbles_make_taylor: bubbles of b's centres, grids and k, lmax max(lmax, t)
-int make_taylor(dage_bubbles b, const std::vector<double> &contam, int t, dage_bubbles *out) {
+// Bubbles_make_taylor: radial values (raw, k of b) of the Taylor bubbles,
+// lmax max(lmax, t), bubble i at vals[off[i]], npts x (lmax_t+1)^2.
+void taylor_values(dage_bubbles b, const std::vector<double> &contam, int t,
+                   std::vector<double> &vals, std::vector<size_t> &off) {
     const int lt = std::max(b->lmax, t), nlt = (lt + 1) * (lt + 1), k = b->k;
     const auto kap = cart_indices(t);
     const int nd = (int)kap.size(), Nt = (t + 1) * (t + 1);
     const std::vector<double> &c2s = cart_to_sph(t);
-    DAGE_TRY(dage_bubbles_create_like(b, lt, k, out));
-    dage_bubbles T = *out;
-    double fact[kMaxTaylorOrder + 1] = {1.0, 1.0, 0.5};
+    const double fact[kMaxTaylorOrder + 1] = {1.0, 1.0, 0.5};
+    off.resize(b->nbub);
+    size_t total = 0;
     for (int i = 0; i < b->nbub; ++i) {
-        RadialHost R;
+        off[i] = total;
+        total += (size_t)b->host[i].npts * nlt;
+    }
+    vals.assign(total, 0.0);
+    const int nl = b->nlip;
+    for (int i = 0; i < b->nbub; ++i) {
         const BubbleHost &h = b->host[i];
-        R.celld.resize(h.ncell);
-        R.steps.resize(h.ncell);
-        DAGE_HIP_CHECK(hipMemcpy(R.celld.data(), h.celld(), 8 * h.ncell, hipMemcpyDeviceToHost));
-        DAGE_HIP_CHECK(hipMemcpy(R.steps.data(), h.steps(), 8 * h.ncell, hipMemcpyDeviceToHost));
-        std::vector<double> f((size_t)h.npts * nlt, 0.0);
-        const int nl = b->nlip;
+        double *f = vals.data() + off[i];
         for (int p = 0; p < h.npts; ++p) {
             const int cell = std::min(p / (nl - 1), h.ncell - 1);
-            const double r = R.celld[cell] + (p - cell * (nl - 1)) * R.steps[cell];
+            const double r = h.h_celld[cell] + (p - cell * (nl - 1)) * h.h_steps[cell];
             const double w = r > 2.220446049250313e-16 ? 0.5 * std::erfc(r - 2.0 / r) : 1.0;
             for (int d = 0; d < nd; ++d) {
                 const int n = kap[d][0] + kap[d][1] + kap[d][2];
@@ -433,8 +435,20 @@ int make_taylor(dage_bubbles b, const std::vector<double> &contam, int t, dage_b
                 }
             }
         }
-        DAGE_HIP_CHECK(hipMemcpy(T->f(i), f.data(), 8 * f.size(), hipMemcpyHostToDevice));
     }
+}
+
+int make_taylor(dage_bubbles b, const std::vector<double> &contam, int t, dage_bubbles *out) {
+    const int lt = std::max(b->lmax, t);
+    std::vector<double> vals;
+    std::vector<size_t> off;
+    taylor_values(b, contam, t, vals, off);
+    DAGE_TRY(dage_bubbles_create_like(b, lt, b->k, out));
+    dage_bubbles T = *out;
+    const int nlt = (lt + 1) * (lt + 1);
+    for (int i = 0; i < b->nbub; ++i)
+        DAGE_HIP_CHECK(hipMemcpy(T->f(i), vals.data() + off[i],
+                                 8 * (size_t)b->host[i].npts * nlt, hipMemcpyHostToDevice));
     T->dirty = true;
     return bubbles_fold_rk(T);
 }
@@ -518,43 +532,62 @@ int dage_function3d_multiply_taylor(dage_grid3d grid, const double *cube1, dage_
                  bub_out->lmax, bub_out->k);
     DAGE_REQUIRE(bub_out != b1 && bub_out != b2, DAGE_ERR_ARG,
                  "the product cannot overwrite a factor");
+    DAGE_REQUIRE(b1->nbub == b2->nbub && b1->nbub == bub_out->nbub, DAGE_ERR_ARG,
+                 "bubble counts differ (%d, %d, %d)", b1->nbub, b2->nbub, bub_out->nbub);
+    for (int i = 0; i < b1->nbub; ++i)
+        DAGE_REQUIRE(b1->host[i].set && b2->host[i].set && bub_out->host[i].set &&
+                         b1->host[i].npts == b2->host[i].npts &&
+                         b1->host[i].npts == bub_out->host[i].npts,
+                     DAGE_ERR_ARG, "bubble %d: radial grids differ or are not set", i);
     DAGE_TRY(use_ctx(grid->ctx));
-    dage_bubbles T1 = nullptr, T2 = nullptr, P1 = nullptr, P2 = nullptr, TT = nullptr;
-    auto cleanup = [&]() {
-        for (dage_bubbles x : {T1, T2, P1, P2, TT})
-            if (x) dage_bubbles_destroy(x);
-    };
-    auto run = [&]() -> int {
-        // the Taylor bubbles of the factors (contaminants from the cube
-        // before the result overwrites it, in case cube_out == cube2)
-        DAGE_TRY(dage_function3d_taylor_bubbles(grid, cube1, b1, t, &T1));
-        const bool same = (cube1 == cube2 && b1 == b2);
-        if (!same) DAGE_TRY(dage_function3d_taylor_bubbles(grid, cube2, b2, t, &T2));
-        dage_bubbles t2 = same ? T1 : T2;
-        // (B1 + T1), (B2 + T2)
-        DAGE_TRY(dage_bubbles_create_like(b1, l1, b1->k, &P1));
-        DAGE_TRY(dage_bubbles_axpby(1.0, b1, 0.0, P1));
-        DAGE_TRY(dage_bubbles_axpby(1.0, T1, 1.0, P1));
-        dage_bubbles p2 = P1;
-        if (!same) {
-            DAGE_TRY(dage_bubbles_create_like(b2, l2, b2->k, &P2));
-            DAGE_TRY(dage_bubbles_axpby(1.0, b2, 0.0, P2));
-            DAGE_TRY(dage_bubbles_axpby(1.0, t2, 1.0, P2));
-            p2 = P2;
-        }
-        DAGE_TRY(dage_bubbles_multiply(P1, p2, bub_out));
-        // minus T1 T2
-        DAGE_TRY(dage_bubbles_create_like(b1, T1->lmax + t2->lmax, T1->k + t2->k, &TT));
-        DAGE_TRY(dage_bubbles_multiply(T1, t2, TT));
-        DAGE_TRY(dage_bubbles_axpby(-1.0, TT, 1.0, bub_out));
-        // cube: (c1 + B1)(c2 + B2) - result bubbles
-        DAGE_TRY(dage_function3d_multiply(grid, cube1, b1, cube2, b2, cube_out, nullptr));
-        DAGE_TRY(dage_bubbles_inject_device(bub_out, grid, 0, -1.0, cube_out));
-        return DAGE_OK;
-    };
-    const int rc = run();
-    cleanup();
-    return rc;
+    dage_ctx ctx = grid->ctx;
+    // the Taylor bubbles of both factors (contaminants read before the
+    // result overwrites anything, cube_out may be cube2), kept in the
+    // context's auxiliary buffer
+    const bool same = (cube1 == cube2 && b1 == b2);
+    std::vector<double> c1, c2, v1, v2;
+    std::vector<size_t> o1, o2;
+    DAGE_TRY(contaminants(grid, cube1, b1, t, c1));
+    taylor_values(b1, c1, t, v1, o1);
+    if (!same) {
+        DAGE_TRY(contaminants(grid, cube2, b2, t, c2));
+        taylor_values(b2, c2, t, v2, o2);
+    }
+    void *ap = nullptr;
+    DAGE_TRY(ctx_aux(ctx, 8 * (v1.size() + v2.size()), &ap));
+    double *d_t1 = (double *)ap, *d_t2 = same ? d_t1 : d_t1 + v1.size();
+    DAGE_HIP_CHECK(hipMemcpyAsync(d_t1, v1.data(), 8 * v1.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    if (!same)
+        DAGE_HIP_CHECK(hipMemcpyAsync(d_t2, v2.data(), 8 * v2.size(), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    const int nb = b1->nbub;
+    std::vector<const double *> f1(nb), f2(nb), t1(nb), t2(nb);
+    std::vector<double *> fo(nb);
+    std::vector<int> np(nb);
+    for (int i = 0; i < nb; ++i) {
+        f1[i] = b1->f(i);
+        f2[i] = b2->f(i);
+        t1[i] = d_t1 + o1[i];
+        t2[i] = d_t2 + (same ? o1[i] : o2[i]);
+        fo[i] = bub_out->f(i);
+        np[i] = b1->host[i].npts;
+    }
+    // result bubbles: B1 B2 + B1 T2 + T1 B2  (= (B1 + T1)(B2 + T2) - T1 T2)
+    DAGE_TRY(dage_bubbles_axpby(0.0, bub_out, 0.0, bub_out));
+    DAGE_TRY(bubble_product_acc(ctx, nb, f1.data(), b1->lmax, f2.data(), b2->lmax, fo.data(),
+                                np.data()));
+    DAGE_TRY(bubble_product_acc(ctx, nb, f1.data(), b1->lmax, t2.data(), l2, fo.data(),
+                                np.data()));
+    DAGE_TRY(bubble_product_acc(ctx, nb, t1.data(), l1, f2.data(), b2->lmax, fo.data(),
+                                np.data()));
+    DAGE_TRY(bubbles_fold_rk(bub_out));
+    // cube: (c1 + B1)(c2 + B2) - result bubbles
+    DAGE_TRY(dage_function3d_multiply(grid, cube1, b1, cube2, b2, cube_out, nullptr));
+    DAGE_TRY(dage_bubbles_inject_device(bub_out, grid, 0, -1.0, cube_out));
+    // the host vectors go out of scope: the uploads must have completed
+    DAGE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return DAGE_OK;
 }
 
 }  // extern "C"
