@@ -550,6 +550,28 @@ def bench_config4(ctx, args):
                  f"converged {r.converged}")
         S.close()
     model.close()
+    # the same molecule with bubble-carrying orbitals: closed-shell Hartree-Fock
+    # (point nuclei -Z/r as bubbles, lmax-4 orbital bubbles on 199-point
+    # radial grids, Taylor-series products, exact exchange), guess plus two
+    # iterations timed
+    hf = scf.BubbleHartreeFock(ctx, grid, test_mol, nocc)
+    c0 = time.perf_counter()
+    e_hf = [hf.initial_guess()]
+    ctx.synchronize()
+    guess_s = time.perf_counter() - c0
+    it_s = []
+    for _ in range(2):
+        c0 = time.perf_counter()
+        e_hf.append(hf.iterate())
+        ctx.synchronize()
+        it_s.append(time.perf_counter() - c0)
+    bubble_hf = {
+        "workload": "12 orbitals = cube + lmax-4 bubbles (199 radial points) on every atom, "
+                    "point nuclei, RHF with exact exchange; guess + 2 iterations",
+        "guess_s": round(guess_s, 2), "s_per_iteration": [round(x, 2) for x in it_s],
+        "energies": e_hf, "eps_last": [round(float(x), 6) for x in hf.eps]}
+    progress(f"config4 bubble RHF: guess {guess_s:.1f} s, iterations {it_s}")
+    hf.close()
     grid.close()
     p_, l_ = runs["plain"], runs["learned"]
     return {
@@ -561,6 +583,7 @@ def bench_config4(ctx, args):
         "time_to_1e-6_s": {"plain": p_["wall_s"] if p_["converged"] else None,
                            "learned": l_["wall_s"] if l_["converged"] else None},
         "wall_ratio_plain_over_learned": round(p_["wall_s"] / l_["wall_s"], 3),
+        "bubble_rhf": bubble_hf,
         "energy_learned_minus_plain": l_["final_energy"] - p_["final_energy"],
         "training": {"molecule_Z": [float(z) for z in train_mol.Z],
                      "plain_iterations": r_train.iterations, "samples": len(samples),

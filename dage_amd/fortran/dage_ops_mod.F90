@@ -30,6 +30,8 @@ module dage_ops_m
     public :: dage_gbfmm_create, dage_gbfmm_apply, dage_gbfmm_box_count
     public :: dage_gbfmm_get_multipoles, dage_gbfmm_destroy, dage_gbfmm_set_energy
     public :: dage_bubbles_inject
+    public :: dage_function3d_contaminants, dage_function3d_taylor_bubbles
+    public :: dage_function3d_multiply_taylor
     public :: dage_helmholtz_batch_create, dage_batch_upload, dage_batch_download
     public :: dage_batch_set_energies, dage_batch_apply, dage_batch_tune, dage_batch_destroy
     public :: DAGE_PREC_BF16X6, DAGE_PREC_BF16X3
@@ -160,6 +162,31 @@ module dage_ops_m
             integer(c_int), value :: lmin
             real(c_double), value :: multiplier
             real(c_double)        :: cube(*)
+        end function
+
+        ! Taylor-series terms of cusp-type products (function3d.F90:1310-1470);
+        ! cubes are device pointers (dage_cube_device_pointer)
+        integer(c_int) function dage_function3d_contaminants(grid, cube, bubbles, order, out) &
+                bind(C, name="dage_function3d_contaminants")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: grid, cube, bubbles
+            integer(c_int), value :: order
+            real(c_double)        :: out(*)
+        end function
+
+        integer(c_int) function dage_function3d_taylor_bubbles(grid, cube, bubbles, order, out) &
+                bind(C, name="dage_function3d_taylor_bubbles")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: grid, cube, bubbles
+            integer(c_int), value :: order
+            type(c_ptr)           :: out
+        end function
+
+        integer(c_int) function dage_function3d_multiply_taylor(grid, cube1, b1, cube2, b2, &
+                order, cube_out, bub_out) bind(C, name="dage_function3d_multiply_taylor")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: grid, cube1, b1, cube2, b2, cube_out, bub_out
+            integer(c_int), value :: order
         end function
 
         integer(c_int) function dage_bubbles_destroy(bubbles) bind(C, name="dage_bubbles_destroy")
