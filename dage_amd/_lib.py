@@ -93,6 +93,8 @@ _SIGNATURES = {
     "dage_bubbles_integrate": [_c_p, _dp],
     "dage_function3d_multiply": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "dage_function3d_contaminants": [_c_p, _c_p, _c_p, _int, _dp],
+    "dage_operator_apply_device_strided": [_c_p, _c_p, _int, _int, _c_p, _int, _int, _int],
+    "dage_bubbles_inject_device_strided": [_c_p, _c_p, _int, _dbl, _c_p, _int, _int],
     "dage_function3d_taylor_bubbles": [_c_p, _c_p, _c_p, _int, ctypes.POINTER(_c_p)],
     "dage_function3d_multiply_taylor": [_c_p, _c_p, _c_p, _c_p, _c_p, _int, _c_p, _c_p],
     "dage_fun_write": [ctypes.c_char_p, ctypes.c_char_p, _int, _ip, _dp, _int, _int, _int, _dp,

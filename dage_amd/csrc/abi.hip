@@ -678,6 +678,79 @@ int dage_operator_apply_device(dage_operator op, const double *in, double *out, 
     return apply_f64(op, in, out);
 }
 
+namespace {
+// element (i, j, k) of an nx x ny x nz box: src[i + sx (j + sy k)] ->
+// dst[i + dx (j + dy k)] (column-major sub-cubes with leading dimensions)
+__global__ void __launch_bounds__(256) copy_strided_kernel(const double *__restrict__ src, long sx,
+                                                           long sy, double *__restrict__ dst,
+                                                           long dx, long dy, int nx, int ny,
+                                                           long total) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long i = e % nx, j = (e / nx) % ny, k = e / ((long)nx * ny);
+        dst[i + dx * (j + dy * k)] = src[i + sx * (j + sy * k)];
+    }
+}
+
+int copy_strided(dage_ctx ctx, const double *src, long sx, long sy, double *dst, long dx, long dy,
+                 const int *dims) {
+    const long total = (long)dims[0] * dims[1] * dims[2];
+    if (total == 0) return DAGE_OK;
+    const int blocks = (int)std::min<long>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(copy_strided_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src, sx, sy,
+                       dst, dx, dy, dims[0], dims[1], total);
+    DAGE_HIP_CHECK(hipGetLastError());
+    return DAGE_OK;
+}
+}  // namespace
+
+int dage_operator_apply_device_strided(dage_operator op, const double *in, int ldx_in, int ldy_in,
+                                       double *out, int ldx_out, int ldy_out, int precision) {
+    DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
+    DAGE_REQUIRE(ldx_in >= op->din[0] && ldy_in >= op->din[1], DAGE_ERR_ARG,
+                 "input leading dimensions (%d, %d) below the input grid's (%d, %d)", ldx_in,
+                 ldy_in, op->din[0], op->din[1]);
+    DAGE_REQUIRE(ldx_out >= op->dout[0] && ldy_out >= op->dout[1], DAGE_ERR_ARG,
+                 "output leading dimensions (%d, %d) below the output grid's (%d, %d)", ldx_out,
+                 ldy_out, op->dout[0], op->dout[1]);
+    const bool pin = ldx_in != op->din[0] || ldy_in != op->din[1];
+    const bool pout = ldx_out != op->dout[0] || ldy_out != op->dout[1];
+    if (!pin && !pout) return dage_operator_apply_device(op, in, out, precision);
+    DAGE_TRY(use_device(op->ctx));
+    dage_ctx ctx = op->ctx;
+    const size_t nin = (size_t)op->din[0] * op->din[1] * op->din[2];
+    const size_t nout = (size_t)op->dout[0] * op->dout[1] * op->dout[2];
+    void *ap = nullptr;
+    DAGE_TRY(ctx_aux(ctx, 8 * ((pin ? nin : 0) + (pout ? nout : 0)), &ap));
+    double *d_in = pin ? (double *)ap : const_cast<double *>(in);
+    double *d_out = pout ? (double *)ap + (pin ? nin : 0) : out;
+    if (pin) DAGE_TRY(copy_strided(ctx, in, ldx_in, ldy_in, d_in, op->din[0], op->din[1], op->din));
+    DAGE_TRY(dage_operator_apply_device(op, d_in, d_out, precision));
+    if (pout)
+        DAGE_TRY(copy_strided(ctx, d_out, op->dout[0], op->dout[1], out, ldx_out, ldy_out,
+                              op->dout));
+    return DAGE_OK;
+}
+
+int dage_bubbles_inject_device_strided(dage_bubbles bubbles, dage_grid3d grid, int lmin,
+                                       double multiplier, double *cube, int ldx, int ldy) {
+    DAGE_REQUIRE(bubbles && grid && cube, DAGE_ERR_ARG, "null argument");
+    const int dims[3] = {grid->shape(0), grid->shape(1), grid->shape(2)};
+    DAGE_REQUIRE(ldx >= dims[0] && ldy >= dims[1], DAGE_ERR_ARG,
+                 "leading dimensions (%d, %d) below the grid's (%d, %d)", ldx, ldy, dims[0],
+                 dims[1]);
+    if (ldx == dims[0] && ldy == dims[1])
+        return dage_bubbles_inject_device(bubbles, grid, lmin, multiplier, cube);
+    DAGE_TRY(use_device(grid->ctx));
+    dage_ctx ctx = grid->ctx;
+    void *ap = nullptr;
+    DAGE_TRY(ctx_aux(ctx, 8 * (size_t)dims[0] * dims[1] * dims[2], &ap));
+    double *box = (double *)ap;
+    DAGE_TRY(copy_strided(ctx, cube, ldx, ldy, box, dims[0], dims[1], dims));
+    DAGE_TRY(dage_bubbles_inject_device(bubbles, grid, lmin, multiplier, box));
+    return copy_strided(ctx, box, dims[0], dims[1], cube, ldx, ldy, dims);
+}
+
 int dage_operator_apply_device_noalloc(dage_operator op, const double *in, double *out) {
     DAGE_REQUIRE(op && in && out, DAGE_ERR_ARG, "null argument");
     DAGE_REQUIRE((const void *)in != (const void *)out, DAGE_ERR_ARG,

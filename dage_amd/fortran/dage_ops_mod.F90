@@ -32,6 +32,7 @@ module dage_ops_m
     public :: dage_bubbles_inject
     public :: dage_function3d_contaminants, dage_function3d_taylor_bubbles
     public :: dage_function3d_multiply_taylor
+    public :: dage_operator_apply_device_strided, dage_bubbles_inject_device_strided
     public :: dage_helmholtz_batch_create, dage_batch_upload, dage_batch_download
     public :: dage_batch_set_energies, dage_batch_apply, dage_batch_tune, dage_batch_destroy
     public :: DAGE_PREC_BF16X6, DAGE_PREC_BF16X3
@@ -162,6 +163,22 @@ module dage_ops_m
             integer(c_int), value :: lmin
             real(c_double), value :: multiplier
             real(c_double)        :: cube(*)
+        end function
+
+        ! sub-cubes: device boxes with leading dimensions (cube.cu:1042-1177)
+        integer(c_int) function dage_operator_apply_device_strided(op, cin, ldx_in, ldy_in, &
+                cout, ldx_out, ldy_out, precision) bind(C, name="dage_operator_apply_device_strided")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: op, cin, cout
+            integer(c_int), value :: ldx_in, ldy_in, ldx_out, ldy_out, precision
+        end function
+
+        integer(c_int) function dage_bubbles_inject_device_strided(bubbles, grid, lmin, &
+                multiplier, cube, ldx, ldy) bind(C, name="dage_bubbles_inject_device_strided")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: bubbles, grid, cube
+            integer(c_int), value :: lmin, ldx, ldy
+            real(c_double), value :: multiplier
         end function
 
         ! Taylor-series terms of cusp-type products (function3d.F90:1310-1470);

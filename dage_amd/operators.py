@@ -133,6 +133,34 @@ class DeviceCube:
     def copy_from(self, src: "DeviceCube"):
         call("dage_cube_copy", self.handle, src.handle)
 
+    def view(self, offset, shape) -> "CubeView":
+        """The sub-cube of `shape` starting at element `offset` (0-based),
+        with this cube's leading dimensions (CudaCube_get_subcube,
+        src/bubbles/cube.cu:1042-1177)."""
+        return CubeView(self, offset, shape)
+
+
+class CubeView:
+    """A box inside a DeviceCube: device pointer of its first element and the
+    parent's leading dimensions (ldx, ldy)."""
+
+    def __init__(self, parent: DeviceCube, offset, shape):
+        off = tuple(int(o) for o in offset)
+        shp = tuple(int(s) for s in shape)
+        if len(off) != 3 or len(shp) != 3:
+            raise ValueError("offset and shape need three entries")
+        for a in range(3):
+            if off[a] < 0 or shp[a] < 1 or off[a] + shp[a] > parent.shape[a]:
+                raise ValueError(f"box {off}+{shp} outside the cube {parent.shape}")
+        self.parent, self.offset, self.shape = parent, off, shp
+        self.ld = (parent.shape[0], parent.shape[1])
+
+    @property
+    def ptr(self) -> int:
+        lx, ly = self.ld
+        o = self.offset
+        return self.parent.ptr + 8 * (o[0] + lx * (o[1] + ly * o[2]))
+
 
 class GaussQuad:
     """t-quadrature: nlin Gauss-Legendre points on [tstart, tlin] plus nlog on a
@@ -310,6 +338,19 @@ class Operator3D:
         call("dage_operator_apply_device", self.handle, ctypes.c_void_p(cubein.ptr),
              ctypes.c_void_p(cubeout.ptr), precision)
 
+    def transform_cuda_cube_strided(self, cubein, cubeout, precision: int = DAGE_PREC_FP64):
+        """transform_cuda_cube on sub-cubes (CubeView or DeviceCube) with
+        their own leading dimensions."""
+        nin, nout = self.get_dims()
+        if cubein.shape != tuple(nin) or cubeout.shape != tuple(nout):
+            raise ValueError(f"device boxes {cubein.shape}->{cubeout.shape} do not match "
+                             f"the operator {tuple(nin)}->{tuple(nout)}")
+        lin = getattr(cubein, "ld", cubein.shape[:2])
+        lout = getattr(cubeout, "ld", cubeout.shape[:2])
+        call("dage_operator_apply_device_strided", self.handle, ctypes.c_void_p(cubein.ptr),
+             int(lin[0]), int(lin[1]), ctypes.c_void_p(cubeout.ptr), int(lout[0]),
+             int(lout[1]), precision)
+
     def transform_cuda_cube_noalloc(self, cubein: DeviceCube, cubeout: DeviceCube):
         """FP64 transform_cuda_cube that allocates nothing: the intermediates
         live in the library's fixed per-GPU scratch (noalloc_limit() bytes,
@@ -415,6 +456,14 @@ class Bubbles:
             raise ValueError(f"cube shape {cube.shape} != grid shape {grid.get_shape()}")
         call("dage_bubbles_inject_device", self.handle, grid.handle, lmin, float(multiplier),
              ctypes.c_void_p(cube.ptr))
+
+    def inject_strided(self, grid: Grid3D, box, lmin: int = 0, multiplier: float = 1.0):
+        """inject into a sub-cube (CubeView) whose shape is the grid's."""
+        if box.shape != grid.get_shape():
+            raise ValueError(f"box shape {box.shape} != grid shape {grid.get_shape()}")
+        ld = getattr(box, "ld", box.shape[:2])
+        call("dage_bubbles_inject_device_strided", self.handle, grid.handle, lmin,
+             float(multiplier), ctypes.c_void_p(box.ptr), int(ld[0]), int(ld[1]))
 
     def eval_3dgrid(self, grid: Grid3D, lmin: int = 0, cube: np.ndarray | None = None,
                     multiplier: float = 1.0) -> np.ndarray:

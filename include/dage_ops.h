@@ -181,6 +181,18 @@ int dage_operator_apply(dage_operator op, const double *cubein,
  * buffers; asynchronous on the context stream. */
 int dage_operator_apply_device(dage_operator op, const double *cubein_dev,
                                double *cubeout_dev, int precision);
+/* The device apply on sub-cubes: the input box (the input grid's shape) at
+ * in with element (i, j, k) at in[i + ldx_in (j + ldy_in k)], the output box
+ * likewise (ld >= the grid's own extent; equal: dage_operator_apply_device).
+ * Replaces the pitched CudaCube slices and sub-cubes the reference applies
+ * to (cudacube_get_subcube_cuda / _get_slice_cuda, src/bubbles/cube.cu:
+ * 1042-1177, 1182-1269).  A strided box is packed into the context's
+ * auxiliary buffer (grown on demand) before the apply and unpacked after it,
+ * so the result equals the contiguous apply's bit for bit; the boxes may
+ * overlap when either is strided.  Asynchronous on the context stream. */
+int dage_operator_apply_device_strided(dage_operator op, const double *in, int ldx_in,
+                                       int ldy_in, double *out, int ldx_out, int ldy_out,
+                                       int precision);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the FP64 apply of dage_operator_apply_device, but with
@@ -226,6 +238,13 @@ int dage_bubbles_set(dage_bubbles bubbles, int ibub, const double center[3],
  * grid (asynchronous on the context stream). */
 int dage_bubbles_inject_device(dage_bubbles bubbles, dage_grid3d grid, int lmin,
                                double multiplier, double *cube_dev);
+/* dage_bubbles_inject_device into a sub-cube: the grid's points are the box
+ * at cube with element (i, j, k) at cube[i + ldx (j + ldy k)] (bubbles_inject
+ * on a CudaCube sub-cube, src/bubbles/bubbles_cuda.cu:3780-3800, cube.cu:
+ * 1042-1177).  Packed through the context's auxiliary buffer; equal to the
+ * contiguous inject bit for bit.  Asynchronous. */
+int dage_bubbles_inject_device_strided(dage_bubbles bubbles, dage_grid3d grid, int lmin,
+                                       double multiplier, double *cube, int ldx, int ldy);
 /* Bubbles_eval_3dgrid (src/bubbles/bubbles.F90:2148-2211) on a host cube:
  * cube += multiplier * bubbles (l >= lmin) on the points of grid.  The cube
  * is uploaded, injected on the GPU and downloaded; returns after the
