@@ -324,7 +324,7 @@ void solid_harmonics_host(int lmax, double x, double y, double z, double *S);
 
 // Yukawa (exp(-kappa r)/r) far-field math (yukawa_host.cpp).  Matrices are
 // column-major N x N, N = (lmax+1)^2, lm = l*l + l + m.
-constexpr int kMaxFmmBasisL = 16;
+constexpr int kMaxFmmBasisL = 24;
 void bessel_i_scaled(int lmax, double x, double *g);  // i_l(x)/x^l, 0 <= x <= 600
 void bessel_k(int lmax, double x, double *k);         // k_l(x), k_0 = (pi/2) e^-x / x
 void yukawa_regular(int lmax, double kappa, const double v[3], double *R);

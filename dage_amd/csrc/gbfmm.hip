@@ -103,7 +103,9 @@ struct dage_gbfmm_s {
 namespace dage {
 namespace {
 
-constexpr int kMaxFmmLmax = 15;  // N <= 256: one thread per lm row
+constexpr int kMaxFmmLmax = 22;  // the reference's GBFMM-Helmholtz default
+constexpr int kMaxFmmN = (kMaxFmmLmax + 1) * (kMaxFmmLmax + 1);
+constexpr int kMaxBoxKernelLmax = 15;  // box_multipoles_kernel<LMAX> instances
 
 int box_index(int vx, int vy, int vz, int level) {
     int idx = 0;
@@ -393,16 +395,17 @@ __global__ void __launch_bounds__(256) sum_parts_kernel(const double *part, int 
 // lower triangle is read), full for the Yukawa kernel.
 __global__ void __launch_bounds__(256) m2m_kernel(const double *W, const double *qchild,
                                                   double *qparent, int N, int full) {
-    const int P = blockIdx.x, i = threadIdx.x;
-    if (i >= N) return;
-    double acc = 0.0;
-    const int send = full ? N : i + 1;
-    for (int o = 0; o < 8; ++o) {
-        const double *Wo = W + (long)o * N * N;
-        const double *qc = qchild + (long)(8 * P + o) * N;
-        for (int s = 0; s < send; ++s) acc += Wo[i + (long)N * s] * qc[s];
+    const int P = blockIdx.x;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {  // one lm row per thread pass
+        double acc = 0.0;
+        const int send = full ? N : i + 1;
+        for (int o = 0; o < 8; ++o) {
+            const double *Wo = W + (long)o * N * N;
+            const double *qc = qchild + (long)(8 * P + o) * N;
+            for (int s = 0; s < send; ++s) acc += Wo[i + (long)N * s] * qc[s];
+        }
+        qparent[(long)P * N + i] = acc;
     }
-    qparent[(long)P * N + i] = acc;
 }
 
 // L(target) += sum_list T(offset) q(source)  (calculate_local_farfield_potential,
@@ -423,23 +426,25 @@ __global__ void __launch_bounds__(256) m2l_gather_kernel(const double *q, const 
 // L[t] += the target's GEMM columns, in interaction-list order (deterministic)
 __global__ void __launch_bounds__(256) m2l_scatter_kernel(const double *out, const int *start,
                                                           const int *pos, int N, double *L) {
-    const int t = blockIdx.x, i = threadIdx.x;
-    if (i >= N) return;
-    double acc = 0.0;
-    for (int e = start[t]; e < start[t + 1]; ++e) acc += out[(long)pos[e] * N + i];
-    L[(long)t * N + i] += acc;
+    const int t = blockIdx.x;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double acc = 0.0;
+        for (int e = start[t]; e < start[t + 1]; ++e) acc += out[(long)pos[e] * N + i];
+        L[(long)t * N + i] += acc;
+    }
 }
 
 // L(child) += W_octant^T L(parent)  (translate_potential, gbfmm_coulomb3d.F90:1562-1622).
 __global__ void __launch_bounds__(256) l2l_kernel(const double *W, const double *Lparent,
                                                   double *Lchild, int N, int full) {
-    const int c = blockIdx.x, i = threadIdx.x;
-    if (i >= N) return;
+    const int c = blockIdx.x;
     const double *Wo = W + (long)(c & 7) * N * N;
     const double *Lp = Lparent + (long)(c >> 3) * N;
-    double acc = 0.0;
-    for (int s = full ? 0 : i; s < N; ++s) acc += Wo[s + (long)N * i] * Lp[s];
-    Lchild[(long)c * N + i] += acc;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double acc = 0.0;
+        for (int s = full ? 0 : i; s < N; ++s) acc += Wo[s + (long)N * i] * Lp[s];
+        Lchild[(long)c * N + i] += acc;
+    }
 }
 
 struct LeArgs {
@@ -459,8 +464,8 @@ struct LeArgs {
 // (GBFMMCoulomb3D_evaluate_le_grid, gbfmm_coulomb3d.cu:233-280).  Block
 // (box, z-plane of the box).
 __global__ void __launch_bounds__(256) local_expansion_kernel(LeArgs A) {
-    __shared__ double coef[3 * 256 + 32];
-    __shared__ double Ls[256];
+    __shared__ double coef[2 * kMaxFmmN + kMaxFmmLmax + 1];
+    __shared__ double Ls[kMaxFmmN];
     const int tid = threadIdx.x;
     const int box = blockIdx.x;
     const int nb = A.nb;
@@ -971,6 +976,9 @@ int build_gbfmm(dage_gbfmm f, dage_grid3d grid, const double *tp, const double *
             f->pb[a] = n0;
         }
         f->uniform = uni;
+        DAGE_REQUIRE(uni || f->lmax <= kMaxBoxKernelLmax, DAGE_ERR_UNSUPPORTED,
+                     "boxes of unequal shape need lmax <= %d (got %d): the per-box multipole "
+                     "kernel is instantiated up to that order", kMaxBoxKernelLmax, f->lmax);
         if (uni) {
             f->P = (long)f->pb[0] * f->pb[1] * f->pb[2];
             f->box_rel = rel;

@@ -397,7 +397,9 @@ int dage_cube_integrate_device(dage_grid3d grid, const double *f_dev,
  * / _evaluate_potential_le_cuda / _destroy_cuda these replace).
  * dage_gbfmm_create (GBFMMCoulomb3D_init + init_nearfield +
  * init_interaction_matrices): the grid's cells split into 2^maxlevel boxes
- * per axis (every axis's cell count must divide), multipoles up to lmax,
+ * per axis (every axis's cell count must divide), multipoles up to lmax
+ * (0..22, 22 being the reference's GBFMM-Helmholtz far-field order; boxes of
+ * unequal shape, which take the per-box multipole kernel, up to 15),
  * near-field operators built on the sub-grids with the t-quadrature
  * (tp, tw, ntp) and Coulomb weights 2/sqrt(pi)*tw, no coda. */
 int dage_gbfmm_create(dage_ctx ctx, dage_grid3d grid, int maxlevel, int lmax,

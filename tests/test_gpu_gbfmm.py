@@ -157,7 +157,7 @@ def test_gbfmm_rejects_bad_arguments(ctx):
     with pytest.raises(dage_amd.DageError):
         dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=1, lmax=4)
     with pytest.raises(dage_amd.DageError):
-        dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=16)
+        dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=23)
     # rho and pot may not overlap: refused before any kernel is launched
     fmm = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=2, lmax=4)
     d = dage_amd.DeviceCube(ctx, (ax.n, ax.n, ax.n))

@@ -71,3 +71,33 @@ def test_gbfmm_helmholtz_linear_and_back_to_coulomb(ctx):
     assert np.array_equal(f.apply(rho), p1)
     f.close()
     g.close()
+
+
+def test_gbfmm_lmax22_far_field(ctx):
+    """lmax 22, the reference's GBFMM-Helmholtz far-field order
+    (gbfmm_helmholtz3d.F90:66-71): with 529 coefficients per box the
+    expansions' truncation falls below the 1e-6 of lmax 12, for the screened
+    kernel (vs the oracle's direct far-field sum) and for 1/r (the oracle at
+    E = 0); boxes of unequal shape stay limited to lmax 15."""
+    ax, g, rho = setup(ctx, 3)
+    gq = dage_amd.GaussQuad(25, 25)
+    for energy in (-0.6, 0.0):
+        ref = O.gbfmm_helmholtz_potential([ax] * 3, rho, 3, energy, gq.tp, gq.tw)
+        errs = {}
+        for lmax in (12, 22):
+            f = dage_amd.GBFMMCoulomb3D(ctx, g, maxlevel=3, lmax=lmax, gauss=gq)
+            if energy != 0.0:
+                f.set_energy(energy)
+            errs[lmax] = rel_maxdiff(f.apply(rho), ref)
+            f.close()
+        print(f"E {energy}: GBFMM vs oracle's direct far field, lmax 12 {errs[12]:.2e}, "
+              f"lmax 22 {errs[22]:.2e}")
+        assert errs[22] < 0.1 * errs[12]
+        assert errs[22] < 2e-7
+    g.close()
+    # cells of two lengths: boxes of unequal shape at maxlevel 2
+    steps = np.array([0.3, 0.3, 0.4, 0.4, 0.3, 0.3, 0.4, 0.4])
+    g2 = dage_amd.Grid3D(ctx, [-5.6] * 3, [steps] * 3, 5)
+    with pytest.raises(dage_amd.DageError):
+        dage_amd.GBFMMCoulomb3D(ctx, g2, maxlevel=2, lmax=16, gauss=gq)
+    g2.close()

@@ -148,3 +148,20 @@ def test_yukawa_matrices_reject_bad_geometry():
                   np.array([1.0, 0.0, 0.0]).ctypes.data_as(_dp), 0.6, T.ctypes.data_as(_dp))
     with pytest.raises(_lib.DageError):
         lib_bessel(4, 0.0)
+
+
+def test_fmm_matrices_at_lmax22():
+    """lmax 22, the reference's GBFMM-Helmholtz far-field order: the 1/r
+    interaction matrix equals the oracle's, and the Yukawa M2L matrix
+    reproduces the direct sum to rounding of the unscreened scale."""
+    L, N = 22, 23 * 23
+    dR = np.array([3.0, -2.0, 1.0])
+    T = np.zeros(N * N)
+    _lib.call("dage_fmm_interaction_matrix", L, dR.ctypes.data_as(_dp), T.ctypes.data_as(_dp))
+    To = O.fmm_interaction(L, dR)
+    assert np.max(np.abs(T.reshape((N, N), order="F") - To)) < 1e-14 * np.max(np.abs(To))
+    box = 3.0
+    e = m2l_error(L, 1.2, np.array([2 * box, box, -box]), box, np.random.default_rng(7))
+    print(f"Yukawa M2L at lmax 22 (kappa 1.2): {e[0]:.1e} of the far field, {e[1]:.1e} of "
+          f"the unscreened scale")
+    assert e[1] < 1e-13
