@@ -188,31 +188,40 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     auto b_dk = [](int r) { return B_KCONTIG ? 0 : (r / NB) * 16; };
     auto b_dn = [](int r) { return B_KCONTIG ? r * (kThreads / KB) : (r % NB) * 16; };
 
+    // The loaded values stay raw in registers and their validity in a bit
+    // mask: the zero for an out-of-range slot is selected when the panel is
+    // stored, so no instruction consumes a prefetched value before the
+    // current panel's MFMAs (a select right after each load made the wave
+    // wait for the next panel's loads before computing on this one).
     double ra[A_PER], rb[B_PER];
+    unsigned amask = 0, bmask = 0;
     auto load_panel = [&](int kp) {
         const int lim = K - kp;  // valid k within this panel: [0, lim)
         const long koff = (long)kp * g.sAk;
+        amask = 0;
+        bmask = 0;
 #pragma unroll
         for (int r = 0; r < A_PER; ++r) {
             const bool ok = A_MCONTIG ? (a_row_ok && a_k + r * A_DK < lim)
                                       : (a_k < lim && m0 + a_m + r * A_DM < g.M);
-            const double v = A[ok ? a_base + r * a_step + koff : 0];
-            ra[r] = ok ? v : 0.0;
+            ra[r] = A[ok ? a_base + r * a_step + koff : 0];
+            amask |= (ok ? 1u : 0u) << r;
         }
         const long kboff = (long)kp * g.sBk;
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
             const bool ok = bk + b_dk(r) < lim && n0 + bn + b_dn(r) < nend;
-            const double v =
-                B[ok ? b_base + (long)b_dk(r) * g.sBk + (long)b_dn(r) * g.sBn + kboff : 0];
-            rb[r] = ok ? v : 0.0;
+            rb[r] = B[ok ? b_base + (long)b_dk(r) * g.sBk + (long)b_dn(r) * g.sBn + kboff : 0];
+            bmask |= (ok ? 1u : 0u) << r;
         }
     };
     auto store_panel = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < A_PER; ++r) As[buf][a_sm0 + r * a_sstep] = ra[r];
+        for (int r = 0; r < A_PER; ++r)
+            As[buf][a_sm0 + r * a_sstep] = ((amask >> r) & 1u) ? ra[r] : 0.0;
 #pragma unroll
-        for (int r = 0; r < B_PER; ++r) Bs[buf][b_sm0 + b_dk(r) * LDB + b_dn(r)] = rb[r];
+        for (int r = 0; r < B_PER; ++r)
+            Bs[buf][b_sm0 + b_dk(r) * LDB + b_dn(r)] = ((bmask >> r) & 1u) ? rb[r] : 0.0;
     };
 
     d4 acc[MW][NB];
@@ -285,6 +294,9 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         const double *as = As[buf];
         const double *bs = Bs[buf];
         // uniform bound: a panel at the end of K skips its all-padding k-steps
+        // (running them as one branch-free block pipelines the LDS reads but
+        // holds every k-step's operands live: 3 instead of 4 waves per SIMD,
+        // 10 % slower at 385^3)
         const int ksteps = min(KB / 4, (K - panel_k(p) + 3) / 4);
 #pragma unroll
         for (int ks = 0; ks < KB / 4; ++ks)
