@@ -727,8 +727,9 @@ def bench_config5(ctx, args):
     # dominant kernel of the step; a mode pass does 2 T n^4 dense FLOPs per
     # right-hand side (sweep: T = 40 for each of 64 orbitals; GBFMM near
     # field: T = 50 once), the fused x/y pass twice that, counted dense
-    # although the all-zero K panels (products with zero matrix entries
-    # outside the bands) of the banded matrices are skipped
+    # although the all-zero K panels of the banded matrices (exact zeros and
+    # the tails stored as zeros below 1e-20 of their row's largest entry) are
+    # skipped: `frac` can exceed 1, `frac_executed` counts the executed FLOPs
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
     mfma_kernels = ("batch_f64_", "gemm_f64_", "batch_xy_fused", "xy_fused")
@@ -743,7 +744,8 @@ def bench_config5(ctx, args):
                     "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
                     "traffic": measured_traffic("config5", dom),
                     "flops": "dense-equivalent (2 T n^4 per pass and right-hand side; the "
-                             "fused x/y kernel does two passes)",
+                             "fused x/y kernel does two passes); the skipped all-zero panels "
+                             "count, so frac can exceed 1: frac_executed is the MFMA use",
                     "avg_launch_ms": dk["ms_per_launch"],
                     "launches_per_step": dk["launches_per_step"],
                     "algorithmic_flops_per_launch": per_launch}

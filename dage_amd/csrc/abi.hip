@@ -580,6 +580,7 @@ int dage_coulomb3d_create(dage_ctx ctx, dage_grid3d gin, dage_grid3d gout, const
     if (rc == DAGE_OK) rc = alloc_dev(&op->d_w, ntp);
     if (rc == DAGE_OK) rc = upload_weights(op);
     if (rc == DAGE_OK) rc = generate_matrices(op);
+    if (rc == DAGE_OK) rc = flush_matrices(op);
     if (rc == DAGE_OK) rc = relayout_fz(op);
     if (rc == DAGE_OK) rc = build_masks(op);
     if (rc == DAGE_OK) {
@@ -643,6 +644,7 @@ int dage_operator_set_matrices(dage_operator op, const double *fx, const double 
                                   hipMemcpyHostToDevice, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_fz, fz, 8 * (size_t)op->din[2] * op->dout[2] * T,
                                   hipMemcpyHostToDevice, s));
+    DAGE_TRY(flush_matrices(op));
     DAGE_TRY(relayout_fz(op));
     DAGE_TRY(build_masks(op));
     op->planes_valid = false;

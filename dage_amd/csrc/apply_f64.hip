@@ -553,6 +553,22 @@ __global__ void __launch_bounds__(256) relayout_fz_kernel(const double *fz, doub
     }
 }
 
+// Entries at most `cutoff` times their row's largest |entry| are set to
+// zero; one thread per row, element (row r, column n) at b[r*rs + n*cs].
+__global__ void __launch_bounds__(256) flush_small_kernel(double *b, long rows, int n, long rs,
+                                                          long cs, double cutoff) {
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows;
+         r += (long)gridDim.x * blockDim.x) {
+        double big = 0.0;
+        for (int c = 0; c < n; ++c) big = fmax(big, fabs(b[r * rs + (long)c * cs]));
+        const double thr = cutoff * big;
+        for (int c = 0; c < n; ++c) {
+            double &v = b[r * rs + (long)c * cs];
+            if (v != 0.0 && fabs(v) <= thr) v = 0.0;
+        }
+    }
+}
+
 // Row masks: one thread per B row of a pass; bit c when the row has a nonzero
 // entry among output columns [16c, 16c + 16).  Element (row r, column n) of a
 // pass's B matrix is at b[r*rs + n*cs] (r local to one t for x and y).
@@ -802,6 +818,52 @@ __global__ void __launch_bounds__(256) splitk_reduce_batch_kernel(const double *
 }
 
 }  // namespace
+
+namespace {
+
+// Entries of the 1D matrices at most kFlushCutoff times the largest |entry|
+// of their row (one input point, one t) are stored as zeros, so that the
+// passes skip the K panels they fill.  The operator matrices' far entries are
+// Gaussian tails exp(-t^2 d^2) that reach exact zero only at underflow;
+// between 1e-300 and 1e-20 of the row maximum they fill about a third of the
+// 16x16 blocks of a 385-point axis at T = 40.  Dropping them changes an
+// output of a pass by at most n * kFlushCutoff * max|F_t| * max|g|, about
+// 1e-4 of the rounding-error bound of the dense FP64 sum itself
+// (n * 2^-53 * max|F_t| * max|g|, n <= 1024).  Every tiling then skips the
+// same entries, so results do not depend on it.  DAGE_FLUSH_CUTOFF
+// overrides the cutoff (0: keep every nonzero entry).
+constexpr double kFlushCutoff = 1e-20;
+
+double flush_cutoff() {
+    if (const char *e = std::getenv("DAGE_FLUSH_CUTOFF")) return std::max(0.0, atof(e));
+    return kFlushCutoff;
+}
+
+}  // namespace
+
+int flush_matrices(dage_operator op) {
+    const double cut = flush_cutoff();
+    if (cut == 0.0) return DAGE_OK;
+    const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
+    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
+    const long T = op->ntp;
+    hipStream_t s = op->ctx->stream;
+    auto grid = [](long r) { return (int)std::min<long>((r + 255) / 256, 4096); };
+    // Fx (nxo, nxi, T): row (xi, t) is contiguous over xo
+    hipLaunchKernelGGL(flush_small_kernel, dim3(grid(T * nxi)), dim3(256), 0, s, op->d_fx,
+                       T * nxi, (int)nxo, nxo, 1L, cut);
+    DAGE_HIP_CHECK(hipGetLastError());
+    // Fy (nyi, nyo, T), Fz (nzi, nzo, T): row (i, t) at i + ni*no*t, stride ni
+    for (long t = 0; t < T; ++t) {
+        hipLaunchKernelGGL(flush_small_kernel, dim3(grid(nyi)), dim3(256), 0, s,
+                           op->d_fy + t * nyi * nyo, nyi, (int)nyo, 1L, nyi, cut);
+        DAGE_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(flush_small_kernel, dim3(grid(nzi)), dim3(256), 0, s,
+                           op->d_fz + t * nzi * nzo, nzi, (int)nzo, 1L, nzi, cut);
+        DAGE_HIP_CHECK(hipGetLastError());
+    }
+    return DAGE_OK;
+}
 
 int relayout_fz(dage_operator op) {
     const int nzi = op->din[2], nzo = op->dout[2], T = op->ntp;

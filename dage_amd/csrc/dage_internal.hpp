@@ -364,6 +364,7 @@ int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, c
 // Row masks of the operator's matrices and the column-block counts they make
 // cheapest; run after every change of the matrices (synchronises).
 int build_masks(dage_operator op);
+int flush_matrices(dage_operator op);
 // Bubbles (inject.hip, abi.hip).
 int bubbles_sync_device(dage_bubbles bub);
 // Bubble ibub's buffers on the radial grid ax (cell-lookup table built from

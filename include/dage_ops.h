@@ -165,10 +165,15 @@ int dage_operator_get_weights(dage_operator op, double *w);
 int dage_operator_set_energy(dage_operator op, double energy);
 /* Upload externally built matrices, as Operator3D_cuda_init uploads
  * f(X_), f(Y_), f(Z_) (function3d.F90:4200-4206).  fx: (nxo,nxi,T), fy:
- * (nyi,nyo,T), fz: (nzi,nzo,T), host memory. */
+ * (nyi,nyo,T), fz: (nzi,nzo,T), host memory.  As for the generated
+ * matrices, entries at most 1e-20 times the largest |entry| of their row
+ * (one input point, one t) are stored as zeros (environment variable
+ * DAGE_FLUSH_CUTOFF: another factor, 0 to keep them): the passes skip the
+ * panels such Gaussian tails fill, and a pass's output moves by at most
+ * n * 1e-20 * max|F_t| * max|input|. */
 int dage_operator_set_matrices(dage_operator op, const double *fx,
                                const double *fy, const double *fz);
-/* Download the matrices (same layouts) to host memory. */
+/* Download the matrices as applied (same layouts) to host memory. */
 int dage_operator_get_matrices(dage_operator op, double *fx, double *fy,
                                double *fz);
 /* Operator3D_transform_cube (function3d.F90:4230-4349): cubeout =

@@ -57,8 +57,17 @@ def test_matrices_vs_oracle(ctx, nlip, ncell):
         worst = max(worst, float(err.max()))
     print(f"matrix max |dF|/max|F_t| = {worst:.3e}")
     assert worst < 2e-9
-    # exact zeros (underflow band) are reproduced exactly
-    assert np.array_equal(fx == 0.0, rx == 0.0)
+    # the closed forms' exact zeros (underflow) stay zero; the other zeros are
+    # the flushed tails, entries at most 1e-20 of their row's largest (one
+    # input point, one t); a 10x margin covers ulp-level differences of
+    # entries next to the cutoff
+    for got, ref, out_axis in ((fx, rx, 0), (fy, ry, 1), (fz, rz, 1)):
+        rowmax = np.max(np.abs(ref), axis=out_axis, keepdims=True)
+        assert np.all(got[ref == 0.0] == 0.0)
+        flushed = (got == 0.0) & (ref != 0.0)
+        assert np.all((np.abs(ref) <= 1e-19 * rowmax)[flushed])
+        assert np.all(got[np.abs(ref) > 1e-19 * rowmax] != 0.0)
+        print(f"flushed entries: {flushed.mean():.3f} of the matrix")
 
 
 def run_both(ctx, axes, nlip, cube, q, coda=True, set_oracle_matrices=False, energy=None):
@@ -341,6 +350,45 @@ def test_direct_z_pass_accumulates_over_t_chunks(ctx, monkeypatch):
         print(f"{label}: direct z pass vs oracle {e_ref:.2e}, vs split-K path {e_split:.2e}")
         assert e_ref < 1e-12
         assert e_split < 1e-14
+
+
+def test_flushed_tails_change_the_apply_at_rounding_level(ctx, monkeypatch):
+    """The operator stores entries at most 1e-20 of their row's largest as
+    zeros, and the passes skip the panels they leave empty.  On a 193-point
+    axis (h = 0.1, the 385^3 headline's spacing) at T = 40 the flush empties
+    a large share of the 16x16 blocks the exact zeros leave; the apply moves
+    by less than 1e-15 of max|V| (the bound is 193 * 1e-20 per pass relative
+    to max|F_t| max|g|), for a smooth density and for random input."""
+    ax = O.Axis.ncell_fixed(-9.6, 9.6, 7, 32)
+    assert ax.n == 193
+    q = quad()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    rng = np.random.default_rng(5)
+    P = ax.points
+    rho = O.gauss_density(np.array([1.0, -0.5]), np.array([1.5, 0.7]),
+                          np.array([[0.3, -0.2, 0.1], [-3.0, 2.5, 4.0]]), P, P, P)
+    rnd = np.asfortranarray(rng.uniform(-1.0, 1.0, (ax.n,) * 3))
+    res, blocks = {}, {}
+    for cut in ("0", None):
+        if cut is None:
+            monkeypatch.delenv("DAGE_FLUSH_CUTOFF", raising=False)
+        else:
+            monkeypatch.setenv("DAGE_FLUSH_CUTOFF", cut)
+        op = dage_amd.Helmholtz3D(ctx, g, g, q, energy=0.0)
+        op.set_energy(-0.3)
+        fx = op.get_matrices()[0]
+        nz = np.abs(fx) > 0
+        pad = np.zeros((208, 208, fx.shape[2]), bool)
+        pad[:193, :193] = nz
+        blocks[cut] = pad.reshape(13, 16, 13, 16, -1).any(axis=(1, 3)).mean()
+        res[cut] = (op.transform_cube(rho), op.transform_cube(rnd))
+        op.close()
+    print(f"nonzero 16x16 blocks of Fx: {blocks['0']:.3f} exact zeros, {blocks[None]:.3f} flushed")
+    assert blocks[None] < 0.85 * blocks["0"]
+    for i, name in enumerate(("density", "random")):
+        e = rel_maxdiff(res[None][i], res["0"][i])
+        print(f"{name}: flushed vs unflushed apply {e:.2e}")
+        assert e < 1e-15
 
 
 def test_apply_far_entries_dominate(ctx):
