@@ -189,6 +189,8 @@ void free_operator_buffers(dage_operator op) {
     op->d_xband = nullptr;
     if (op->d_yband) (void)hipFree(op->d_yband);
     op->d_yband = nullptr;
+    if (op->d_band) (void)hipFree(op->d_band);
+    op->d_band = nullptr;
     op->planes_valid = false;
     op->ws_bytes = 0;
 }
@@ -796,6 +798,12 @@ int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip
         nb[p] = op->pitch[p] > 0 ? op->nbp[p] : op->nb_pick[p];
         skip[p] = op->mask_used[p] ? 1 : 0;
     }
+    return DAGE_OK;
+}
+
+int dage_operator_get_band(dage_operator op, int *t0) {
+    DAGE_REQUIRE(op && t0, DAGE_ERR_ARG, "null argument");
+    *t0 = op->d_band ? op->band_t0 : op->ntp;
     return DAGE_OK;
 }
 

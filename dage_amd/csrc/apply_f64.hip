@@ -1076,6 +1076,7 @@ int build_masks(dage_operator op) {
                                   hipMemcpyHostToDevice, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_yband, yb.data(), sizeof(int2) * yb.size(),
                                   hipMemcpyHostToDevice, s));
+    DAGE_TRY(build_bands(op, hs));
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
     // tuning knob "x,y,z": fixed column-block counts, masks on wherever defined
     if (const char *e = std::getenv("DAGE_APPLY_NB")) {
@@ -1106,6 +1107,13 @@ ApplyMats operator_mats(dage_operator op) {
     m.yband = op->d_yband;
     m.nxt = op->nxt;
     m.nyc = op->nyc;
+    m.band_t0 = op->ntp;
+    if (op->d_band) {
+        m.band_t0 = op->band_t0;
+        m.band_y = op->d_band;
+        m.band_z = op->d_band + (size_t)(op->ntp - op->band_t0) * op->dout[1] * kBandW;
+        m.band_h = op->band_h.data();
+    }
     if (op->d_mask) {
         const long rx = (long)op->ntp * op->din[0], ry = (long)op->ntp * op->din[1];
         const uint64_t *base[3] = {op->d_mask, op->d_mask + rx, op->d_mask + rx + ry};
@@ -1202,8 +1210,14 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const char *nm_y = bat ? "batch_f64_ymode" : "gemm_f64_ymode";
     const char *nm_z = bat ? "batch_f64_zmode" : "gemm_f64_zmode";
     const char *nm_r = bat ? "batch_splitk_reduce" : "splitk_reduce_kernel";
-    for (int t0 = 0; t0 < T; t0 += tcmax) {
-        const int tc = std::min(tcmax, T - t0);
+    const char *nm_by = bat ? "batch_band_ymode" : "band_ymode";
+    const char *nm_bz = bat ? "batch_band_zmode" : "band_zmode";
+    // t-points from tb on (narrow bands) take the banded y and z passes; the
+    // MFMA chunks stop there
+    const int tb = (nt == 0 && m.band_y) ? m.band_t0 : T;
+    for (int t0 = 0; t0 < T;) {
+        const bool banded = t0 >= tb;
+        const int tc = std::min(tcmax, (banded ? T : tb) - t0);
         if (nt > 0) {
             // passes 1 and 2 fused: U_b((xo,yo),(zi,t)) straight from in_b
             XYF f{};
@@ -1239,6 +1253,9 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         set_tiles(g1, 0, (long)t0 * nxi, nxi, &nb1);
         DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
 
+        if (banded) {
+            DAGE_TRY(band_pass_y(ctx, m, V, U, W, wS, t0, tc, B, nm_by));
+        } else {
         // pass 2, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};
         g2.A = V;
@@ -1255,6 +1272,21 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         int nb2;
         set_tiles(g2, 1, (long)t0 * nyi, nyi, &nb2);
         DAGE_TRY(launch_gemm<1>(ctx, nm_y, g2, tc * B, nb2));
+        }
+        }
+
+        if (banded) {
+            // into partial 0 of each right-hand side (the others are zero when
+            // no MFMA chunk ran before), or straight into out
+            const long slab = M3 * nzo;
+            if (t0 == 0 && !direct)
+                for (int b = 0; b < B && plan.splits > 1; ++b)
+                    DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + 1) * slab,
+                                       (size_t)(plan.splits - 1) * slab));
+            DAGE_TRY(band_pass_z(ctx, m, U, direct ? out : P, (long)plan.splits * slab, t0 > 0,
+                                 t0, tc, B, nm_bz));
+            t0 += tc;
+            continue;
         }
 
         // pass 3, batch (split, b): P_{b,s}((xo,yo), zo) (+)= U_b((xo,yo), k) . Gz(k, zo)
@@ -1280,6 +1312,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
                 DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + splits) * M3 * nzo,
                                    (size_t)(plan.splits - splits) * M3 * nzo));
         DAGE_TRY(launch_gemm<2>(ctx, nm_z, g3, splits * B, nb3));
+        t0 += tc;
     }
     if (direct) return DAGE_OK;
     const long len = M3 * nzo;

@@ -1361,6 +1361,11 @@ int dage_gbfmm_box_count(dage_gbfmm f, int *nbox_total) {
     return DAGE_OK;
 }
 
+int dage_gbfmm_get_band(dage_gbfmm f, int *t0) {
+    DAGE_REQUIRE(f && t0, DAGE_ERR_ARG, "null argument");
+    return dage_operator_get_band(f->nf_op, t0);
+}
+
 int dage_gbfmm_get_multipoles(dage_gbfmm f, double *q) {
     DAGE_REQUIRE(f && q, DAGE_ERR_ARG, "null argument");
     DAGE_TRY(use_device(f->ctx));

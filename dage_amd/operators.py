@@ -375,10 +375,14 @@ class Operator3D:
 
     def tiling(self) -> dict:
         """Column tiling of the FP64 passes (x, y, z): column blocks per tile,
-        tile pitch (0: 16*nb columns) and whether zero K panels are skipped."""
+        tile pitch (0: 16*nb columns), whether zero K panels are skipped, and
+        the first t-point of the banded y/z passes (T: none)."""
         nb, pitch, skip = (ctypes.c_int * 3)(), (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
         call("dage_operator_get_tiling", self.handle, nb, pitch, skip)
-        return {"nb": list(nb), "pitch": list(pitch), "skip": [bool(v) for v in skip]}
+        t0 = ctypes.c_int()
+        call("dage_operator_get_band", self.handle, ctypes.byref(t0))
+        return {"nb": list(nb), "pitch": list(pitch), "skip": [bool(v) for v in skip],
+                "band_t0": t0.value}
 
 
 class Coulomb3D(Operator3D):
@@ -553,6 +557,12 @@ class GBFMMCoulomb3D:
         n = ctypes.c_int()
         call("dage_gbfmm_box_count", h, ctypes.byref(n))
         self.nbox_total = n.value
+
+    def band_t0(self) -> int:
+        """First t-point whose near-field y and z passes run banded (T: none)."""
+        t0 = ctypes.c_int()
+        call("dage_gbfmm_get_band", self.handle, ctypes.byref(t0))
+        return t0.value
 
     def close(self):
         if self.handle:

@@ -220,6 +220,11 @@ int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
  * pitch[p]: tile pitch in points, 0 for tiles of 16*nb[p] columns;
  * skip[p]: 1 when all-zero K panels are skipped. */
 int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip[3]);
+/* No reference counterpart (diagnostic): the first t-point whose y and z
+ * passes run banded on the FP64 VALU (every nonzero entry of Fy_t and Fz_t
+ * within 24 points of the diagonal, for this and all later t); *t0 = T when
+ * none (or with DAGE_APPLY_BAND=0 at operator set-up). */
+int dage_operator_get_band(dage_operator op, int *t0);
 int dage_operator_destroy(dage_operator op);
 
 /* ----------------------------------------------------------------- bubbles
@@ -436,6 +441,9 @@ int dage_gbfmm_apply(dage_gbfmm gbfmm, const double *rho, double *pot,
  * -1/(2 pi) of the Helmholtz update). */
 int dage_gbfmm_set_energy(dage_gbfmm gbfmm, double energy);
 int dage_gbfmm_box_count(dage_gbfmm gbfmm, int *nbox_total);
+/* No reference counterpart (diagnostic): dage_operator_get_band of the
+ * near-field operator. */
+int dage_gbfmm_get_band(dage_gbfmm gbfmm, int *t0);
 /* After an apply: multipole moments and local expansions, (lmax+1)^2 x
  * nbox_total column-major, boxes in the reference's order (level maxlevel
  * first, children of a box consecutive), as multipole_moments(:, :) and

@@ -242,8 +242,9 @@ def test_profiling_counts_kernel_launches(ctx):
     ctx.profile(True)
     op.transform_cube(np.ones((ax.n,) * 3, order="F"))
     ms, n = ctx.profile_query("gemm_f64_zmode")
+    ms_b, n_b = ctx.profile_query("band_zmode")  # a 19-point axis is banded at every t
     ctx.profile(False)
-    assert n >= 1 and ms > 0.0
+    assert n + n_b >= 1 and ms + ms_b > 0.0
 
 
 def test_error_paths(ctx):
@@ -369,6 +370,9 @@ def test_flushed_tails_change_the_apply_at_rounding_level(ctx, monkeypatch):
                           np.array([[0.3, -0.2, 0.1], [-3.0, 2.5, 4.0]]), P, P, P)
     rnd = np.asfortranarray(rng.uniform(-1.0, 1.0, (ax.n,) * 3))
     res, blocks = {}, {}
+    # the same (MFMA) passes for both, so that only the flush differs (the
+    # wider unflushed bands would leave fewer t-points to the banded passes)
+    monkeypatch.setenv("DAGE_APPLY_BAND", "0")
     for cut in ("0", None):
         if cut is None:
             monkeypatch.delenv("DAGE_FLUSH_CUTOFF", raising=False)
@@ -389,6 +393,48 @@ def test_flushed_tails_change_the_apply_at_rounding_level(ctx, monkeypatch):
         e = rel_maxdiff(res[None][i], res["0"][i])
         print(f"{name}: flushed vs unflushed apply {e:.2e}")
         assert e < 1e-15
+
+
+@pytest.mark.parametrize("n,energy", [(97, None), (97, -0.4), (25, None), (61, -1.5)])
+def test_banded_passes_match_the_mfma_passes(ctx, monkeypatch, n, energy):
+    """t-points whose Fy_t and Fz_t keep every nonzero entry within 24 points
+    of the diagonal (for that t and all larger ones) take banded y and z
+    passes on the FP64 VALU.  At h = 0.1 and T = 40 the large-t tail is
+    banded (every t on a 25-point axis); with DAGE_APPLY_BAND=0 every t takes
+    the MFMA passes.  The same sums in another order: 1e-14 of max|V|; the
+    oracle's dense apply with the operator's own matrices: 1e-12."""
+    ax = O.Axis.ncell_fixed(-0.05 * (n - 1), 0.05 * (n - 1), 7, (n - 1) // 6)
+    assert ax.n == n
+    q = quad()
+    T = q.get_npoints()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    P = ax.points
+    rho = O.gauss_density(np.array([1.0, 0.4]), np.array([2.0, 0.8]),
+                          np.array([[0.2, -0.1, 0.3], [-0.9, 0.7, -0.4]]), P, P, P)
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("DAGE_APPLY_BAND", flag)
+        if energy is None:
+            op = dage_amd.Coulomb3D(ctx, g, g, q)
+        else:
+            op = dage_amd.Helmholtz3D(ctx, g, g, q, energy=0.0)
+            op.set_energy(energy)
+        res[flag] = (op.transform_cube(rho), op.tiling()["band_t0"], op.get_matrices())
+        op.close()
+    t0 = res["1"][1]
+    print(f"n {n}: banded from t-point {t0} of {T}")
+    assert res["0"][1] == T and 0 <= t0 < T
+    if n == 25:
+        assert t0 == 0
+    e = rel_maxdiff(res["1"][0], res["0"][0])
+    print(f"banded vs MFMA passes: {e:.2e}")
+    assert e < 1e-14
+    fx, fy, fz = res["1"][2]
+    w = O.coulomb_weights(q.tw) if energy is None else O.helmholtz_weights(q.tp, q.tw, energy)
+    ref = O.transform_cube(fx, fy, fz, w, O.coda(q.tlog), rho)
+    e = rel_maxdiff(res["1"][0], ref)
+    print(f"banded apply vs oracle: {e:.2e}")
+    assert e < 1e-12
 
 
 def test_apply_far_entries_dominate(ctx):
