@@ -701,8 +701,10 @@ def bench_config5(ctx, args):
     # batch_*: the sweep's passes (the batch's own event names); gemm_f64_*:
     # the GBFMM near field's
     names = ("batch_xy_fused_kernel", "batch_f64_xmode", "batch_f64_ymode", "batch_f64_zmode",
-             "batch_band_ymode", "batch_band_zmode", "batch_splitk_reduce", "xy_fused_kernel",
-             "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "band_ymode", "band_zmode",
+             "batch_band_xmode", "batch_band_ymode", "batch_band_zmode", "batch_splitk_reduce",
+             "xy_fused_kernel",
+             "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "band_xmode", "band_ymode",
+             "band_zmode",
              "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",

@@ -1111,7 +1111,8 @@ ApplyMats operator_mats(dage_operator op) {
     if (op->d_band) {
         m.band_t0 = op->band_t0;
         m.band_y = op->d_band;
-        m.band_z = op->d_band + (size_t)(op->ntp - op->band_t0) * op->dout[1] * kBandW;
+        m.band_z = op->d_band + op->band_off[0];
+        m.band_x = op->d_band + op->band_off[1];
         m.band_h = op->band_h.data();
     }
     if (op->d_mask) {
@@ -1210,6 +1211,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const char *nm_y = bat ? "batch_f64_ymode" : "gemm_f64_ymode";
     const char *nm_z = bat ? "batch_f64_zmode" : "gemm_f64_zmode";
     const char *nm_r = bat ? "batch_splitk_reduce" : "splitk_reduce_kernel";
+    const char *nm_bx = bat ? "batch_band_xmode" : "band_xmode";
     const char *nm_by = bat ? "batch_band_ymode" : "band_ymode";
     const char *nm_bz = bat ? "batch_band_zmode" : "band_zmode";
     // t-points from tb on (narrow bands) take the banded y and z passes; the
@@ -1251,7 +1253,10 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
         int nb1;
         set_tiles(g1, 0, (long)t0 * nxi, nxi, &nb1);
-        DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
+        if (banded)
+            DAGE_TRY(band_pass_x(ctx, m, in, V, t0, tc, B, nm_bx));
+        else
+            DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
 
         if (banded) {
             DAGE_TRY(band_pass_y(ctx, m, V, U, W, wS, t0, tc, B, nm_by));

@@ -1,12 +1,13 @@
 // Banded FP64 passes for the t-points whose 1D operator matrices are narrow
-// bands (y and z mode products of the tensor-quadrature apply, the same sums
-// as passes 2 and 3 of apply_f64.hip over a contiguous range of t).
+// bands (x, y and z mode products of the tensor-quadrature apply, the same
+// sums as the three passes of apply_f64.hip over a contiguous range of t).
 //
 // At large t the Gaussian exp(-t^2 (x - x')^2) confines every nonzero entry
 // of F_t (after the matrices' tails are stored as zeros) to |o - i| <= H_t
-// with H_t of a few points: at 385 points and T = 40 the last 16 t-points
-// have H_t <= 24, the last 8 H_t <= 6 (13 taps per output where a 16x16 MFMA
-// tile of the band executes over 100).  These passes run on the FP64 VALU
+// with H_t of a few points: at 385 points and T = 40 the last 11 t-points
+// have H_t <= 8, the last 8 H_t <= 6 (13 taps per output where a 16x16 MFMA
+// tile of the band executes over 100; wider bands, whose sums run at a small
+// fraction of the VALU peak here, stay on the MFMA passes).  These passes run on the FP64 VALU
 // (same 78.6 TFLOP/s peak as FP64 MFMA on gfx950) instead:
 //
 //   * one thread per (line, R consecutive outputs along the contracted axis);
@@ -20,10 +21,13 @@
 //     output once (partial slab 0 of the split-K reduction, or the output
 //     itself when the z pass writes straight to it).
 //
-// Coefficient table (built once per operator): tab[((t - t0)*n_out + o)*kBandW
-// + j] = F_t(o, o - kBandH + j), zero where that input index is outside the
-// axis, so a kernel with half-width HC <= kBandH reads taps
-// j in [kBandH - HC, kBandH + HC].
+// Coefficient tables (built once per operator) hold F_t(o, o - kBandH + j),
+// zero where that input index is outside the axis, so a kernel with
+// half-width HC <= kBandH reads taps j in [kBandH - HC, kBandH + HC]: the y
+// and z passes sum one output at a time from a row-major table (an output's
+// taps are one contiguous scalar read); the x pass sums tap by tap over its
+// R outputs from a transposed table (a tap's R coefficients are contiguous),
+// which gives its FMAs R independent sums.
 #include "dage_internal.hpp"
 
 #include <algorithm>
@@ -42,20 +46,33 @@ __device__ inline unsigned xcd_order(unsigned L, unsigned total) {
     return x * q + min(x, r) + (L >> 3);
 }
 
-// tab[((tl*nout) + o)*kBandW + j] = F_{t0+tl}(o, o - kBandH + j) with
-// F_t(o, i) = f[o*so + i*si + t*st] (0 for i outside [0, nin)).
+// F_{t0+tl}(o, o - kBandH + j), F_t(o, i) = f[o*so + i*si + t*st] (0 for i
+// outside [0, nin) and for the padding o >= nout), stored
+//   row-major  (nop == 0): tab[(tl*nout + o)*kBandW + j]
+//   transposed (nop > 0):  tab[(tl*kBandW + j)*nop + o], o < nop
 __global__ void __launch_bounds__(256) band_table_kernel(const double *f, long so, long si,
-                                                         long st, int nout, int nin, int t0,
-                                                         int nt, double *tab) {
-    const long total = (long)nt * nout * kBandW;
+                                                         long st, int nout, int nin, int nop,
+                                                         int t0, int nt, double *tab) {
+    const int no = nop > 0 ? nop : nout;
+    const long total = (long)nt * kBandW * no;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
          e += (long)gridDim.x * blockDim.x) {
-        const int j = (int)(e % kBandW);
-        const long r = e / kBandW;
-        const int o = (int)(r % nout);
-        const int tl = (int)(r / nout);
+        int o, j, tl;
+        if (nop > 0) {
+            o = (int)(e % nop);
+            const long r = e / nop;
+            j = (int)(r % kBandW);
+            tl = (int)(r / kBandW);
+        } else {
+            j = (int)(e % kBandW);
+            const long r = e / kBandW;
+            o = (int)(r % nout);
+            tl = (int)(r / nout);
+        }
         const int i = o - kBandH + j;
-        tab[e] = (i >= 0 && i < nin) ? f[(long)o * so + (long)i * si + (long)(t0 + tl) * st] : 0.0;
+        tab[e] = (o < nout && i >= 0 && i < nin)
+                     ? f[(long)o * so + (long)i * si + (long)(t0 + tl) * st]
+                     : 0.0;
     }
 }
 
@@ -66,8 +83,9 @@ struct BandY {
     long wS;
     long Vt, Ut;
     int nx, nyi, nyo, nz;
-    int t0, tc, B;
-    long tab_ts;      // nyo * kBandW
+    int t0, tc, B;    // chunk start and length (tc: stride of b in V and U)
+    int tl0, tr;      // this launch: t-points tl0..tl0+tr-1 of the chunk
+    long tab_ts;      // nyo * kBandW (row-major table)
     int ngroups, nblk;  // line groups of 256, output blocks along y
 };
 
@@ -78,24 +96,27 @@ __global__ void __launch_bounds__(256) band_y_kernel(BandY a, const double *__re
     const int yb = (int)(L % (unsigned)a.nblk);
     const unsigned rest = L / (unsigned)a.nblk;
     const int lg = (int)(rest % (unsigned)a.ngroups);
-    const int bt = (int)(rest / (unsigned)a.ngroups);  // b*tc + tl
-    const int tl = bt % a.tc, b = bt / a.tc;
+    const int bt = (int)(rest / (unsigned)a.ngroups);
+    const int tl = a.tl0 + bt % a.tr, b = bt / a.tr;
     const long lines = (long)a.nx * a.nz;
     const long line = (long)lg * 256 + threadIdx.x;
     const bool ok = line < lines;
     const long l = ok ? line : lines - 1;
     const long xo = l % a.nx, zi = l / a.nx;
-    const double *src = a.V + (long)bt * a.Vt + xo + (long)a.nx * a.nyi * zi;
-    double *dst = a.U + (long)bt * a.Ut + xo + (long)a.nx * a.nyo * zi;
+    const long bt_off = (long)b * a.tc + tl;
+    const double *src = a.V + bt_off * a.Vt + xo + (long)a.nx * a.nyi * zi;
+    double *dst = a.U + bt_off * a.Ut + xo + (long)a.nx * a.nyo * zi;
     const double alpha = a.w[(long)b * a.wS + a.t0 + tl];
-    const double *tt = tab + (long)tl * a.tab_ts + (kBandH - HC);
     const int y0 = yb * R;
+    const double *tt = tab + (long)(tl - a.tl0) * a.tab_ts + (kBandH - HC);
     double win[R + 2 * HC];
 #pragma unroll
     for (int i = 0; i < R + 2 * HC; ++i) {
         const int yi = min(max(y0 - HC + i, 0), a.nyi - 1);  // taps outside carry 0
         win[i] = src[(long)a.nx * yi];
     }
+    // one output at a time: its 2HC+1 coefficients are one contiguous scalar
+    // read, and the stores stream out as the sums finish
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int yo = y0 + r;
@@ -107,13 +128,90 @@ __global__ void __launch_bounds__(256) band_y_kernel(BandY a, const double *__re
     }
 }
 
+struct BandX {
+    const double *in;  // in_b(xi, line) at in + b*nin + xi + nxi*line, line = yi + nyi*zi
+    double *V;         // V_{b,tl}(xo, line) at V + (b*tc + tl)*Vt + xo + nxo*line
+    long nin, Vt, lines;
+    int nxi, nxo;
+    int tc, tl0, tr, B;  // chunk length (stride of b in V); this launch's t-points
+    int nop;             // padded output count of the table rows
+    long tab_ts;         // kBandW * nop
+    int nxt, ngroups;    // x tiles, line groups of 64
+};
+
+// V_{b,t}(xo, line) = sum_{|xi - xo| <= HC} F_t(xo, xi) in_b(xi, line) for the
+// launch's t-points.  The contracted axis is the contiguous one, so the
+// block stages the input of 64 lines x (4R + 2HC) points in LDS (coalesced),
+// the lanes of a wavefront take one line each (coefficients uniform again:
+// the wavefront's R outputs share xo) and hold their R + 2HC window in
+// registers for every t of the launch, and the 64 x 4R outputs go back
+// through LDS so the stores run along x.  Odd LDS row strides keep the
+// lane-per-line accesses on distinct banks.
+template <int HC, int R>
+__global__ void __launch_bounds__(256) band_x_kernel(BandX a, const double *__restrict__ tab) {
+    constexpr int XT = 4 * R;        // outputs per block along x
+    constexpr int WI = XT + 2 * HC;  // input points per line
+    constexpr int LSI = WI | 1, LSO = XT | 1;
+    // the input tile is in registers (win) before the first output tile is
+    // written, so both share one LDS buffer
+    __shared__ double S[64 * LSI];
+    double *Sin = S, *Sout = S;
+    const unsigned L = xcd_order(blockIdx.x, gridDim.x);
+    const int xt = (int)(L % (unsigned)a.nxt);
+    const unsigned rest = L / (unsigned)a.nxt;
+    const int lg = (int)(rest % (unsigned)a.ngroups);
+    const int b = (int)(rest / (unsigned)a.ngroups);
+    const int x0 = xt * XT;
+    const long l0 = (long)lg * 64;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // wavefront-uniform, made visible to the compiler so that the coefficient
+    // addresses stay scalar
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const double *src = a.in + (long)b * a.nin;
+    for (int e = tid; e < 64 * WI; e += 256) {
+        const int line = e / WI, xl = e - line * WI;
+        const long l = min(l0 + line, a.lines - 1);
+        const int x = min(max(x0 - HC + xl, 0), a.nxi - 1);  // taps outside carry 0
+        Sin[line * LSI + xl] = src[x + (long)a.nxi * l];
+    }
+    __syncthreads();
+    double win[R + 2 * HC];
+#pragma unroll
+    for (int i = 0; i < R + 2 * HC; ++i) win[i] = Sin[lane * LSI + wave * R + i];
+    __syncthreads();  // every wavefront holds its window: the buffer takes outputs now
+    for (int k = 0; k < a.tr; ++k) {
+        const double *tt =
+            tab + (long)k * a.tab_ts + (long)(kBandH - HC) * a.nop + x0 + wave * R;
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0;
+#pragma unroll
+        for (int j = 0; j <= 2 * HC; ++j) {
+            const double *c = tt + (long)j * a.nop;
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = fma(c[r], win[r + j], acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) Sout[lane * LSO + wave * R + r] = acc[r];
+        __syncthreads();
+        double *dst = a.V + ((long)b * a.tc + a.tl0 + k) * a.Vt;
+        for (int e = tid; e < 64 * XT; e += 256) {
+            const int line = e / XT, xl = e - line * XT;
+            const int xo = x0 + xl;
+            const long l = l0 + line;
+            if (xo < a.nxo && l < a.lines) dst[xo + (long)a.nxo * l] = Sout[line * LSO + xl];
+        }
+        __syncthreads();
+    }
+}
+
 struct BandZ {
     const double *U;  // U_{b,tl}(l, zi) at U + (b*tc + tl)*Ut + l + M3*zi, l = xo + nxo*yo
     double *C;        // C_b(l, zo) at C + b*cS + l + M3*zo
     long Ut, cS, M3;
     int nzi, nzo, tc, B;
     int accumulate;
-    long tab_ts;      // nzo * kBandW
+    long tab_ts;      // nzo * kBandW (row-major table)
     int ngroups, nblk;
 };
 
@@ -163,16 +261,44 @@ __global__ void __launch_bounds__(256) band_z_kernel(BandZ a, const double *__re
 
 // Outputs per thread along the contracted axis for each half-width class:
 // R + 2HC input values stay in registers.
-template <int HC> constexpr int band_rows() { return HC <= 8 ? 16 : (HC <= 16 ? 12 : 8); }
+template <int HC> constexpr int band_rows() { return 16; }
 
-int band_class(int h) { return h <= 8 ? 8 : (h <= 16 ? 16 : 24); }
+// One half-width class: every banded t-point runs the kBandH kernels (a
+// narrower band only carries zero taps).
+int band_class(int) { return kBandH; }
+
+// Table rows padded past the last output (zeros) by more than the widest
+// output tile (4R = 56 points in the x pass), so the coefficients a tile
+// reads for its outputs past the axis end stay inside the row.
+int band_nop(long n) { return (int)n + 64; }
+
+// x pass: R outputs per wavefront along x (4R per block, 37 KB of LDS: four
+// blocks per CU); 4R = 56 leaves little of a 385-point axis to a last,
+// mostly empty tile
+template <int HC> constexpr int band_rows_x() { return 14; }
+
+template <int HC>
+int launch_x(dage_ctx ctx, BandX a, const double *tab, const char *name) {
+    constexpr int R = band_rows_x<HC>();
+    a.nxt = (a.nxo + 4 * R - 1) / (4 * R);
+    a.ngroups = (int)((a.lines + 63) / 64);
+    const long blocks = (long)a.nxt * a.ngroups * a.B;
+    DAGE_REQUIRE(blocks < (1L << 31), DAGE_ERR_ARG, "banded x pass too large (%ld blocks)", blocks);
+    hipEvent_t ev;
+    prof_begin(ctx, name, &ev);
+    hipLaunchKernelGGL((band_x_kernel<HC, R>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a,
+                       tab);
+    DAGE_HIP_CHECK(hipGetLastError());
+    prof_end(ctx, name, ev);
+    return DAGE_OK;
+}
 
 template <int HC>
 int launch_y(dage_ctx ctx, BandY a, const double *tab, const char *name) {
     constexpr int R = band_rows<HC>();
     a.ngroups = (int)(((long)a.nx * a.nz + 255) / 256);
     a.nblk = (a.nyo + R - 1) / R;
-    const long blocks = (long)a.ngroups * a.nblk * a.tc * a.B;
+    const long blocks = (long)a.ngroups * a.nblk * a.tr * a.B;
     DAGE_REQUIRE(blocks < (1L << 31), DAGE_ERR_ARG, "banded y pass too large (%ld blocks)", blocks);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
@@ -208,10 +334,10 @@ bool band_enabled() {
 
 int build_bands(dage_operator op, const std::vector<int> &span) {
     const long nxi = op->din[0], nyi = op->din[1], nzi = op->din[2];
-    const long nyo = op->dout[1], nzo = op->dout[2];
+    const long nxo = op->dout[0], nyo = op->dout[1], nzo = op->dout[2];
     const int T = op->ntp;
     const long rx = (long)T * nxi, ry = (long)T * nyi;
-    // half-width of t: the largest |o - i| of a nonzero entry of Fy_t or Fz_t
+    // half-width of t: the largest |o - i| of a nonzero entry of Fx_t, Fy_t or Fz_t
     // (spans: first and last nonzero output of each input row)
     op->band_h.assign(T, 0);
     auto widen = [&](long row, long i, int t) {
@@ -220,6 +346,7 @@ int build_bands(dage_operator op, const std::vector<int> &span) {
         op->band_h[t] = std::max(op->band_h[t], (int)std::max(i - lo, (long)hi - i));
     };
     for (int t = 0; t < T; ++t) {
+        for (long xi = 0; xi < nxi; ++xi) widen(t * nxi + xi, xi, t);
         for (long yi = 0; yi < nyi; ++yi) widen(rx + t * nyi + yi, yi, t);
         for (long zi = 0; zi < nzi; ++zi) widen(rx + ry + zi + nzi * t, zi, t);
     }
@@ -230,29 +357,78 @@ int build_bands(dage_operator op, const std::vector<int> &span) {
     op->band_t0 = T;
     if (t0 == T || !band_enabled()) return DAGE_OK;
     const int nt = T - t0;
-    const size_t ny = (size_t)nt * nyo * kBandW, nz = (size_t)nt * nzo * kBandW;
-    DAGE_HIP_CHECK(hipMalloc((void **)&op->d_band, 8 * (ny + nz)));
+    const size_t ny = (size_t)nt * kBandW * nyo, nz = (size_t)nt * kBandW * nzo;
+    const size_t nx = (size_t)nt * kBandW * band_nop(nxo);
+    DAGE_HIP_CHECK(hipMalloc((void **)&op->d_band, 8 * (ny + nz + nx)));
     hipStream_t s = op->ctx->stream;
     auto grid = [](size_t n) { return (int)std::min<size_t>((n + 255) / 256, 65536); };
     // Fy (nyi, nyo, T): F_t(yo, yi) = fy[yi + nyi*yo + nyi*nyo*t]
     hipLaunchKernelGGL(band_table_kernel, dim3(grid(ny)), dim3(256), 0, s, op->d_fy, nyi, 1L,
-                       nyi * nyo, (int)nyo, (int)nyi, t0, nt, op->d_band);
+                       nyi * nyo, (int)nyo, (int)nyi, 0, t0, nt, op->d_band);
     DAGE_HIP_CHECK(hipGetLastError());
     // Fz (nzi, nzo, T): F_t(zo, zi) = fz[zi + nzi*zo + nzi*nzo*t]
     hipLaunchKernelGGL(band_table_kernel, dim3(grid(nz)), dim3(256), 0, s, op->d_fz, nzi, 1L,
-                       nzi * nzo, (int)nzo, (int)nzi, t0, nt, op->d_band + ny);
+                       nzi * nzo, (int)nzo, (int)nzi, 0, t0, nt, op->d_band + ny);
     DAGE_HIP_CHECK(hipGetLastError());
+    // Fx (nxo, nxi, T): F_t(xo, xi) = fx[xo + nxo*xi + nxo*nxi*t]
+    hipLaunchKernelGGL(band_table_kernel, dim3(grid(nx)), dim3(256), 0, s, op->d_fx, 1L, nxo,
+                       nxo * nxi, (int)nxo, (int)nxi, band_nop(nxo), t0, nt,
+                       op->d_band + ny + nz);
+    DAGE_HIP_CHECK(hipGetLastError());
+    op->band_off[0] = ny;
+    op->band_off[1] = ny + nz;
     op->band_t0 = t0;
     return DAGE_OK;
+}
+
+namespace {
+
+// Runs of consecutive t-points of one half-width class within [t0, t0+tc):
+// calls fn(first, count, class) for each.
+template <typename Fn>
+int for_runs(const ApplyMats &m, int t0, int tc, Fn fn) {
+    DAGE_REQUIRE(t0 >= m.band_t0 && t0 + tc <= m.ntp, DAGE_ERR_STATE,
+                 "t-points %d..%d are not banded", t0, t0 + tc - 1);
+    for (int a = t0; a < t0 + tc;) {
+        DAGE_REQUIRE(m.band_h[a] <= kBandH, DAGE_ERR_STATE, "t-point %d is not banded", a);
+        const int c = band_class(m.band_h[a]);
+        int e = a + 1;
+        while (e < t0 + tc && band_class(m.band_h[e]) == c) ++e;
+        DAGE_TRY(fn(a, e - a, c));
+        a = e;
+    }
+    return DAGE_OK;
+}
+
+}  // namespace
+
+int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, int t0, int tc,
+                int B, const char *name) {
+    BandX a{};
+    a.in = in;
+    a.V = V;
+    a.nxi = m.din[0];
+    a.nxo = m.dout[0];
+    a.lines = (long)m.din[1] * m.din[2];
+    a.nin = (long)a.nxi * a.lines;
+    a.Vt = (long)a.nxo * a.lines;
+    a.tc = tc;
+    a.B = B;
+    a.nop = band_nop(a.nxo);
+    a.tab_ts = (long)kBandW * a.nop;
+    return for_runs(m, t0, tc, [&](int first, int count, int c) {
+        BandX r = a;
+        r.tl0 = first - t0;
+        r.tr = count;
+        const double *tab = m.band_x + (long)(first - m.band_t0) * a.tab_ts;
+        (void)c;
+        return launch_x<kBandH>(ctx, r, tab, name);
+    });
 }
 
 int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, const double *w,
                 long wS, int t0, int tc, int B, const char *name) {
     const int nx = m.dout[0], nyi = m.din[1], nyo = m.dout[1], nz = m.din[2];
-    int h = 0;
-    for (int t = t0; t < t0 + tc; ++t) h = std::max(h, m.band_h[t]);
-    DAGE_REQUIRE(t0 >= m.band_t0 && h <= kBandH, DAGE_ERR_STATE, "t-points %d..%d are not banded",
-                 t0, t0 + tc - 1);
     BandY a{};
     a.V = V;
     a.U = U;
@@ -263,12 +439,14 @@ int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, co
     a.nx = nx; a.nyi = nyi; a.nyo = nyo; a.nz = nz;
     a.t0 = t0; a.tc = tc; a.B = B;
     a.tab_ts = (long)nyo * kBandW;
-    const double *tab = m.band_y + (long)(t0 - m.band_t0) * a.tab_ts;
-    switch (band_class(h)) {
-    case 8: return launch_y<8>(ctx, a, tab, name);
-    case 16: return launch_y<16>(ctx, a, tab, name);
-    default: return launch_y<24>(ctx, a, tab, name);
-    }
+    return for_runs(m, t0, tc, [&](int first, int count, int c) {
+        BandY r = a;
+        r.tl0 = first - t0;
+        r.tr = count;
+        const double *tab = m.band_y + (long)(first - m.band_t0) * a.tab_ts;
+        (void)c;
+        return launch_y<kBandH>(ctx, r, tab, name);
+    });
 }
 
 int band_pass_z(dage_ctx ctx, const ApplyMats &m, const double *U, double *C, long cS,
@@ -288,11 +466,7 @@ int band_pass_z(dage_ctx ctx, const ApplyMats &m, const double *U, double *C, lo
     a.accumulate = accumulate ? 1 : 0;
     a.tab_ts = (long)nzo * kBandW;
     const double *tab = m.band_z + (long)(t0 - m.band_t0) * a.tab_ts;
-    switch (band_class(h)) {
-    case 8: return launch_z<8>(ctx, a, tab, name);
-    case 16: return launch_z<16>(ctx, a, tab, name);
-    default: return launch_z<24>(ctx, a, tab, name);
-    }
+    return launch_z<kBandH>(ctx, a, tab, name);
 }
 
 }  // namespace dage

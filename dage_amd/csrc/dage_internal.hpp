@@ -59,7 +59,7 @@ double pi();
 constexpr int kMaxNlip = 8;
 // Banded y/z passes (band.hip): largest half-width |o - i| of a nonzero
 // matrix entry they handle, and the row length of their coefficient tables.
-constexpr int kBandH = 24;
+constexpr int kBandH = 8;
 constexpr int kBandW = 2 * kBandH + 1;
 
 struct Axis {
@@ -225,10 +225,12 @@ struct dage_operator_s {
     void *d_planes = nullptr;   // split-BF16 matrix planes (built on first mixed apply)
     bool planes_valid = false;  // cleared when the matrices change
     MixedLayout mixed;
-    // Banded y/z passes for t >= band_t0 (band_t0 == ntp: none): band_h[t] =
-    // largest |o - i| of a nonzero entry of Fy_t or Fz_t; d_band holds the
-    // coefficient tables of Fy then Fz for t = band_t0..ntp-1 (band.hip)
+    // Banded passes for t >= band_t0 (band_t0 == ntp: none): band_h[t] =
+    // largest |o - i| of a nonzero entry of Fx_t, Fy_t or Fz_t; d_band holds
+    // the coefficient tables of Fy, Fz, then Fx for t = band_t0..ntp-1
+    // (band.hip)
     double *d_band = nullptr;
+    size_t band_off[2] = {0, 0};  // element offsets of the Fz and Fx tables
     int band_t0 = 0;
     std::vector<int> band_h;
 };
@@ -278,7 +280,7 @@ struct ApplyMats {
     const int2 *xband = nullptr, *yband = nullptr;
     int nxt = 0, nyc = 0;
     // banded y/z passes for t >= band_t0 (see dage_operator_s)
-    const double *band_y = nullptr, *band_z = nullptr;
+    const double *band_y = nullptr, *band_z = nullptr, *band_x = nullptr;
     int band_t0 = 0;
     const int *band_h = nullptr;
 };
@@ -380,10 +382,13 @@ int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, c
 int build_masks(dage_operator op);
 int flush_matrices(dage_operator op);
 // Banded passes (band.hip): half-widths and coefficient tables from the row
-// spans build_masks computes; the y pass U = w * (V . Fy_t) and the z pass
+// spans build_masks computes; the x pass V = Fx_t . in, the y pass
+// U = w * (V . Fy_t) and the z pass
 // C (+)= sum_t U_t . Fz_t over t = t0..t0+tc-1 (all >= band_t0), for B
 // right-hand sides, in the layouts of apply_f64.hip's passes 2 and 3.
 int build_bands(dage_operator op, const std::vector<int> &span);
+int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, int t0, int tc,
+                int B, const char *name);
 int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, const double *w,
                 long wS, int t0, int tc, int B, const char *name);
 int band_pass_z(dage_ctx ctx, const ApplyMats &m, const double *U, double *C, long cS,

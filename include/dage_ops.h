@@ -220,10 +220,10 @@ int dage_operator_workspace_bytes(dage_operator op, size_t *bytes);
  * pitch[p]: tile pitch in points, 0 for tiles of 16*nb[p] columns;
  * skip[p]: 1 when all-zero K panels are skipped. */
 int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip[3]);
-/* No reference counterpart (diagnostic): the first t-point whose y and z
- * passes run banded on the FP64 VALU (every nonzero entry of Fy_t and Fz_t
- * within 24 points of the diagonal, for this and all later t); *t0 = T when
- * none (or with DAGE_APPLY_BAND=0 at operator set-up). */
+/* No reference counterpart (diagnostic): the first t-point whose passes run
+ * banded on the FP64 VALU (every nonzero entry of Fx_t, Fy_t and Fz_t within
+ * 8 points of the diagonal, for this and all later t); *t0 = T when none (or
+ * with DAGE_APPLY_BAND=0 at operator set-up). */
 int dage_operator_get_band(dage_operator op, int *t0);
 int dage_operator_destroy(dage_operator op);
 
