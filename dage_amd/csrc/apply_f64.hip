@@ -1114,6 +1114,7 @@ ApplyMats operator_mats(dage_operator op) {
         m.band_z = op->d_band + op->band_off[0];
         m.band_x = op->d_band + op->band_off[1];
         m.band_h = op->band_h.data();
+        m.band_mode = op->band_mode;
     }
     if (op->d_mask) {
         const long rx = (long)op->ntp * op->din[0], ry = (long)op->ntp * op->din[1];
@@ -1216,7 +1217,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const char *nm_bz = bat ? "batch_band_zmode" : "band_zmode";
     // t-points from tb on (narrow bands) take the banded y and z passes; the
     // MFMA chunks stop there
-    const int tb = (nt == 0 && m.band_y) ? m.band_t0 : T;
+    const int tb = (nt == 0 && band_pays(m, B)) ? m.band_t0 : T;
     for (int t0 = 0; t0 < T;) {
         const bool banded = t0 >= tb;
         const int tc = std::min(tcmax, (banded ? T : tb) - t0);

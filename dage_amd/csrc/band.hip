@@ -325,9 +325,12 @@ int launch_z(dage_ctx ctx, BandZ a, const double *tab, const char *name) {
     return DAGE_OK;
 }
 
-bool band_enabled() {
+// DAGE_APPLY_BAND: 0 no banded passes, 2 banded passes at every size,
+// otherwise (default 1) where the launch is large enough to fill the GPU
+int band_mode() {
     const char *e = std::getenv("DAGE_APPLY_BAND");
-    return !(e && e[0] == '0');
+    if (!e) return 1;
+    return e[0] == '0' ? 0 : (e[0] == '2' ? 2 : 1);
 }
 
 }  // namespace
@@ -355,7 +358,8 @@ int build_bands(dage_operator op, const std::vector<int> &span) {
     if (op->d_band) DAGE_HIP_CHECK(hipFree(op->d_band));
     op->d_band = nullptr;
     op->band_t0 = T;
-    if (t0 == T || !band_enabled()) return DAGE_OK;
+    op->band_mode = band_mode();
+    if (t0 == T || op->band_mode == 0) return DAGE_OK;
     const int nt = T - t0;
     const size_t ny = (size_t)nt * kBandW * nyo, nz = (size_t)nt * kBandW * nzo;
     const size_t nx = (size_t)nt * kBandW * band_nop(nxo);
@@ -401,6 +405,16 @@ int for_runs(const ApplyMats &m, int t0, int tc, Fn fn) {
 }
 
 }  // namespace
+
+bool band_pays(const ApplyMats &m, int B) {
+    if (!m.band_y) return false;
+    if (m.band_mode == 2) return true;
+    // the z pass has the fewest blocks (no t in its grid): below ~4 per CU
+    // the MFMA passes, whose z pass splits K, fill the GPU better
+    const long m3 = (long)m.dout[0] * m.dout[1];
+    const long zblocks = (m3 + 255) / 256 * ((m.dout[2] + band_rows<kBandH>() - 1) / band_rows<kBandH>()) * B;
+    return zblocks >= 1024;
+}
 
 int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, int t0, int tc,
                 int B, const char *name) {

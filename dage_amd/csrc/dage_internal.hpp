@@ -232,6 +232,7 @@ struct dage_operator_s {
     double *d_band = nullptr;
     size_t band_off[2] = {0, 0};  // element offsets of the Fz and Fx tables
     int band_t0 = 0;
+    int band_mode = 1;  // 1: where the launch fills the GPU, 2: always
     std::vector<int> band_h;
 };
 
@@ -281,7 +282,7 @@ struct ApplyMats {
     int nxt = 0, nyc = 0;
     // banded y/z passes for t >= band_t0 (see dage_operator_s)
     const double *band_y = nullptr, *band_z = nullptr, *band_x = nullptr;
-    int band_t0 = 0;
+    int band_t0 = 0, band_mode = 1;
     const int *band_h = nullptr;
 };
 // Device scratch owned by the caller, grown on demand.
@@ -387,6 +388,8 @@ int flush_matrices(dage_operator op);
 // C (+)= sum_t U_t . Fz_t over t = t0..t0+tc-1 (all >= band_t0), for B
 // right-hand sides, in the layouts of apply_f64.hip's passes 2 and 3.
 int build_bands(dage_operator op, const std::vector<int> &span);
+// Whether an apply of B right-hand sides takes the banded passes.
+bool band_pays(const ApplyMats &m, int B);
 int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, int t0, int tc,
                 int B, const char *name);
 int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, const double *w,

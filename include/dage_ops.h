@@ -223,7 +223,8 @@ int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip
 /* No reference counterpart (diagnostic): the first t-point whose passes run
  * banded on the FP64 VALU (every nonzero entry of Fx_t, Fy_t and Fz_t within
  * 8 points of the diagonal, for this and all later t); *t0 = T when none (or
- * with DAGE_APPLY_BAND=0 at operator set-up). */
+ * with DAGE_APPLY_BAND=0 at operator set-up).  An apply takes them when it
+ * is large enough to fill the GPU (DAGE_APPLY_BAND=2: always). */
 int dage_operator_get_band(dage_operator op, int *t0);
 int dage_operator_destroy(dage_operator op);
 
