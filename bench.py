@@ -194,7 +194,7 @@ def bench_config2(ctx, args):
     ms = ctx.time_end() / steps
     wall = time.perf_counter() - wall0
     names = ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
-             "band_ymode", "band_zmode", "splitk_reduce_kernel",
+             "band_xmode", "band_ymode", "band_zmode", "lowrank", "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm")
@@ -213,8 +213,9 @@ def bench_config2(ctx, args):
     dk = kernels[dom]
     if dom.startswith("gemm_f64") or dom == "xy_fused_kernel":
         passes = 2.0 if dom == "xy_fused_kernel" else 1.0
-        # the MFMA y and z passes cover the t-points before the banded tail
-        t_mfma = fmm.band_t0() if dom in ("gemm_f64_ymode", "gemm_f64_zmode") else T
+        # the MFMA passes cover the t-points between the low-rank head and the
+        # banded tail
+        t_mfma = fmm.band_t0() - fmm.lowrank_t1() if dom.startswith("gemm_f64") else T
         per_launch = passes * 2.0 * t_mfma * float(n) ** 4 / dk["launches_per_step"]
         achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
@@ -336,7 +337,7 @@ def bench_config1(ctx, args):
     ms = ctx.time_end() / steps
     kernels = {}
     for name in ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
-                 "band_ymode", "band_zmode", "splitk_reduce_kernel"):
+                 "band_xmode", "band_ymode", "band_zmode", "lowrank", "splitk_reduce_kernel"):
         tot, k = ctx.profile_query(name)
         if k:
             kernels[name] = {"ms_per_launch": round(tot / k, 4), "launches_per_apply": k // steps}
@@ -701,10 +702,11 @@ def bench_config5(ctx, args):
     # batch_*: the sweep's passes (the batch's own event names); gemm_f64_*:
     # the GBFMM near field's
     names = ("batch_xy_fused_kernel", "batch_f64_xmode", "batch_f64_ymode", "batch_f64_zmode",
-             "batch_band_xmode", "batch_band_ymode", "batch_band_zmode", "batch_splitk_reduce",
+             "batch_band_xmode", "batch_band_ymode", "batch_band_zmode", "batch_lowrank",
+             "batch_splitk_reduce",
              "xy_fused_kernel",
              "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "band_xmode", "band_ymode",
-             "band_zmode",
+             "band_zmode", "lowrank",
              "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
@@ -740,16 +742,17 @@ def bench_config5(ctx, args):
     mfma_kernels = ("batch_f64_", "gemm_f64_", "batch_xy_fused", "xy_fused")
     if dom.startswith(mfma_kernels):
         passes = 2.0 if "xy_fused" in dom else 1.0
-        # the MFMA y and z passes cover the t-points before the banded tail
-        # (whose y and z passes run banded on the VALU); x covers every t
-        if "xy_fused" in dom or dom.endswith("xmode"):
+        # the MFMA passes cover the t-points between the low-rank head (applied
+        # through factors) and the banded tail (VALU passes)
+        if "xy_fused" in dom:
             tsum = T * nrhs if dom.startswith("batch_") else Tn
         elif dom.startswith("batch_"):
             probe = dage_amd.Helmholtz3D(ctx, grid, grid, hq)
-            tsum = probe.tiling()["band_t0"] * nrhs
+            til = probe.tiling()
+            tsum = (til["band_t0"] - til["lowrank_t1"]) * nrhs
             probe.close()
         else:
-            tsum = fmm.band_t0()
+            tsum = fmm.band_t0() - fmm.lowrank_t1()
         per_step = passes * 2.0 * float(n) ** 4 * tsum
         per_launch = per_step / dk["launches_per_step"]
         achieved = per_launch / (dk["ms_per_launch"] * 1e-3) / 1e12

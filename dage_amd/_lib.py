@@ -71,6 +71,7 @@ _SIGNATURES = {
     "dage_operator_noalloc_limit": [ctypes.POINTER(ctypes.c_size_t)],
     "dage_operator_workspace_bytes": [_c_p, ctypes.POINTER(ctypes.c_size_t)],
     "dage_operator_get_band": [_c_p, ctypes.POINTER(ctypes.c_int)],
+    "dage_operator_get_lowrank": [_c_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "dage_operator_get_tiling": [_c_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int)],
     "dage_operator_destroy": [_c_p],
@@ -111,6 +112,7 @@ _SIGNATURES = {
     "dage_gbfmm_set_energy": [_c_p, _dbl],
     "dage_gbfmm_box_count": [_c_p, _ip],
     "dage_gbfmm_get_band": [_c_p, _ip],
+    "dage_gbfmm_get_lowrank": [_c_p, _ip, _ip],
     "dage_gbfmm_get_multipoles": [_c_p, _dp],
     "dage_gbfmm_get_local_expansions": [_c_p, _dp],
     "dage_gbfmm_destroy": [_c_p],

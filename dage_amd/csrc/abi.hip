@@ -191,6 +191,9 @@ void free_operator_buffers(dage_operator op) {
     op->d_yband = nullptr;
     if (op->d_band) (void)hipFree(op->d_band);
     op->d_band = nullptr;
+    if (op->d_lr) (void)hipFree(op->d_lr);
+    op->d_lr = nullptr;
+    op->lr = LowRank{};
     op->planes_valid = false;
     op->ws_bytes = 0;
 }
@@ -804,6 +807,13 @@ int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip
 int dage_operator_get_band(dage_operator op, int *t0) {
     DAGE_REQUIRE(op && t0, DAGE_ERR_ARG, "null argument");
     *t0 = op->d_band ? op->band_t0 : op->ntp;
+    return DAGE_OK;
+}
+
+int dage_operator_get_lowrank(dage_operator op, int *t1, int *kx_sum) {
+    DAGE_REQUIRE(op && t1 && kx_sum, DAGE_ERR_ARG, "null argument");
+    *t1 = op->d_lr ? op->lr.t1 : 0;
+    *kx_sum = op->d_lr ? op->lr.Kx : 0;
     return DAGE_OK;
 }
 

@@ -1078,6 +1078,7 @@ int build_masks(dage_operator op) {
                                   hipMemcpyHostToDevice, s));
     DAGE_TRY(build_bands(op, hs));
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    DAGE_TRY(build_lowrank(op));
     // tuning knob "x,y,z": fixed column-block counts, masks on wherever defined
     if (const char *e = std::getenv("DAGE_APPLY_NB")) {
         int v[3] = {0, 0, 0};
@@ -1116,6 +1117,7 @@ ApplyMats operator_mats(dage_operator op) {
         m.band_h = op->band_h.data();
         m.band_mode = op->band_mode;
     }
+    if (op->d_lr && op->lr.t1 > 0) m.lr = &op->lr;
     if (op->d_mask) {
         const long rx = (long)op->ntp * op->din[0], ry = (long)op->ntp * op->din[1];
         const uint64_t *base[3] = {op->d_mask, op->d_mask + rx, op->d_mask + rx + ry};
@@ -1215,10 +1217,28 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const char *nm_bx = bat ? "batch_band_xmode" : "band_xmode";
     const char *nm_by = bat ? "batch_band_ymode" : "band_ymode";
     const char *nm_bz = bat ? "batch_band_zmode" : "band_zmode";
-    // t-points from tb on (narrow bands) take the banded y and z passes; the
-    // MFMA chunks stop there
+    // t-points from tb on (narrow bands) take the banded passes; the MFMA
+    // chunks stop there
     const int tb = (nt == 0 && band_pays(m, B)) ? m.band_t0 : T;
-    for (int t0 = 0; t0 < T;) {
+    // the low-rank t-points (the smallest t) first, when their intermediates
+    // fit the passes' workspace: partial 0 (or out) is overwritten, the other
+    // partials cleared, and the chunks below accumulate
+    int tlr = 0;
+    if (nt == 0 && lowrank_pays(m, B)) {
+        size_t w1 = 0, w2 = 0;
+        lowrank_workspace(m, B, &w1, &w2);
+        if (w1 <= plan.v_elems && w2 <= plan.u_elems) {
+            const long slab = M3 * nzo;
+            if (!direct)
+                for (int b = 0; b < B && plan.splits > 1; ++b)
+                    DAGE_TRY(fill_zero(ctx, P + ((size_t)b * plan.splits + 1) * slab,
+                                       (size_t)(plan.splits - 1) * slab));
+            DAGE_TRY(lowrank_apply(ctx, m, in, direct ? out : P, (long)plan.splits * slab, B, W,
+                                   wS, V, U, bat ? "batch_lowrank" : "lowrank"));
+            tlr = m.lr->t1;
+        }
+    }
+    for (int t0 = tlr; t0 < T;) {
         const bool banded = t0 >= tb;
         const int tc = std::min(tcmax, (banded ? T : tb) - t0);
         if (nt > 0) {
@@ -1358,6 +1378,20 @@ int gemm_colmajor(dage_ctx ctx, const char *name, int M, int N, int K, const dou
                        nsplit, len, 1, (const double *)nullptr, 0.0, C);
     DAGE_HIP_CHECK(hipGetLastError());
     return DAGE_OK;
+}
+
+int gemm_colmajor_general(dage_ctx ctx, const char *name, const ColGemm &c, int batch) {
+    Gemm g{};
+    g.A = c.A; g.B = c.B; g.C = c.C;
+    g.M = c.M; g.N = c.N; g.K = c.K;
+    g.M0 = c.M; g.sAm0 = 1; g.sAm1 = 0; g.sAk = c.sAk;
+    g.sBk = 1; g.sBn = c.sBn;
+    g.sCm0 = 1; g.sCm1 = 0; g.sCn = c.sCn;
+    g.bA = c.bA; g.bB = c.bB; g.bC = c.bC;
+    g.Z0 = c.Z0; g.bA1 = c.bA1; g.bB1 = c.bB1; g.bC1 = c.bC1;
+    g.alpha = c.alpha; g.aS1 = c.aS1;
+    g.accumulate = c.accumulate;
+    return launch_gemm<3>(ctx, name, g, batch);
 }
 
 int gemm_colmajor_batched(dage_ctx ctx, const char *name, int M, int N, int K, const double *A,

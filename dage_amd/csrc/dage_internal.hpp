@@ -189,6 +189,20 @@ struct MixedLayout {
     size_t off_x = 0, off_y = 0, off_z = 0;  // element offsets into the plane buffer
 };
 
+// Factors of the low-rank t-points (lowrank.hip), element offsets into one
+// device buffer `base`: RxAll (Kx x nxi) and QxAll (nxo x Kx) with the x
+// factors of t in rows / columns koff[t].., and per t R_y^T, R_z^T
+// (n_in x k) and Q_y^T, Q_z^T (k x n_out), column-major.
+struct LowRank {
+    int t1 = 0;    // t-points 0..t1-1
+    int mode = 1;  // 1: where the apply is large enough, 2: always
+    int Kx = 0;
+    std::vector<int> k[3], koff;
+    size_t off_rx = 0, off_qx = 0;
+    std::vector<size_t> off_rt[2], off_qt[2];  // [0]: y, [1]: z
+    const double *base = nullptr;
+};
+
 struct dage_operator_s {
     dage_ctx ctx = nullptr;
     dage_grid3d gin = nullptr, gout = nullptr;  // set during creation only
@@ -229,6 +243,8 @@ struct dage_operator_s {
     // largest |o - i| of a nonzero entry of Fx_t, Fy_t or Fz_t; d_band holds
     // the coefficient tables of Fy, Fz, then Fx for t = band_t0..ntp-1
     // (band.hip)
+    double *d_lr = nullptr;  // low-rank factors (lowrank.hip), described by lr
+    LowRank lr;
     double *d_band = nullptr;
     size_t band_off[2] = {0, 0};  // element offsets of the Fz and Fx tables
     int band_t0 = 0;
@@ -284,6 +300,8 @@ struct ApplyMats {
     const double *band_y = nullptr, *band_z = nullptr, *band_x = nullptr;
     int band_t0 = 0, band_mode = 1;
     const int *band_h = nullptr;
+    // low-rank t-points 0..lr->t1-1 (null: none)
+    const LowRank *lr = nullptr;
 };
 // Device scratch owned by the caller, grown on demand.
 struct Workspace {
@@ -390,6 +408,33 @@ int flush_matrices(dage_operator op);
 int build_bands(dage_operator op, const std::vector<int> &span);
 // Whether an apply of B right-hand sides takes the banded passes.
 bool band_pays(const ApplyMats &m, int B);
+// Column-major FP64 MFMA GEMM, batched: for batch index z (z_lo = z % Z0,
+// z_hi = z / Z0 when Z0 > 0, else z_lo = z, z_hi = 0)
+//   C_z (M x N, column stride sCn) (+)= alpha * A_z (M x K, column stride sAk)
+//                                              . B_z (K x N, column stride sBn)
+// with X_z = X + z_lo*bX + z_hi*bX1 and alpha = alpha[z_lo + aS1*z_hi] (null: 1).
+struct ColGemm {
+    const double *A = nullptr, *B = nullptr;
+    double *C = nullptr;
+    int M = 0, N = 0, K = 0;
+    long sAk = 0, sBn = 0, sCn = 0;
+    long bA = 0, bB = 0, bC = 0;
+    int Z0 = 0;
+    long bA1 = 0, bB1 = 0, bC1 = 0;
+    const double *alpha = nullptr;
+    long aS1 = 0;
+    int accumulate = 0;
+};
+int gemm_colmajor_general(dage_ctx ctx, const char *name, const ColGemm &c, int batch);
+// Low-rank t-points (lowrank.hip): factors from the operator's matrices
+// (after build_masks), workspace (doubles: first region, second region) and
+// the apply of t = 0..lr->t1-1 for B right-hand sides into C_b = C + b*cS
+// (overwritten).
+int build_lowrank(dage_operator op);
+bool lowrank_pays(const ApplyMats &m, int B);
+void lowrank_workspace(const ApplyMats &m, int B, size_t *first, size_t *second);
+int lowrank_apply(dage_ctx ctx, const ApplyMats &m, const double *in, double *C, long cS, int B,
+                  const double *W, long wS, double *ws1, double *ws2, const char *name);
 int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, int t0, int tc,
                 int B, const char *name);
 int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, const double *w,

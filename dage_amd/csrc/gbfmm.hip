@@ -1366,6 +1366,11 @@ int dage_gbfmm_get_band(dage_gbfmm f, int *t0) {
     return dage_operator_get_band(f->nf_op, t0);
 }
 
+int dage_gbfmm_get_lowrank(dage_gbfmm f, int *t1, int *kx_sum) {
+    DAGE_REQUIRE(f && t1 && kx_sum, DAGE_ERR_ARG, "null argument");
+    return dage_operator_get_lowrank(f->nf_op, t1, kx_sum);
+}
+
 int dage_gbfmm_get_multipoles(dage_gbfmm f, double *q) {
     DAGE_REQUIRE(f && q, DAGE_ERR_ARG, "null argument");
     DAGE_TRY(use_device(f->ctx));

@@ -375,14 +375,16 @@ class Operator3D:
 
     def tiling(self) -> dict:
         """Column tiling of the FP64 passes (x, y, z): column blocks per tile,
-        tile pitch (0: 16*nb columns), whether zero K panels are skipped, and
-        the first t-point of the banded y/z passes (T: none)."""
+        tile pitch (0: 16*nb columns), whether zero K panels are skipped, the
+        first t-point of the banded passes (T: none), and the number of
+        low-rank t-points with the sum of their x ranks."""
         nb, pitch, skip = (ctypes.c_int * 3)(), (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
         call("dage_operator_get_tiling", self.handle, nb, pitch, skip)
-        t0 = ctypes.c_int()
+        t0, t1, kx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         call("dage_operator_get_band", self.handle, ctypes.byref(t0))
+        call("dage_operator_get_lowrank", self.handle, ctypes.byref(t1), ctypes.byref(kx))
         return {"nb": list(nb), "pitch": list(pitch), "skip": [bool(v) for v in skip],
-                "band_t0": t0.value}
+                "band_t0": t0.value, "lowrank_t1": t1.value, "lowrank_kx": kx.value}
 
 
 class Coulomb3D(Operator3D):
@@ -559,10 +561,16 @@ class GBFMMCoulomb3D:
         self.nbox_total = n.value
 
     def band_t0(self) -> int:
-        """First t-point whose near-field y and z passes run banded (T: none)."""
+        """First t-point whose near-field passes run banded (T: none)."""
         t0 = ctypes.c_int()
         call("dage_gbfmm_get_band", self.handle, ctypes.byref(t0))
         return t0.value
+
+    def lowrank_t1(self) -> int:
+        """Number of leading near-field t-points applied through low-rank factors."""
+        t1, kx = ctypes.c_int(), ctypes.c_int()
+        call("dage_gbfmm_get_lowrank", self.handle, ctypes.byref(t1), ctypes.byref(kx))
+        return t1.value
 
     def close(self):
         if self.handle:

@@ -226,6 +226,13 @@ int dage_operator_get_tiling(dage_operator op, int nb[3], int pitch[3], int skip
  * with DAGE_APPLY_BAND=0 at operator set-up).  An apply takes them when it
  * is large enough to fill the GPU (DAGE_APPLY_BAND=2: always). */
 int dage_operator_get_band(dage_operator op, int *t0);
+/* No reference counterpart (diagnostic): the t-points 0..t1-1 applied
+ * through low-rank factors of their 1D matrices (column-pivoted
+ * Gram-Schmidt, truncated where the largest residual column norm falls to
+ * 1e-14 of the largest column norm; every axis's rank at most half its
+ * length), and the sum of their x ranks.  t1 = 0: none (or with
+ * DAGE_APPLY_LOWRANK=0 at operator set-up). */
+int dage_operator_get_lowrank(dage_operator op, int *t1, int *kx_sum);
 int dage_operator_destroy(dage_operator op);
 
 /* ----------------------------------------------------------------- bubbles
@@ -445,6 +452,9 @@ int dage_gbfmm_box_count(dage_gbfmm gbfmm, int *nbox_total);
 /* No reference counterpart (diagnostic): dage_operator_get_band of the
  * near-field operator. */
 int dage_gbfmm_get_band(dage_gbfmm gbfmm, int *t0);
+/* No reference counterpart (diagnostic): dage_operator_get_lowrank of the
+ * near-field operator. */
+int dage_gbfmm_get_lowrank(dage_gbfmm gbfmm, int *t1, int *kx_sum);
 /* After an apply: multipole moments and local expansions, (lmax+1)^2 x
  * nbox_total column-major, boxes in the reference's order (level maxlevel
  * first, children of a box consecutive), as multipole_moments(:, :) and

@@ -371,8 +371,10 @@ def test_flushed_tails_change_the_apply_at_rounding_level(ctx, monkeypatch):
     rnd = np.asfortranarray(rng.uniform(-1.0, 1.0, (ax.n,) * 3))
     res, blocks = {}, {}
     # the same (MFMA) passes for both, so that only the flush differs (the
-    # wider unflushed bands would leave fewer t-points to the banded passes)
+    # wider unflushed bands would leave fewer t-points to the banded passes,
+    # and the low-rank factors of the small t would differ at 1e-14)
     monkeypatch.setenv("DAGE_APPLY_BAND", "0")
+    monkeypatch.setenv("DAGE_APPLY_LOWRANK", "0")
     for cut in ("0", None):
         if cut is None:
             monkeypatch.delenv("DAGE_FLUSH_CUTOFF", raising=False)
@@ -435,6 +437,50 @@ def test_banded_passes_match_the_mfma_passes(ctx, monkeypatch, n, energy):
     e = rel_maxdiff(res["2"][0], ref)
     print(f"banded apply vs oracle: {e:.2e}")
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("n,energy", [(97, None), (97, -0.4), (193, -1.2)])
+def test_lowrank_points_match_the_dense_passes(ctx, monkeypatch, n, energy):
+    """The smallest t-points, whose 1D matrices are numerically of low rank
+    (truncated at a residual column norm of 1e-14 of the largest), are
+    applied through their factors (reduce each axis to the rank, expand
+    back).  Against the dense passes (DAGE_APPLY_LOWRANK=0) the apply moves
+    by the truncation, ~1e-14 of max|V| (bar 1e-13); against the oracle's
+    dense apply with the operator's own matrices 1e-12."""
+    ax = O.Axis.ncell_fixed(-0.05 * (n - 1), 0.05 * (n - 1), 7, (n - 1) // 6)
+    assert ax.n == n
+    q = quad()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    P = ax.points
+    rho = O.gauss_density(np.array([1.0, 0.4]), np.array([2.0, 0.8]),
+                          np.array([[0.2, -0.1, 0.3], [-0.9, 0.7, -0.4]]), P, P, P)
+    rng = np.random.default_rng(9)
+    rnd = np.asfortranarray(rng.uniform(-1.0, 1.0, rho.shape))
+    res = {}
+    for flag in ("2", "0"):  # 2: also below the size where it pays
+        monkeypatch.setenv("DAGE_APPLY_LOWRANK", flag)
+        if energy is None:
+            op = dage_amd.Coulomb3D(ctx, g, g, q)
+        else:
+            op = dage_amd.Helmholtz3D(ctx, g, g, q, energy=0.0)
+            op.set_energy(energy)
+        til = op.tiling()
+        res[flag] = (op.transform_cube(rho), op.transform_cube(rnd), til, op.get_matrices())
+        op.close()
+    t1, kx = res["2"][2]["lowrank_t1"], res["2"][2]["lowrank_kx"]
+    print(f"n {n}: {t1} low-rank t-points, x ranks summing to {kx}")
+    assert res["0"][2]["lowrank_t1"] == 0 and t1 >= 3
+    for i, name in enumerate(("density", "random")):
+        e = rel_maxdiff(res["2"][i], res["0"][i])
+        print(f"{name}: low-rank vs dense passes {e:.2e}")
+        assert e < 1e-13
+    if n <= 97:
+        fx, fy, fz = res["2"][3]
+        w = O.coulomb_weights(q.tw) if energy is None else O.helmholtz_weights(q.tp, q.tw, energy)
+        ref = O.transform_cube(fx, fy, fz, w, O.coda(q.tlog), rho)
+        e = rel_maxdiff(res["2"][0], ref)
+        print(f"low-rank apply vs oracle: {e:.2e}")
+        assert e < 1e-12
 
 
 def test_apply_far_entries_dominate(ctx):

@@ -60,18 +60,19 @@ def test_fused_equals_separate_passes(ctx, case):
     q = quad()
     gin = lib_grid(ctx, ain, nlip)
     gout = gin if same else lib_grid(ctx, aout, nlip)
-    # the fused path has no banded y/z passes for the large-t points, so the
-    # separate passes are compared on an operator without them
+    # the fused path has neither the low-rank small-t points nor the banded
+    # large-t passes, so the separate passes are compared on an operator
+    # without them
     if energy is None:
-        op = with_env({"DAGE_APPLY_BAND": "0"},
+        op = with_env({"DAGE_APPLY_BAND": "0", "DAGE_APPLY_LOWRANK": "0"},
                       lambda: dage_amd.Coulomb3D(ctx, gin, gout, q, coda=same))
         w = O.coulomb_weights(q.tw)
     else:
-        op = with_env({"DAGE_APPLY_BAND": "0"},
+        op = with_env({"DAGE_APPLY_BAND": "0", "DAGE_APPLY_LOWRANK": "0"},
                       lambda: dage_amd.Helmholtz3D(ctx, gin, gout, q, energy=0.0, coda=same))
         op.set_energy(energy)
         w = O.helmholtz_weights(q.tp, q.tw, energy)
-    assert op.tiling()["band_t0"] == q.get_npoints()
+    assert op.tiling()["band_t0"] == q.get_npoints() and op.tiling()["lowrank_t1"] == 0
     fused = with_env({"DAGE_APPLY_TCHUNK": str(tchunk), "DAGE_APPLY_FUSED": "1"},
                      lambda: op.transform_cube(rho))
     sep = with_env({"DAGE_APPLY_TCHUNK": str(tchunk), "DAGE_APPLY_FUSED": "0"},
