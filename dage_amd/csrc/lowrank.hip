@@ -37,6 +37,7 @@ namespace dage {
 namespace {
 
 constexpr double kLowRankTol = 1e-14;
+constexpr int kLowRankMaxN = 800;  // longest axis factored at set-up
 
 // Factor M (no x ni, column-major, M(o, i) = m[o + no*i]) as Q R with Q
 // orthonormal (no x k) and R = Q^T M (k x ni).  Returns k, or -1 when the
@@ -129,6 +130,10 @@ int build_lowrank(dage_operator op) {
     if (L.mode == 0 || tmax == 0) return DAGE_OK;
     const int n_in[3] = {op->din[0], op->din[1], op->din[2]};
     const int n_out[3] = {op->dout[0], op->dout[1], op->dout[2]};
+    // the host factorization costs O(n^2 k) per t and axis: fine at a few
+    // hundred points (385: ~0.1 s per t), too slow for set-up beyond ~800
+    for (int a = 0; a < 3; ++a)
+        if (n_in[a] > kLowRankMaxN || n_out[a] > kLowRankMaxN) return DAGE_OK;
     hipStream_t s = op->ctx->stream;
     // host copies of the (flushed) matrices
     std::vector<double> h[3];
