@@ -992,8 +992,27 @@ int build_masks(dage_operator op) {
     DAGE_HIP_CHECK(hipMemcpyAsync(h.data(), op->d_mask, 8 * h.size(), hipMemcpyDeviceToHost, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(hs.data(), d_span, 4 * hs.size(), hipMemcpyDeviceToHost, s));
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    // banded large-t passes and low-rank small-t points (both need the
+    // spans and the final matrices)
+    DAGE_TRY(build_bands(op, hs));
+    DAGE_HIP_CHECK(hipStreamSynchronize(s));
+    DAGE_TRY(build_lowrank(op));
+    // The tiles are chosen for the t-points the MFMA passes run at one
+    // right-hand side: between the low-rank head and the banded tail when
+    // those paths are taken at that size.
+    int tlo = 0, thi = (int)T;
+    {
+        const ApplyMats m1 = operator_mats(op);
+        if (lowrank_pays(m1, 1)) tlo = op->lr.t1;
+        if (band_pays(m1, 1)) thi = op->band_t0;
+        if (tlo >= thi) tlo = 0, thi = (int)T;
+    }
     const long n[3] = {nxo, nyo, nzo};
     const long klen[3] = {nxi, nyi, rz}, nseg[3] = {T, T, 1}, off[3] = {0, rx, rx + ry};
+    // the rows of those t-points: t*klen.. for x and y, (zi + nzi*t) for z
+    const long rlo[3] = {tlo * nxi, tlo * nyi, nzi * tlo}, rhi[3] = {thi * nxi, thi * nyi, nzi * thi};
+    const long sseg[3] = {thi - tlo, thi - tlo, 1};
+    const long slen[3] = {nxi, nyi, nzi * (thi - tlo)};
     std::vector<uint64_t> ht(rows, 0);
     for (int p = 0; p < 3; ++p) {
         op->nb_pick[p] = 0;
@@ -1001,27 +1020,28 @@ int build_masks(dage_operator op) {
         op->pitch[p] = 0;
         op->nbp[p] = 0;
         if (n[p] > 1024) continue;
-        std::vector<uint64_t> mp(h.begin() + off[p], h.begin() + off[p] + nseg[p] * klen[p]);
+        std::vector<uint64_t> mp(h.begin() + off[p] + rlo[p], h.begin() + off[p] + rhi[p]);
         bool use = false;
-        op->nb_pick[p] = pick_nb_masked(mp, nseg[p], klen[p], (int)n[p], &use);
+        op->nb_pick[p] = pick_nb_masked(mp, sseg[p], slen[p], (int)n[p], &use);
         op->mask_used[p] = use;
         // tiles that follow the zero blocks (FP64 passes only), when they cut
         // the dense estimate by 7 % and beat the 16-column grid by 5 %
-        std::vector<int> sp(hs.begin() + 2 * off[p], hs.begin() + 2 * (off[p] + nseg[p] * klen[p]));
+        std::vector<int> sp(hs.begin() + 2 * (off[p] + rlo[p]), hs.begin() + 2 * (off[p] + rhi[p]));
         int nbq = 0, P = 0;
         double pc = 0.0;
-        pick_pitch(sp, nseg[p], klen[p], (int)n[p], &nbq, &P, &pc);
+        pick_pitch(sp, sseg[p], slen[p], (int)n[p], &nbq, &P, &pc);
         double dense = 1e300;
         for (int nb = 1; nb <= 8; ++nb)
-            dense = std::min(dense, pass_cost(mp, nseg[p], klen[p], (int)n[p], nb, true));
+            dense = std::min(dense, pass_cost(mp, sseg[p], slen[p], (int)n[p], nb, true));
         const double gc =
-            use ? pass_cost(mp, nseg[p], klen[p], (int)n[p], op->nb_pick[p], false) : dense;
+            use ? pass_cost(mp, sseg[p], slen[p], (int)n[p], op->nb_pick[p], false) : dense;
         if (P > 0 && pc < 0.93 * dense && pc < 0.95 * gc) {
             op->pitch[p] = P;
             op->nbp[p] = nbq;
             const int nt = pitch_tiles((int)n[p], P);
+            const int *spf = hs.data() + 2 * off[p];  // every row's span (all t)
             for (long r = 0; r < nseg[p] * klen[p]; ++r) {
-                const int lo = sp[2 * r], hi = sp[2 * r + 1];
+                const int lo = spf[2 * r], hi = spf[2 * r + 1];
                 uint64_t m = 0;
                 for (int j = 0; j < nt; ++j) {
                     const int a = j * P, b = (j + 1 < nt) ? a + P : (int)n[p];
@@ -1076,9 +1096,7 @@ int build_masks(dage_operator op) {
                                   hipMemcpyHostToDevice, s));
     DAGE_HIP_CHECK(hipMemcpyAsync(op->d_yband, yb.data(), sizeof(int2) * yb.size(),
                                   hipMemcpyHostToDevice, s));
-    DAGE_TRY(build_bands(op, hs));
     DAGE_HIP_CHECK(hipStreamSynchronize(s));
-    DAGE_TRY(build_lowrank(op));
     // tuning knob "x,y,z": fixed column-block counts, masks on wherever defined
     if (const char *e = std::getenv("DAGE_APPLY_NB")) {
         int v[3] = {0, 0, 0};
