@@ -1013,31 +1013,58 @@ int build_masks(dage_operator op) {
     const long rlo[3] = {tlo * nxi, tlo * nyi, nzi * tlo}, rhi[3] = {thi * nxi, thi * nyi, nzi * thi};
     const long sseg[3] = {thi - tlo, thi - tlo, 1};
     const long slen[3] = {nxi, nyi, nzi * (thi - tlo)};
+    // tiles for the rows [r0, r1) of pass p: column blocks, whether masks
+    // pay, and a pitch tiling that follows the zero blocks when it cuts the
+    // dense estimate by 7 % and beats the 16-column grid by 5 % (pitch 0: none)
+    struct Choice {
+        int nb = 0, pitch = 0, nbp = 0;
+        bool use = false;
+    };
+    auto choose = [&](int p, long r0, long r1, long ns, long kl) {
+        Choice c;
+        std::vector<uint64_t> mp(h.begin() + off[p] + r0, h.begin() + off[p] + r1);
+        c.nb = pick_nb_masked(mp, ns, kl, (int)n[p], &c.use);
+        std::vector<int> sp(hs.begin() + 2 * (off[p] + r0), hs.begin() + 2 * (off[p] + r1));
+        int nbq = 0, P = 0;
+        double pc = 0.0;
+        pick_pitch(sp, ns, kl, (int)n[p], &nbq, &P, &pc);
+        double dense = 1e300;
+        for (int nb = 1; nb <= 8; ++nb)
+            dense = std::min(dense, pass_cost(mp, ns, kl, (int)n[p], nb, true));
+        const double gc = c.use ? pass_cost(mp, ns, kl, (int)n[p], c.nb, false) : dense;
+        if (P > 0 && pc < 0.93 * dense && pc < 0.95 * gc) {
+            c.pitch = P;
+            c.nbp = nbq;
+        }
+        return c;
+    };
     std::vector<uint64_t> ht(rows, 0);
     for (int p = 0; p < 3; ++p) {
         op->nb_pick[p] = 0;
         op->mask_used[p] = false;
         op->pitch[p] = 0;
         op->nbp[p] = 0;
+        op->mix_nb_pick[p] = 0;
+        op->mix_mask_used[p] = false;
+        op->mix_pitch[p] = 0;
+        op->mix_nbp[p] = 0;
         if (n[p] > 1024) continue;
-        std::vector<uint64_t> mp(h.begin() + off[p] + rlo[p], h.begin() + off[p] + rhi[p]);
-        bool use = false;
-        op->nb_pick[p] = pick_nb_masked(mp, sseg[p], slen[p], (int)n[p], &use);
-        op->mask_used[p] = use;
-        // tiles that follow the zero blocks (FP64 passes only), when they cut
-        // the dense estimate by 7 % and beat the 16-column grid by 5 %
-        std::vector<int> sp(hs.begin() + 2 * (off[p] + rlo[p]), hs.begin() + 2 * (off[p] + rhi[p]));
-        int nbq = 0, P = 0;
-        double pc = 0.0;
-        pick_pitch(sp, sseg[p], slen[p], (int)n[p], &nbq, &P, &pc);
-        double dense = 1e300;
-        for (int nb = 1; nb <= 8; ++nb)
-            dense = std::min(dense, pass_cost(mp, sseg[p], slen[p], (int)n[p], nb, true));
-        const double gc =
-            use ? pass_cost(mp, sseg[p], slen[p], (int)n[p], op->nb_pick[p], false) : dense;
-        if (P > 0 && pc < 0.93 * dense && pc < 0.95 * gc) {
+        // FP64 passes: their t-range; split-BF16 passes: every t (the
+        // pitch only where it equals the FP64 one: one tile-mask array)
+        const Choice cf = choose(p, rlo[p], rhi[p], sseg[p], slen[p]);
+        const Choice cm = choose(p, 0, nseg[p] * klen[p], nseg[p], klen[p]);
+        op->nb_pick[p] = cf.nb;
+        op->mask_used[p] = cf.use;
+        op->mix_nb_pick[p] = cm.nb;
+        op->mix_mask_used[p] = cm.use;
+        if (cm.pitch > 0 && cm.pitch == cf.pitch) {
+            op->mix_pitch[p] = cm.pitch;
+            op->mix_nbp[p] = cm.nbp;
+        }
+        if (cf.pitch > 0) {
+            const int P = cf.pitch;
             op->pitch[p] = P;
-            op->nbp[p] = nbq;
+            op->nbp[p] = cf.nbp;
             const int nt = pitch_tiles((int)n[p], P);
             const int *spf = hs.data() + 2 * off[p];  // every row's span (all t)
             for (long r = 0; r < nseg[p] * klen[p]; ++r) {
@@ -1103,9 +1130,9 @@ int build_masks(dage_operator op) {
         if (std::sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
             for (int p = 0; p < 3; ++p)
                 if (v[p] >= 1 && v[p] <= 8) {
-                    op->nb_pick[p] = v[p];
-                    op->mask_used[p] = n[p] <= 1024;
-                    op->pitch[p] = 0;
+                    op->nb_pick[p] = op->mix_nb_pick[p] = v[p];
+                    op->mask_used[p] = op->mix_mask_used[p] = n[p] <= 1024;
+                    op->pitch[p] = op->mix_pitch[p] = 0;
                 }
     }
     return DAGE_OK;

@@ -583,18 +583,18 @@ int apply_mixed_t(dage_operator op, const double *in, double *out, const BatchSp
     // passes whose zero-panel masks pay use them, with the column-block count
     // the mask cost model chose
     auto nb_of = [&](int p, int dflt) {
-        if (!bs.nb[p] && op->pitch[p] > 0) return op->nbp[p];
-        return bs.nb[p] ? bs.nb[p] : (op->mask_used[p] ? op->nb_pick[p] : dflt);
+        if (!bs.nb[p] && op->mix_pitch[p] > 0) return op->mix_nbp[p];
+        return bs.nb[p] ? bs.nb[p] : (op->mix_mask_used[p] ? op->mix_nb_pick[p] : dflt);
     };
     const int nbx = nb_of(0, L.nbx), nby = nb_of(1, L.nby), nbz = nb_of(2, L.nbz);
     const long mrx = (long)T * nxi, mry = (long)T * nyi;
     // masks of pass p starting at row r: the pitch tiling's when it is used
     // (no caller column-block count), else the 16-column ones if they pay
     auto set_mask = [&](SGemm &g, int p, long r, long mS) {
-        if (!bs.nb[p] && op->pitch[p] > 0) {
-            g.pitch = op->pitch[p];
+        if (!bs.nb[p] && op->mix_pitch[p] > 0) {
+            g.pitch = op->mix_pitch[p];
             g.mask = op->d_tmask + r;
-        } else if (op->mask_used[p]) {
+        } else if (op->mix_mask_used[p]) {
             g.mask = op->d_mask + r;
         }
         if (g.mask) g.mS = mS;

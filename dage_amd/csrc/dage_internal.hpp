@@ -230,6 +230,11 @@ struct dage_operator_s {
     // then holds per row one bit per such tile, same row order as d_mask
     uint64_t *d_tmask = nullptr;
     int pitch[3] = {0, 0, 0}, nbp[3] = {0, 0, 0};
+    // the same choice for the split-BF16 passes, which run every t (the FP64
+    // ones are chosen for the t-points between the low-rank head and the
+    // banded tail); mix_pitch[p] is 0 or pitch[p]
+    int mix_nb_pick[3] = {0, 0, 0}, mix_pitch[3] = {0, 0, 0}, mix_nbp[3] = {0, 0, 0};
+    bool mix_mask_used[3] = {false, false, false};
     // Nonzero bands for the fused x/y pass: d_xband[t*nxt + j] = [first,
     // last+1) k-step (4 input x points) of Fx_t rows reaching output x tile j
     // (16 points); d_yband[t*nyc + c] = [first, last+1) 16-point output y tile
