@@ -194,7 +194,8 @@ def bench_config2(ctx, args):
     ms = ctx.time_end() / steps
     wall = time.perf_counter() - wall0
     names = ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
-             "band_xmode", "band_ymode", "band_zmode", "lowrank", "splitk_reduce_kernel",
+             "band_xmode", "band_ymode", "band_xymode", "band_zmode", "lowrank",
+             "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",
              "local_expansion_gemm")
@@ -337,7 +338,8 @@ def bench_config1(ctx, args):
     ms = ctx.time_end() / steps
     kernels = {}
     for name in ("xy_fused_kernel", "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode",
-                 "band_xmode", "band_ymode", "band_zmode", "lowrank", "splitk_reduce_kernel"):
+                 "band_xmode", "band_ymode", "band_xymode", "band_zmode", "lowrank",
+                 "splitk_reduce_kernel"):
         tot, k = ctx.profile_query(name)
         if k:
             kernels[name] = {"ms_per_launch": round(tot / k, 4), "launches_per_apply": k // steps}
@@ -702,11 +704,12 @@ def bench_config5(ctx, args):
     # batch_*: the sweep's passes (the batch's own event names); gemm_f64_*:
     # the GBFMM near field's
     names = ("batch_xy_fused_kernel", "batch_f64_xmode", "batch_f64_ymode", "batch_f64_zmode",
-             "batch_band_xmode", "batch_band_ymode", "batch_band_zmode", "batch_lowrank",
+             "batch_band_xmode", "batch_band_ymode", "batch_band_xymode", "batch_band_zmode",
+             "batch_lowrank",
              "batch_splitk_reduce",
              "xy_fused_kernel",
              "gemm_f64_xmode", "gemm_f64_ymode", "gemm_f64_zmode", "band_xmode", "band_ymode",
-             "band_zmode", "lowrank",
+             "band_xymode", "band_zmode", "lowrank",
              "splitk_reduce_kernel",
              "bubbles_inject_kernel", "box_multipoles_kernel", "box_multipoles_gemm",
              "m2m_kernel", "m2l_gemm", "l2l_kernel", "local_expansion_kernel",

@@ -1260,6 +1260,7 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
     const char *nm_z = bat ? "batch_f64_zmode" : "gemm_f64_zmode";
     const char *nm_r = bat ? "batch_splitk_reduce" : "splitk_reduce_kernel";
     const char *nm_bx = bat ? "batch_band_xmode" : "band_xmode";
+    const char *nm_bxy = bat ? "batch_band_xymode" : "band_xymode";
     const char *nm_by = bat ? "batch_band_ymode" : "band_ymode";
     const char *nm_bz = bat ? "batch_band_zmode" : "band_zmode";
     // t-points from tb on (narrow bands) take the banded passes; the MFMA
@@ -1319,13 +1320,15 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
         int nb1;
         set_tiles(g1, 0, (long)t0 * nxi, nxi, &nb1);
-        if (banded)
+        if (banded && band_xy_fused())
+            DAGE_TRY(band_pass_xy(ctx, m, in, U, W, wS, t0, tc, B, nm_bxy));
+        else if (banded)
             DAGE_TRY(band_pass_x(ctx, m, in, V, t0, tc, B, nm_bx));
         else
             DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
 
         if (banded) {
-            DAGE_TRY(band_pass_y(ctx, m, V, U, W, wS, t0, tc, B, nm_by));
+            if (!band_xy_fused()) DAGE_TRY(band_pass_y(ctx, m, V, U, W, wS, t0, tc, B, nm_by));
         } else {
         // pass 2, batch (t, b): U_b((xo,yo),(zi,t)) = w_{b,t} V_{b,t}((xo,zi), yi) . Fy_t(yi, yo)
         Gemm g2{};

@@ -205,6 +205,97 @@ __global__ void __launch_bounds__(256) band_x_kernel(BandX a, const double *__re
     }
 }
 
+struct BandXY {
+    const double *in;  // in_b(xi, yi, zi) at in + b*nin + xi + nxi*(yi + nyi*zi)
+    double *U;         // U_{b,tl}(xo, yo, zi) at U + (b*tc + tl)*Ut + xo + nxo*(yo + nyo*zi)
+    const double *w;   // weight of (b, t): w[b*wS + t0 + tl]
+    long wS, nin, Ut;
+    int nxi, nyi, nzi, nxo, nyo;
+    int t0, tc, tl0, tr, B;  // chunk start and length; this launch's t-points
+    int nopx;                // padded row length of the transposed x table
+    long tabx_ts, taby_ts;   // per-t strides of the x (transposed) and y (row-major) tables
+    int nxt, nyt;            // tiles along x and y
+};
+
+// The banded x and y passes of one (right-hand side, z-plane) tile without
+// V in memory: U(xo, yo) = w_t sum_yi F_y(yo, yi) sum_xi F_x(xo, xi) in(xi, yi)
+// for xo in a tile of 4R, yo in a tile of 4RY, and the launch's t-points.
+// The block stages in(x-window, y-window) in LDS (64 rows = 4RY + 2HC y
+// points); the x sums run as in band_x_kernel (one y row per lane, window in
+// registers for every t, tap by tap over R outputs); V (64 rows x 4R) goes
+// through the same LDS buffer to the y sums, which run as in band_y_kernel
+// (one x point per lane, coefficients uniform, an output's taps in order).
+// The same sums in the same order as the two kernels: bit-identical U.
+template <int HC, int R, int RY>
+__global__ void __launch_bounds__(256) band_xy_kernel(BandXY a, const double *__restrict__ tabx,
+                                                      const double *__restrict__ taby) {
+    constexpr int XT = 4 * R, YT = 4 * RY;
+    static_assert(YT + 2 * HC == 64, "one y row per lane");
+    constexpr int WI = XT + 2 * HC;  // input points per row
+    constexpr int LSI = WI | 1, LSV = XT | 1;
+    __shared__ double S[64 * LSI];
+    const unsigned L = xcd_order(blockIdx.x, gridDim.x);
+    const int xt = (int)(L % (unsigned)a.nxt);
+    unsigned rest = L / (unsigned)a.nxt;
+    const int yt = (int)(rest % (unsigned)a.nyt);
+    rest /= (unsigned)a.nyt;
+    const int zi = (int)(rest % (unsigned)a.nzi);
+    const int b = (int)(rest / (unsigned)a.nzi);
+    const int x0 = xt * XT, y0 = yt * YT;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const double *src = a.in + (long)b * a.nin + (long)a.nxi * a.nyi * zi;
+    for (int e = tid; e < 64 * WI; e += 256) {
+        const int row = e / WI, xl = e - row * WI;
+        const int yi = min(max(y0 - HC + row, 0), a.nyi - 1);  // rows outside carry 0 weight
+        const int x = min(max(x0 - HC + xl, 0), a.nxi - 1);
+        S[row * LSI + xl] = src[x + (long)a.nxi * yi];
+    }
+    __syncthreads();
+    double win[R + 2 * HC];
+#pragma unroll
+    for (int i = 0; i < R + 2 * HC; ++i) win[i] = S[lane * LSI + wave * R + i];
+    __syncthreads();  // the buffer holds V from here on
+    for (int k = 0; k < a.tr; ++k) {
+        const int tl = a.tl0 + k;
+        // x sums: V(row = lane, xo = x0 + wave*R + r)
+        const double *tt =
+            tabx + (long)k * a.tabx_ts + (long)(kBandH - HC) * a.nopx + x0 + wave * R;
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0;
+#pragma unroll
+        for (int j = 0; j <= 2 * HC; ++j) {
+            const double *c = tt + (long)j * a.nopx;
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = fma(c[r], win[r + j], acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) S[lane * LSV + wave * R + r] = acc[r];
+        __syncthreads();
+        // y sums: U(xo = x0 + lane, yo = y0 + wave*RY + ry)
+        if (lane < XT) {
+            const int xo = x0 + lane;
+            double wy[RY + 2 * HC];
+#pragma unroll
+            for (int i = 0; i < RY + 2 * HC; ++i) wy[i] = S[(wave * RY + i) * LSV + lane];
+            const double alpha = a.w[(long)b * a.wS + a.t0 + tl];
+            const double *ty = taby + (long)k * a.taby_ts + (kBandH - HC);
+            double *dst = a.U + ((long)b * a.tc + tl) * a.Ut + xo + (long)a.nxo * a.nyo * zi;
+#pragma unroll
+            for (int ry = 0; ry < RY; ++ry) {
+                const int yo = y0 + wave * RY + ry;
+                const double *c = ty + (long)min(yo, a.nyo - 1) * kBandW;
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j <= 2 * HC; ++j) s = fma(c[j], wy[ry + j], s);
+                if (xo < a.nxo && yo < a.nyo) dst[(long)a.nxo * yo] = alpha * s;
+            }
+        }
+        __syncthreads();  // V read before the next t overwrites it
+    }
+}
+
 struct BandZ {
     const double *U;  // U_{b,tl}(l, zi) at U + (b*tc + tl)*Ut + l + M3*zi, l = xo + nxo*yo
     double *C;        // C_b(l, zo) at C + b*cS + l + M3*zo
@@ -405,6 +496,48 @@ int for_runs(const ApplyMats &m, int t0, int tc, Fn fn) {
 }
 
 }  // namespace
+
+int band_pass_xy(dage_ctx ctx, const ApplyMats &m, const double *in, double *U, const double *w,
+                 long wS, int t0, int tc, int B, const char *name) {
+    constexpr int R = 14, RY = 12;
+    BandXY a{};
+    a.in = in;
+    a.U = U;
+    a.w = w;
+    a.wS = wS;
+    a.nxi = m.din[0]; a.nyi = m.din[1]; a.nzi = m.din[2];
+    a.nxo = m.dout[0]; a.nyo = m.dout[1];
+    a.nin = (long)a.nxi * a.nyi * a.nzi;
+    a.Ut = (long)a.nxo * a.nyo * a.nzi;
+    a.t0 = t0; a.tc = tc; a.B = B;
+    a.nopx = band_nop(a.nxo);
+    a.tabx_ts = (long)kBandW * a.nopx;
+    a.taby_ts = (long)a.nyo * kBandW;
+    a.nxt = (a.nxo + 4 * R - 1) / (4 * R);
+    a.nyt = (a.nyo + 4 * RY - 1) / (4 * RY);
+    return for_runs(m, t0, tc, [&](int first, int count, int) {
+        BandXY r = a;
+        r.tl0 = first - t0;
+        r.tr = count;
+        const double *tx = m.band_x + (long)(first - m.band_t0) * a.tabx_ts;
+        const double *ty = m.band_y + (long)(first - m.band_t0) * a.taby_ts;
+        const long blocks = (long)r.nxt * r.nyt * r.nzi * r.B;
+        DAGE_REQUIRE(blocks < (1L << 31), DAGE_ERR_ARG, "banded x/y pass too large (%ld blocks)",
+                     blocks);
+        hipEvent_t ev;
+        prof_begin(ctx, name, &ev);
+        hipLaunchKernelGGL((band_xy_kernel<kBandH, R, RY>), dim3((unsigned)blocks), dim3(256), 0,
+                           ctx->stream, r, tx, ty);
+        DAGE_HIP_CHECK(hipGetLastError());
+        prof_end(ctx, name, ev);
+        return DAGE_OK;
+    });
+}
+
+bool band_xy_fused() {
+    const char *e = std::getenv("DAGE_APPLY_BANDXY");
+    return !(e && e[0] == '0');
+}
 
 bool band_pays(const ApplyMats &m, int B) {
     if (!m.band_y) return false;

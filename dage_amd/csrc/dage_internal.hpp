@@ -444,6 +444,11 @@ int band_pass_x(dage_ctx ctx, const ApplyMats &m, const double *in, double *V, i
                 int B, const char *name);
 int band_pass_y(dage_ctx ctx, const ApplyMats &m, const double *V, double *U, const double *w,
                 long wS, int t0, int tc, int B, const char *name);
+// The banded x and y passes in one kernel (no V in memory; the same sums in
+// the same order as band_pass_x then band_pass_y); off with DAGE_APPLY_BANDXY=0.
+int band_pass_xy(dage_ctx ctx, const ApplyMats &m, const double *in, double *U, const double *w,
+                 long wS, int t0, int tc, int B, const char *name);
+bool band_xy_fused();
 int band_pass_z(dage_ctx ctx, const ApplyMats &m, const double *U, double *C, long cS,
                 bool accumulate, int t0, int tc, int B, const char *name);
 // Bubbles (inject.hip, abi.hip).

@@ -483,6 +483,37 @@ def test_lowrank_points_match_the_dense_passes(ctx, monkeypatch, n, energy):
         assert e < 1e-12
 
 
+@pytest.mark.parametrize("n,energy", [(61, -0.7), (97, None)])
+def test_fused_banded_xy_pass_is_bit_identical(ctx, monkeypatch, n, energy):
+    """The banded x and y passes in one kernel (V kept in LDS) against the
+    two separate banded kernels: the same sums in the same order, so the
+    apply is bit-identical (axes of 61 and 97 points leave partial 56-point
+    x tiles and 48-point y tiles)."""
+    ax = O.Axis.ncell_fixed(-0.05 * (n - 1), 0.05 * (n - 1), 7, (n - 1) // 6)
+    q = quad()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    P = ax.points
+    rng = np.random.default_rng(21)
+    rho = O.gauss_density(np.array([1.0, -0.3]), np.array([1.7, 0.6]),
+                          np.array([[0.1, 0.2, -0.3], [1.1, -0.8, 0.5]]), P, P, P)
+    rho = np.asfortranarray(rho + 1e-3 * rng.uniform(-1.0, 1.0, rho.shape))
+    monkeypatch.setenv("DAGE_APPLY_BAND", "2")
+    if energy is None:
+        op = dage_amd.Coulomb3D(ctx, g, g, q)
+    else:
+        op = dage_amd.Helmholtz3D(ctx, g, g, q, energy=0.0)
+        op.set_energy(energy)
+    assert op.tiling()["band_t0"] < q.get_npoints()
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("DAGE_APPLY_BANDXY", flag)
+        res[flag] = op.transform_cube(rho)
+    op.close()
+    ndiff = int(np.count_nonzero(res["1"] != res["0"]))
+    print(f"n {n}: fused banded x/y vs separate: {ndiff} points differ")
+    assert ndiff == 0
+
+
 def test_apply_far_entries_dominate(ctx):
     """Two compact unit charges 20 bohr apart on a 161^3 grid: around the
     midpoint the density is zero to underflow and the whole potential comes
