@@ -58,6 +58,11 @@ struct Gemm {
     int gx, gy, gz;
     int remap;  // 0: plain row-tile-fastest order (for comparison runs)
     int batch_inner;  // remap order walks z_lo before the row tile (x pass)
+    // PASS 4 (x pass, t-groups): a block covers 4 t-points of tcount (z_lo
+    // = t-group) with the same A; column group j of a tile is t = j / (NB/4),
+    // columns (j % (NB/4))*16.., at B + t*tB and C + t*tC
+    long tB, tC;
+    int tcount;
 };
 
 // Block coordinates of a one-dimensional launch.  Hardware hands block L to
@@ -118,7 +123,11 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     // a CU, and independent blocks hide more latency than a second buffer
     // does (x and z passes 5 % faster at 241^3 than double buffering at four
     // blocks per CU).  The column-major GEMMs keep two buffers.
-    constexpr int NBUF = (PASS <= 2) ? 1 : 2;
+    constexpr int NBUF = (PASS <= 2 || PASS == 4) ? 1 : 2;
+    // PASS 4: the x pass with TG t-points per block sharing the A panels
+    constexpr bool TGRP = PASS == 4;
+    constexpr int TG = TGRP ? 4 : 1;
+    constexpr int CG = TGRP ? (NB >= 4 ? NB / 4 : 1) : NB;  // column groups per t
     __shared__ double As[NBUF][KB * LDA];
     __shared__ double Bs[NBUF][KB * LDB];
 
@@ -128,7 +137,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     int bx, by, bz;
     block_coords(g, &bx, &by, &bz);
     const int m0 = bx * BM;
-    const int n0 = g.pitch > 0 ? by * g.pitch : by * BN;
+    const int n0 = g.pitch > 0 ? by * g.pitch : by * 16 * CG;
     const int nend = g.pitch > 0 && by + 1 < g.gy ? n0 + g.pitch : g.N;
 
     const double *A = g.A;
@@ -152,8 +161,9 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         if (g.alpha) alpha = g.alpha[zlo + g.aS1 * zhi];
     }
     const int K = kend - kbeg;
+    const int tval = TGRP ? min(TG, g.tcount - zlo * TG) : 1;  // t-points of this block
     const uint64_t *mrow = nullptr;
-    if (g.mask) mrow = g.mask + (g.Ksplit > 0 ? (long)kbeg : (long)zlo * g.mS);
+    if (g.mask) mrow = g.mask + (g.Ksplit > 0 ? (long)kbeg : (long)zlo * TG * g.mS);
 
     // Loader geometry: every thread loads A_PER elements of the A panel and NB
     // of the B panel at fixed (m, k) / (k, n) steps from one base position, so
@@ -187,6 +197,10 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     const int b_sm0 = bk * LDB + bn;
     auto b_dk = [](int r) { return B_KCONTIG ? 0 : (r / NB) * 16; };
     auto b_dn = [](int r) { return B_KCONTIG ? r * (kThreads / KB) : (r % NB) * 16; };
+    // global position of slot r: column offset within the tile and t-point
+    // (PASS 4), the LDS position (b_dn) keeps the group order
+    auto b_gn = [](int r) { return B_KCONTIG ? r * (kThreads / KB) : ((r % NB) % CG) * 16; };
+    auto b_gt = [](int r) { return TGRP ? (r % NB) / CG : 0; };
 
     // The loaded values stay raw in registers and their validity in a bit
     // mask: the zero for an out-of-range slot is selected when the panel is
@@ -210,8 +224,10 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
         const long kboff = (long)kp * g.sBk;
 #pragma unroll
         for (int r = 0; r < B_PER; ++r) {
-            const bool ok = bk + b_dk(r) < lim && n0 + bn + b_dn(r) < nend;
-            rb[r] = B[ok ? b_base + (long)b_dk(r) * g.sBk + (long)b_dn(r) * g.sBn + kboff : 0];
+            const bool ok = bk + b_dk(r) < lim && n0 + bn + b_gn(r) < nend && b_gt(r) < tval;
+            rb[r] = B[ok ? b_base + (long)b_dk(r) * g.sBk + (long)b_gn(r) * g.sBn +
+                               (long)b_gt(r) * g.tB + kboff
+                         : 0];
             bmask |= (ok ? 1u : 0u) << r;
         }
     };
@@ -259,7 +275,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
     if (listed) {
         if (wave == 0) {
             const uint64_t cols = g.pitch > 0 ? 1ull << by
-                                              : ((1ull << NB) - 1ull) << (n0 / 16);
+                                              : ((1ull << CG) - 1ull) << (n0 / 16);
             int cnt = 0;
             for (int base = 0; base < npanels; base += 64) {
                 const int p = base + lane;
@@ -268,7 +284,8 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
                     const int k0 = p * KB;
                     const int kn = min(KB, K - k0);
                     uint64_t any = 0;
-                    for (int j = 0; j < kn; ++j) any |= mrow[k0 + j];
+                    for (int tl = 0; tl < tval; ++tl)  // PASS 4: any t of the block
+                        for (int j = 0; j < kn; ++j) any |= mrow[(long)tl * g.mS + k0 + j];
                     need = (any & cols) != 0;
                 }
                 const uint64_t bal = __ballot(need);
@@ -359,9 +376,11 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Gemm g) {
             const long crow = (long)(gm % g.M0) * g.sCm0 + (long)(gm / g.M0) * g.sCm1;
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                const int gn = n0 + j * 16 + lrow;
+                const int jt = TGRP ? j / CG : 0, jc = TGRP ? j % CG : j;
+                if (jt >= tval) continue;
+                const int gn = n0 + jc * 16 + lrow;
                 if (gn >= nend) continue;
-                double *dst = C + crow + (long)gn * g.sCn;
+                double *dst = C + (long)jt * g.tC + crow + (long)gn * g.sCn;
                 const double v = alpha * acc[i][j][r];
                 *dst = g.accumulate ? *dst + v : v;
             }
@@ -634,8 +653,11 @@ int remap_default(double a_bytes) {
 template <bool AM, bool BK, int PASS>
 int launch_nb(const Gemm &g0, int nb, int zdim, hipStream_t s) {
     const int BM = 64 * kMW;
-    const int ntiles =
-        g0.pitch > 0 ? pitch_tiles(g0.N, g0.pitch) : (g0.N + 16 * nb - 1) / (16 * nb);
+    // PASS 4 tiles are 16*nb/4 columns wide (nb/4 groups for each of 4 t)
+    const int tw = PASS == 4 ? 16 * std::max(1, nb / 4) : 16 * nb;
+    DAGE_REQUIRE(PASS != 4 || (g0.pitch <= 0 && (nb == 4 || nb == 8)), DAGE_ERR_STATE,
+                 "t-grouped x pass needs 16-column tiles and 4 or 8 column groups");
+    const int ntiles = g0.pitch > 0 ? pitch_tiles(g0.N, g0.pitch) : (g0.N + tw - 1) / tw;
     DAGE_REQUIRE(g0.pitch <= 0 || (ntiles <= 64 && g0.pitch <= 16 * nb &&
                                    g0.N - (ntiles - 1) * g0.pitch <= 16 * nb),
                  DAGE_ERR_STATE, "column pitch %d does not fit %d-column tiles", g0.pitch, 16 * nb);
@@ -644,13 +666,19 @@ int launch_nb(const Gemm &g0, int nb, int zdim, hipStream_t s) {
     g.gy = ntiles;
     g.gz = zdim;
     g.remap = remap_default(8.0 * g.M * (double)(g.Ksplit > 0 ? g.Ksplit : g.K));
-    g.batch_inner = PASS == 0;  // the x pass's A (the input cube) is the same for every t
+    g.batch_inner = PASS == 0 || PASS == 4;  // the x pass's A (the input) is the same for every t
     const long blocks = (long)g.gx * g.gy * g.gz;
     DAGE_REQUIRE(g.Z0 <= 0 || zdim % g.Z0 == 0, DAGE_ERR_STATE,
                  "batch grid %d is not a multiple of %d", zdim, g.Z0);
     DAGE_REQUIRE(blocks > 0 && blocks < (1L << 31), DAGE_ERR_ARG,
                  "GEMM of %ld tiles does not fit one launch", blocks);
     const dim3 grid((unsigned)blocks);
+    if constexpr (PASS == 4) {
+        if (nb == 4) launch_t<4, AM, BK, PASS>(g, grid, s);
+        else launch_t<8, AM, BK, PASS>(g, grid, s);
+        DAGE_HIP_CHECK(hipGetLastError());
+        return DAGE_OK;
+    }
     switch (nb) {
     case 1: launch_t<1, AM, BK, PASS>(g, grid, s); break;
     case 2: launch_t<2, AM, BK, PASS>(g, grid, s); break;
@@ -670,12 +698,13 @@ int launch_nb(const Gemm &g0, int nb, int zdim, hipStream_t s) {
 template <int PASS>
 int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim, int nb_override = 0) {
     // the k-contiguous A loader steps rows linearly: one-level row index only
-    DAGE_REQUIRE(PASS != 0 || g.M0 >= g.M, DAGE_ERR_STATE, "x-mode GEMM needs a one-level M");
+    DAGE_REQUIRE((PASS != 0 && PASS != 4) || g.M0 >= g.M, DAGE_ERR_STATE,
+                 "x-mode GEMM needs a one-level M");
     const int nb = (nb_override >= 1 && nb_override <= 8) ? nb_override : pick_nb(g.N);
     hipEvent_t ev;
     prof_begin(ctx, name, &ev);
     int rc;
-    if constexpr (PASS == 0) rc = launch_nb<false, false, PASS>(g, nb, zdim, ctx->stream);
+    if constexpr (PASS == 0 || PASS == 4) rc = launch_nb<false, false, PASS>(g, nb, zdim, ctx->stream);
     else rc = launch_nb<true, true, PASS>(g, nb, zdim, ctx->stream);
     prof_end(ctx, name, ev);
     return rc;
@@ -689,6 +718,16 @@ int launch_gemm(dage_ctx ctx, const char *name, const Gemm &g, int zdim, int nb_
 // wavefront per SIMD once the y accumulators pass 128 registers.  Returns
 // the kernel's NT (0: separate passes).
 constexpr int kMaxNT = 28;
+
+// The x pass with 4 t-points per block (PASS 4) where the cost model picks
+// 16- or 32-column tiles; DAGE_APPLY_TGROUP=0 keeps one t per block.
+// Measured on the 385^3 sweep's middle t-points: 23.6 ms per launch against
+// 24.1 ms for one t per block on 32-column tiles (the A panel of 64 input
+// rows feeds 4 column groups instead of 2).
+bool tgroup_x() {
+    const char *e = std::getenv("DAGE_APPLY_TGROUP");
+    return !(e && e[0] == '0');
+}
 
 int fused_nt(const ApplyMats &m) {
     if (!m.xband || !m.yband) return 0;
@@ -1320,12 +1359,25 @@ int apply_f64_batch(dage_ctx ctx, const ApplyMats &m, const double *in, double *
         g1.Z0 = tc; g1.bA1 = nin; g1.bB1 = 0; g1.bC1 = Vt * tc;
         int nb1;
         set_tiles(g1, 0, (long)t0 * nxi, nxi, &nb1);
-        if (banded && band_xy_fused())
+        if (banded && band_xy_fused()) {
             DAGE_TRY(band_pass_xy(ctx, m, in, U, W, wS, t0, tc, B, nm_bxy));
-        else if (banded)
+        } else if (banded) {
             DAGE_TRY(band_pass_x(ctx, m, in, V, t0, tc, B, nm_bx));
-        else
+        } else if (tgroup_x() && tc >= 2 && g1.pitch <= 0 && (nb1 == 1 || nb1 == 2)) {
+            // 4 t-points per block share each loaded input panel, on 16-column
+            // tiles (4 column groups: 5 wavefronts per SIMD; 32-column tiles,
+            // 8 groups at 3 per SIMD, were slower)
+            Gemm gt = g1;
+            gt.Z0 = (tc + 3) / 4;
+            gt.bB = 4L * nxo * nxi;
+            gt.bC = 4L * Vt;
+            gt.tB = (long)nxo * nxi;
+            gt.tC = Vt;
+            gt.tcount = tc;
+            DAGE_TRY(launch_gemm<4>(ctx, nm_x, gt, gt.Z0 * B, 4));
+        } else {
             DAGE_TRY(launch_gemm<0>(ctx, nm_x, g1, tc * B, nb1));
+        }
 
         if (banded) {
             if (!band_xy_fused()) DAGE_TRY(band_pass_y(ctx, m, V, U, W, wS, t0, tc, B, nm_by));

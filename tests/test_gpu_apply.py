@@ -514,6 +514,36 @@ def test_fused_banded_xy_pass_is_bit_identical(ctx, monkeypatch, n, energy):
     assert ndiff == 0
 
 
+@pytest.mark.parametrize("n,energy", [(97, -0.5), (61, None)])
+def test_t_grouped_x_pass_is_bit_identical(ctx, monkeypatch, n, energy):
+    """The x pass with 4 t-points per block sharing each loaded input panel
+    (32-column tiles of 4 t) against one t per block: the same K panels (the
+    extra ones of the union over t add exact zeros) in the same order, so
+    the apply is bit-identical; t-chunks of 3 and 4 (the last group of a
+    chunk short)."""
+    ax = O.Axis.ncell_fixed(-0.05 * (n - 1), 0.05 * (n - 1), 7, (n - 1) // 6)
+    q = quad()
+    g = lib_grid(ctx, [ax] * 3, 7)
+    rng = np.random.default_rng(33)
+    cube = np.asfortranarray(rng.uniform(-1.0, 1.0, (ax.n,) * 3))
+    monkeypatch.setenv("DAGE_APPLY_NB", "2,2,2")
+    for tchunk in ("3", "4"):
+        monkeypatch.setenv("DAGE_APPLY_TCHUNK", tchunk)
+        if energy is None:
+            op = dage_amd.Coulomb3D(ctx, g, g, q)
+        else:
+            op = dage_amd.Helmholtz3D(ctx, g, g, q, energy=0.0)
+            op.set_energy(energy)
+        res = {}
+        for flag in ("1", "0"):
+            monkeypatch.setenv("DAGE_APPLY_TGROUP", flag)
+            res[flag] = op.transform_cube(cube)
+        op.close()
+        ndiff = int(np.count_nonzero(res["1"] != res["0"]))
+        print(f"n {n}, t-chunk {tchunk}: t-grouped vs one-t x pass: {ndiff} points differ")
+        assert ndiff == 0
+
+
 def test_apply_far_entries_dominate(ctx):
     """Two compact unit charges 20 bohr apart on a 161^3 grid: around the
     midpoint the density is zero to underflow and the whole potential comes
